@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the MI355X path-tracing hot path.
+
+Workload (default): the triangle-mesh scene — Cornell box (10 triangles + quad light of
+radiance 15) around a value-noise-displaced icosphere of 1,310,720 triangles with a GGX
+material, 1920x1080, 64 spp, max depth 8, next-event estimation, FP32.
+
+One step = one full 64-spp render of one 1920x1080 frame (a fresh job with its own seed,
+inputs already resident in HBM). The timed region covers K such renders, bracketed by
+device synchronisation. Mrays/s counts every traced segment (extension + shadow rays).
+
+Printed: one JSON line with the metric, the per-kernel roofline (algorithmic bytes from
+the traversal counters / kernel time from HIP events on the library's stream), and the
+CPU baseline (the FP64 oracle on a bounded pixel subset of the same job, all host cores),
+which also gives the relative RMSE of the GPU image on that subset with identical
+random-number streams.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Mrays/s at ≤1% RMSE vs FP64 oracle; p50/p99 job latency, batched vs plain"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per fetch of each device record (layout.h)
+NODE_B, PRIM_B, MAT_B, LIGHT_B = 64, 48, 96, 48
+PIXEL_RMW_B = 2 * (16 + 16 + 4)  # acc + comp + count, read and written once per launch
+
+
+class HipEvents:
+    """hipEvent timing on an arbitrary stream through libamdhip64."""
+
+    def __init__(self):
+        self.h = C.CDLL("libamdhip64.so")
+        for n in ("hipEventCreate", "hipEventRecord", "hipEventSynchronize", "hipEventElapsedTime",
+                  "hipEventDestroy"):
+            getattr(self.h, n).restype = C.c_int
+        self.h.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.h.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.h.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def create(self):
+        e = C.c_void_p()
+        assert self.h.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.h.hipEventRecord(e, C.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.h.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.h.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return ms.value
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--detail", type=int, default=8, help="icosphere subdivision level (8 = 1,310,720 tris)")
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--depth", type=int, default=8)
+    ap.add_argument("--path", choices=["plain", "batched"], default="plain")
+    ap.add_argument("--cpu-stride", type=int, default=8, help="CPU baseline pixel subset: every Nth row/col")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args(argv)
+
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
+              "(the sharding design is described in DESIGN.md).", file=sys.stderr)
+        return 2
+
+    import rayon_amd as ra
+
+    log("generating scene")
+    g = ra.Generated.generate(ra.GEN.CORNELL_ICOSPHERE, 7, detail=a.detail)
+    ntris = g.desc.meshes[0].num_triangles + 10
+    ctx = ra.Context(0)
+    t0 = time.perf_counter()
+    sc = ctx.scene(g.desc_ptr())
+    info = sc.info()
+    log(f"scene: {info.num_prims} prims, {info.num_nodes} pair nodes, depth {info.max_depth}, "
+        f"build {info.build_ms:.0f} ms, upload {info.upload_ms:.0f} ms, {info.device_bytes / 2**20:.1f} MiB")
+    lf, la, up, fov = g.camera()
+    cam = ra.camera_frame(lf, la, up, fov, a.width, a.height)
+    flags = ra.FLAG.NEE
+    W, H, spp = a.width, a.height, a.spp
+
+    def new_job(seed, extra=0):
+        return ctx.job(sc, ra.job_params(W, H, cam, spp=spp, max_depth=a.depth, seed=seed, flags=flags | extra))
+
+    # one counting pass (untimed) for the algorithmic bytes per segment
+    log("traversal-count pass")
+    cj = new_job(1000, ra.FLAG.COUNT_TRAVERSAL)
+    cj.render_plain(spp)
+    cs = cj.stats()
+    cj.close()
+    seg_c = cs.segments
+    bytes_per_seg = (NODE_B * cs.node_fetches + PRIM_B * cs.prim_fetches + MAT_B * cs.material_fetches
+                     + LIGHT_B * cs.light_fetches) / seg_c
+    log(f"per segment: {cs.node_fetches / seg_c:.2f} node pairs, {cs.prim_fetches / seg_c:.2f} prims, "
+        f"{bytes_per_seg:.0f} B")
+
+    jobs = [new_job(1 + i) for i in range(a.warmup + a.steps)]
+    ev = HipEvents()
+    stream = ctx.stream
+    for j in jobs[:a.warmup]:
+        log("warmup step")
+        j.render_plain(spp)
+    ctx.sync()
+
+    timed = jobs[a.warmup:]
+    evs = [(ev.create(), ev.create()) for _ in timed]
+    ctx.sync()
+    t_start = time.perf_counter()
+    for j, (e0, e1) in zip(timed, evs):
+        ev.record(e0, stream)
+        j.render_plain(spp)
+        ev.record(e1, stream)
+    ctx.sync()
+    t_wall = time.perf_counter() - t_start
+    kern_ms = [ev.elapsed_ms(e0, e1) for e0, e1 in evs]
+    stats = [j.stats() for j in timed]
+    segs = sum(s.segments for s in stats)
+    prim_samples = sum(s.primary_samples for s in stats)
+    value = segs / t_wall / 1e6
+    avg_kernel_ms = float(np.mean(kern_ms))
+    seg_per_launch = segs / len(timed)
+    alg_bytes = seg_per_launch * bytes_per_seg + W * H * PIXEL_RMW_B
+    achieved = alg_bytes / (avg_kernel_ms * 1e-3) / 1e9
+    log(f"timed: {len(timed)} steps, {t_wall:.3f} s, kernel avg {avg_kernel_ms:.1f} ms, {value:.0f} Mrays/s")
+
+    result = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(t_wall / len(timed) * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (procedural scene generated in-process)",
+        "config": {
+            "workload": f"cornell_icosphere: Cornell box + displaced icosphere ({ntris} triangles), "
+                        f"{W}x{H}, {spp} spp, max depth {a.depth}, NEE, GGX",
+            "path": a.path, "kernel": "k_plain (per-pixel megakernel, wave64)",
+            "width": W, "height": H, "spp": spp, "max_depth": a.depth, "triangles": int(ntris),
+        },
+        "primary_samples_per_s": round(prim_samples / t_wall, 1),
+        "segments_per_step": int(seg_per_launch),
+        "bvh": {"build_ms": round(info.build_ms, 1), "pair_nodes": int(info.num_nodes),
+                "depth": int(info.max_depth), "device_MiB": round(info.device_bytes / 2**20, 1)},
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_plain", "avg_launch_ms": round(avg_kernel_ms, 3),
+            "alg_bytes_per_segment": round(bytes_per_seg, 1),
+        },
+    }
+
+    if not a.no_cpu_baseline:
+        from oracle import OracleScene
+        log("cpu baseline: oracle on a pixel subset")
+        job0 = timed[0]
+        img = job0.resolve_linear().reshape(-1, 3).astype(np.float64)
+        ys, xs = np.meshgrid(np.arange(a.cpu_stride // 2, H, a.cpu_stride),
+                             np.arange(a.cpu_stride // 2, W, a.cpu_stride), indexing="ij")
+        pix = (ys * W + xs).reshape(-1).astype(np.int64)
+        t_b = time.perf_counter()
+        osc = OracleScene(g.desc_ptr())
+        t_build = time.perf_counter() - t_b
+        cores = os.cpu_count() or 1
+        t_r = time.perf_counter()
+        sums, osegs = osc.render(job0.params, pixels=pix, nsamples=spp, threads=cores, camera=cam)
+        t_cpu = time.perf_counter() - t_r
+        ref = sums / spp
+        err = float(np.sqrt(((img[pix] - ref) ** 2).sum() / (ref ** 2).sum()))
+        result["cpu_baseline"] = {
+            "value": round(osegs / t_cpu / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 oracle, every {a.cpu_stride}th row and column of the same job ({len(pix)} pixels x "
+                      f"{spp} spp, {osegs} segments, {t_cpu:.1f} s; oracle BVH build {t_build:.1f} s excluded)",
+            "cpu": cpu_model(),
+        }
+        result["rel_rmse_vs_oracle"] = round(err, 5)
+        result["rmse_note"] = f"same random streams, {spp} spp, {len(pix)}-pixel subset, linear RGB"
+        log(f"cpu baseline {osegs / t_cpu / 1e6:.2f} Mrays/s on {cores} cores, rel RMSE {err:.5f}")
+
+    for j in jobs:
+        j.close()
+    sc.close()
+    ctx.close()
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,320 @@
+/*
+ * rayon_hip.h — C ABI of the MI355X path-tracing library (librayon_hip.so).
+ *
+ * The reference drives its GPU render mode through the extern "C" functions of
+ * src/rayon/gpu_renderers/renderer_cuda_device.cuh:8-68 plus the C++ statics
+ * Scene::CudaSceneBuilder::buildGPUScene/freeGPUScene (scenes/scene_builder.hpp:234-239).
+ * This header is the re-designed, reentrant boundary: every call takes a context,
+ * scenes and jobs are opaque handles owned by the library, every function returns a
+ * rayon_status and leaves a thread-local message for rayon_last_error().
+ * The reference's own entry-point names are exported as well; they are declared in
+ * rayon_legacy.h and implemented on top of this API.
+ *
+ * Plain pointers and sizes only: no C++ or framework types cross this boundary.
+ */
+#ifndef RAYON_HIP_H
+#define RAYON_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RAYON_ABI_VERSION 1
+
+typedef enum rayon_status {
+   RAYON_OK = 0,
+   RAYON_ERR_INVALID = 1,   /* bad argument, size or handle                              */
+   RAYON_ERR_HIP = 2,       /* a HIP runtime call failed (message names it)               */
+   RAYON_ERR_OOM = 3,       /* device or host allocation failed                          */
+   RAYON_ERR_BUDGET = 4,    /* admission refused: the HBM budget would be exceeded       */
+   RAYON_ERR_IO = 5,        /* file could not be read                                    */
+   RAYON_ERR_PARSE = 6,     /* scene file could not be parsed                            */
+   RAYON_ERR_UNSUPPORTED = 7
+} rayon_status;
+
+/* ---------------------------------------------------------------------------------
+ * Scene description (host memory, read during rayon_scene_create only).
+ * ------------------------------------------------------------------------------- */
+
+/* Material types. Numbering follows CudaScene::MaterialType (cuda_scene.cuh:26-41);
+ * the last two are additions of this library (no counterpart in the reference). */
+enum {
+   RAYON_MAT_LAMBERTIAN = 0,
+   RAYON_MAT_METAL = 1,
+   RAYON_MAT_MIRROR = 2,
+   RAYON_MAT_ROUGH_MIRROR = 3,
+   RAYON_MAT_GLASS = 4,
+   RAYON_MAT_DIELECTRIC = 5,
+   RAYON_MAT_LIGHT = 6,
+   RAYON_MAT_CONSTANT = 7,
+   RAYON_MAT_SHOW_NORMALS = 8,
+   RAYON_MAT_SDF_MATERIAL = 9,
+   RAYON_MAT_ANISOTROPIC_METAL = 10,
+   RAYON_MAT_THIN_FILM = 11,
+   RAYON_MAT_CLEAR_COAT = 12,
+   RAYON_MAT_GGX_CONDUCTOR = 13,  /* isotropic GGX conductor, VNDF sampled             */
+   RAYON_MAT_GGX_DIELECTRIC = 14  /* rough GGX dielectric (reflect/refract, VNDF)      */
+};
+
+/* Procedural patterns, numbering of CudaScene::ProceduralPattern (cuda_scene.cuh:46-52). */
+enum { RAYON_PATTERN_NONE = 0, RAYON_PATTERN_FIBONACCI_DOTS = 1 };
+
+typedef struct rayon_material {
+   int32_t type;
+   float albedo[3];
+   float emission[3];
+   float roughness;
+   float metallic;
+   float refractive_index;
+   float transmission;
+   float anisotropy;
+   float eta[3];          /* conductor complex IOR, real part                            */
+   float k[3];            /* conductor complex IOR, imaginary part                       */
+   float film_thickness;  /* nm                                                          */
+   float film_ior;
+   int32_t pattern;
+   float pattern_color[3];
+   float pattern_param1;
+   float pattern_param2;
+} rayon_material;
+
+/* Geometry types, numbering of CudaScene::GeometryType (cuda_scene.cuh:95-104). */
+enum {
+   RAYON_GEOM_SPHERE = 0,
+   RAYON_GEOM_RECTANGLE = 1,
+   RAYON_GEOM_DISPLACED_SPHERE = 3,
+   RAYON_GEOM_TRIANGLE = 4
+};
+
+typedef struct rayon_geometry {
+   int32_t type;
+   int32_t material_id;
+   int32_t visible;      /* 0: camera rays pass through (cuda_raytracer.cuh:879-883)    */
+   int32_t has_normals;  /* triangles: interpolate n0..n2                               */
+   float p[9];           /* sphere: cx,cy,cz,r | rectangle: corner,u,v | triangle: v0,v1,v2 */
+   float n[9];           /* triangle vertex normals n0,n1,n2                            */
+} rayon_geometry;
+
+/* Indexed triangle mesh; every triangle becomes one primitive. */
+typedef struct rayon_mesh {
+   const float* positions;   /* 3 floats per vertex                                  */
+   const float* normals;     /* 3 floats per vertex, or NULL (flat shading)          */
+   const int32_t* indices;   /* 3 per triangle                                       */
+   int64_t num_vertices;
+   int64_t num_triangles;
+   int32_t material_id;
+   int32_t visible;
+} rayon_mesh;
+
+/* Analytic SDF primitives, combined by polynomial smooth union into one field. */
+enum { RAYON_SDF_SPHERE = 0, RAYON_SDF_ROUNDED_BOX = 1, RAYON_SDF_TORUS = 2 };
+
+typedef struct rayon_sdf_prim {
+   int32_t type;
+   float center[3];
+   float params[4];  /* sphere: r | rounded box: hx,hy,hz,round | torus: R,r (XZ plane) */
+} rayon_sdf_prim;
+
+typedef struct rayon_scene_desc {
+   const rayon_material* materials;
+   int32_t num_materials;
+   const rayon_geometry* geometries;
+   int64_t num_geometries;
+   const rayon_mesh* meshes;
+   int32_t num_meshes;
+   const rayon_sdf_prim* sdf_prims;
+   int32_t num_sdf_prims;
+   float sdf_smooth_k;       /* smooth-union k                                          */
+   int32_t sdf_material_id;
+   int32_t sdf_max_steps;    /* sphere-tracing step cap                                 */
+   float sdf_eps;            /* hit threshold                                           */
+} rayon_scene_desc;
+
+/* BVH build options. */
+enum {
+   RAYON_BVH_DEFAULT = 0,
+   RAYON_BVH_NO_QUANTIZED = 1 << 0   /* skip the 8-bit quantized wide-node layout       */
+};
+
+typedef struct rayon_scene_info {
+   int64_t num_prims;
+   int64_t num_nodes;         /* binary pair nodes                                       */
+   int64_t num_qnodes;        /* quantized wide nodes                                    */
+   int32_t max_depth;         /* binary tree depth (root = 0)                            */
+   int32_t num_lights;
+   int64_t device_bytes;      /* device memory held by the scene                         */
+   double build_ms;           /* host BVH build time                                     */
+   double upload_ms;
+} rayon_scene_info;
+
+/* ---------------------------------------------------------------------------------
+ * Camera (reference CameraBase::initialize, camera/camera_base.hpp:76-99, FP64).
+ * ------------------------------------------------------------------------------- */
+typedef struct rayon_camera {
+   double center[3];
+   double pixel00[3];
+   double delta_u[3];
+   double delta_v[3];
+   double u[3], v[3], w[3];
+} rayon_camera;
+
+rayon_status rayon_camera_frame(const double look_from[3], const double look_at[3], const double vup[3],
+                                double vfov_deg, int32_t width, int32_t height, rayon_camera* out);
+
+/* ---------------------------------------------------------------------------------
+ * Jobs: one image (camera, resolution, sample budget) over one scene.
+ * ------------------------------------------------------------------------------- */
+enum {
+   RAYON_FLAG_REF_QUIRKS = 1 << 0,    /* metal roughness 0 -> 0.3, fuzz x metal_fuzziness,
+                                          glass uses glass_ior (cuda_raytracer.cuh:432-437,
+                                          :647-660)                                          */
+   RAYON_FLAG_REF_RNG = 1 << 1,       /* sequential per-pixel stream of cuda_utils.cuh:24-36
+                                          seeded as cuda_utils.cu:26; default: counter-based */
+   RAYON_FLAG_NEE = 1 << 2,           /* next-event estimation on Lambertian surfaces       */
+   RAYON_FLAG_ADAPTIVE_REF = 1 << 3,  /* batch-to-batch luminance change test
+                                          (render_acc_kernel.cu:240-271)                     */
+   RAYON_FLAG_ADAPTIVE_STDERR = 1 << 4,/* per-pixel relative standard error test            */
+   RAYON_FLAG_DOF = 1 << 5,           /* thin-lens depth of field (render_acc_kernel.cu:209-221) */
+   RAYON_FLAG_COUNT_TRAVERSAL = 1 << 6 /* also count node/primitive/material fetches
+                                          (rayon_job_stats.node_fetches...); slower      */
+};
+
+typedef struct rayon_job_params {
+   int32_t width, height;
+   float camera_center[3];
+   float pixel00[3];
+   float delta_u[3];
+   float delta_v[3];
+   float cam_u[3];
+   float cam_v[3];
+   int32_t max_depth;
+   int32_t spp;                  /* sample budget per pixel                              */
+   uint64_t seed;
+   uint32_t flags;
+   float light_intensity;        /* reference default 1.0                                */
+   float background_intensity;   /* reference default 1.0                                */
+   float metal_fuzziness;        /* used with RAYON_FLAG_REF_QUIRKS; reference 0.8        */
+   float glass_ior;              /* used with RAYON_FLAG_REF_QUIRKS; reference 1.5        */
+   float dof_aperture;
+   float dof_focus_distance;
+   int32_t min_adaptive_samples; /* adaptive tests start at this many samples            */
+   float adaptive_threshold;     /* REF: relative luminance change; STDERR: rel. std-error */
+} rayon_job_params;
+
+/* Fills params with the reference's defaults (renderer_cuda_device.cu:18-26, :24 of the header). */
+void rayon_job_params_default(rayon_job_params* p);
+/* Copies an FP64 camera frame into the FP32 job fields (the cast of renderer_cuda_device.cu:279-284). */
+void rayon_job_params_set_camera(rayon_job_params* p, const rayon_camera* cam);
+
+typedef struct rayon_job_stats {
+   int64_t samples_done;         /* uniform samples per pixel completed                  */
+   int64_t primary_samples;      /* camera samples traced (adaptive pixels may skip)    */
+   int64_t segments;             /* extension + shadow ray segments traced              */
+   int64_t shadow_segments;
+   int64_t converged_pixels;
+   int32_t done;                 /* sample budget exhausted or every pixel converged    */
+   int32_t pad_;
+   int64_t node_fetches;         /* 64 B pair nodes read (RAYON_FLAG_COUNT_TRAVERSAL)    */
+   int64_t prim_fetches;         /* 48 B primitive records read                         */
+   int64_t material_fetches;     /* 96 B material records read                          */
+   int64_t light_fetches;        /* emitter primitive records read for NEE              */
+} rayon_job_stats;
+
+typedef struct rayon_ctx rayon_ctx;
+typedef struct rayon_scene rayon_scene;
+typedef struct rayon_job rayon_job;
+
+const char* rayon_last_error(void);
+int32_t rayon_abi_version(void);
+
+rayon_status rayon_ctx_create(int32_t device, rayon_ctx** out);
+rayon_status rayon_ctx_destroy(rayon_ctx* ctx);
+/* The HIP stream every kernel of this context is launched on (hipStream_t). */
+void* rayon_ctx_stream(rayon_ctx* ctx);
+rayon_status rayon_ctx_sync(rayon_ctx* ctx);
+
+/* Builds the BVH (binned SAH, 16 bins, leaves <= 4 primitives) on the host and uploads
+ * primitives, materials, nodes and lights. Replaces buildGPUScene (scene_builder_cuda.cu:148-303). */
+rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* desc, uint32_t bvh_flags,
+                                rayon_scene** out);
+rayon_status rayon_scene_info_get(const rayon_scene* scene, rayon_scene_info* out);
+/* Host only, no device needed: builds the BVH of a description and verifies its
+ * invariants (every primitive in exactly one leaf, leaves of 1..4, every child box
+ * containing what lies below it, depth within the traversal stack). Fills counts. */
+rayon_status rayon_bvh_check(const rayon_scene_desc* desc, rayon_scene_info* out);
+/* Index (into rayon_scene_desc order: geometries first, then mesh triangles in order)
+ * of the primitive stored at BVH slot `slot`; for host-side checks. */
+rayon_status rayon_scene_prim_order(const rayon_scene* scene, int64_t* out_orig_ids, int64_t n);
+rayon_status rayon_scene_destroy(rayon_scene* scene);
+
+rayon_status rayon_job_create(rayon_ctx* ctx, rayon_scene* scene, const rayon_job_params* params,
+                              rayon_job** out);
+rayon_status rayon_job_destroy(rayon_job* job);
+/* Clears accumulation, counts and RNG state (camera change; resetDeviceAccumBuffer). */
+rayon_status rayon_job_reset(rayon_job* job);
+
+/* Plain path: one launch of the per-pixel megakernel adds `samples` samples to every
+ * unconverged pixel of the job (renderAccKernel, render_acc_kernel.cu:128-282).
+ * Asynchronous on the context stream. */
+rayon_status rayon_job_render_plain(rayon_ctx* ctx, rayon_job* job, int32_t samples);
+
+/* Batched wavefront path: one progressive pass over `n` jobs, each advanced by
+ * `samples` samples (clamped to its budget), with generate / extend / shade / shadow /
+ * accumulate stages over shared device queues. Asynchronous on the context stream. */
+rayon_status rayon_render_batched(rayon_ctx* ctx, rayon_job* const* jobs, int32_t n, int32_t samples);
+
+/* Copies the per-pixel mean linear radiance (3 floats per pixel) to host memory. */
+rayon_status rayon_job_resolve_linear(rayon_ctx* ctx, rayon_job* job, float* rgb_out);
+/* Gamma-corrected 8-bit image (gammaCorrectKernel, render_acc_kernel.cu:11-44). */
+rayon_status rayon_job_resolve_u8(rayon_ctx* ctx, rayon_job* job, uint8_t* out, int32_t channels, float gamma);
+/* Per-pixel sample counts (negative = converged, as the reference's count buffer). */
+rayon_status rayon_job_sample_counts(rayon_ctx* ctx, rayon_job* job, int32_t* out);
+rayon_status rayon_job_stats_get(rayon_ctx* ctx, rayon_job* job, rayon_job_stats* out);
+
+/* Closest-hit query for explicit rays (origins/dirs: 3 floats each, host memory).
+ * Writes the original primitive index (-1 on miss) and hit distance. For parity tests
+ * of traversal and intersection; runs the same device code as the renderers. */
+rayon_status rayon_trace_closest(rayon_ctx* ctx, rayon_scene* scene, const float* origins, const float* dirs,
+                                 int64_t n, float t_min, float t_max, int64_t* out_prim, float* out_t);
+
+/* Primary camera rays of sample `sample` of a job, exactly as the renderers generate
+ * them (origins/dirs 3 floats per pixel). For parity tests. */
+rayon_status rayon_job_primary_rays(rayon_ctx* ctx, rayon_job* job, int32_t sample, float* origins,
+                                    float* dirs);
+
+/* ---------------------------------------------------------------------------------
+ * Synthetic scene generators (host) and the YAML scene loader.
+ * ------------------------------------------------------------------------------- */
+typedef struct rayon_generated rayon_generated;
+
+enum {
+   RAYON_GEN_BOOK_SPHERES = 0,   /* ground + 3 large + jittered r=0.2 grid             */
+   RAYON_GEN_CORNELL_ICOSPHERE = 1,
+   RAYON_GEN_SDF_TERRAIN = 2
+};
+
+typedef struct rayon_gen_params {
+   int32_t kind;
+   uint64_t seed;
+   int32_t detail;     /* book: unused | cornell: icosphere subdivision level | sdf: terrain grid */
+   int32_t count;      /* sdf: number of SDF primitives                                  */
+} rayon_gen_params;
+
+rayon_status rayon_generate(const rayon_gen_params* params, rayon_generated** out);
+rayon_status rayon_load_yaml(const char* path, rayon_generated** out);
+/* The description stays valid until rayon_generated_free. */
+const rayon_scene_desc* rayon_generated_desc(const rayon_generated* g);
+/* Camera from the generator or the YAML camera section: look_from, look_at, vup, vfov. */
+rayon_status rayon_generated_camera(const rayon_generated* g, double look_from[3], double look_at[3],
+                                    double vup[3], double* vfov);
+float rayon_generated_background_intensity(const rayon_generated* g);
+void rayon_generated_free(rayon_generated* g);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RAYON_HIP_H */

@@ -1,0 +1,138 @@
+"""ctypes binding of liboracle.so — the FP64 CPU restatement used as the checker.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the CPU-baseline
+leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from rayon_amd.abi import Camera, JobParams, Material, SceneDesc
+
+LIB_PATH = Path(__file__).resolve().parent / "liboracle.so"
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python -m rayon_amd.build --oracle`")
+        L = C.CDLL(str(LIB_PATH))
+        vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+        P = C.POINTER
+        L.oracle_scene_create.restype = vp
+        L.oracle_scene_create.argtypes = [P(SceneDesc)]
+        L.oracle_scene_free.argtypes = [vp]
+        L.oracle_scene_num_prims.restype = i64
+        L.oracle_scene_num_prims.argtypes = [vp]
+        L.oracle_scene_num_lights.restype = i32
+        L.oracle_scene_num_lights.argtypes = [vp]
+        L.oracle_trace_closest.argtypes = [vp, P(C.c_float), P(C.c_float), i64, C.c_double, C.c_double, i32,
+                                           P(i64), P(C.c_double)]
+        L.oracle_trace_any.argtypes = [vp, P(C.c_float), P(C.c_float), i64, C.c_double, C.c_double, P(i32)]
+        L.oracle_render.argtypes = [vp, P(JobParams), P(Camera), P(i64), i64, i32, i32, i32, P(C.c_double),
+                                    P(C.c_uint32), P(i64)]
+        L.oracle_render_samples.argtypes = [vp, P(JobParams), P(Camera), P(i64), i64, i32, i32, i32,
+                                            P(C.c_double)]
+        L.oracle_ref_rng_init.restype = C.c_uint32
+        L.oracle_ref_rng_init.argtypes = [i64]
+        L.oracle_ref_rng_next.restype = C.c_float
+        L.oracle_ref_rng_next.argtypes = [P(C.c_uint32)]
+        L.oracle_counter_rng_init.restype = C.c_uint32
+        L.oracle_counter_rng_init.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32]
+        L.oracle_counter_rng_next.restype = C.c_float
+        L.oracle_counter_rng_next.argtypes = [P(C.c_uint32)]
+        L.oracle_camera_frame.argtypes = [P(C.c_double), P(C.c_double), P(C.c_double), C.c_double, i32, i32,
+                                          P(Camera)]
+        L.oracle_adaptive_ref.restype = i32
+        L.oracle_adaptive_ref.argtypes = [i32, P(C.c_double), P(C.c_double), i32, i32, C.c_double]
+        L.oracle_adaptive_stderr.restype = i32
+        L.oracle_adaptive_stderr.argtypes = [C.c_double, C.c_double, i64, i32, C.c_double]
+        L.oracle_gamma_u8.restype = C.c_uint8
+        L.oracle_gamma_u8.argtypes = [C.c_double, i64, C.c_double]
+        L.oracle_scatter_probe.restype = i32
+        L.oracle_scatter_probe.argtypes = [P(Material), C.c_uint32, C.c_float, C.c_float, C.c_float,
+                                           P(C.c_double), P(C.c_double), i32, P(C.c_double), P(C.c_float), i32,
+                                           P(C.c_double), P(C.c_double), P(C.c_double), P(i32)]
+        _lib = L
+    return _lib
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class OracleScene:
+    def __init__(self, desc_ptr):
+        self._h = lib().oracle_scene_create(desc_ptr)
+
+    @property
+    def num_prims(self) -> int:
+        return lib().oracle_scene_num_prims(self._h)
+
+    @property
+    def num_lights(self) -> int:
+        return lib().oracle_scene_num_lights(self._h)
+
+    def trace_closest(self, origins, dirs, t_min=0.001, t_max=3.4e38, brute=False):
+        o = np.ascontiguousarray(origins, np.float32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        n = o.shape[0]
+        prim = np.empty(n, np.int64)
+        t = np.empty(n, np.float64)
+        lib().oracle_trace_closest(self._h, _fp(o), _fp(d), n, t_min, t_max, int(brute),
+                                   prim.ctypes.data_as(C.POINTER(C.c_int64)), _dp(t))
+        return prim, t
+
+    def trace_any(self, origins, dirs, t_min, t_max):
+        o = np.ascontiguousarray(origins, np.float32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        out = np.empty(o.shape[0], np.int32)
+        lib().oracle_trace_any(self._h, _fp(o), _fp(d), o.shape[0], t_min, t_max,
+                               out.ctypes.data_as(C.POINTER(C.c_int32)))
+        return out
+
+    def render(self, params: JobParams, pixels=None, sample0=0, nsamples=1, threads=0, camera: Camera = None,
+               rng_state=None):
+        """Sums of nsamples samples per listed pixel (all pixels if None): (npix, 3) float64."""
+        if pixels is None:
+            pix = None
+            npix = params.width * params.height
+        else:
+            pix = np.ascontiguousarray(pixels, np.int64)
+            npix = pix.shape[0]
+        sums = np.zeros((npix, 3), np.float64)
+        segs = C.c_int64()
+        rs = None
+        if rng_state is not None:
+            rs = rng_state.ctypes.data_as(C.POINTER(C.c_uint32))
+        lib().oracle_render(self._h, C.byref(params), C.byref(camera) if camera is not None else None,
+                            pix.ctypes.data_as(C.POINTER(C.c_int64)) if pix is not None else None, npix, sample0,
+                            nsamples, threads, _dp(sums), rs, C.byref(segs))
+        return sums, segs.value
+
+    def render_samples(self, params: JobParams, pixels, sample0, nsamples, threads=0):
+        pix = np.ascontiguousarray(pixels, np.int64)
+        out = np.zeros((pix.shape[0], nsamples, 3), np.float64)
+        lib().oracle_render_samples(self._h, C.byref(params), None, pix.ctypes.data_as(C.POINTER(C.c_int64)),
+                                    pix.shape[0], sample0, nsamples, threads, _dp(out))
+        return out
+
+    def close(self):
+        if self._h:
+            lib().oracle_scene_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,247 @@
+"""Python binding of librayon_hip.so (the C ABI in include/rayon_hip.h).
+
+The library is the product: a C++/HIP path tracer for MI355X. This module only wraps its
+entry points with ctypes for tests, the benchmark and scripting. It never falls back to
+anything else: if the shared library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from .abi import (  # noqa: F401  (re-exported)
+    Camera, GenParams, Geometry, JobParams, JobStats, Material, Mesh, SceneDesc, SceneInfo, SdfPrim,
+    MAT, GEOM, SDF, FLAG, GEN, STATUS, declare,
+)
+
+LIB_PATH = Path(__file__).resolve().parent / "librayon_hip.so"
+_lib = None
+
+
+class RayonError(RuntimeError):
+    pass
+
+
+def lib() -> C.CDLL:
+    """Loads librayon_hip.so (built in-tree by rayon_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RayonError(f"{LIB_PATH} is not built; run `python -m rayon_amd.build`")
+        _lib = declare(C.CDLL(str(LIB_PATH)))
+    return _lib
+
+
+def _check(status: int, what: str) -> None:
+    if status != 0:
+        msg = lib().rayon_last_error().decode(errors="replace")
+        raise RayonError(f"{what}: {STATUS.get(status, status)}: {msg}")
+
+
+def camera_frame(look_from, look_at, vup, vfov: float, width: int, height: int) -> Camera:
+    cam = Camera()
+    _check(lib().rayon_camera_frame((C.c_double * 3)(*look_from), (C.c_double * 3)(*look_at),
+                                    (C.c_double * 3)(*vup), float(vfov), int(width), int(height),
+                                    C.byref(cam)), "rayon_camera_frame")
+    return cam
+
+
+def bvh_check(desc_ptr) -> SceneInfo:
+    """Host-only BVH build + invariant check (no GPU needed)."""
+    info = SceneInfo()
+    _check(lib().rayon_bvh_check(desc_ptr, C.byref(info)), "rayon_bvh_check")
+    return info
+
+
+def job_params(width: int, height: int, camera: Camera, *, spp: int = 64, max_depth: int = 16,
+               seed: int = 1234, flags: int = 0, **kw) -> JobParams:
+    p = JobParams()
+    lib().rayon_job_params_default(C.byref(p))
+    p.width, p.height, p.spp, p.max_depth, p.seed, p.flags = width, height, spp, max_depth, seed, flags
+    lib().rayon_job_params_set_camera(C.byref(p), C.byref(camera))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(k)
+        setattr(p, k, v)
+    return p
+
+
+class Generated:
+    """A synthetic or YAML-loaded scene description held by the library."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def generate(cls, kind: int, seed: int, detail: int = 0, count: int = 0) -> "Generated":
+        gp = GenParams(kind=kind, seed=seed, detail=detail, count=count)
+        h = C.c_void_p()
+        _check(lib().rayon_generate(C.byref(gp), C.byref(h)), "rayon_generate")
+        return cls(h)
+
+    @classmethod
+    def load_yaml(cls, path: str) -> "Generated":
+        h = C.c_void_p()
+        _check(lib().rayon_load_yaml(str(path).encode(), C.byref(h)), "rayon_load_yaml")
+        return cls(h)
+
+    @property
+    def desc(self) -> SceneDesc:
+        return lib().rayon_generated_desc(self._h).contents
+
+    def desc_ptr(self):
+        return lib().rayon_generated_desc(self._h)
+
+    def camera(self):
+        lf, la, up = (C.c_double * 3)(), (C.c_double * 3)(), (C.c_double * 3)()
+        fov = C.c_double()
+        _check(lib().rayon_generated_camera(self._h, lf, la, up, C.byref(fov)), "rayon_generated_camera")
+        return list(lf), list(la), list(up), fov.value
+
+    def background_intensity(self) -> float:
+        return lib().rayon_generated_background_intensity(self._h)
+
+    def close(self):
+        if self._h:
+            lib().rayon_generated_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib().rayon_ctx_create(device, C.byref(h)), "rayon_ctx_create")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def stream(self) -> int:
+        return lib().rayon_ctx_stream(self._h) or 0
+
+    def sync(self):
+        _check(lib().rayon_ctx_sync(self._h), "rayon_ctx_sync")
+
+    def scene(self, desc_ptr, bvh_flags: int = 0) -> "Scene":
+        h = C.c_void_p()
+        _check(lib().rayon_scene_create(self._h, desc_ptr, bvh_flags, C.byref(h)), "rayon_scene_create")
+        return Scene(self, h)
+
+    def job(self, scene: "Scene", params: JobParams) -> "Job":
+        h = C.c_void_p()
+        _check(lib().rayon_job_create(self._h, scene._h, C.byref(params), C.byref(h)), "rayon_job_create")
+        return Job(self, scene, h, params)
+
+    def render_batched(self, jobs, samples: int):
+        arr = (C.c_void_p * len(jobs))(*[j._h for j in jobs])
+        _check(lib().rayon_render_batched(self._h, arr, len(jobs), samples), "rayon_render_batched")
+
+    def close(self):
+        if self._h:
+            lib().rayon_ctx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Scene:
+    def __init__(self, ctx: Context, h):
+        self.ctx, self._h = ctx, h
+
+    def info(self) -> SceneInfo:
+        i = SceneInfo()
+        _check(lib().rayon_scene_info_get(self._h, C.byref(i)), "rayon_scene_info_get")
+        return i
+
+    def prim_order(self) -> np.ndarray:
+        n = self.info().num_prims
+        out = np.empty(n, np.int64)
+        _check(lib().rayon_scene_prim_order(self._h, out.ctypes.data_as(C.POINTER(C.c_int64)), n),
+               "rayon_scene_prim_order")
+        return out
+
+    def trace_closest(self, origins: np.ndarray, dirs: np.ndarray, t_min: float = 0.001,
+                      t_max: float = 3.4e38):
+        o = np.ascontiguousarray(origins, np.float32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        n = o.shape[0]
+        prim = np.empty(n, np.int64)
+        t = np.empty(n, np.float32)
+        _check(lib().rayon_trace_closest(self.ctx._h, self._h, o.ctypes.data_as(C.POINTER(C.c_float)),
+                                         d.ctypes.data_as(C.POINTER(C.c_float)), n, t_min, t_max,
+                                         prim.ctypes.data_as(C.POINTER(C.c_int64)),
+                                         t.ctypes.data_as(C.POINTER(C.c_float))), "rayon_trace_closest")
+        return prim, t
+
+    def close(self):
+        if self._h:
+            lib().rayon_scene_destroy(self._h)
+            self._h = None
+
+
+class Job:
+    def __init__(self, ctx: Context, scene: Scene, h, params: JobParams):
+        self.ctx, self.scene, self._h, self.params = ctx, scene, h, params
+
+    @property
+    def shape(self):
+        return self.params.height, self.params.width
+
+    def render_plain(self, samples: int):
+        _check(lib().rayon_job_render_plain(self.ctx._h, self._h, samples), "rayon_job_render_plain")
+
+    def reset(self):
+        _check(lib().rayon_job_reset(self._h), "rayon_job_reset")
+
+    def resolve_linear(self) -> np.ndarray:
+        h, w = self.shape
+        out = np.empty((h, w, 3), np.float32)
+        _check(lib().rayon_job_resolve_linear(self.ctx._h, self._h, out.ctypes.data_as(C.POINTER(C.c_float))),
+               "rayon_job_resolve_linear")
+        return out
+
+    def resolve_u8(self, channels: int = 3, gamma: float = 2.0) -> np.ndarray:
+        h, w = self.shape
+        out = np.empty((h, w, channels), np.uint8)
+        _check(lib().rayon_job_resolve_u8(self.ctx._h, self._h, out.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                          channels, gamma), "rayon_job_resolve_u8")
+        return out
+
+    def sample_counts(self) -> np.ndarray:
+        h, w = self.shape
+        out = np.empty((h, w), np.int32)
+        _check(lib().rayon_job_sample_counts(self.ctx._h, self._h, out.ctypes.data_as(C.POINTER(C.c_int32))),
+               "rayon_job_sample_counts")
+        return out
+
+    def stats(self) -> JobStats:
+        s = JobStats()
+        _check(lib().rayon_job_stats_get(self.ctx._h, self._h, C.byref(s)), "rayon_job_stats_get")
+        return s
+
+    def primary_rays(self, sample: int = 0):
+        h, w = self.shape
+        o = np.empty((h * w, 3), np.float32)
+        d = np.empty((h * w, 3), np.float32)
+        _check(lib().rayon_job_primary_rays(self.ctx._h, self._h, sample, o.ctypes.data_as(C.POINTER(C.c_float)),
+                                            d.ctypes.data_as(C.POINTER(C.c_float))), "rayon_job_primary_rays")
+        return o, d
+
+    def close(self):
+        if self._h:
+            lib().rayon_job_destroy(self._h)
+            self._h = None

@@ -1,0 +1,132 @@
+"""ctypes mirror of the structs and functions of include/rayon_hip.h."""
+from __future__ import annotations
+
+import ctypes as C
+
+f3 = C.c_float * 3
+d3 = C.c_double * 3
+
+STATUS = {0: "RAYON_OK", 1: "RAYON_ERR_INVALID", 2: "RAYON_ERR_HIP", 3: "RAYON_ERR_OOM",
+          4: "RAYON_ERR_BUDGET", 5: "RAYON_ERR_IO", 6: "RAYON_ERR_PARSE", 7: "RAYON_ERR_UNSUPPORTED"}
+
+
+class _NS:
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+MAT = _NS(LAMBERTIAN=0, METAL=1, MIRROR=2, ROUGH_MIRROR=3, GLASS=4, DIELECTRIC=5, LIGHT=6, CONSTANT=7,
+          SHOW_NORMALS=8, SDF_MATERIAL=9, ANISOTROPIC_METAL=10, THIN_FILM=11, CLEAR_COAT=12,
+          GGX_CONDUCTOR=13, GGX_DIELECTRIC=14)
+GEOM = _NS(SPHERE=0, RECTANGLE=1, DISPLACED_SPHERE=3, TRIANGLE=4)
+SDF = _NS(SPHERE=0, ROUNDED_BOX=1, TORUS=2)
+FLAG = _NS(REF_QUIRKS=1, REF_RNG=2, NEE=4, ADAPTIVE_REF=8, ADAPTIVE_STDERR=16, DOF=32, COUNT_TRAVERSAL=64)
+GEN = _NS(BOOK_SPHERES=0, CORNELL_ICOSPHERE=1, SDF_TERRAIN=2)
+
+
+class Material(C.Structure):
+    _fields_ = [("type", C.c_int32), ("albedo", f3), ("emission", f3), ("roughness", C.c_float),
+                ("metallic", C.c_float), ("refractive_index", C.c_float), ("transmission", C.c_float),
+                ("anisotropy", C.c_float), ("eta", f3), ("k", f3), ("film_thickness", C.c_float),
+                ("film_ior", C.c_float), ("pattern", C.c_int32), ("pattern_color", f3),
+                ("pattern_param1", C.c_float), ("pattern_param2", C.c_float)]
+
+
+class Geometry(C.Structure):
+    _fields_ = [("type", C.c_int32), ("material_id", C.c_int32), ("visible", C.c_int32),
+                ("has_normals", C.c_int32), ("p", C.c_float * 9), ("n", C.c_float * 9)]
+
+
+class Mesh(C.Structure):
+    _fields_ = [("positions", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
+                ("indices", C.POINTER(C.c_int32)), ("num_vertices", C.c_int64), ("num_triangles", C.c_int64),
+                ("material_id", C.c_int32), ("visible", C.c_int32)]
+
+
+class SdfPrim(C.Structure):
+    _fields_ = [("type", C.c_int32), ("center", f3), ("params", C.c_float * 4)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("materials", C.POINTER(Material)), ("num_materials", C.c_int32),
+                ("geometries", C.POINTER(Geometry)), ("num_geometries", C.c_int64),
+                ("meshes", C.POINTER(Mesh)), ("num_meshes", C.c_int32),
+                ("sdf_prims", C.POINTER(SdfPrim)), ("num_sdf_prims", C.c_int32),
+                ("sdf_smooth_k", C.c_float), ("sdf_material_id", C.c_int32), ("sdf_max_steps", C.c_int32),
+                ("sdf_eps", C.c_float)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("num_prims", C.c_int64), ("num_nodes", C.c_int64), ("num_qnodes", C.c_int64),
+                ("max_depth", C.c_int32), ("num_lights", C.c_int32), ("device_bytes", C.c_int64),
+                ("build_ms", C.c_double), ("upload_ms", C.c_double)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("center", d3), ("pixel00", d3), ("delta_u", d3), ("delta_v", d3), ("u", d3), ("v", d3),
+                ("w", d3)]
+
+
+class JobParams(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("camera_center", f3), ("pixel00", f3),
+                ("delta_u", f3), ("delta_v", f3), ("cam_u", f3), ("cam_v", f3), ("max_depth", C.c_int32),
+                ("spp", C.c_int32), ("seed", C.c_uint64), ("flags", C.c_uint32),
+                ("light_intensity", C.c_float), ("background_intensity", C.c_float),
+                ("metal_fuzziness", C.c_float), ("glass_ior", C.c_float), ("dof_aperture", C.c_float),
+                ("dof_focus_distance", C.c_float), ("min_adaptive_samples", C.c_int32),
+                ("adaptive_threshold", C.c_float)]
+
+
+class JobStats(C.Structure):
+    _fields_ = [("samples_done", C.c_int64), ("primary_samples", C.c_int64), ("segments", C.c_int64),
+                ("shadow_segments", C.c_int64), ("converged_pixels", C.c_int64), ("done", C.c_int32),
+                ("pad_", C.c_int32), ("node_fetches", C.c_int64), ("prim_fetches", C.c_int64),
+                ("material_fetches", C.c_int64), ("light_fetches", C.c_int64)]
+
+
+class GenParams(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("seed", C.c_uint64), ("detail", C.c_int32), ("count", C.c_int32)]
+
+
+def declare(lib: C.CDLL) -> C.CDLL:
+    vp, i32, i64, f32, st = C.c_void_p, C.c_int32, C.c_int64, C.c_float, C.c_int
+    P = C.POINTER
+    sig = {
+        "rayon_last_error": (C.c_char_p, []),
+        "rayon_abi_version": (i32, []),
+        "rayon_camera_frame": (st, [P(C.c_double), P(C.c_double), P(C.c_double), C.c_double, i32, i32,
+                                    P(Camera)]),
+        "rayon_job_params_default": (None, [P(JobParams)]),
+        "rayon_job_params_set_camera": (None, [P(JobParams), P(Camera)]),
+        "rayon_ctx_create": (st, [i32, P(vp)]),
+        "rayon_ctx_destroy": (st, [vp]),
+        "rayon_ctx_stream": (vp, [vp]),
+        "rayon_ctx_sync": (st, [vp]),
+        "rayon_scene_create": (st, [vp, P(SceneDesc), C.c_uint32, P(vp)]),
+        "rayon_scene_info_get": (st, [vp, P(SceneInfo)]),
+        "rayon_bvh_check": (st, [P(SceneDesc), P(SceneInfo)]),
+        "rayon_scene_prim_order": (st, [vp, P(i64), i64]),
+        "rayon_scene_destroy": (st, [vp]),
+        "rayon_job_create": (st, [vp, vp, P(JobParams), P(vp)]),
+        "rayon_job_destroy": (st, [vp]),
+        "rayon_job_reset": (st, [vp]),
+        "rayon_job_render_plain": (st, [vp, vp, i32]),
+        "rayon_render_batched": (st, [vp, P(vp), i32, i32]),
+        "rayon_job_resolve_linear": (st, [vp, vp, P(f32)]),
+        "rayon_job_resolve_u8": (st, [vp, vp, P(C.c_uint8), i32, f32]),
+        "rayon_job_sample_counts": (st, [vp, vp, P(i32)]),
+        "rayon_job_stats_get": (st, [vp, vp, P(JobStats)]),
+        "rayon_trace_closest": (st, [vp, vp, P(f32), P(f32), i64, f32, f32, P(i64), P(f32)]),
+        "rayon_job_primary_rays": (st, [vp, vp, i32, P(f32), P(f32)]),
+        "rayon_generate": (st, [P(GenParams), P(vp)]),
+        "rayon_load_yaml": (st, [C.c_char_p, P(vp)]),
+        "rayon_generated_desc": (P(SceneDesc), [vp]),
+        "rayon_generated_camera": (st, [vp, P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_double)]),
+        "rayon_generated_background_intensity": (f32, [vp]),
+        "rayon_generated_free": (None, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib

@@ -1,0 +1,89 @@
+"""Build recipe for librayon_hip.so (HIP for gfx950) and the oracle's liboracle.so.
+
+    python -m rayon_amd.build            # both libraries, in-tree
+    python -m rayon_amd.build --product  # only librayon_hip.so
+
+Objects go to rayon_amd/_build/; the libraries land next to the sources that load them
+(rayon_amd/librayon_hip.so, oracle/liboracle.so) so they travel with the tree.
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent.parent
+PKG = ROOT / "rayon_amd"
+CSRC = PKG / "csrc"
+OBJ = PKG / "_build"
+LIB = PKG / "librayon_hip.so"
+ORACLE_DIR = ROOT / "oracle"
+ORACLE_LIB = ORACLE_DIR / "liboracle.so"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+# Packed FP32 ops are disabled for every device compile; the Neumaier sums in the
+# accumulation rely on plain IEEE adds, so no fast-math either.
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result", "-Wno-unused-command-line-argument",
+]
+
+
+def _run(cmd: list[str]) -> None:
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + proc.stdout + proc.stderr)
+        raise RuntimeError(f"build step failed: {cmd[-1]}")
+    warn = (proc.stdout + proc.stderr).strip()
+    if warn and "packed-fp32-ops" not in warn:
+        sys.stderr.write(warn + "\n")
+
+
+def build_product(jobs: int = 8) -> Path:
+    OBJ.mkdir(exist_ok=True)
+    srcs = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
+    cmds = []
+    objs = []
+    for s in srcs:
+        o = OBJ / (s.name + ".o")
+        objs.append(o)
+        lang = ["-x", "hip"] if s.suffix == ".hip" else []
+        cmds.append([HIPCC, *HIP_FLAGS, *lang, "-c", str(s), "-o", str(o)])
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        list(ex.map(_run, cmds))
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lpthread"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle() -> Path:
+    tmp = ORACLE_LIB.with_suffix(".so.tmp")
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-pthread", "-Wall",
+          "-o", str(tmp), str(ORACLE_DIR / "oracle.cpp")])
+    os.replace(tmp, ORACLE_LIB)
+    return ORACLE_LIB
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--product", action="store_true")
+    ap.add_argument("--oracle", action="store_true")
+    ap.add_argument("-j", type=int, default=8)
+    a = ap.parse_args(argv)
+    both = not (a.product or a.oracle)
+    if a.product or both:
+        print(build_product(a.j))
+    if a.oracle or both:
+        print(build_oracle())
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,595 @@
+// api.cpp — the C ABI of include/rayon_hip.h: contexts, scene upload, jobs, the plain
+// render path, resolves and parity queries.
+//
+// Reference counterparts: renderer_cuda_device.cu:157-478 (accumulative render,
+// resolve, adaptive buffers) and scene_builder_cuda.cu:148-335 (scene upload/free).
+// Unlike the reference, every call is reentrant (no function-static buffers or global
+// __constant__ knobs), failures return a status instead of printf-and-continue
+// (renderer_cuda_device.cu:262-266, :286-296), and device memory stays owned by the
+// library's handles.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "internal.h"
+#include "kernels.h"
+#include "wavefront.h"
+
+namespace ry {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+   va_list ap;
+   va_start(ap, fmt);
+   vsnprintf(g_err, sizeof(g_err), fmt, ap);
+   va_end(ap);
+}
+
+rayon_status hip_fail(hipError_t e, const char* what) {
+   set_error("%s failed: %s", what, hipGetErrorString(e));
+   return e == hipErrorOutOfMemory ? RAYON_ERR_OOM : RAYON_ERR_HIP;
+}
+
+static rayon_status dev_alloc(std::vector<DevBuf>& bufs, size_t bytes, void** out) {
+   void* p = nullptr;
+   if (bytes) {
+      hipError_t e = hipMalloc(&p, bytes);
+      if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+   }
+   bufs.push_back({p, bytes});
+   *out = p;
+   return RAYON_OK;
+}
+
+static void free_all(std::vector<DevBuf>& bufs) {
+   for (auto& b : bufs)
+      if (b.p) (void)hipFree(b.p);
+   bufs.clear();
+}
+
+template <class T>
+static rayon_status upload(std::vector<DevBuf>& bufs, const std::vector<T>& host, T** out) {
+   void* p = nullptr;
+   rayon_status s = dev_alloc(bufs, host.size() * sizeof(T), &p);
+   if (s != RAYON_OK) return s;
+   if (!host.empty()) RY_HIP(hipMemcpy(p, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice));
+   *out = (T*)p;
+   return RAYON_OK;
+}
+
+static float bits_as_float(uint32_t u) {
+   float f;
+   std::memcpy(&f, &u, 4);
+   return f;
+}
+
+}  // namespace ry
+
+using namespace ry;
+
+extern "C" {
+
+const char* rayon_last_error(void) { return ry::g_err; }
+int32_t rayon_abi_version(void) { return RAYON_ABI_VERSION; }
+
+rayon_status rayon_camera_frame(const double lf[3], const double la[3], const double up[3], double vfov,
+                                int32_t W, int32_t H, rayon_camera* out) {
+   if (!lf || !la || !up || !out || W <= 0 || H <= 0) {
+      set_error("rayon_camera_frame: bad arguments");
+      return RAYON_ERR_INVALID;
+   }
+   // camera_base.hpp:76-99, FP64
+   double w[3], u[3], v[3];
+   double f2 = 0;
+   for (int k = 0; k < 3; ++k) {
+      w[k] = lf[k] - la[k];
+      f2 += w[k] * w[k];
+   }
+   const double focal = std::sqrt(f2);
+   if (!(focal > 0)) {
+      set_error("rayon_camera_frame: look_from equals look_at");
+      return RAYON_ERR_INVALID;
+   }
+   for (int k = 0; k < 3; ++k) w[k] /= focal;
+   const double vh = 2.0 * std::tan(vfov * M_PI / 180.0 / 2.0) * focal;
+   const double vw = vh * ((double)W / H);
+   double c[3] = {up[1] * w[2] - up[2] * w[1], up[2] * w[0] - up[0] * w[2], up[0] * w[1] - up[1] * w[0]};
+   double cl = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+   for (int k = 0; k < 3; ++k) u[k] = c[k] / cl;
+   v[0] = w[1] * u[2] - w[2] * u[1];
+   v[1] = w[2] * u[0] - w[0] * u[2];
+   v[2] = w[0] * u[1] - w[1] * u[0];
+   for (int k = 0; k < 3; ++k) {
+      const double vu = vw * u[k], vv = -vh * v[k];
+      out->center[k] = lf[k];
+      out->delta_u[k] = vu / W;
+      out->delta_v[k] = vv / H;
+      const double ul = lf[k] - focal * w[k] - vu / 2 - vv / 2;
+      out->pixel00[k] = ul + 0.5 * (out->delta_u[k] + out->delta_v[k]);
+      out->u[k] = u[k];
+      out->v[k] = v[k];
+      out->w[k] = w[k];
+   }
+   return RAYON_OK;
+}
+
+void rayon_job_params_default(rayon_job_params* p) {
+   std::memset(p, 0, sizeof(*p));
+   p->width = 640;
+   p->height = 360;
+   p->max_depth = 16;  // constants.hpp:21
+   p->spp = 64;        // constants.hpp:20
+   p->seed = 1234;     // cuda_utils.cu:26 seed of the reference stream
+   p->light_intensity = 1.0f;
+   p->background_intensity = 1.0f;
+   p->metal_fuzziness = 0.8f;  // renderer_cuda_device.cu:20
+   p->glass_ior = 1.5f;        // renderer_cuda_device.cu:21
+   p->dof_aperture = 0.1f;     // renderer_cuda_device.cu:25
+   p->dof_focus_distance = 10.0f;
+   p->min_adaptive_samples = 32;  // renderer_cuda_device.cuh:24
+   p->adaptive_threshold = 0.01f;
+}
+
+void rayon_job_params_set_camera(rayon_job_params* p, const rayon_camera* c) {
+   for (int k = 0; k < 3; ++k) {
+      p->camera_center[k] = (float)c->center[k];
+      p->pixel00[k] = (float)c->pixel00[k];
+      p->delta_u[k] = (float)c->delta_u[k];
+      p->delta_v[k] = (float)c->delta_v[k];
+      p->cam_u[k] = (float)c->u[k];
+      p->cam_v[k] = (float)c->v[k];
+   }
+}
+
+rayon_status rayon_ctx_create(int32_t device, rayon_ctx** out) {
+   if (!out) {
+      set_error("rayon_ctx_create: out is NULL");
+      return RAYON_ERR_INVALID;
+   }
+   int n = 0;
+   RY_HIP(hipGetDeviceCount(&n));
+   if (device < 0 || device >= n) {
+      set_error("rayon_ctx_create: device %d not present (%d devices)", device, n);
+      return RAYON_ERR_INVALID;
+   }
+   RY_HIP(hipSetDevice(device));
+   auto* c = new rayon_ctx();
+   c->device = device;
+   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+   if (e != hipSuccess) {
+      delete c;
+      return hip_fail(e, "hipStreamCreateWithFlags");
+   }
+   *out = c;
+   return RAYON_OK;
+}
+
+rayon_status rayon_ctx_destroy(rayon_ctx* c) {
+   if (!c) return RAYON_OK;
+   (void)hipSetDevice(c->device);
+   (void)hipStreamSynchronize(c->stream);
+   wave_destroy(c);
+   (void)hipStreamDestroy(c->stream);
+   delete c;
+   return RAYON_OK;
+}
+
+void* rayon_ctx_stream(rayon_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+rayon_status rayon_ctx_sync(rayon_ctx* c) {
+   if (!c) {
+      set_error("rayon_ctx_sync: NULL context");
+      return RAYON_ERR_INVALID;
+   }
+   RY_HIP(hipSetDevice(c->device));
+   RY_HIP(hipStreamSynchronize(c->stream));
+   return RAYON_OK;
+}
+
+rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* d, uint32_t bvh_flags, rayon_scene** out) {
+   (void)bvh_flags;
+   if (!ctx || !out) {
+      set_error("rayon_scene_create: NULL argument");
+      return RAYON_ERR_INVALID;
+   }
+   std::vector<HostPrim> prims;
+   rayon_status st = gather_prims(d, prims);
+   if (st != RAYON_OK) return st;
+   auto t0 = std::chrono::steady_clock::now();
+   BuildResult br;
+   if (!build_bvh(prims, br)) {
+      set_error("BVH depth %d exceeds the traversal stack (%d)", br.max_depth, kMaxDepth);
+      return RAYON_ERR_INVALID;
+   }
+   auto t1 = std::chrono::steady_clock::now();
+   const size_t n = prims.size();
+   std::vector<PrimRec> recs(n);
+   std::vector<float4> normals;
+   bool any_normals = false;
+   for (auto& p : prims) any_normals |= (p.flags & F_NORMALS) != 0;
+   if (any_normals) normals.assign(3 * n, float4{0, 0, 0, 0});
+   auto* sc = new rayon_scene();
+   sc->ctx = ctx;
+   sc->orig_of_slot.resize(n);
+   std::vector<int> slot_of(n);
+   for (size_t s = 0; s < n; ++s) {
+      const HostPrim& p = prims[br.order[s]];
+      slot_of[br.order[s]] = (int)s;
+      sc->orig_of_slot[s] = p.orig;
+      const uint32_t info = (p.kind << 28) | p.flags | p.mat;
+      PrimRec& r = recs[s];
+      r.a = {p.a[0], p.a[1], p.a[2], bits_as_float(info)};
+      const float of = bits_as_float((uint32_t)p.orig);
+      if (p.kind == K_TRI) {
+         r.b = {p.b[0] - p.a[0], p.b[1] - p.a[1], p.b[2] - p.a[2], of};
+         r.c = {p.c[0] - p.a[0], p.c[1] - p.a[1], p.c[2] - p.a[2], 0.0f};
+         if (p.flags & F_NORMALS) {
+            for (int v = 0; v < 3; ++v) normals[3 * s + v] = {p.n[3 * v], p.n[3 * v + 1], p.n[3 * v + 2], 0.0f};
+         }
+      } else if (p.kind == K_SPHERE || p.kind == K_DISPLACED) {
+         r.b = {p.b[0], 0.0f, 0.0f, of};
+         r.c = {0, 0, 0, 0};
+      } else if (p.kind == K_RECT) {
+         r.b = {p.b[0], p.b[1], p.b[2], of};
+         r.c = {p.c[0], p.c[1], p.c[2], 0.0f};
+      } else {  // SDF field: bounds
+         r.a = {p.lo[0], p.lo[1], p.lo[2], bits_as_float(info)};
+         r.b = {p.hi[0], p.hi[1], p.hi[2], of};
+         r.c = {0, 0, 0, 0};
+      }
+   }
+   std::vector<MatRec> mats(d->num_materials);
+   for (int i = 0; i < d->num_materials; ++i) mats[i] = pack_material(d->materials[i]);
+   std::vector<int> lslots;
+   std::vector<float> larea;
+   for (size_t i = 0; i < n; ++i) {
+      const HostPrim& p = prims[i];
+      if (!(p.flags & F_EMITTER)) continue;
+      double area = 0;
+      if (p.kind == K_RECT) {
+         double cx = (double)p.b[1] * p.c[2] - (double)p.b[2] * p.c[1];
+         double cy = (double)p.b[2] * p.c[0] - (double)p.b[0] * p.c[2];
+         double cz = (double)p.b[0] * p.c[1] - (double)p.b[1] * p.c[0];
+         area = std::sqrt(cx * cx + cy * cy + cz * cz);
+      } else if (p.kind == K_TRI) {
+         double e1[3], e2[3];
+         for (int k = 0; k < 3; ++k) {
+            e1[k] = (double)p.b[k] - p.a[k];
+            e2[k] = (double)p.c[k] - p.a[k];
+         }
+         double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                cz = e1[0] * e2[1] - e1[1] * e2[0];
+         area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+      } else if (p.kind == K_SPHERE) {
+         area = 4.0 * M_PI * (double)p.b[0] * p.b[0];
+      } else {
+         continue;
+      }
+      lslots.push_back(slot_of[i]);
+      larea.push_back((float)area);
+   }
+   std::vector<SdfRec> sdf;
+   for (int i = 0; i < d->num_sdf_prims; ++i) {
+      const rayon_sdf_prim& q = d->sdf_prims[i];
+      sdf.push_back({{q.center[0], q.center[1], q.center[2], (float)q.type},
+                     {q.params[0], q.params[1], q.params[2], q.params[3]}});
+   }
+   (void)hipSetDevice(ctx->device);
+   SceneView& V = sc->view;
+   PrimRec* dp = nullptr;
+   float4* dn = nullptr;
+   Node2* dnodes = nullptr;
+   MatRec* dm = nullptr;
+   int* dl = nullptr;
+   float* da = nullptr;
+   SdfRec* ds = nullptr;
+   if ((st = upload(sc->bufs, recs, &dp)) != RAYON_OK || (st = upload(sc->bufs, normals, &dn)) != RAYON_OK ||
+       (st = upload(sc->bufs, br.nodes, &dnodes)) != RAYON_OK || (st = upload(sc->bufs, mats, &dm)) != RAYON_OK ||
+       (st = upload(sc->bufs, lslots, &dl)) != RAYON_OK || (st = upload(sc->bufs, larea, &da)) != RAYON_OK ||
+       (st = upload(sc->bufs, sdf, &ds)) != RAYON_OK) {
+      free_all(sc->bufs);
+      delete sc;
+      return st;
+   }
+   auto t2 = std::chrono::steady_clock::now();
+   V.prims = dp;
+   V.normals = dn;
+   V.nodes = dnodes;
+   V.mats = dm;
+   V.light_slots = dl;
+   V.light_area = da;
+   V.sdf = ds;
+   V.num_lights = (int)lslots.size();
+   V.num_sdf = (int)sdf.size();
+   V.root = br.root;
+   V.num_prims = (int)n;
+   V.sdf_k = d->num_sdf_prims > 0 ? d->sdf_smooth_k : 0.1f;
+   V.sdf_eps = d->num_sdf_prims > 0 ? d->sdf_eps : 1e-4f;
+   V.sdf_steps = d->num_sdf_prims > 0 ? d->sdf_max_steps : 0;
+   rayon_scene_info& I = sc->info;
+   I.num_prims = (int64_t)n;
+   I.num_nodes = (int64_t)br.nodes.size();
+   I.num_qnodes = 0;
+   I.max_depth = br.max_depth;
+   I.num_lights = V.num_lights;
+   for (auto& b : sc->bufs) I.device_bytes += (int64_t)b.bytes;
+   I.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+   I.upload_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+   *out = sc;
+   return RAYON_OK;
+}
+
+rayon_status rayon_scene_info_get(const rayon_scene* s, rayon_scene_info* out) {
+   if (!s || !out) {
+      set_error("rayon_scene_info_get: NULL argument");
+      return RAYON_ERR_INVALID;
+   }
+   *out = s->info;
+   return RAYON_OK;
+}
+
+rayon_status rayon_scene_prim_order(const rayon_scene* s, int64_t* out, int64_t n) {
+   if (!s || !out || n != (int64_t)s->orig_of_slot.size()) {
+      set_error("rayon_scene_prim_order: NULL argument or n != number of primitives");
+      return RAYON_ERR_INVALID;
+   }
+   std::memcpy(out, s->orig_of_slot.data(), sizeof(int64_t) * (size_t)n);
+   return RAYON_OK;
+}
+
+rayon_status rayon_scene_destroy(rayon_scene* s) {
+   if (!s) return RAYON_OK;
+   (void)hipSetDevice(s->ctx->device);
+   (void)hipStreamSynchronize(s->ctx->stream);
+   free_all(s->bufs);
+   delete s;
+   return RAYON_OK;
+}
+
+static void fill_jobview(rayon_job* j) {
+   JobView& J = j->view;
+   const rayon_job_params& P = j->params;
+   J.scene = j->scene->view;
+   for (int k = 0; k < 3; ++k) {
+      J.cam_c[k] = P.camera_center[k];
+      J.p00[k] = P.pixel00[k];
+      J.du[k] = P.delta_u[k];
+      J.dv[k] = P.delta_v[k];
+      J.cu[k] = P.cam_u[k];
+      J.cv[k] = P.cam_v[k];
+   }
+   J.width = P.width;
+   J.height = P.height;
+   J.max_depth = P.max_depth;
+   J.flags = P.flags;
+   J.seed_lo = (uint32_t)P.seed;
+   J.seed_hi = (uint32_t)(P.seed >> 32);
+   J.light_intensity = P.light_intensity;
+   J.background_intensity = P.background_intensity;
+   J.metal_fuzziness = P.metal_fuzziness;
+   J.glass_ior = P.glass_ior;
+   J.dof_aperture = P.dof_aperture;
+   J.dof_focus = P.dof_focus_distance;
+   J.min_adaptive = P.min_adaptive_samples;
+   J.adaptive_threshold = P.adaptive_threshold;
+   J.spp_target = P.spp;
+   J.sample_base = (int)j->samples_done;
+}
+
+rayon_status rayon_job_reset(rayon_job* j) {
+   if (!j) {
+      set_error("rayon_job_reset: NULL job");
+      return RAYON_ERR_INVALID;
+   }
+   const size_t npix = (size_t)j->params.width * j->params.height;
+   hipStream_t st = j->ctx->stream;
+   RY_HIP(hipSetDevice(j->ctx->device));
+   // fills by kernel, not hipMemsetAsync, so a caller may capture this in a graph
+   RY_HIP(launch_fill_u32((uint32_t*)j->view.acc, 0u, (long long)npix * 4, st));
+   RY_HIP(launch_fill_u32((uint32_t*)j->view.comp, 0u, (long long)npix * 4, st));
+   RY_HIP(launch_fill_u32((uint32_t*)j->view.counts, 0u, (long long)npix, st));
+   RY_HIP(launch_fill_u32((uint32_t*)j->view.counters, 0u, 16, st));  // 8 x u64
+   if (j->view.rng) RY_HIP(launch_ref_rng_init(j->view.rng, (int)npix, st));
+   j->samples_done = 0;
+   return RAYON_OK;
+}
+
+rayon_status rayon_job_create(rayon_ctx* ctx, rayon_scene* sc, const rayon_job_params* p, rayon_job** out) {
+   if (!ctx || !sc || !p || !out) {
+      set_error("rayon_job_create: NULL argument");
+      return RAYON_ERR_INVALID;
+   }
+   if (p->width <= 0 || p->height <= 0 || (int64_t)p->width * p->height > (1ll << 28) || p->max_depth < 1 ||
+       p->max_depth > 1024 || p->spp < 0) {
+      set_error("rayon_job_create: bad resolution (%dx%d), max_depth %d or spp %d", p->width, p->height,
+                p->max_depth, p->spp);
+      return RAYON_ERR_INVALID;
+   }
+   if ((p->flags & (RAYON_FLAG_ADAPTIVE_REF | RAYON_FLAG_ADAPTIVE_STDERR)) ==
+       (RAYON_FLAG_ADAPTIVE_REF | RAYON_FLAG_ADAPTIVE_STDERR)) {
+      set_error("rayon_job_create: choose one adaptive test");
+      return RAYON_ERR_INVALID;
+   }
+   (void)hipSetDevice(ctx->device);
+   auto* j = new rayon_job();
+   j->ctx = ctx;
+   j->scene = sc;
+   j->params = *p;
+   const size_t npix = (size_t)p->width * p->height;
+   void *acc, *comp, *counts, *rng = nullptr, *ctr;
+   rayon_status st;
+   if ((st = dev_alloc(j->bufs, npix * sizeof(float4), &acc)) != RAYON_OK ||
+       (st = dev_alloc(j->bufs, npix * sizeof(float4), &comp)) != RAYON_OK ||
+       (st = dev_alloc(j->bufs, npix * sizeof(int), &counts)) != RAYON_OK ||
+       (st = dev_alloc(j->bufs, 8 * sizeof(unsigned long long), &ctr)) != RAYON_OK ||
+       ((p->flags & RAYON_FLAG_REF_RNG) && (st = dev_alloc(j->bufs, npix * sizeof(uint32_t), &rng)) != RAYON_OK)) {
+      free_all(j->bufs);
+      delete j;
+      return st;
+   }
+   j->view.acc = (float4*)acc;
+   j->view.comp = (float4*)comp;
+   j->view.counts = (int*)counts;
+   j->view.rng = (uint32_t*)rng;
+   j->view.counters = (unsigned long long*)ctr;
+   fill_jobview(j);
+   st = rayon_job_reset(j);
+   if (st != RAYON_OK) {
+      free_all(j->bufs);
+      delete j;
+      return st;
+   }
+   *out = j;
+   return RAYON_OK;
+}
+
+rayon_status rayon_job_destroy(rayon_job* j) {
+   if (!j) return RAYON_OK;
+   (void)hipSetDevice(j->ctx->device);
+   (void)hipStreamSynchronize(j->ctx->stream);
+   free_all(j->bufs);
+   delete j;
+   return RAYON_OK;
+}
+
+rayon_status rayon_job_render_plain(rayon_ctx* ctx, rayon_job* j, int32_t samples) {
+   if (!ctx || !j || samples < 0) {
+      set_error("rayon_job_render_plain: bad arguments");
+      return RAYON_ERR_INVALID;
+   }
+   int64_t left = j->params.spp - j->samples_done;
+   if (samples > left) samples = (int32_t)left;
+   if (samples <= 0) return RAYON_OK;
+   RY_HIP(hipSetDevice(ctx->device));
+   fill_jobview(j);
+   RY_HIP(launch_plain(j->view, samples, ctx->stream));
+   j->samples_done += samples;
+   return RAYON_OK;
+}
+
+rayon_status rayon_job_resolve_linear(rayon_ctx* ctx, rayon_job* j, float* rgb) {
+   if (!ctx || !j || !rgb) {
+      set_error("rayon_job_resolve_linear: NULL argument");
+      return RAYON_ERR_INVALID;
+   }
+   RY_HIP(hipSetDevice(ctx->device));
+   const size_t bytes = (size_t)j->params.width * j->params.height * 3 * sizeof(float);
+   float* d = nullptr;
+   RY_HIP(hipMalloc(&d, bytes));
+   hipError_t e = launch_resolve_linear(j->view, d, ctx->stream);
+   if (e == hipSuccess) e = hipMemcpyAsync(rgb, d, bytes, hipMemcpyDeviceToHost, ctx->stream);
+   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+   (void)hipFree(d);
+   if (e != hipSuccess) return hip_fail(e, "resolve_linear");
+   return RAYON_OK;
+}
+
+rayon_status rayon_job_resolve_u8(rayon_ctx* ctx, rayon_job* j, uint8_t* out, int32_t channels, float gamma) {
+   if (!ctx || !j || !out || (channels != 3 && channels != 4) || !(gamma > 0.0f)) {
+      set_error("rayon_job_resolve_u8: bad arguments");
+      return RAYON_ERR_INVALID;
+   }
+   RY_HIP(hipSetDevice(ctx->device));
+   const size_t bytes = (size_t)j->params.width * j->params.height * channels;
+   uint8_t* d = nullptr;
+   RY_HIP(hipMalloc(&d, bytes));
+   hipError_t e = launch_resolve_u8(j->view, d, channels, gamma, ctx->stream);
+   if (e == hipSuccess) e = hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, ctx->stream);
+   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+   (void)hipFree(d);
+   if (e != hipSuccess) return hip_fail(e, "resolve_u8");
+   return RAYON_OK;
+}
+
+rayon_status rayon_job_sample_counts(rayon_ctx* ctx, rayon_job* j, int32_t* out) {
+   if (!ctx || !j || !out) {
+      set_error("rayon_job_sample_counts: NULL argument");
+      return RAYON_ERR_INVALID;
+   }
+   RY_HIP(hipSetDevice(ctx->device));
+   RY_HIP(hipMemcpyAsync(out, j->view.counts, (size_t)j->params.width * j->params.height * sizeof(int),
+                         hipMemcpyDeviceToHost, ctx->stream));
+   RY_HIP(hipStreamSynchronize(ctx->stream));
+   return RAYON_OK;
+}
+
+rayon_status rayon_job_stats_get(rayon_ctx* ctx, rayon_job* j, rayon_job_stats* out) {
+   if (!ctx || !j || !out) {
+      set_error("rayon_job_stats_get: NULL argument");
+      return RAYON_ERR_INVALID;
+   }
+   unsigned long long c[8];
+   RY_HIP(hipSetDevice(ctx->device));
+   RY_HIP(hipMemcpyAsync(c, j->view.counters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+   RY_HIP(hipStreamSynchronize(ctx->stream));
+   out->samples_done = j->samples_done;
+   out->segments = (int64_t)c[0];
+   out->shadow_segments = (int64_t)c[1];
+   out->primary_samples = (int64_t)c[2];
+   out->converged_pixels = (int64_t)c[3];
+   out->pad_ = 0;
+   out->node_fetches = (int64_t)c[4];
+   out->prim_fetches = (int64_t)c[5];
+   out->material_fetches = (int64_t)c[6];
+   out->light_fetches = (int64_t)c[7];
+   const int64_t npix = (int64_t)j->params.width * j->params.height;
+   out->done = (j->samples_done >= j->params.spp || out->converged_pixels >= npix) ? 1 : 0;
+   return RAYON_OK;
+}
+
+rayon_status rayon_trace_closest(rayon_ctx* ctx, rayon_scene* sc, const float* o, const float* d, int64_t n,
+                                 float tmin, float tmax, int64_t* prim, float* t) {
+   if (!ctx || !sc || !o || !d || !prim || !t || n < 0) {
+      set_error("rayon_trace_closest: bad arguments");
+      return RAYON_ERR_INVALID;
+   }
+   if (n == 0) return RAYON_OK;
+   RY_HIP(hipSetDevice(ctx->device));
+   float *dO = nullptr, *dD = nullptr, *dT = nullptr;
+   long long* dP = nullptr;
+   const size_t b3 = (size_t)n * 3 * sizeof(float);
+   hipError_t e = hipMalloc(&dO, b3);
+   if (e == hipSuccess) e = hipMalloc(&dD, b3);
+   if (e == hipSuccess) e = hipMalloc(&dT, (size_t)n * sizeof(float));
+   if (e == hipSuccess) e = hipMalloc(&dP, (size_t)n * sizeof(long long));
+   if (e == hipSuccess) e = hipMemcpyAsync(dO, o, b3, hipMemcpyHostToDevice, ctx->stream);
+   if (e == hipSuccess) e = hipMemcpyAsync(dD, d, b3, hipMemcpyHostToDevice, ctx->stream);
+   if (e == hipSuccess) e = launch_trace_closest(sc->view, dO, dD, n, tmin, tmax, dP, dT, ctx->stream);
+   if (e == hipSuccess) e = hipMemcpyAsync(prim, dP, (size_t)n * sizeof(long long), hipMemcpyDeviceToHost, ctx->stream);
+   if (e == hipSuccess) e = hipMemcpyAsync(t, dT, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
+   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+   (void)hipFree(dO);
+   (void)hipFree(dD);
+   (void)hipFree(dT);
+   (void)hipFree(dP);
+   if (e != hipSuccess) return hip_fail(e, "trace_closest");
+   return RAYON_OK;
+}
+
+rayon_status rayon_job_primary_rays(rayon_ctx* ctx, rayon_job* j, int32_t sample, float* o, float* d) {
+   if (!ctx || !j || !o || !d || sample < 0) {
+      set_error("rayon_job_primary_rays: bad arguments");
+      return RAYON_ERR_INVALID;
+   }
+   RY_HIP(hipSetDevice(ctx->device));
+   const size_t b3 = (size_t)j->params.width * j->params.height * 3 * sizeof(float);
+   float *dO = nullptr, *dD = nullptr;
+   hipError_t e = hipMalloc(&dO, b3);
+   if (e == hipSuccess) e = hipMalloc(&dD, b3);
+   fill_jobview(j);
+   if (e == hipSuccess) e = launch_primary(j->view, sample, dO, dD, ctx->stream);
+   if (e == hipSuccess) e = hipMemcpyAsync(o, dO, b3, hipMemcpyDeviceToHost, ctx->stream);
+   if (e == hipSuccess) e = hipMemcpyAsync(d, dD, b3, hipMemcpyDeviceToHost, ctx->stream);
+   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+   (void)hipFree(dO);
+   (void)hipFree(dD);
+   if (e != hipSuccess) return hip_fail(e, "primary_rays");
+   return RAYON_OK;
+}
+
+}  // extern "C"

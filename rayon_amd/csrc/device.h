@@ -1,0 +1,860 @@
+// device.h — per-ray device code shared by the plain megakernel and the wavefront
+// stages: RNG streams, BVH traversal with an LDS stack, primitive intersection, surface
+// reconstruction, material scattering and light sampling. FP32 throughout.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/rayon_hip.h"
+#include "layout.h"
+
+namespace ry {
+
+constexpr int kStack = 32;            // LDS traversal stack entries per lane
+constexpr float kPiF = 3.14159265358979f;
+
+// ------------------------------------------------------------------------------------
+// small vector type
+// ------------------------------------------------------------------------------------
+struct V3 {
+   float x, y, z;
+};
+__device__ __forceinline__ V3 mk(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ V3 mk(const float4& a) { return {a.x, a.y, a.z}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ V3 operator*(float s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ V3 operator*(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+   return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float len2(V3 a) { return dot(a, a); }
+__device__ __forceinline__ V3 unit(V3 a) { return (1.0f / sqrtf(dot(a, a))) * a; }
+__device__ __forceinline__ float maxc(V3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
+__device__ __forceinline__ V3 ld3(const float* p) { return {p[0], p[1], p[2]}; }
+
+// sin/cos of 2*pi*u for u in [0,1]: v_sin_f32 / v_cos_f32 take revolutions.
+__device__ __forceinline__ float sin2pi(float u) { return __builtin_amdgcn_sinf(u); }
+__device__ __forceinline__ float cos2pi(float u) { return __builtin_amdgcn_cosf(u); }
+
+// ------------------------------------------------------------------------------------
+// RNG: the reference's LCG + output permutation (cuda_utils.cuh:24-36). REF mode keeps
+// one sequential stream per pixel (seeded as cuda_utils.cu:26) and converts with
+// float(r)*2^-32; the default counter mode restarts the stream per (seed, pixel, sample)
+// and converts the top 24 bits, so each sample is independent of every other.
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lcg_step(uint32_t s) { return s * 747796405u + 2891336453u; }
+__device__ __forceinline__ uint32_t permute(uint32_t s) {
+   uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+   return (w >> 22u) ^ w;
+}
+__device__ __forceinline__ uint32_t hash32(uint32_t v) { return permute(lcg_step(v)); }
+
+__device__ __forceinline__ uint32_t counter_init(uint32_t seed_lo, uint32_t seed_hi, uint32_t pixel,
+                                                 uint32_t sample) {
+   uint32_t h = hash32(seed_lo ^ hash32(seed_hi + 0x9E3779B9u));
+   h = hash32(h + pixel);
+   return hash32(h ^ (sample * 0x85EBCA6Bu + 0xC2B2AE35u));
+}
+
+struct Rng {
+   uint32_t s;
+   bool ref;
+   __device__ __forceinline__ float next() {
+      s = lcg_step(s);
+      uint32_t r = permute(s);
+      if (ref) return (float)r * (1.0f / 4294967296.0f);
+      return (float)(r >> 8) * (1.0f / 16777216.0f);
+   }
+};
+
+// ------------------------------------------------------------------------------------
+// traversal
+// ------------------------------------------------------------------------------------
+struct RayInv {
+   V3 o, d, inv, ood;
+};
+
+__device__ __forceinline__ float safe_inv(float d) {
+   float a = fabsf(d) < 1e-20f ? copysignf(1e-20f, d) : d;
+   return 1.0f / a;
+}
+
+__device__ __forceinline__ RayInv make_rayinv(V3 o, V3 d) {
+   RayInv r;
+   r.o = o;
+   r.d = d;
+   r.inv = mk(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
+   r.ood = r.o * r.inv;
+   return r;
+}
+
+// Slab test of both children of a pair node; returns entry distances, hit flags.
+__device__ __forceinline__ void box2(const RayInv& r, const Node2& n, float tmin, float tmax, float& tn0,
+                                     float& tn1, bool& h0, bool& h1) {
+   float ax0 = fmaf(n.bx.x, r.inv.x, -r.ood.x), bx0 = fmaf(n.bx.y, r.inv.x, -r.ood.x);
+   float ay0 = fmaf(n.by.x, r.inv.y, -r.ood.y), by0 = fmaf(n.by.y, r.inv.y, -r.ood.y);
+   float az0 = fmaf(n.bz.x, r.inv.z, -r.ood.z), bz0 = fmaf(n.bz.y, r.inv.z, -r.ood.z);
+   float ax1 = fmaf(n.bx.z, r.inv.x, -r.ood.x), bx1 = fmaf(n.bx.w, r.inv.x, -r.ood.x);
+   float ay1 = fmaf(n.by.z, r.inv.y, -r.ood.y), by1 = fmaf(n.by.w, r.inv.y, -r.ood.y);
+   float az1 = fmaf(n.bz.z, r.inv.z, -r.ood.z), bz1 = fmaf(n.bz.w, r.inv.z, -r.ood.z);
+   tn0 = fmaxf(fmaxf(fminf(ax0, bx0), fminf(ay0, by0)), fmaxf(fminf(az0, bz0), tmin));
+   float tf0 = fminf(fminf(fmaxf(ax0, bx0), fmaxf(ay0, by0)), fminf(fmaxf(az0, bz0), tmax));
+   tn1 = fmaxf(fmaxf(fminf(ax1, bx1), fminf(ay1, by1)), fmaxf(fminf(az1, bz1), tmin));
+   float tf1 = fminf(fminf(fmaxf(ax1, bx1), fmaxf(ay1, by1)), fminf(fmaxf(az1, bz1), tmax));
+   // widen the exit by 2^-22 relative so FP32 rounding of the slab distances can never
+   // reject a box the exact test accepts (flat boxes included)
+   h0 = tn0 <= tf0 * 1.00000024f;
+   h1 = tn1 <= tf1 * 1.00000024f;
+}
+
+struct HitMin {
+   float t;
+   int slot;
+   int orig;
+   float u, v;
+};
+
+// Moller-Trumbore on (v0, e1, e2), the arithmetic of hit_triangle
+// (cuda_raytracer.cuh:277-304) with the inclusive [tmin, tmax] range.
+__device__ __forceinline__ bool isect_tri(const PrimRec& p, V3 o, V3 d, float tmin, float tmax, float& t, float& u,
+                                          float& v) {
+   V3 v0 = mk(p.a), e1 = mk(p.b), e2 = mk(p.c);
+   V3 h = cross(d, e2);
+   float a = dot(e1, h);
+   if (fabsf(a) < 1e-8f) return false;
+   float f = 1.0f / a;
+   V3 s = o - v0;
+   u = f * dot(s, h);
+   if (u < 0.0f || u > 1.0f) return false;
+   V3 q = cross(s, e1);
+   v = f * dot(d, q);
+   if (v < 0.0f || u + v > 1.0f) return false;
+   t = f * dot(e2, q);
+   return t >= tmin && t <= tmax;
+}
+
+// hit_sphere root selection (cuda_raytracer.cuh:203-222).
+__device__ __forceinline__ bool isect_sphere(V3 c, float rad, V3 o, V3 d, float tmin, float tmax, float& t) {
+   V3 oc = o - c;
+   float a = dot(d, d);
+   float hb = dot(oc, d);
+   float cc = dot(oc, oc) - rad * rad;
+   float disc = hb * hb - a * cc;
+   if (disc < 0.0f) return false;
+   float sq = sqrtf(disc);
+   float root = (-hb - sq) / a;
+   if (root < tmin || tmax < root) {
+      root = (-hb + sq) / a;
+      if (root < tmin || tmax < root) return false;
+   }
+   t = root;
+   return true;
+}
+
+// hit_rectangle (cuda_raytracer.cuh:235-262).
+__device__ __forceinline__ bool isect_rect(V3 q, V3 u, V3 v, V3 o, V3 d, float tmin, float tmax, float& t) {
+   V3 n = unit(cross(u, v));
+   float denom = dot(n, d);
+   if (fabsf(denom) < 1e-8f) return false;
+   t = dot(n, q - o) / denom;
+   if (t < tmin || t > tmax) return false;
+   V3 rel = (o + t * d) - q;
+   float al = dot(rel, u) / dot(u, u);
+   float be = dot(rel, v) / dot(v, v);
+   return !(al < 0.0f || al > 1.0f || be < 0.0f || be > 1.0f);
+}
+
+// ---- SDF field: smooth union of analytic primitives (sdf.hpp:105-143, :410-414)
+__device__ __forceinline__ float sdf_prim(const SdfRec& s, V3 p) {
+   V3 o = p - mk(s.c);
+   int type = (int)s.c.w;
+   if (type == RAYON_SDF_SPHERE) return sqrtf(dot(o, o)) - s.p.x;
+   if (type == RAYON_SDF_ROUNDED_BOX) {
+      float rr = s.p.w;
+      float qx = fabsf(o.x) - (s.p.x - rr), qy = fabsf(o.y) - (s.p.y - rr), qz = fabsf(o.z) - (s.p.z - rr);
+      V3 m = mk(fmaxf(qx, 0.0f), fmaxf(qy, 0.0f), fmaxf(qz, 0.0f));
+      return sqrtf(dot(m, m)) + fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f) - rr;
+   }
+   float qx = sqrtf(o.x * o.x + o.z * o.z) - s.p.x;
+   return sqrtf(qx * qx + o.y * o.y) - s.p.y;
+}
+
+__device__ __forceinline__ float sdf_field(const SceneView& S, V3 p) {
+   float d = sdf_prim(S.sdf[0], p);
+   const float k = S.sdf_k, ik = 1.0f / S.sdf_k;
+   for (int i = 1; i < S.num_sdf; ++i) {
+      float d2 = sdf_prim(S.sdf[i], p);
+      float h = fminf(fmaxf(0.5f + 0.5f * (d2 - d) * ik, 0.0f), 1.0f);
+      d = d2 * (1.0f - h) + d * h - k * h * (1.0f - h);
+   }
+   return d;
+}
+
+// Sphere tracing from the entry of the field's box along the unit direction
+// (rayMarch, sdf.hpp:518-563). Returns the ray parameter of the hit.
+__device__ inline bool isect_sdf(const SceneView& S, const PrimRec& p, V3 o, V3 d, float tmin, float tmax, float& t) {
+   float dl = sqrtf(dot(d, d));
+   float idl = 1.0f / dl;
+   V3 dn = idl * d;
+   float t0 = tmin * dl, t1 = tmax * dl;
+   float lo[3] = {p.a.x, p.a.y, p.a.z}, hi[3] = {p.b.x, p.b.y, p.b.z};
+   float oo[3] = {o.x, o.y, o.z}, dd[3] = {dn.x, dn.y, dn.z};
+#pragma unroll
+   for (int k = 0; k < 3; ++k) {
+      float inv = safe_inv(dd[k]);
+      float a = (lo[k] - oo[k]) * inv, b = (hi[k] - oo[k]) * inv;
+      t0 = fmaxf(t0, fminf(a, b));
+      t1 = fminf(t1, fmaxf(a, b));
+   }
+   if (t0 > t1) return false;
+   float tw = t0;
+   for (int step = 0; step < S.sdf_steps; ++step) {
+      float dist = sdf_field(S, o + tw * dn);
+      if (dist < S.sdf_eps) {
+         t = tw * idl;
+         return true;
+      }
+      tw += dist;
+      if (tw > t1) break;
+   }
+   return false;
+}
+
+// Closest hit over the scene. Exact t ties go to the larger original primitive index,
+// which is the linear-scan rule of the reference (cuda_raytracer.cuh:568-585) made
+// independent of traversal order. `stack` points at this lane's first LDS entry; entry
+// i lives at stack[i * STRIDE].
+// Optional traversal statistics (node pairs fetched, primitive records fetched).
+struct TravCount {
+   unsigned nodes = 0, prims = 0;
+};
+
+template <int STRIDE, bool CNT = false>
+__device__ inline HitMin closest_hit(const SceneView& S, V3 o, V3 d, float tmin, float tmax, int* stack,
+                                     TravCount* tc = nullptr) {
+   HitMin best;
+   best.t = tmax;
+   best.slot = -1;
+   best.orig = -1;
+   best.u = best.v = 0.0f;
+   int cur = S.root;
+   if (cur == kSentinel) return best;
+   const RayInv r = make_rayinv(o, d);
+   int sp = 0;
+   for (;;) {
+      while (cur >= 0) {
+         const Node2 n = S.nodes[cur];
+         if constexpr (CNT) ++tc->nodes;
+         float tn0, tn1;
+         bool h0, h1;
+         box2(r, n, tmin, best.t, tn0, tn1, h0, h1);
+         int r0 = n.meta.z ? pack_leaf(n.meta.x, n.meta.z) : n.meta.x;
+         int r1 = n.meta.w ? pack_leaf(n.meta.y, n.meta.w) : n.meta.y;
+         if (h0 && h1) {
+            int nr = r0, fr = r1;
+            if (tn1 < tn0) {
+               nr = r1;
+               fr = r0;
+            }
+            if (sp < kStack) stack[(sp++) * STRIDE] = fr;
+            cur = nr;
+         } else if (h0) {
+            cur = r0;
+         } else if (h1) {
+            cur = r1;
+         } else {
+            cur = sp > 0 ? stack[(--sp) * STRIDE] : kSentinel;
+         }
+      }
+      if (cur == kSentinel) break;
+      const int packed = ~cur;
+      const int first = packed >> 3, cnt = (packed & 7) + 1;
+      for (int i = first; i < first + cnt; ++i) {
+         const PrimRec p = S.prims[i];
+         if constexpr (CNT) ++tc->prims;
+         const uint32_t info = __float_as_uint(p.a.w);
+         const uint32_t kind = info_kind(info);
+         float t = 0.0f, u = 0.0f, v = 0.0f;
+         bool hit;
+         if (kind == K_TRI)
+            hit = isect_tri(p, o, d, tmin, best.t, t, u, v);
+         else if (kind == K_SPHERE || kind == K_DISPLACED)
+            hit = isect_sphere(mk(p.a), p.b.x, o, d, tmin, best.t, t);
+         else if (kind == K_RECT)
+            hit = isect_rect(mk(p.a), mk(p.b), mk(p.c), o, d, tmin, best.t, t);
+         else
+            hit = isect_sdf(S, p, o, d, tmin, best.t, t);
+         if (hit) {
+            const int orig = __float_as_int(p.b.w);
+            if (t < best.t || orig > best.orig) {
+               best.t = t;
+               best.slot = i;
+               best.orig = orig;
+               best.u = u;
+               best.v = v;
+            }
+         }
+      }
+      cur = sp > 0 ? stack[(--sp) * STRIDE] : kSentinel;
+      if (cur == kSentinel) break;
+   }
+   return best;
+}
+
+// Any hit in [tmin, tmax] by a primitive that is not an emitter (shadow rays).
+template <int STRIDE, bool CNT = false>
+__device__ inline bool any_hit(const SceneView& S, V3 o, V3 d, float tmin, float tmax, int* stack,
+                               TravCount* tc = nullptr) {
+   int cur = S.root;
+   if (cur == kSentinel) return false;
+   const RayInv r = make_rayinv(o, d);
+   int sp = 0;
+   for (;;) {
+      while (cur >= 0) {
+         const Node2 n = S.nodes[cur];
+         if constexpr (CNT) ++tc->nodes;
+         float tn0, tn1;
+         bool h0, h1;
+         box2(r, n, tmin, tmax, tn0, tn1, h0, h1);
+         int r0 = n.meta.z ? pack_leaf(n.meta.x, n.meta.z) : n.meta.x;
+         int r1 = n.meta.w ? pack_leaf(n.meta.y, n.meta.w) : n.meta.y;
+         if (h0 && h1) {
+            if (sp < kStack) stack[(sp++) * STRIDE] = r1;
+            cur = r0;
+         } else if (h0) {
+            cur = r0;
+         } else if (h1) {
+            cur = r1;
+         } else {
+            cur = sp > 0 ? stack[(--sp) * STRIDE] : kSentinel;
+         }
+      }
+      if (cur == kSentinel) return false;
+      const int packed = ~cur;
+      const int first = packed >> 3, cnt = (packed & 7) + 1;
+      for (int i = first; i < first + cnt; ++i) {
+         const PrimRec p = S.prims[i];
+         if constexpr (CNT) ++tc->prims;
+         const uint32_t info = __float_as_uint(p.a.w);
+         if (info & F_EMITTER) continue;
+         const uint32_t kind = info_kind(info);
+         float t, u, v;
+         bool hit;
+         if (kind == K_TRI)
+            hit = isect_tri(p, o, d, tmin, tmax, t, u, v);
+         else if (kind == K_SPHERE || kind == K_DISPLACED)
+            hit = isect_sphere(mk(p.a), p.b.x, o, d, tmin, tmax, t);
+         else if (kind == K_RECT)
+            hit = isect_rect(mk(p.a), mk(p.b), mk(p.c), o, d, tmin, tmax, t);
+         else
+            hit = isect_sdf(S, p, o, d, tmin, tmax, t);
+         if (hit) return true;
+      }
+      cur = sp > 0 ? stack[(--sp) * STRIDE] : kSentinel;
+      if (cur == kSentinel) return false;
+   }
+}
+
+// ------------------------------------------------------------------------------------
+// surface reconstruction for the winning primitive
+// ------------------------------------------------------------------------------------
+struct Surf {
+   V3 p, n, tangent, center;
+   bool front;
+   uint32_t info;
+};
+
+__device__ __forceinline__ V3 fib_point(int i, int n) {  // shader_golf.cu:4-13
+   const float ga = 2.39996323f;
+   float k = (float)i + 0.5f;
+   float z = 1.0f - 2.0f * (k / (float)n);
+   float s = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+   float th = ga * k;
+   return mk(cosf(th) * s, sinf(th) * s, z);
+}
+
+__device__ inline float dimple(V3 p) {  // shader_golf.cu:15-43
+   V3 q = unit(p);
+   float best = -1.0f;
+   for (int i = 0; i < 150; ++i) best = fmaxf(best, dot(q, fib_point(i, 150)));
+   best = fmaxf(fminf(best, 1.0f), -1.0f);
+   float ang = acosf(best);
+   if (ang < 0.24f) return -0.35f * cosf(ang / 0.24f * kPiF * 0.5f);
+   return 0.0f;
+}
+
+__device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
+   float t = fmaxf(0.0f, fminf(1.0f, (x - e0) / (e1 - e0)));
+   return t * t * (3.0f - 2.0f * t);
+}
+
+__device__ inline Surf surface(const SceneView& S, const HitMin& h, V3 o, V3 d) {
+   const PrimRec p = S.prims[h.slot];
+   Surf s;
+   s.info = __float_as_uint(p.a.w);
+   const uint32_t kind = info_kind(s.info);
+   s.p = o + h.t * d;
+   s.center = mk(0.0f, 0.0f, 0.0f);
+   if (kind == K_TRI) {  // cuda_raytracer.cuh:306-346
+      V3 e1 = mk(p.b), e2 = mk(p.c);
+      V3 geo = unit(cross(e1, e2));
+      s.front = dot(d, geo) < 0.0f;
+      V3 sn = geo;
+      const bool smooth = (s.info & F_NORMALS) != 0;
+      if (smooth) {
+         const float4* nn = S.normals + 3 * h.slot;
+         float w = 1.0f - h.u - h.v;
+         sn = unit(w * mk(nn[0]) + h.u * mk(nn[1]) + h.v * mk(nn[2]));
+         if (dot(sn, geo) < 0.0f) sn = -sn;
+      }
+      s.n = s.front ? sn : -sn;
+      if (smooth && dot(s.n, -d) <= 0.0f) s.n = s.front ? geo : -geo;
+      s.tangent = unit(e1);
+   } else if (kind == K_RECT) {  // cuda_raytracer.cuh:264-270
+      V3 n = unit(cross(mk(p.b), mk(p.c)));
+      s.front = dot(d, n) < 0.0f;
+      s.n = s.front ? n : -n;
+      s.tangent = unit(mk(p.b));
+   } else if (kind == K_SDF) {  // central differences, sdf.hpp:499-507
+      const float e = S.sdf_eps;
+      V3 g = mk(sdf_field(S, s.p + mk(e, 0, 0)) - sdf_field(S, s.p - mk(e, 0, 0)),
+                sdf_field(S, s.p + mk(0, e, 0)) - sdf_field(S, s.p - mk(0, e, 0)),
+                sdf_field(S, s.p + mk(0, 0, e)) - sdf_field(S, s.p - mk(0, 0, e)));
+      V3 n = unit(g);
+      s.front = dot(d, n) < 0.0f;
+      s.n = s.front ? n : -n;
+      V3 tg = cross(mk(0.0f, 1.0f, 0.0f), n);
+      s.tangent = len2(tg) < 1e-6f ? mk(1.0f, 0.0f, 0.0f) : unit(tg);
+   } else {  // sphere, cuda_raytracer.cuh:223-231
+      V3 c = mk(p.a);
+      float rad = p.b.x;
+      s.center = c;
+      V3 out = (1.0f / rad) * (s.p - c);
+      s.front = dot(d, out) < 0.0f;
+      s.n = s.front ? out : -out;
+      V3 tg = cross(mk(0.0f, 1.0f, 0.0f), out);
+      s.tangent = len2(tg) < 1e-6f ? mk(1.0f, 0.0f, 0.0f) : unit(tg);
+      if (kind == K_DISPLACED) {  // golf-ball displacement, shader_golf.cu:61-138
+         V3 sp = s.p;
+         V3 bo = unit(sp - c);
+         float disp = dimple(bo);
+         float dn = fminf(1.0f, fmaxf(0.0f, -disp / 0.35f));
+         s.p = sp + (rad * 0.35f * (1.0f - dn)) * bo;
+         V3 bn = unit(s.p - c);
+         V3 nrm = bn;
+         if (disp < -0.001f) {
+            V3 helper = fabsf(bn.x) > 0.8f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+            V3 t1 = unit(cross(helper, bn));
+            V3 t2 = cross(bn, t1);
+            const float hs = 0.015f;
+            float d0 = dimple(bn);
+            float d1 = dimple(unit(bn + hs * t1));
+            float d2 = dimple(unit(bn + hs * t2));
+            V3 g = ((d1 - d0) / hs) * t1 + ((d2 - d0) / hs) * t2;
+            V3 dnv = -0.2f * g;
+            float ndv = fmaxf(0.0f, dot(bn, unit(-d)));
+            dnv = smoothstepf(0.1f, 0.4f, ndv) * dnv;
+            float l = sqrtf(len2(dnv));
+            if (l > 0.4f && l > 1e-6f) dnv = (0.4f / l) * dnv;
+            V3 pert = unit(bn + dnv);
+            if (dot(pert, bn) < 0.0f) pert = -pert;
+            if (!(pert.x == pert.x) || !(pert.y == pert.y) || !(pert.z == pert.z)) pert = bn;
+            nrm = pert;
+         }
+         float ndotv = dot(d, nrm);
+         if (!(ndotv == ndotv)) {
+            nrm = bn;
+            ndotv = dot(d, nrm);
+         }
+         s.front = ndotv < 0.0f;
+         s.n = s.front ? nrm : -nrm;
+         s.p = s.p + 1e-3f * s.n;
+      }
+   }
+   return s;
+}
+
+// ------------------------------------------------------------------------------------
+// materials
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ V3 reflect(V3 v, V3 n) { return v - (2.0f * dot(v, n)) * n; }
+
+__device__ __forceinline__ V3 refract(V3 uv, V3 n, float eta) {  // cuda_raytracer.cuh:86-92
+   float c = fminf(dot(-uv, n), 1.0f);
+   V3 perp = eta * (uv + c * n);
+   V3 par = (-sqrtf(fabsf(1.0f - len2(perp)))) * n;
+   return perp + par;
+}
+
+__device__ __forceinline__ float schlick(float c, float ri) {  // cuda_raytracer.cuh:94-99
+   float r0 = (1.0f - ri) / (1.0f + ri);
+   r0 *= r0;
+   float x = 1.0f - c;
+   float x2 = x * x;
+   return r0 + (1.0f - r0) * (x2 * x2 * x);
+}
+
+__device__ __forceinline__ void onb(V3 n, V3& u, V3& v) {  // cuda_utils.cuh:99-107
+   if (fabsf(n.x) > fabsf(n.z))
+      u = unit(mk(-n.y, n.x, 0.0f));
+   else
+      u = unit(mk(0.0f, -n.z, n.y));
+   v = cross(n, u);
+}
+
+__device__ __forceinline__ V3 on_sphere(Rng& R) {  // cuda_utils.cuh:66-77
+   float a = R.next();
+   float z = 1.0f - 2.0f * R.next();
+   float s = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+   return mk(s * cos2pi(a), s * sin2pi(a), z);
+}
+
+__device__ __forceinline__ V3 cosine_dir(V3 n, Rng& R) {  // cuda_raytracer.cuh:621-638
+   V3 w = unit(n), u, v;
+   onb(w, u, v);
+   float u1 = R.next(), u2 = R.next();
+   float r = sqrtf(u1);
+   return (r * cos2pi(u2)) * u + (r * sin2pi(u2)) * v + sqrtf(fmaxf(0.0f, 1.0f - u1)) * w;
+}
+
+__device__ __forceinline__ float lambda_ggx(V3 w, float ax, float ay) {  // microfacet_ggx.cuh:21-28
+   float wz2 = w.z * w.z;
+   if (wz2 < 1e-16f) return 0.0f;
+   float a2 = (ax * w.x) * (ax * w.x) + (ay * w.y) * (ay * w.y);
+   return (sqrtf(1.0f + a2 / wz2) - 1.0f) * 0.5f;
+}
+
+__device__ __forceinline__ V3 sample_vndf(V3 wo, float ax, float ay, float u1, float u2) {  // :45-76
+   V3 wh = unit(mk(ax * wo.x, ay * wo.y, wo.z));
+   if (wh.z < 0.0f) wh = -wh;
+   V3 t1 = wh.z < 0.99999f ? unit(cross(mk(0.0f, 0.0f, 1.0f), wh)) : mk(1.0f, 0.0f, 0.0f);
+   V3 t2 = cross(wh, t1);
+   float r = sqrtf(u1);
+   float px = r * cos2pi(u2), py = r * sin2pi(u2);
+   float hh = sqrtf(fmaxf(0.0f, 1.0f - px * px));
+   float sm = (1.0f + wh.z) * 0.5f;
+   py = (1.0f - sm) * hh + sm * py;
+   float pz = sqrtf(fmaxf(0.0f, 1.0f - px * px - py * py));
+   V3 nh = px * t1 + py * t2 + pz * wh;
+   return unit(mk(ax * nh.x, ay * nh.y, fmaxf(1e-6f, nh.z)));
+}
+
+__device__ __forceinline__ float fr_complex1(float cos2, float sin2, float ci, float eta, float k) {  // :83-101
+   float eta2 = eta * eta, k2 = k * k;
+   float t0 = eta2 - k2 - sin2;
+   float ab = sqrtf(fmaxf(t0 * t0 + 4.0f * eta2 * k2, 0.0f));
+   float a = sqrtf(fmaxf((ab + t0) * 0.5f, 0.0f));
+   float rs = (ab + cos2 - 2.0f * a * ci) / fmaxf(ab + cos2 + 2.0f * a * ci, 1e-10f);
+   float rp = rs * (ab * cos2 + sin2 * sin2 - 2.0f * a * ci * sin2) /
+              fmaxf(ab * cos2 + sin2 * sin2 + 2.0f * a * ci * sin2, 1e-10f);
+   return (rs + rp) * 0.5f;
+}
+
+__device__ __forceinline__ V3 fr_complex(float ci, V3 eta, V3 k) {
+   ci = fminf(fmaxf(ci, 0.0f), 1.0f);
+   float c2 = ci * ci, s2 = 1.0f - c2;
+   return mk(fr_complex1(c2, s2, ci, eta.x, k.x), fr_complex1(c2, s2, ci, eta.y, k.y),
+             fr_complex1(c2, s2, ci, eta.z, k.z));
+}
+
+__device__ __forceinline__ float fresnel_dielectric(float ci, float eta) {
+   float s2t = eta * eta * fmaxf(0.0f, 1.0f - ci * ci);
+   if (s2t >= 1.0f) return 1.0f;
+   float ct = sqrtf(1.0f - s2t);
+   float rs = (eta * ci - ct) / (eta * ci + ct);
+   float rp = (ci - eta * ct) / (ci + eta * ct);
+   return 0.5f * (rs * rs + rp * rp);
+}
+
+struct Scatter {
+   V3 o, d, atten, emitted;
+   bool scattered;
+   bool lambert;
+};
+
+// apply_material's type mapping (cuda_raytracer.cuh:418-485) and scatter_material
+// (:613-850); RAYON_MAT_GGX_* are additions of this library.
+__device__ inline void scatter(const MatRec& M, V3 color, const JobView& J, const Surf& s, V3 dir, Rng& R,
+                               Scatter& o) {
+   const uint32_t mt = __float_as_uint(M.m0.w) & 0xFFu;
+   const bool quirks = (J.flags & RAYON_FLAG_REF_QUIRKS) != 0;
+   const V3 n = s.n;
+   o.emitted = mk(0.0f, 0.0f, 0.0f);
+   o.atten = mk(1.0f, 1.0f, 1.0f);
+   o.scattered = false;
+   o.lambert = false;
+   switch (mt) {
+   case RAYON_MAT_LAMBERTIAN:
+   case RAYON_MAT_SDF_MATERIAL: {
+      o.d = cosine_dir(n, R);
+      o.o = s.p + 1e-4f * n;
+      o.atten = color;
+      o.scattered = true;
+      o.lambert = true;
+      return;
+   }
+   case RAYON_MAT_MIRROR: {
+      V3 rf = reflect(unit(dir), n);
+      o.o = s.p;
+      o.d = rf;
+      o.atten = color;
+      o.scattered = dot(rf, n) > 0.0f;
+      return;
+   }
+   case RAYON_MAT_METAL:
+   case RAYON_MAT_ROUGH_MIRROR: {
+      float rough = M.m1.w, fz = 1.0f;
+      if (quirks) {
+         rough = rough > 0.0f ? rough : 0.3f;
+         fz = J.metal_fuzziness;
+      }
+      V3 sph = on_sphere(R);
+      V3 pn = unit(n + (rough * fz) * sph);
+      V3 rf = reflect(unit(dir), pn);
+      o.o = s.p;
+      o.d = rf;
+      o.atten = color;
+      o.scattered = dot(rf, n) > 0.0f;
+      return;
+   }
+   case RAYON_MAT_GLASS:
+   case RAYON_MAT_DIELECTRIC: {
+      V3 ud = unit(dir);
+      float ri = quirks ? J.glass_ior : M.m2.w;
+      float ratio = s.front ? 1.0f / ri : ri;
+      float ct = fminf(dot(-ud, n), 1.0f);
+      float st = sqrtf(1.0f - ct * ct);
+      bool tir = ratio * st > 1.0f;
+      if (tir || schlick(ct, ratio) > R.next())
+         o.d = reflect(ud, n);
+      else
+         o.d = refract(ud, n, ratio);
+      o.o = s.p;
+      o.scattered = true;
+      return;
+   }
+   case RAYON_MAT_LIGHT:
+      o.emitted = J.light_intensity * mk(M.m1);
+      return;
+   case RAYON_MAT_CONSTANT:
+      o.emitted = color;
+      return;
+   case RAYON_MAT_SHOW_NORMALS:
+      o.emitted = 0.5f * (n + mk(1.0f, 1.0f, 1.0f));
+      return;
+   case RAYON_MAT_ANISOTROPIC_METAL:
+   case RAYON_MAT_GGX_CONDUCTOR: {
+      const bool aniso = mt == RAYON_MAT_ANISOTROPIC_METAL;
+      const float rough = M.m1.w;
+      float ax, ay;
+      if (aniso) {
+         float aspect = sqrtf(fmaxf(1e-4f, 1.0f - 0.9f * M.m3.w));
+         float r2 = rough * rough;
+         ax = fmaxf(1e-4f, r2 / aspect);
+         ay = fmaxf(1e-4f, r2 * aspect);
+      } else {
+         ax = ay = fmaxf(1e-4f, rough * rough);
+      }
+      const V3 eta = mk(M.m2), kk = mk(M.m3);
+      if (fmaxf(ax, ay) < 1e-3f) {
+         V3 rf = reflect(unit(dir), n);
+         o.o = s.p + 1e-4f * n;
+         o.d = rf;
+         float ci = fmaxf(dot(-unit(dir), n), 0.0f);
+         o.atten = color * fr_complex(ci, eta, kk);
+         o.scattered = dot(rf, n) > 0.0f;
+         return;
+      }
+      V3 N = unit(n), T, B;
+      if (aniso) {
+         T = unit(s.tangent - dot(s.tangent, N) * N);
+         B = cross(N, T);
+      } else {
+         onb(N, T, B);
+      }
+      V3 wo = unit(-dir);
+      V3 wol = mk(dot(wo, T), dot(wo, B), dot(wo, N));
+      if (wol.z <= 0.0f) return;
+      float u1 = R.next(), u2 = R.next();
+      V3 wm = sample_vndf(wol, ax, ay, u1, u2);
+      V3 wil = reflect(-wol, wm);
+      if (wil.z <= 0.0f) return;
+      V3 F = fr_complex(fmaxf(dot(wol, wm), 0.0f), eta, kk);
+      float lo = lambda_ggx(wol, ax, ay), li = lambda_ggx(wil, ax, ay);
+      float w = (1.0f / (1.0f + lo + li)) / fmaxf(1.0f / (1.0f + lo), 1e-6f);
+      o.o = s.p + 1e-4f * n;
+      o.d = wil.x * T + wil.y * B + wil.z * N;
+      o.atten = color * (w * F);
+      o.scattered = true;
+      return;
+   }
+   case RAYON_MAT_GGX_DIELECTRIC: {
+      const float rough = M.m1.w;
+      float a = fmaxf(1e-4f, rough * rough);
+      float ri = M.m2.w;
+      float eta = s.front ? 1.0f / ri : ri;
+      V3 N = unit(n), T, B;
+      onb(N, T, B);
+      V3 wo = unit(-dir);
+      V3 wol = mk(dot(wo, T), dot(wo, B), dot(wo, N));
+      if (wol.z <= 0.0f) return;
+      float u1 = R.next(), u2 = R.next();
+      V3 wm = sample_vndf(wol, a, a, u1, u2);
+      float F = fresnel_dielectric(dot(wol, wm), eta);
+      bool refl = R.next() < F;
+      V3 wil;
+      if (refl) {
+         wil = reflect(-wol, wm);
+         if (wil.z <= 0.0f) return;
+      } else {
+         wil = refract(-wol, wm, eta);
+         if (wil.z >= 0.0f) return;
+      }
+      float lo = lambda_ggx(wol, a, a), li = lambda_ggx(wil, a, a);
+      float w = (1.0f / (1.0f + lo + li)) / fmaxf(1.0f / (1.0f + lo), 1e-6f);
+      o.o = refl ? s.p + 1e-4f * n : s.p - 1e-4f * n;
+      o.d = wil.x * T + wil.y * B + wil.z * N;
+      o.atten = w * color;
+      o.scattered = true;
+      return;
+   }
+   case RAYON_MAT_THIN_FILM: {  // cuda_raytracer.cuh:753-809
+      V3 ud = unit(dir);
+      float ci = fmaxf(fabsf(dot(-ud, n)), 0.001f);
+      float n0 = M.m2.w, n1 = M.m4.y, dd = M.m4.x;
+      float si = sqrtf(fmaxf(0.0f, 1.0f - ci * ci));
+      float st = (n0 / n1) * si;
+      float ct = st >= 1.0f ? 0.0f : sqrtf(1.0f - st * st);
+      float r01 = (n0 - n1) / (n0 + n1);
+      r01 *= r01;
+      float x01 = 1.0f - ci;
+      float R01 = r01 + (1.0f - r01) * x01 * x01 * x01 * x01 * x01;
+      float r12 = (n1 - n0) / (n1 + n0);
+      r12 *= r12;
+      float x12 = 1.0f - ct;
+      float R12 = r12 + (1.0f - r12) * x12 * x12 * x12 * x12 * x12;
+      float sq = sqrtf(R01 * R12);
+      float base = 4.0f * kPiF * n1 * dd * ct;
+      float c0 = cosf(base / 650.0f), c1 = cosf(base / 550.0f), c2 = cosf(base / 450.0f);
+      float Rr = (R01 + R12 + 2.0f * sq * c0) / fmaxf(1.0f + R01 * R12 + 2.0f * sq * c0, 1e-8f);
+      float Rg = (R01 + R12 + 2.0f * sq * c1) / fmaxf(1.0f + R01 * R12 + 2.0f * sq * c1, 1e-8f);
+      float Rb = (R01 + R12 + 2.0f * sq * c2) / fmaxf(1.0f + R01 * R12 + 2.0f * sq * c2, 1e-8f);
+      float avg = (Rr + Rg + Rb) / 3.0f;
+      if (R.next() < avg) {
+         V3 rf = reflect(ud, n);
+         o.o = s.p + 1e-4f * n;
+         o.d = rf;
+         o.atten = (1.0f / fmaxf(avg, 0.001f)) * mk(Rr, Rg, Rb);
+         o.scattered = dot(rf, n) > 0.0f;
+      } else {
+         o.o = s.p - 1e-4f * n;
+         o.d = ud;
+         o.atten = (1.0f / fmaxf(1.0f - avg, 0.001f)) * mk(1.0f - Rr, 1.0f - Rg, 1.0f - Rb);
+         o.scattered = true;
+      }
+      return;
+   }
+   case RAYON_MAT_CLEAR_COAT: {  // cuda_raytracer.cuh:810-846
+      V3 ud = unit(dir);
+      float ct = fminf(dot(-ud, n), 1.0f);
+      float ior = M.m2.w > 1.0f ? M.m2.w : 1.5f;
+      float F = schlick(ct, ior);
+      if (R.next() < F) {
+         const float rough = M.m1.w;
+         V3 pn = n;
+         if (rough > 1e-3f) pn = unit(n + rough * on_sphere(R));
+         V3 rf = reflect(ud, pn);
+         o.o = s.p + 1e-4f * n;
+         o.d = rf;
+         o.scattered = dot(rf, n) > 0.0f;
+      } else {
+         o.d = cosine_dir(n, R);
+         o.o = s.p + 1e-4f * n;
+         o.atten = color;
+         o.scattered = true;
+      }
+      return;
+   }
+   default:
+      return;
+   }
+}
+
+// apply_procedural_pattern (cuda_raytracer.cuh:377-416)
+__device__ inline V3 surface_color(const MatRec& M, V3 p, V3 center) {
+   V3 base = mk(M.m0);
+   const uint32_t pat = (__float_as_uint(M.m0.w) >> 8) & 0xFFu;
+   if (pat != RAYON_PATTERN_FIBONACCI_DOTS) return base;
+   V3 dir = unit(p - center);
+   int N = (int)M.m4.z;
+   float best = -1.0f;
+   for (int i = 0; i < N; ++i) best = fmaxf(best, dot(dir, fib_point(i, N)));
+   best = fmaxf(fminf(best, 1.0f), -1.0f);
+   float mask = acosf(best) < M.m4.w ? 0.0f : 1.0f;
+   return mask * base + (1.0f - mask) * mk(M.m5);
+}
+
+// Uniform point on an emitter (rectangle, triangle, sphere surface).
+__device__ __forceinline__ void sample_emitter(const PrimRec& p, float a, float b, V3& q, V3& nl, bool& one_sided) {
+   const uint32_t kind = info_kind(__float_as_uint(p.a.w));
+   one_sided = false;
+   if (kind == K_RECT) {
+      q = mk(p.a) + a * mk(p.b) + b * mk(p.c);
+      nl = unit(cross(mk(p.b), mk(p.c)));
+   } else if (kind == K_TRI) {
+      float su = sqrtf(a);
+      q = mk(p.a) + (su * (1.0f - b)) * mk(p.b) + (su * b) * mk(p.c);
+      nl = unit(cross(mk(p.b), mk(p.c)));
+   } else {
+      float z = 1.0f - 2.0f * a;
+      float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+      nl = mk(r * cos2pi(b), r * sin2pi(b), z);
+      q = mk(p.a) + p.b.x * nl;
+      one_sided = true;
+   }
+}
+
+__device__ __forceinline__ V3 sky(const JobView& J, V3 d) {  // cuda_raytracer.cuh:924-929
+   V3 ud = unit(d);
+   float t = 0.5f * (ud.y + 1.0f);
+   return J.background_intensity * mk(1.0f - 0.5f * t, 1.0f - 0.3f * t, 1.0f);
+}
+
+// Neumaier-compensated add; plain adds only, so no contraction can change it.
+__device__ __forceinline__ void neumaier(float& sum, float& c, float x) {
+   float t = sum + x;
+   if (fabsf(sum) >= fabsf(x))
+      c += (sum - t) + x;
+   else
+      c += (x - t) + sum;
+   sum = t;
+}
+
+__device__ __forceinline__ float luminance(V3 c) { return 0.299f * c.x + 0.587f * c.y + 0.114f * c.z; }
+
+// Camera sample: render_acc_kernel.cu:200-221.
+__device__ __forceinline__ void camera_ray(const JobView& J, int x, int y, Rng& R, V3& o, V3& d) {
+   float ou = R.next() - 0.5f;
+   float ov = R.next() - 0.5f;
+   V3 C = ld3(J.cam_c);
+   V3 pc = ld3(J.p00) + ((float)x + ou) * ld3(J.du) + ((float)y + ov) * ld3(J.dv);
+   o = C;
+   d = pc - C;
+   if ((J.flags & RAYON_FLAG_DOF) && J.dof_aperture > 0.0f) {
+      V3 nd = unit(d);
+      V3 focus = C + J.dof_focus * nd;
+      float dx, dy;
+      do {
+         dx = 2.0f * R.next() - 1.0f;
+         dy = 2.0f * R.next() - 1.0f;
+      } while (dx * dx + dy * dy >= 1.0f);
+      o = C + J.dof_aperture * (dx * ld3(J.cu) + dy * ld3(J.cv));
+      d = focus - o;
+   }
+}
+
+}  // namespace ry

@@ -1,0 +1,82 @@
+// internal.h — host-side objects behind the opaque C-ABI handles.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/rayon_hip.h"
+#include "layout.h"
+
+namespace ry {
+
+void set_error(const char* fmt, ...);
+rayon_status hip_fail(hipError_t e, const char* what);
+
+#define RY_HIP(call)                                               \
+   do {                                                            \
+      hipError_t e_ = (call);                                      \
+      if (e_ != hipSuccess) return ::ry::hip_fail(e_, #call);      \
+   } while (0)
+
+// Device allocation owned by a scene, job or context.
+struct DevBuf {
+   void* p = nullptr;
+   size_t bytes = 0;
+};
+
+// Host primitive before the BVH reorder.
+struct HostPrim {
+   uint32_t kind;
+   uint32_t flags;
+   uint32_t mat;
+   int32_t orig;
+   float a[3], b[3], c[3];
+   float n[9];
+   float lo[3], hi[3];
+};
+
+struct BuildResult {
+   std::vector<int32_t> order;  // slot -> host prim index
+   std::vector<Node2> nodes;
+   int root = 0;                // packed root reference
+   int max_depth = 0;
+};
+
+// Binned-SAH build (16 bins, leaves <= 4 primitives, depth <= kMaxDepth).
+constexpr int kMaxDepth = 32;
+bool build_bvh(const std::vector<HostPrim>& prims, BuildResult& out);
+
+// Flattens a description into host primitives, validates it.
+rayon_status gather_prims(const rayon_scene_desc* d, std::vector<HostPrim>& prims);
+MatRec pack_material(const rayon_material& m);
+
+}  // namespace ry
+
+struct rayon_ctx {
+   int device = 0;
+   hipStream_t stream = nullptr;
+   std::mutex mu;
+   // wavefront state (shared by all batched launches of this context)
+   struct Wave* wave = nullptr;
+};
+
+struct rayon_scene {
+   rayon_ctx* ctx = nullptr;
+   ry::SceneView view{};
+   std::vector<ry::DevBuf> bufs;
+   rayon_scene_info info{};
+   std::vector<int64_t> orig_of_slot;
+};
+
+struct rayon_job {
+   rayon_ctx* ctx = nullptr;
+   rayon_scene* scene = nullptr;
+   rayon_job_params params{};
+   ry::JobView view{};
+   std::vector<ry::DevBuf> bufs;
+   int64_t samples_done = 0;
+   int job_slot = -1;  // index in the batched job table
+};

@@ -1,0 +1,20 @@
+// kernels.h — host-side launchers of the device kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "layout.h"
+
+namespace ry {
+
+hipError_t launch_plain(const JobView& J, int samples, hipStream_t st);
+hipError_t launch_resolve_u8(const JobView& J, uint8_t* out, int channels, float gamma, hipStream_t st);
+hipError_t launch_resolve_linear(const JobView& J, float* out, hipStream_t st);
+hipError_t launch_trace_closest(const SceneView& S, const float* o, const float* d, long long n, float tmin,
+                                float tmax, long long* prim, float* t, hipStream_t st);
+hipError_t launch_primary(const JobView& J, int sample, float* o, float* d, hipStream_t st);
+hipError_t launch_ref_rng_init(uint32_t* p, int n, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t st);
+
+}  // namespace ry

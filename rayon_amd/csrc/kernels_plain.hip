@@ -1,0 +1,198 @@
+// kernels_plain.hip — the plain path: one launch of a per-pixel megakernel per job and
+// pass (the role of renderAccKernel, shaders/render_acc_kernel.cu:128-282), plus the
+// resolve, closest-hit query and primary-ray kernels used by the C ABI.
+//
+// Launch shape: 256-lane workgroups (4 wave64) covering a 16x16 pixel tile, one lane per
+// pixel, samples looped in the lane. Workgroups are dispatched round-robin over the 8
+// XCDs; the tile index is remapped so each XCD walks a contiguous band of tiles and its
+// L2 sees spatially coherent BVH traffic.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "pathtrace.h"
+
+namespace ry {
+
+constexpr int kTile = 16;
+constexpr int kBlock = kTile * kTile;
+
+__device__ __forceinline__ int xcd_tile(int b, int per_xcd) { return (b & 7) * per_xcd + (b >> 3); }
+
+template <bool CNT>
+__global__ __launch_bounds__(kBlock) void k_plain(JobView J, int samples, int tiles_x, int ntiles, int per_xcd) {
+   __shared__ int stack[kStack * kBlock];
+   const int tid = threadIdx.x;
+   const int tile = xcd_tile(blockIdx.x, per_xcd);
+   PathCounts pc;
+   int primary = 0, converged = 0;
+   if (tile < ntiles) {
+      const int x = (tile % tiles_x) * kTile + (tid & (kTile - 1));
+      const int y = (tile / tiles_x) * kTile + (tid / kTile);
+      if (x < J.width && y < J.height) {
+         const int pix = y * J.width + x;
+         const int cnt = J.counts[pix];
+         int todo = samples;
+         if (J.spp_target > 0) todo = min(todo, J.spp_target - cnt);
+         if (cnt >= 0 && todo > 0) {
+            const float4 a0 = J.acc[pix];
+            float4 a = a0, c = J.comp[pix];
+            const bool refrng = (J.flags & RAYON_FLAG_REF_RNG) != 0;
+            Rng R{refrng ? J.rng[pix] : 0u, refrng};
+            for (int s = 0; s < todo; ++s) {
+               if (!refrng) R.s = counter_init(J.seed_lo, J.seed_hi, (uint32_t)pix, (uint32_t)(cnt + s));
+               V3 o, d;
+               camera_ray(J, x, y, R, o, d);
+               const V3 L = trace_path<kBlock, CNT>(J, o, d, R, stack + tid, pc);
+               accumulate(a, c, L);
+            }
+            primary = todo;
+            J.acc[pix] = a;
+            J.comp[pix] = c;
+            if (refrng) J.rng[pix] = R.s;
+            const int n = adaptive_count(J, a0, cnt, a, c, cnt + todo);
+            converged = n < 0 ? 1 : 0;
+            J.counts[pix] = n;
+         }
+      }
+   }
+   unsigned long long v0 = wave_sum((unsigned long long)pc.segments + (unsigned long long)pc.shadow);
+   unsigned long long v1 = wave_sum((unsigned long long)pc.shadow);
+   unsigned long long v2 = wave_sum((unsigned long long)primary);
+   unsigned long long v3 = wave_sum((unsigned long long)converged);
+   if ((tid & 63) == 0) {
+      if (v0) atomicAdd(&J.counters[0], v0);
+      if (v1) atomicAdd(&J.counters[1], v1);
+      if (v2) atomicAdd(&J.counters[2], v2);
+      if (v3) atomicAdd(&J.counters[3], v3);
+   }
+   if constexpr (CNT) {
+      unsigned long long c4 = wave_sum((unsigned long long)pc.trav.nodes);
+      unsigned long long c5 = wave_sum((unsigned long long)pc.trav.prims);
+      unsigned long long c6 = wave_sum((unsigned long long)pc.mats);
+      unsigned long long c7 = wave_sum((unsigned long long)pc.light_prims);
+      if ((tid & 63) == 0) {
+         atomicAdd(&J.counters[4], c4);
+         atomicAdd(&J.counters[5], c5);
+         atomicAdd(&J.counters[6], c6);
+         atomicAdd(&J.counters[7], c7);
+      }
+   }
+}
+
+// gammaCorrectKernel semantics (render_acc_kernel.cu:11-44) on the compensated sums.
+__global__ void k_resolve_u8(JobView J, uint8_t* out, int channels, float inv_gamma, int n_total) {
+   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+   if (pix >= J.width * J.height) return;
+   const float4 a = J.acc[pix], c = J.comp[pix];
+   int n = J.counts ? abs(J.counts[pix]) : n_total;
+   if (n <= 0) n = 1;
+   const float inv = 1.0f / (float)n;
+   const float r = fminf(powf(fmaxf((a.x + c.x) * inv, 0.0f), inv_gamma), 0.999f);
+   const float g = fminf(powf(fmaxf((a.y + c.y) * inv, 0.0f), inv_gamma), 0.999f);
+   const float b = fminf(powf(fmaxf((a.z + c.z) * inv, 0.0f), inv_gamma), 0.999f);
+   uint8_t* o = out + (size_t)pix * channels;
+   o[0] = (uint8_t)(256.0f * r);
+   o[1] = (uint8_t)(256.0f * g);
+   o[2] = (uint8_t)(256.0f * b);
+   if (channels == 4) o[3] = 255;
+}
+
+__global__ void k_resolve_linear(JobView J, float* out) {
+   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+   if (pix >= J.width * J.height) return;
+   const float4 a = J.acc[pix], c = J.comp[pix];
+   int n = abs(J.counts[pix]);
+   const float inv = n > 0 ? 1.0f / (float)n : 0.0f;
+   out[3 * (size_t)pix + 0] = (a.x + c.x) * inv;
+   out[3 * (size_t)pix + 1] = (a.y + c.y) * inv;
+   out[3 * (size_t)pix + 2] = (a.z + c.z) * inv;
+}
+
+__global__ __launch_bounds__(256) void k_trace_closest(SceneView S, const float* o, const float* d, long long n,
+                                                       float tmin, float tmax, long long* prim, float* t) {
+   __shared__ int stack[kStack * 256];
+   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+   if (i >= n) return;
+   const V3 oo = mk(o[3 * i], o[3 * i + 1], o[3 * i + 2]);
+   const V3 dd = mk(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
+   const HitMin h = closest_hit<256>(S, oo, dd, tmin, tmax, stack + threadIdx.x);
+   prim[i] = h.slot < 0 ? -1 : (long long)h.orig;
+   t[i] = h.slot < 0 ? -1.0f : h.t;
+}
+
+__global__ void k_primary(JobView J, int sample, float* o, float* d) {
+   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+   if (pix >= J.width * J.height) return;
+   const bool refrng = (J.flags & RAYON_FLAG_REF_RNG) != 0;
+   Rng R{refrng ? J.rng[pix] : counter_init(J.seed_lo, J.seed_hi, (uint32_t)pix, (uint32_t)sample), refrng};
+   V3 oo, dd;
+   camera_ray(J, pix % J.width, pix / J.width, R, oo, dd);
+   o[3 * (size_t)pix] = oo.x;
+   o[3 * (size_t)pix + 1] = oo.y;
+   o[3 * (size_t)pix + 2] = oo.z;
+   d[3 * (size_t)pix] = dd.x;
+   d[3 * (size_t)pix + 1] = dd.y;
+   d[3 * (size_t)pix + 2] = dd.z;
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, long long n) {
+   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) p[i] = v;
+}
+
+// Reference stream seeds: (uint32)(1234 + idx * 747796405), cuda_utils.cu:26.
+__global__ void k_ref_rng_init(uint32_t* st, int n) {
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) st[i] = 1234u + (uint32_t)i * 747796405u;
+}
+
+// ---- host launchers --------------------------------------------------------------
+hipError_t launch_plain(const JobView& J, int samples, hipStream_t st) {
+   const int tx = (J.width + kTile - 1) / kTile, ty = (J.height + kTile - 1) / kTile;
+   const int ntiles = tx * ty;
+   const int per = (ntiles + 7) / 8;
+   if (J.flags & RAYON_FLAG_COUNT_TRAVERSAL)
+      hipLaunchKernelGGL(k_plain<true>, dim3(8 * per), dim3(kBlock), 0, st, J, samples, tx, ntiles, per);
+   else
+      hipLaunchKernelGGL(k_plain<false>, dim3(8 * per), dim3(kBlock), 0, st, J, samples, tx, ntiles, per);
+   return hipGetLastError();
+}
+
+hipError_t launch_resolve_u8(const JobView& J, uint8_t* out, int channels, float gamma, hipStream_t st) {
+   const int n = J.width * J.height;
+   hipLaunchKernelGGL(k_resolve_u8, dim3((n + 255) / 256), dim3(256), 0, st, J, out, channels, 1.0f / gamma, 1);
+   return hipGetLastError();
+}
+
+hipError_t launch_resolve_linear(const JobView& J, float* out, hipStream_t st) {
+   const int n = J.width * J.height;
+   hipLaunchKernelGGL(k_resolve_linear, dim3((n + 255) / 256), dim3(256), 0, st, J, out);
+   return hipGetLastError();
+}
+
+hipError_t launch_trace_closest(const SceneView& S, const float* o, const float* d, long long n, float tmin,
+                                float tmax, long long* prim, float* t, hipStream_t st) {
+   if (n <= 0) return hipSuccess;
+   hipLaunchKernelGGL(k_trace_closest, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, S, o, d, n, tmin, tmax,
+                      prim, t);
+   return hipGetLastError();
+}
+
+hipError_t launch_primary(const JobView& J, int sample, float* o, float* d, hipStream_t st) {
+   const int n = J.width * J.height;
+   hipLaunchKernelGGL(k_primary, dim3((n + 255) / 256), dim3(256), 0, st, J, sample, o, d);
+   return hipGetLastError();
+}
+
+hipError_t launch_ref_rng_init(uint32_t* p, int n, hipStream_t st) {
+   hipLaunchKernelGGL(k_ref_rng_init, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+   return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t st) {
+   if (n <= 0) return hipSuccess;
+   hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, v, n);
+   return hipGetLastError();
+}
+
+}  // namespace ry

@@ -1,0 +1,118 @@
+// layout.h — device-resident scene and job layouts, shared by host and device code.
+//
+// HBM layout (one scene):
+//   prims   : PrimRec[num_prims], 48 B, leaf order (the slot order of the BVH leaves,
+//             as the reference reorders geometries, scene_description.hpp:735-777)
+//   normals : float4[3*num_prims] vertex normals, only read for smooth triangles
+//   nodes   : Node2[num_nodes], 64 B = one cache line per visited pair of children
+//   qnodes  : QNode8[...], 8-wide quantized nodes (80 B), see bvh_quant.*
+//   mats    : MatRec[num_mats], 96 B
+//   lights  : int slot + float area per emitter
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define RY_HD __host__ __device__
+#else
+#define RY_HD
+struct alignas(16) float4 {
+   float x, y, z, w;
+};
+struct alignas(16) int4 {
+   int x, y, z, w;
+};
+#endif
+
+namespace ry {
+
+// Packed reference meaning "no node" (empty scene root, empty stack).
+constexpr int kSentinel = (int)0x80000000;
+
+// primitive types held in PrimRec::a.w bits 28..31
+enum PrimKind : uint32_t { K_TRI = 0, K_SPHERE = 1, K_RECT = 2, K_DISPLACED = 3, K_SDF = 4 };
+enum PrimFlag : uint32_t { F_VISIBLE = 1u << 24, F_NORMALS = 1u << 25, F_EMITTER = 1u << 26 };
+
+// Primitive record, 3 x float4:
+//   triangle : a = (v0, info)  b = (v1-v0, orig_id)  c = (v2-v0, 0)
+//   sphere   : a = (center, info)  b = (radius, 0, 0, orig_id)
+//   rectangle: a = (corner, info)  b = (u, orig_id)  c = (v, 0)
+//   sdf field: a = (bounds lo, info) b = (bounds hi, orig_id)
+// info = kind << 28 | flags | material (bits 0..23)
+struct alignas(16) PrimRec {
+   float4 a, b, c;
+};
+
+RY_HD inline uint32_t info_kind(uint32_t info) { return info >> 28; }
+RY_HD inline uint32_t info_mat(uint32_t info) { return info & 0xFFFFFFu; }
+
+// Binary BVH stored as pair nodes: node i holds the boxes of both children of one
+// interior node. Child reference: interior -> node index (>= 0) with count 0;
+// leaf -> first primitive slot with count in 1..8.
+struct alignas(64) Node2 {
+   float4 bx;  // c0.lo.x, c0.hi.x, c1.lo.x, c1.hi.x
+   float4 by;
+   float4 bz;
+   int4 meta;  // ref0, ref1, count0, count1
+};
+
+// Packed child reference for stacks: interior >= 0, leaf = ~(first << 3 | (count - 1)).
+RY_HD inline int pack_leaf(int first, int count) { return ~((first << 3) | (count - 1)); }
+
+// Material record, 6 x float4.
+struct alignas(16) MatRec {
+   float4 m0;  // albedo, type (bits 0..7) | pattern (bits 8..15)
+   float4 m1;  // emission, roughness
+   float4 m2;  // eta, refractive_index
+   float4 m3;  // k, anisotropy
+   float4 m4;  // film_thickness, film_ior, pattern_param1, pattern_param2
+   float4 m5;  // pattern_color, 0
+};
+
+struct alignas(16) SdfRec {
+   float4 c;  // center, type
+   float4 p;  // params
+};
+
+// Device view of one scene.
+struct SceneView {
+   const PrimRec* prims;
+   const float4* normals;
+   const Node2* nodes;
+   const MatRec* mats;
+   const int* light_slots;
+   const float* light_area;
+   const SdfRec* sdf;
+   int num_lights;
+   int num_sdf;
+   int root;        // packed child reference of the root, or INT_MIN if empty
+   int num_prims;
+   float sdf_k;
+   float sdf_eps;
+   int sdf_steps;
+   int pad_;
+};
+
+// Per-job constants of one launch.
+struct JobView {
+   SceneView scene;
+   float4* acc;        // per pixel: sum.rgb, sum of luminance^2
+   float4* comp;       // per pixel: Neumaier compensation rgb, luminance compensation
+   int* counts;        // per pixel sample count, negative = converged
+   uint32_t* rng;      // per pixel sequential stream (RAYON_FLAG_REF_RNG only)
+   unsigned long long* counters;  // segments, shadow segments, primary samples, converged
+   float cam_c[3], p00[3], du[3], dv[3], cu[3], cv[3];
+   int width, height;
+   int max_depth;
+   uint32_t flags;
+   uint32_t seed_lo, seed_hi;
+   float light_intensity, background_intensity, metal_fuzziness, glass_ior;
+   float dof_aperture, dof_focus;
+   int min_adaptive;
+   float adaptive_threshold;
+   int sample_base;    // index of the first sample of this pass
+   int spp_target;
+};
+
+}  // namespace ry

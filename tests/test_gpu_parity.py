@@ -1,0 +1,182 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle on the same
+seeded inputs.
+
+Tolerances (stated here, checked below):
+  * primary-hit primitive IDs: bit-exact on the identical FP32 rays, except near-ties;
+    mismatches are at most 1e-5 of the rays and every mismatch must be a near-tie (the
+    oracle's own answer changes under a 1e-5 relative perturbation of the ray);
+  * hit distance of agreeing hits: relative 1e-4 (FP32 vs FP64 arithmetic);
+  * images: relative RMSE sqrt(sum|I-O|^2 / sum|O|^2) over linear RGB <= 1% with the same
+    random-number streams on both sides.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import rayon_amd as ra
+from oracle import OracleScene
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel_rmse(img, ref):
+    return float(np.sqrt(((img - ref) ** 2).sum() / (ref ** 2).sum()))
+
+
+def _perturbed_disagree(osc, o, d, prim, eps=1e-5, k=6, seed=0):
+    """True where the oracle's closest primitive changes under small ray perturbations."""
+    rng = np.random.default_rng(seed)
+    unstable = np.zeros(len(o), bool)
+    for _ in range(k):
+        dd = d * (1 + eps * rng.standard_normal(d.shape)).astype(np.float32)
+        p2, _ = osc.trace_closest(o, dd, brute=True)
+        unstable |= p2 != prim
+    return unstable
+
+
+def _setup(ctx, kind, seed, detail, W, H, **kw):
+    g = ra.Generated.generate(kind, seed, detail=detail)
+    sc = ctx.scene(g.desc_ptr())
+    lf, la, up, fov = g.camera()
+    cam = ra.camera_frame(lf, la, up, fov, W, H)
+    P = ra.job_params(W, H, cam, **kw)
+    job = ctx.job(sc, P)
+    return g, sc, cam, P, job
+
+
+@pytest.mark.parametrize("kind,seed,detail,W,H", [
+    (ra.GEN.BOOK_SPHERES, 1984, 0, 640, 360),
+    (ra.GEN.CORNELL_ICOSPHERE, 7, 4, 320, 180),
+])
+def test_primary_hit_ids_bit_exact(ctx, kind, seed, detail, W, H):
+    log(f"primary hits kind={kind} {W}x{H}")
+    g, sc, cam, P, job = _setup(ctx, kind, seed, detail, W, H, spp=1)
+    o, d = job.primary_rays(0)
+    gp, gt = sc.trace_closest(o, d)
+    osc = OracleScene(g.desc_ptr())
+    op, ot = osc.trace_closest(o, d, brute=True)
+    mism = np.nonzero(gp != op)[0]
+    log(f"  rays={len(o)} hits={int((op >= 0).sum())} mismatches={len(mism)}")
+    assert len(mism) <= max(1e-5 * len(o), 1)
+    if len(mism):
+        unstable = _perturbed_disagree(osc, o[mism], d[mism], op[mism])
+        assert unstable.all(), "a primary-hit mismatch is not a near-tie"
+    ok = (gp == op) & (op >= 0)
+    np.testing.assert_allclose(gt[ok], ot[ok], rtol=1e-4)
+    job.close()
+    sc.close()
+
+
+def test_random_rays_closest_hit_mesh(ctx):
+    log("random rays on the displaced icosphere (detail 5)")
+    g = ra.Generated.generate(ra.GEN.CORNELL_ICOSPHERE, 7, detail=5)
+    sc = ctx.scene(g.desc_ptr())
+    info = sc.info()
+    assert info.max_depth <= 32
+    rng = np.random.default_rng(11)
+    n = 200000
+    o = rng.uniform(0, 555, (n, 3)).astype(np.float32)
+    d = rng.normal(0, 1, (n, 3)).astype(np.float32)
+    gp, gt = sc.trace_closest(o, d)
+    osc = OracleScene(g.desc_ptr())
+    op, ot = osc.trace_closest(o, d)
+    mism = np.nonzero(gp != op)[0]
+    log(f"  mismatches={len(mism)} / {n}")
+    assert len(mism) <= max(1e-5 * n, 2)
+    if len(mism):
+        assert _perturbed_disagree(osc, o[mism], d[mism], op[mism]).all()
+    sc.close()
+
+
+def test_white_furnace_gpu(ctx):
+    import ctypes as C
+    from test_oracle import make_desc, mat, sphere
+    desc, keep = make_desc([mat(0, (0.3, 0.6, 1.0))], [sphere((0, 0, 0), 1.0, 0)])
+    sc = ctx.scene(C.pointer(desc))
+    cam = ra.camera_frame((0, 0, 4), (0, 0, 0), (0, 1, 0), 40.0, 64, 48)
+    job = ctx.job(sc, ra.job_params(64, 48, cam, spp=16, max_depth=16))
+    job.render_plain(16)
+    img = job.resolve_linear()
+    np.testing.assert_allclose(img[..., 2], 1.0, rtol=2e-6)
+    assert img[..., 0].min() < 0.5
+    job.close()
+    sc.close()
+
+
+@pytest.mark.parametrize("name,kind,seed,detail,W,H,spp,depth,flags", [
+    ("book", ra.GEN.BOOK_SPHERES, 1984, 0, 128, 72, 64, 50, 0),
+    ("cornell_nee", ra.GEN.CORNELL_ICOSPHERE, 7, 4, 96, 96, 64, 8, ra.FLAG.NEE),
+    ("cornell_bsdf", ra.GEN.CORNELL_ICOSPHERE, 7, 4, 64, 64, 64, 8, 0),
+])
+def test_image_rel_rmse_same_streams(ctx, name, kind, seed, detail, W, H, spp, depth, flags):
+    log(f"image parity {name} {W}x{H} spp={spp} depth={depth}")
+    g, sc, cam, P, job = _setup(ctx, kind, seed, detail, W, H, spp=spp, max_depth=depth, flags=flags, seed=77)
+    job.render_plain(spp)
+    img = job.resolve_linear().reshape(-1, 3).astype(np.float64)
+    st = job.stats()
+    osc = OracleScene(g.desc_ptr())
+    sums, segs = osc.render(P, nsamples=spp, camera=cam)
+    ref = sums / spp
+    err = rel_rmse(img, ref)
+    log(f"  rel_rmse={err:.5f} gpu_segments={st.segments} oracle_segments={segs}")
+    assert err <= 0.01
+    assert st.primary_samples == W * H * spp
+    assert abs(st.segments - segs) / segs < 0.01
+    job.close()
+    sc.close()
+
+
+def test_reference_rng_stream_primary_rays(ctx):
+    """REF_RNG: the per-pixel stream seeded as cuda_utils.cu:26 drives the jitter; the
+    device's sample-0 directions equal the ones built from the oracle's stream."""
+    from oracle import lib as olib
+    import ctypes as C
+    W, H = 64, 32
+    g, sc, cam, P, job = _setup(ctx, ra.GEN.BOOK_SPHERES, 1984, 0, W, H, spp=1, flags=ra.FLAG.REF_RNG)
+    o, d = job.primary_rays(0)
+    exp = np.empty_like(d, dtype=np.float64)
+    p00, du, dv, c0 = (np.array(list(getattr(P, k)), np.float64) for k in ("pixel00", "delta_u", "delta_v",
+                                                                           "camera_center"))
+    for pix in range(W * H):
+        st = C.c_uint32(olib().oracle_ref_rng_init(pix))
+        ou = olib().oracle_ref_rng_next(C.byref(st)) - 0.5
+        ov = olib().oracle_ref_rng_next(C.byref(st)) - 0.5
+        x, y = pix % W, pix // W
+        exp[pix] = p00 + (x + ou) * du + (y + ov) * dv - c0
+    np.testing.assert_allclose(d, exp, rtol=1e-5, atol=1e-5)
+    job.close()
+    sc.close()
+
+
+def test_reference_adaptive_rule(ctx):
+    """RAYON_FLAG_ADAPTIVE_REF follows render_acc_kernel.cu:240-271 pass by pass."""
+    from oracle import lib as olib
+    import ctypes as C
+    W, H = 48, 32
+    g, sc, cam, P, job = _setup(ctx, ra.GEN.BOOK_SPHERES, 1984, 0, W, H, spp=96, max_depth=8,
+                                flags=ra.FLAG.ADAPTIVE_REF, min_adaptive_samples=16, adaptive_threshold=0.02)
+    prev_cnt = np.zeros(W * H, np.int32)
+    prev_sum = np.zeros((W * H, 3))
+    agree = total = 0
+    for _ in range(6):
+        job.render_plain(16)
+        cnt = job.sample_counts().reshape(-1)
+        mean = job.resolve_linear().reshape(-1, 3).astype(np.float64)
+        s = mean * np.abs(cnt)[:, None]
+        for i in range(W * H):
+            exp = olib().oracle_adaptive_ref(int(prev_cnt[i]), (C.c_double * 3)(*prev_sum[i]),
+                                             (C.c_double * 3)(*s[i]), 16, 16, 0.02)
+            total += 1
+            agree += exp == cnt[i]
+        prev_cnt, prev_sum = cnt, s
+    log(f"  adaptive agreement {agree}/{total}, converged {(prev_cnt < 0).sum()}")
+    assert agree / total > 0.995
+    assert (prev_cnt < 0).sum() > 0
+    job.close()
+    sc.close()
