@@ -33,6 +33,8 @@ def lib() -> C.CDLL:
         L.oracle_scene_num_lights.argtypes = [vp]
         L.oracle_trace_closest.argtypes = [vp, P(C.c_float), P(C.c_float), i64, C.c_double, C.c_double, i32,
                                            P(i64), P(C.c_double)]
+        L.oracle_intersect_prim.restype = C.c_double
+        L.oracle_intersect_prim.argtypes = [vp, i64, P(C.c_float), P(C.c_float), C.c_double, C.c_double]
         L.oracle_trace_any.argtypes = [vp, P(C.c_float), P(C.c_float), i64, C.c_double, C.c_double, P(i32)]
         L.oracle_render.argtypes = [vp, P(JobParams), P(Camera), P(i64), i64, i32, i32, i32, P(C.c_double),
                                     P(C.c_uint32), P(i64)]
@@ -91,6 +93,11 @@ class OracleScene:
         lib().oracle_trace_closest(self._h, _fp(o), _fp(d), n, t_min, t_max, int(brute),
                                    prim.ctypes.data_as(C.POINTER(C.c_int64)), _dp(t))
         return prim, t
+
+    def intersect_prim(self, prim, origin, d, t_min=0.001, t_max=3.4e38) -> float:
+        o = np.ascontiguousarray(origin, np.float32)
+        dd = np.ascontiguousarray(d, np.float32)
+        return lib().oracle_intersect_prim(self._h, int(prim), _fp(o), _fp(dd), t_min, t_max)
 
     def trace_any(self, origins, dirs, t_min, t_max):
         o = np.ascontiguousarray(origins, np.float32)

@@ -1144,6 +1144,14 @@ void oracle_trace_closest(const oracle_scene* s, const float* origins, const flo
    });
 }
 
+double oracle_intersect_prim(const oracle_scene* s, int64_t prim, const float* o, const float* d, double t_min,
+                             double t_max) {
+   if (prim < 0 || prim >= (int64_t)s->prims.size()) return -1;
+   Ray r{fv(o), fv(d)};
+   Hit h;
+   return isect_prim(*s, s->prims[prim], r, t_min, t_max, h) ? h.t : -1.0;
+}
+
 void oracle_trace_any(const oracle_scene* s, const float* origins, const float* dirs, int64_t n, double t_min,
                       double t_max, int32_t* out_hit) {
    parallel_for(n, 0, [&](int64_t i) {

@@ -31,6 +31,10 @@ int32_t oracle_scene_num_lights(const oracle_scene* s);
 void oracle_trace_closest(const oracle_scene* s, const float* origins, const float* dirs, int64_t n,
                           double t_min, double t_max, int32_t brute, int64_t* out_prim, double* out_t);
 
+/* FP64 hit distance of one primitive (input order index) along a ray, -1 if missed. */
+double oracle_intersect_prim(const oracle_scene* s, int64_t prim, const float* origin, const float* dir,
+                             double t_min, double t_max);
+
 /* 1 if any primitive other than an emitter is hit in [t_min, t_max]. */
 void oracle_trace_any(const oracle_scene* s, const float* origins, const float* dirs, int64_t n, double t_min,
                       double t_max, int32_t* out_hit);

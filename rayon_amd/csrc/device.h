@@ -137,18 +137,56 @@ __device__ __forceinline__ bool isect_tri(const PrimRec& p, V3 o, V3 d, float tm
    return t >= tmin && t <= tmax;
 }
 
-// hit_sphere root selection (cuda_raytracer.cuh:203-222).
+// |oc|^2 - r^2 with error-free FMA products and compensated sums (Ogita-Rump-Oishi
+// Dot2): for a large sphere seen from near its surface (the r=1000 ground) the direct
+// FP32 form cancels ~16 of 24 bits.
+__device__ __forceinline__ float sphere_c(V3 f, float r) {
+   float p = f.x * f.x;
+   float s = fmaf(f.x, f.x, -p);
+   float terms[3] = {f.y, f.z, r};
+#pragma unroll
+   for (int i = 0; i < 3; ++i) {
+      const float x = terms[i];
+      const float y = i < 2 ? x : -x;
+      const float h = x * y;
+      const float e = fmaf(x, y, -h);
+      const float sum = p + h;
+      const float bb = sum - p;
+      const float q = (p - (sum - bb)) + (h - bb);
+      p = sum;
+      s += q + e;
+   }
+   return p + s;
+}
+
+// hit_sphere's root selection (cuda_raytracer.cuh:203-222): the nearer root if it lies in
+// [tmin, tmax], else the farther one. Same roots as the reference's expressions in exact
+// arithmetic, formed for FP32 accuracy: c = |oc|^2 - r^2 compensated; the discriminant
+// as b^2 - a c with exact products when the origin is within ~1.4 r of the centre (large
+// spheres seen from near their surface), else from the perpendicular distance
+// a r^2 - |d x oc|^2 (small or distant spheres); roots q/a and c/q with
+// q = -b - sign(b) sqrt(disc), which never subtracts nearly equal numbers.
 __device__ __forceinline__ bool isect_sphere(V3 c, float rad, V3 o, V3 d, float tmin, float tmax, float& t) {
-   V3 oc = o - c;
-   float a = dot(d, d);
-   float hb = dot(oc, d);
-   float cc = dot(oc, oc) - rad * rad;
-   float disc = hb * hb - a * cc;
+   const V3 oc = o - c;
+   const float a = dot(d, d);
+   const float hb = dot(oc, d);
+   const float cc = sphere_c(oc, rad);
+   float disc;
+   if (cc < 0.5f * dot(oc, oc)) {
+      const float h2 = hb * hb, ac = a * cc;
+      disc = (h2 - ac) + (fmaf(hb, hb, -h2) - fmaf(a, cc, -ac));
+   } else {
+      const V3 x = cross(d, oc);
+      disc = a * (rad * rad) - dot(x, x);
+   }
    if (disc < 0.0f) return false;
-   float sq = sqrtf(disc);
-   float root = (-hb - sq) / a;
+   const float q = -hb - copysignf(sqrtf(disc), hb);
+   float r0 = q / a, r1 = cc / q;
+   if (q == 0.0f) r0 = r1 = 0.0f;
+   const float tn = fminf(r0, r1), tf = fmaxf(r0, r1);
+   float root = tn;
    if (root < tmin || tmax < root) {
-      root = (-hb + sq) / a;
+      root = tf;
       if (root < tmin || tmax < root) return false;
    }
    t = root;

@@ -3,8 +3,9 @@ seeded inputs.
 
 Tolerances (stated here, checked below):
   * primary-hit primitive IDs: bit-exact on the identical FP32 rays, except near-ties;
-    mismatches are at most 1e-5 of the rays and every mismatch must be a near-tie (the
-    oracle's own answer changes under a 1e-5 relative perturbation of the ray);
+    mismatches are at most 1e-5 of the rays and every mismatch must be a near-tie: in
+    FP64 both primitives are hit within a relative distance gap of 1e-4, or the oracle's
+    own answer changes under a 1e-5 relative perturbation of the ray (grazing hits);
   * hit distance of agreeing hits: relative 1e-4 (FP32 vs FP64 arithmetic);
   * images: relative RMSE sqrt(sum|I-O|^2 / sum|O|^2) over linear RGB <= 1% with the same
     random-number streams on both sides.
@@ -40,8 +41,23 @@ def _perturbed_disagree(osc, o, d, prim, eps=1e-5, k=6, seed=0):
     return unstable
 
 
-def _setup(ctx, kind, seed, detail, W, H, **kw):
-    g = ra.Generated.generate(kind, seed, detail=detail)
+def _near_ties(osc, o, d, gpu_prim, ora_prim):
+    """Classifies mismatches: True where the two answers are within 1e-4 relative
+    distance in FP64, or where the oracle's answer is unstable under perturbation."""
+    tie = np.zeros(len(o), bool)
+    for i in range(len(o)):
+        ta = osc.intersect_prim(gpu_prim[i], o[i], d[i]) if gpu_prim[i] >= 0 else -1.0
+        tb = osc.intersect_prim(ora_prim[i], o[i], d[i]) if ora_prim[i] >= 0 else -1.0
+        if ta > 0 and tb > 0 and abs(ta - tb) <= 1e-4 * tb:
+            tie[i] = True
+    rest = ~tie
+    if rest.any():
+        tie[rest] = _perturbed_disagree(osc, o[rest], d[rest], ora_prim[rest])
+    return tie
+
+
+def _setup(ctx, kind, scene_seed, detail, W, H, **kw):
+    g = ra.Generated.generate(kind, scene_seed, detail=detail)
     sc = ctx.scene(g.desc_ptr())
     lf, la, up, fov = g.camera()
     cam = ra.camera_frame(lf, la, up, fov, W, H)
@@ -65,8 +81,7 @@ def test_primary_hit_ids_bit_exact(ctx, kind, seed, detail, W, H):
     log(f"  rays={len(o)} hits={int((op >= 0).sum())} mismatches={len(mism)}")
     assert len(mism) <= max(1e-5 * len(o), 1)
     if len(mism):
-        unstable = _perturbed_disagree(osc, o[mism], d[mism], op[mism])
-        assert unstable.all(), "a primary-hit mismatch is not a near-tie"
+        assert _near_ties(osc, o[mism], d[mism], gp[mism], op[mism]).all(), "a mismatch is not a near-tie"
     ok = (gp == op) & (op >= 0)
     np.testing.assert_allclose(gt[ok], ot[ok], rtol=1e-4)
     job.close()
@@ -90,7 +105,7 @@ def test_random_rays_closest_hit_mesh(ctx):
     log(f"  mismatches={len(mism)} / {n}")
     assert len(mism) <= max(1e-5 * n, 2)
     if len(mism):
-        assert _perturbed_disagree(osc, o[mism], d[mism], op[mism]).all()
+        assert _near_ties(osc, o[mism], d[mism], gp[mism], op[mism]).all()
     sc.close()
 
 
