@@ -266,6 +266,32 @@ rayon_status rayon_job_render_plain(rayon_ctx* ctx, rayon_job* job, int32_t samp
  * accumulate stages over shared device queues. Asynchronous on the context stream. */
 rayon_status rayon_render_batched(rayon_ctx* ctx, rayon_job* const* jobs, int32_t n, int32_t samples);
 
+/* Context options for the batched path. */
+enum {
+   RAYON_OPT_WAVE_CAPACITY = 0,   /* paths in flight per round (default 4M)                   */
+   RAYON_OPT_WAVE_WG_PER_CU = 1,  /* persistent workgroups per CU of the queue stages (4)     */
+   RAYON_OPT_GRAPH = 2,           /* capture each pass into a hipGraph and replay it (1)      */
+   RAYON_OPT_STAGE_TIMING = 3,    /* time every stage launch with HIP events (0; disables graphs) */
+   RAYON_OPT_COUNT_TRAVERSAL = 4  /* count node/primitive fetches of the queue stages (0)     */
+};
+rayon_status rayon_ctx_set_option(rayon_ctx* ctx, int32_t option, int64_t value);
+rayon_status rayon_ctx_get_option(rayon_ctx* ctx, int32_t option, int64_t* value);
+
+/* Accumulated statistics of the batched path since the last reset. Stages: 0 generate,
+ * 1 extend, 2 shade, 3 shadow, 4 accumulate, 5 bookkeeping. */
+typedef struct rayon_wave_stats {
+   double stage_ms[6];            /* summed HIP-event time per stage (RAYON_OPT_STAGE_TIMING) */
+   int64_t launches[6];
+   int64_t rounds;
+   int64_t extend_node_fetches;   /* RAYON_OPT_COUNT_TRAVERSAL                               */
+   int64_t extend_prim_fetches;
+   int64_t shadow_node_fetches;
+   int64_t shadow_prim_fetches;
+   int64_t material_fetches;
+   int64_t light_fetches;
+} rayon_wave_stats;
+rayon_status rayon_ctx_wave_stats(rayon_ctx* ctx, rayon_wave_stats* out, int32_t reset);
+
 /* Copies the per-pixel mean linear radiance (3 floats per pixel) to host memory. */
 rayon_status rayon_job_resolve_linear(rayon_ctx* ctx, rayon_job* job, float* rgb_out);
 /* Gamma-corrected 8-bit image (gammaCorrectKernel, render_acc_kernel.cu:11-44). */

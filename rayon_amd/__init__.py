@@ -12,8 +12,8 @@ from pathlib import Path
 import numpy as np
 
 from .abi import (  # noqa: F401  (re-exported)
-    Camera, GenParams, Geometry, JobParams, JobStats, Material, Mesh, SceneDesc, SceneInfo, SdfPrim,
-    MAT, GEOM, SDF, FLAG, GEN, STATUS, declare,
+    Camera, GenParams, Geometry, JobParams, JobStats, Material, Mesh, SceneDesc, SceneInfo, SdfPrim, WaveStats,
+    MAT, GEOM, SDF, FLAG, GEN, OPT, STATUS, declare,
 )
 
 LIB_PATH = Path(__file__).resolve().parent / "librayon_hip.so"
@@ -145,6 +145,19 @@ class Context:
     def render_batched(self, jobs, samples: int):
         arr = (C.c_void_p * len(jobs))(*[j._h for j in jobs])
         _check(lib().rayon_render_batched(self._h, arr, len(jobs), samples), "rayon_render_batched")
+
+    def set_option(self, opt: int, value: int):
+        _check(lib().rayon_ctx_set_option(self._h, opt, value), "rayon_ctx_set_option")
+
+    def get_option(self, opt: int) -> int:
+        v = C.c_int64()
+        _check(lib().rayon_ctx_get_option(self._h, opt, C.byref(v)), "rayon_ctx_get_option")
+        return v.value
+
+    def wave_stats(self, reset: bool = False) -> WaveStats:
+        s = WaveStats()
+        _check(lib().rayon_ctx_wave_stats(self._h, C.byref(s), int(reset)), "rayon_ctx_wave_stats")
+        return s
 
     def close(self):
         if self._h:

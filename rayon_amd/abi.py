@@ -22,6 +22,7 @@ GEOM = _NS(SPHERE=0, RECTANGLE=1, DISPLACED_SPHERE=3, TRIANGLE=4)
 SDF = _NS(SPHERE=0, ROUNDED_BOX=1, TORUS=2)
 FLAG = _NS(REF_QUIRKS=1, REF_RNG=2, NEE=4, ADAPTIVE_REF=8, ADAPTIVE_STDERR=16, DOF=32, COUNT_TRAVERSAL=64)
 GEN = _NS(BOOK_SPHERES=0, CORNELL_ICOSPHERE=1, SDF_TERRAIN=2)
+OPT = _NS(WAVE_CAPACITY=0, WAVE_WG_PER_CU=1, GRAPH=2, STAGE_TIMING=3, COUNT_TRAVERSAL=4)
 
 
 class Material(C.Structure):
@@ -84,6 +85,13 @@ class JobStats(C.Structure):
                 ("material_fetches", C.c_int64), ("light_fetches", C.c_int64)]
 
 
+class WaveStats(C.Structure):
+    _fields_ = [("stage_ms", C.c_double * 6), ("launches", C.c_int64 * 6), ("rounds", C.c_int64),
+                ("extend_node_fetches", C.c_int64), ("extend_prim_fetches", C.c_int64),
+                ("shadow_node_fetches", C.c_int64), ("shadow_prim_fetches", C.c_int64),
+                ("material_fetches", C.c_int64), ("light_fetches", C.c_int64)]
+
+
 class GenParams(C.Structure):
     _fields_ = [("kind", C.c_int32), ("seed", C.c_uint64), ("detail", C.c_int32), ("count", C.c_int32)]
 
@@ -112,6 +120,9 @@ def declare(lib: C.CDLL) -> C.CDLL:
         "rayon_job_reset": (st, [vp]),
         "rayon_job_render_plain": (st, [vp, vp, i32]),
         "rayon_render_batched": (st, [vp, P(vp), i32, i32]),
+        "rayon_ctx_set_option": (st, [vp, i32, i64]),
+        "rayon_ctx_get_option": (st, [vp, i32, P(i64)]),
+        "rayon_ctx_wave_stats": (st, [vp, P(WaveStats), i32]),
         "rayon_job_resolve_linear": (st, [vp, vp, P(f32)]),
         "rayon_job_resolve_u8": (st, [vp, vp, P(C.c_uint8), i32, f32]),
         "rayon_job_sample_counts": (st, [vp, vp, P(i32)]),

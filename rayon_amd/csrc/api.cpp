@@ -72,6 +72,36 @@ static float bits_as_float(uint32_t u) {
 
 using namespace ry;
 
+void ry::fill_jobview(rayon_job* j) {
+   JobView& J = j->view;
+   const rayon_job_params& P = j->params;
+   J.scene = j->scene->view;
+   for (int k = 0; k < 3; ++k) {
+      J.cam_c[k] = P.camera_center[k];
+      J.p00[k] = P.pixel00[k];
+      J.du[k] = P.delta_u[k];
+      J.dv[k] = P.delta_v[k];
+      J.cu[k] = P.cam_u[k];
+      J.cv[k] = P.cam_v[k];
+   }
+   J.width = P.width;
+   J.height = P.height;
+   J.max_depth = P.max_depth;
+   J.flags = P.flags;
+   J.seed_lo = (uint32_t)P.seed;
+   J.seed_hi = (uint32_t)(P.seed >> 32);
+   J.light_intensity = P.light_intensity;
+   J.background_intensity = P.background_intensity;
+   J.metal_fuzziness = P.metal_fuzziness;
+   J.glass_ior = P.glass_ior;
+   J.dof_aperture = P.dof_aperture;
+   J.dof_focus = P.dof_focus_distance;
+   J.min_adaptive = P.min_adaptive_samples;
+   J.adaptive_threshold = P.adaptive_threshold;
+   J.spp_target = P.spp;
+   J.sample_base = (int)j->samples_done;
+}
+
 extern "C" {
 
 const char* rayon_last_error(void) { return ry::g_err; }
@@ -351,35 +381,6 @@ rayon_status rayon_scene_destroy(rayon_scene* s) {
    return RAYON_OK;
 }
 
-static void fill_jobview(rayon_job* j) {
-   JobView& J = j->view;
-   const rayon_job_params& P = j->params;
-   J.scene = j->scene->view;
-   for (int k = 0; k < 3; ++k) {
-      J.cam_c[k] = P.camera_center[k];
-      J.p00[k] = P.pixel00[k];
-      J.du[k] = P.delta_u[k];
-      J.dv[k] = P.delta_v[k];
-      J.cu[k] = P.cam_u[k];
-      J.cv[k] = P.cam_v[k];
-   }
-   J.width = P.width;
-   J.height = P.height;
-   J.max_depth = P.max_depth;
-   J.flags = P.flags;
-   J.seed_lo = (uint32_t)P.seed;
-   J.seed_hi = (uint32_t)(P.seed >> 32);
-   J.light_intensity = P.light_intensity;
-   J.background_intensity = P.background_intensity;
-   J.metal_fuzziness = P.metal_fuzziness;
-   J.glass_ior = P.glass_ior;
-   J.dof_aperture = P.dof_aperture;
-   J.dof_focus = P.dof_focus_distance;
-   J.min_adaptive = P.min_adaptive_samples;
-   J.adaptive_threshold = P.adaptive_threshold;
-   J.spp_target = P.spp;
-   J.sample_base = (int)j->samples_done;
-}
 
 rayon_status rayon_job_reset(rayon_job* j) {
    if (!j) {
@@ -452,6 +453,10 @@ rayon_status rayon_job_destroy(rayon_job* j) {
    if (!j) return RAYON_OK;
    (void)hipSetDevice(j->ctx->device);
    (void)hipStreamSynchronize(j->ctx->stream);
+   {
+      std::lock_guard<std::mutex> lk(j->ctx->mu);
+      wave_forget_job(j->ctx, j);
+   }
    free_all(j->bufs);
    delete j;
    return RAYON_OK;

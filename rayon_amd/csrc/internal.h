@@ -53,6 +53,8 @@ bool build_bvh(const std::vector<HostPrim>& prims, BuildResult& out);
 rayon_status gather_prims(const rayon_scene_desc* d, std::vector<HostPrim>& prims);
 MatRec pack_material(const rayon_material& m);
 
+struct Wave;  // wavefront.hip
+
 }  // namespace ry
 
 struct rayon_ctx {
@@ -60,7 +62,7 @@ struct rayon_ctx {
    hipStream_t stream = nullptr;
    std::mutex mu;
    // wavefront state (shared by all batched launches of this context)
-   struct Wave* wave = nullptr;
+   ry::Wave* wave = nullptr;
 };
 
 struct rayon_scene {
@@ -78,5 +80,9 @@ struct rayon_job {
    ry::JobView view{};
    std::vector<ry::DevBuf> bufs;
    int64_t samples_done = 0;
-   int job_slot = -1;  // index in the batched job table
+   float4* prev = nullptr;  // sums at pass start (batched passes with the reference adaptive test)
 };
+
+namespace ry {
+void fill_jobview(rayon_job* j);
+}

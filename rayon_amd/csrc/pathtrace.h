@@ -14,10 +14,30 @@ struct PathCounts {
    unsigned light_prims = 0;
 };
 
-// Emitter sample + shadow ray for a Lambertian hit; returns the contribution before the
-// path throughput is applied.
-template <int STRIDE, bool CNT = false>
-__device__ inline V3 nee_lambert(const JobView& J, const Surf& s, V3 color, Rng& R, int* stack, PathCounts& pc) {
+// State of one path between bounces.
+struct PathState {
+   V3 o, d, T, L;
+   bool prev_lambert;
+};
+
+// A shadow ray and the radiance it carries if the segment is unoccluded.
+struct ShadowReq {
+   V3 o, d, contrib;
+   bool valid;
+};
+
+// Products rounded separately, so the contribution is the same value whether it is
+// added at once (plain path) or by a later shadow stage (wavefront).
+__device__ __forceinline__ V3 mul_rn(V3 a, V3 b) {
+   return mk(__fmul_rn(a.x, b.x), __fmul_rn(a.y, b.y), __fmul_rn(a.z, b.z));
+}
+
+// Emitter sample for a Lambertian hit (spec addition): picks an emitter uniformly, a
+// point uniformly on it, and forms the shadow ray and its unoccluded contribution
+// T * f_r * Le * cos_s * cos_l / d^2 / pdf.
+template <bool CNT>
+__device__ inline void nee_request(const JobView& J, const Surf& s, V3 color, V3 T, Rng& R, ShadowReq& sh,
+                                   PathCounts& pc) {
    const SceneView& S = J.scene;
    float ul = R.next(), ua = R.next(), ub = R.next();
    int li = min((int)(ul * (float)S.num_lights), S.num_lights - 1);
@@ -27,63 +47,76 @@ __device__ inline V3 nee_lambert(const JobView& J, const Surf& s, V3 color, Rng&
    V3 q, nl;
    bool one_sided;
    sample_emitter(lp, ua, ub, q, nl, one_sided);
-   V3 org = s.p + 1e-4f * s.n;
-   V3 wd = q - org;
-   float d2 = len2(wd);
-   V3 wi = (1.0f / sqrtf(d2)) * wd;
+   sh.o = s.p + 1e-4f * s.n;
+   sh.d = q - sh.o;
+   float d2 = len2(sh.d);
+   V3 wi = (1.0f / sqrtf(d2)) * sh.d;
    float cs = dot(s.n, wi);
    float cl = dot(nl, -wi);
    if (!one_sided) cl = fabsf(cl);
-   V3 out = mk(0.0f, 0.0f, 0.0f);
-   if (cs > 0.0f && cl > 0.0f) {
+   sh.valid = cs > 0.0f && cl > 0.0f;
+   if (sh.valid) {
       ++pc.shadow;
-      if (!any_hit<STRIDE, CNT>(S, org, wd, 1e-4f, 1.0f - 1e-4f, stack, &pc.trav)) {
-         const MatRec LM = S.mats[info_mat(__float_as_uint(lp.a.w))];
-         if constexpr (CNT) ++pc.mats;
-         float g = cs * cl / d2 * S.light_area[li] * (float)S.num_lights * (1.0f / kPiF);
-         out = (g * J.light_intensity) * (color * mk(LM.m1));
-      }
+      const MatRec LM = S.mats[info_mat(__float_as_uint(lp.a.w))];
+      if constexpr (CNT) ++pc.mats;
+      float g = cs * cl / d2 * S.light_area[li] * (float)S.num_lights * (1.0f / kPiF);
+      sh.contrib = mul_rn(T, (g * J.light_intensity) * (color * mk(LM.m1)));
    }
-   return out;
+}
+
+// One bounce of ray_color (cuda_raytracer.cuh:869-931) given the closest hit of ps's ray:
+// sky on a miss, invisible pass-through at bounce 0, emission, scattering, an optional
+// shadow request (NEE on Lambertian surfaces) and Russian roulette from bounce 1.
+// Returns true while the path continues with ps.o / ps.d. Shared by the megakernel and
+// the wavefront shade stage so both compute identical per-sample radiance.
+template <bool CNT>
+__device__ inline bool shade_step(const JobView& J, const HitMin& h, int bounce, PathState& ps, Rng& R,
+                                  ShadowReq& sh, PathCounts& pc) {
+   const SceneView& S = J.scene;
+   const bool nee = (J.flags & RAYON_FLAG_NEE) && S.num_lights > 0;
+   sh.valid = false;
+   if (h.slot < 0) {
+      ps.L = ps.L + ps.T * sky(J, ps.d);
+      return false;
+   }
+   const Surf s = surface(S, h, ps.o, ps.d);
+   if (!(s.info & F_VISIBLE) && bounce == 0) {  // cuda_raytracer.cuh:879-883
+      ps.o = s.p + 0.01f * ps.d;
+      return true;
+   }
+   const MatRec M = S.mats[info_mat(s.info)];
+   if constexpr (CNT) ++pc.mats;
+   const V3 color = surface_color(M, s.p, s.center);
+   Scatter sc;
+   scatter(M, color, J, s, ps.d, R, sc);
+   if (len2(sc.emitted) > 0.0f && !(nee && ps.prev_lambert)) ps.L = ps.L + ps.T * sc.emitted;
+   if (!sc.scattered) return false;
+   if (nee && sc.lambert) nee_request<CNT>(J, s, color, ps.T, R, sh, pc);
+   ps.T = ps.T * sc.atten;
+   ps.prev_lambert = sc.lambert;
+   ps.o = sc.o;
+   ps.d = sc.d;
+   if (bounce > 0) {  // Russian roulette, cuda_raytracer.cuh:906-920
+      float q = fminf(maxc(ps.T), 0.95f);
+      if (R.next() > q) return false;
+      ps.T = (1.0f / q) * ps.T;
+   }
+   return true;
 }
 
 template <int STRIDE, bool CNT = false>
 __device__ inline V3 trace_path(const JobView& J, V3 o, V3 d, Rng& R, int* stack, PathCounts& pc) {
-   const SceneView& S = J.scene;
-   const bool nee = (J.flags & RAYON_FLAG_NEE) && S.num_lights > 0;
-   V3 L = mk(0.0f, 0.0f, 0.0f), T = mk(1.0f, 1.0f, 1.0f);
-   bool prev_lambert = false;
+   PathState ps{o, d, mk(1.0f, 1.0f, 1.0f), mk(0.0f, 0.0f, 0.0f), false};
+   ShadowReq sh;
    for (int bounce = 0; bounce < J.max_depth; ++bounce) {
       ++pc.segments;
-      const HitMin h = closest_hit<STRIDE, CNT>(S, o, d, 0.001f, 3.402823466e38f, stack, &pc.trav);
-      if (h.slot < 0) {
-         L = L + T * sky(J, d);
-         break;
-      }
-      const Surf s = surface(S, h, o, d);
-      if (!(s.info & F_VISIBLE) && bounce == 0) {  // cuda_raytracer.cuh:879-883
-         o = s.p + 0.01f * d;
-         continue;
-      }
-      const MatRec M = S.mats[info_mat(s.info)];
-      if constexpr (CNT) ++pc.mats;
-      const V3 color = surface_color(M, s.p, s.center);
-      Scatter sc;
-      scatter(M, color, J, s, d, R, sc);
-      if (len2(sc.emitted) > 0.0f && !(nee && prev_lambert)) L = L + T * sc.emitted;
-      if (!sc.scattered) break;
-      if (nee && sc.lambert) L = L + T * nee_lambert<STRIDE, CNT>(J, s, color, R, stack, pc);
-      T = T * sc.atten;
-      prev_lambert = sc.lambert;
-      o = sc.o;
-      d = sc.d;
-      if (bounce > 0) {  // Russian roulette, cuda_raytracer.cuh:906-920
-         float q = fminf(maxc(T), 0.95f);
-         if (R.next() > q) break;
-         T = (1.0f / q) * T;
-      }
+      const HitMin h = closest_hit<STRIDE, CNT>(J.scene, ps.o, ps.d, 0.001f, 3.402823466e38f, stack, &pc.trav);
+      const bool cont = shade_step<CNT>(J, h, bounce, ps, R, sh, pc);
+      if (sh.valid && !any_hit<STRIDE, CNT>(J.scene, sh.o, sh.d, 1e-4f, 1.0f - 1e-4f, stack, &pc.trav))
+         ps.L = ps.L + sh.contrib;
+      if (!cont) break;
    }
-   return L;
+   return ps.L;
 }
 
 // Adds one sample's radiance to a pixel's compensated sums.

@@ -7,5 +7,7 @@
 namespace ry {
 
 void wave_destroy(rayon_ctx* ctx);
+// Drops cached pass graphs that were captured for `job` (called before the job is freed).
+void wave_forget_job(rayon_ctx* ctx, rayon_job* job);
 
 }  // namespace ry

@@ -1,27 +1,789 @@
-// wavefront.hip — batched wavefront renderer. (Stage kernels follow.)
+// wavefront.hip — batched wavefront renderer: many jobs advance together through
+// generate -> (extend -> shade -> shadow) x depth -> accumulate over shared device queues.
+//
+// This is the MI355X replacement for running renderAccKernel once per job
+// (shaders/render_acc_kernel.cu:128-282): the per-thread megakernel loop is split into
+// stages so that each stage keeps few registers (traversal needs the LDS stack, shading
+// does not), waves stay full as paths terminate (queues are compacted with one
+// wave-aggregated atomic per 64 lanes), and jobs of different sizes share every launch.
+//
+// Work of one pass = for every job, `samples` samples of every pixel, enumerated
+// job-major, then sample, then pixel. It is processed in rounds of at most `capacity`
+// paths; path slot i of a round holds work item w0 + i. Queues carry slot indices. The
+// extend/shade/shadow kernels are persistent: a fixed grid of workgroups whose waves pull
+// 64 queue entries at a time until the queue (whose length the previous stage left in
+// device memory) is drained, so a whole pass needs no host synchronisation and can be
+// captured once into a hipGraph and replayed.
+//
+// Determinism: a path's random numbers depend only on (job seed, pixel, sample); its
+// radiance is accumulated into the pixel by the accumulate stage in sample order. A job
+// therefore produces the same image alone or batched with any other jobs.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <vector>
+
 #include "internal.h"
+#include "kernels.h"
+#include "pathtrace.h"
 #include "wavefront.h"
 
 namespace ry {
 
-void wave_destroy(rayon_ctx*) {}
+constexpr int kWfBlock = 256;
+constexpr uint32_t kSkip = 1u;        // ids.w bit: pixel converged, no path this sample
+constexpr uint32_t kPrevLambert = 2u; // ids.w bit: the last scatter was Lambertian
+// ids.w bits 8..19: extension segments, bits 20..31: shadow segments
+
+struct WfView {
+   float4* o;     // origin, rng state (bits)
+   float4* d;     // direction
+   float4* thr;   // throughput
+   float4* rad;   // radiance
+   uint4* ids;    // job, pixel, sample, flags
+   float4* hit;   // t, primitive slot (bits), u, v
+   float4* so;    // shadow queue: origin, path slot (bits)
+   float4* sd;    // shadow queue: direction
+   float4* sc;    // shadow queue: contribution
+   int* q0;
+   int* q1;
+   unsigned* ctl;           // per level: extend count, extend head, shadow count, shadow head
+   const JobView* jobs;
+   const long long* base;   // work prefix per job (njobs + 1 entries)
+   unsigned long long* stats;  // traversal statistics (optional)
+   int njobs;
+   int levels;
+};
+
+struct Wave {
+   int capacity = 0;
+   int levels = 0;
+   int max_jobs = 0;
+   std::vector<DevBuf> bufs;
+   WfView v{};
+   JobView* jobs_dev = nullptr;
+   long long* base_dev = nullptr;
+   unsigned long long* stats_dev = nullptr;
+   // options
+   int opt_capacity = 1 << 22;
+   int opt_wg_per_cu = 4;
+   int opt_graph = 1;
+   int opt_timing = 0;
+   int opt_count = 0;
+   int cus = 0;
+   // timing
+   double stage_ms[6] = {0, 0, 0, 0, 0, 0};
+   long long launches[6] = {0, 0, 0, 0, 0, 0};
+   long long rounds = 0;
+   // graph cache
+   struct GraphEntry {
+      hipGraphExec_t exec = nullptr;
+      hipGraph_t graph = nullptr;
+   };
+   std::map<std::vector<long long>, GraphEntry> graphs;
+   std::vector<JobView> host_jobs;
+   std::vector<long long> host_base;
+};
+
+// ---------------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+
+// Position of this lane's entry in a queue, or -1; one atomic per wave.
+__device__ __forceinline__ int wave_push(unsigned* counter, bool pred) {
+   const unsigned long long m = __ballot(pred);
+   if (m == 0) return -1;
+   const int leader = __ffsll((long long)m) - 1;
+   unsigned base = 0;
+   if (lane_id() == leader) base = atomicAdd(counter, (unsigned)__popcll(m));
+   base = __shfl(base, leader);
+   const unsigned long long below = m & ((1ull << lane_id()) - 1ull);
+   return pred ? (int)(base + (unsigned)__popcll(below)) : -1;
+}
+
+__device__ __forceinline__ int find_job(const WfView& V, long long w) {
+   int lo = 0, hi = V.njobs - 1;
+   while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (V.base[mid] <= w)
+         lo = mid;
+      else
+         hi = mid - 1;
+   }
+   return lo;
+}
+
+// ---------------------------------------------------------------------------------
+// stages
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(kWfBlock) void k_wf_generate(WfView V, long long w0, int n) {
+   const int i = blockIdx.x * kWfBlock + threadIdx.x;
+   bool live = false;
+   if (i < n) {
+      const long long w = w0 + i;
+      const int j = find_job(V, w);
+      const JobView& J = V.jobs[j];
+      const long long npix = (long long)J.width * J.height;
+      const long long l = w - V.base[j];
+      const int s = (int)(l / npix);
+      const int p = (int)(l - (long long)s * npix);
+      const int sample = J.sample_base + s;
+      uint32_t flags = 0;
+      if (J.counts[p] < 0) {
+         flags = kSkip;
+      } else {
+         Rng R{counter_init(J.seed_lo, J.seed_hi, (uint32_t)p, (uint32_t)sample), false};
+         V3 o, d;
+         camera_ray(J, p % J.width, p / J.width, R, o, d);
+         V.o[i] = make_float4(o.x, o.y, o.z, __uint_as_float(R.s));
+         V.d[i] = make_float4(d.x, d.y, d.z, 0.0f);
+         V.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+         V.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+         live = true;
+      }
+      V.ids[i] = make_uint4((uint32_t)j, (uint32_t)p, (uint32_t)sample, flags);
+   }
+   const int pos = wave_push(&V.ctl[0], live);
+   if (live) V.q0[pos] = i;
+}
+
+template <bool CNT>
+__global__ __launch_bounds__(kWfBlock) void k_wf_extend(WfView V, int level) {
+   __shared__ int stack[kStack * kWfBlock];
+   const unsigned* ctl = V.ctl + 4 * level;
+   const unsigned count = ctl[0];
+   const int* q = (level & 1) ? V.q1 : V.q0;
+   TravCount tc;
+   for (;;) {
+      unsigned base = 0;
+      if (lane_id() == 0) base = atomicAdd((unsigned*)&ctl[1], 64u);
+      base = __shfl(base, 0);
+      if (base >= count) break;
+      const unsigned idx = base + (unsigned)lane_id();
+      if (idx < count) {
+         const int slot = q[idx];
+         const float4 o = V.o[slot], d = V.d[slot];
+         const SceneView& S = V.jobs[V.ids[slot].x].scene;
+         const HitMin h = closest_hit<kWfBlock, CNT>(S, mk(o), mk(d), 0.001f, 3.402823466e38f,
+                                                    stack + threadIdx.x, &tc);
+         V.hit[slot] = make_float4(h.t, __int_as_float(h.slot), h.u, h.v);
+      }
+   }
+   if constexpr (CNT) {
+      unsigned long long a = wave_sum((unsigned long long)tc.nodes), b = wave_sum((unsigned long long)tc.prims);
+      if (lane_id() == 0) {
+         atomicAdd(&V.stats[0], a);
+         atomicAdd(&V.stats[1], b);
+      }
+   }
+}
+
+template <bool CNT>
+__global__ __launch_bounds__(kWfBlock) void k_wf_shade(WfView V, int level) {
+   const unsigned* ctl = V.ctl + 4 * level;
+   const unsigned count = ctl[0];
+   const int* q = (level & 1) ? V.q1 : V.q0;
+   int* qn = (level & 1) ? V.q0 : V.q1;
+   unsigned* ctl_next = V.ctl + 4 * (level + 1);
+   unsigned* sctl = V.ctl + 4 * level + 2;
+   PathCounts pc;
+   for (;;) {
+      unsigned base = 0;
+      if (lane_id() == 0) base = atomicAdd((unsigned*)&ctl[1], 64u);
+      base = __shfl(base, 0);
+      if (base >= count) break;
+      const unsigned idx = base + (unsigned)lane_id();
+      bool cont = false, shadow = false;
+      int slot = -1;
+      ShadowReq sh;
+      sh.valid = false;
+      if (idx < count) {
+         slot = q[idx];
+         uint4 id = V.ids[slot];
+         const JobView& J = V.jobs[id.x];
+         const float4 o = V.o[slot], d = V.d[slot], t = V.thr[slot], r = V.rad[slot], hf = V.hit[slot];
+         PathState ps{mk(o), mk(d), mk(t), mk(r), (id.w & kPrevLambert) != 0};
+         Rng R{__float_as_uint(o.w), false};
+         HitMin h;
+         h.t = hf.x;
+         h.slot = __float_as_int(hf.y);
+         h.u = hf.z;
+         h.v = hf.w;
+         cont = shade_step<CNT>(J, h, level, ps, R, sh, pc);
+         cont = cont && level + 1 < J.max_depth;
+         shadow = sh.valid;
+         id.w = (id.w & ~kPrevLambert) | (ps.prev_lambert ? kPrevLambert : 0u);
+         id.w += 1u << 8;
+         if (shadow) id.w += 1u << 20;
+         V.ids[slot] = id;
+         V.o[slot] = make_float4(ps.o.x, ps.o.y, ps.o.z, __uint_as_float(R.s));
+         V.d[slot] = make_float4(ps.d.x, ps.d.y, ps.d.z, 0.0f);
+         V.thr[slot] = make_float4(ps.T.x, ps.T.y, ps.T.z, 0.0f);
+         V.rad[slot] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
+      }
+      const int pn = wave_push(&ctl_next[0], cont);
+      if (cont) qn[pn] = slot;
+      const int ps_pos = wave_push(sctl, shadow);
+      if (shadow) {
+         V.so[ps_pos] = make_float4(sh.o.x, sh.o.y, sh.o.z, __int_as_float(slot));
+         V.sd[ps_pos] = make_float4(sh.d.x, sh.d.y, sh.d.z, 0.0f);
+         V.sc[ps_pos] = make_float4(sh.contrib.x, sh.contrib.y, sh.contrib.z, 0.0f);
+      }
+   }
+   if constexpr (CNT) {
+      unsigned long long a = wave_sum((unsigned long long)pc.mats), b = wave_sum((unsigned long long)pc.light_prims);
+      if (lane_id() == 0) {
+         atomicAdd(&V.stats[4], a);
+         atomicAdd(&V.stats[5], b);
+      }
+   }
+}
+
+template <bool CNT>
+__global__ __launch_bounds__(kWfBlock) void k_wf_shadow(WfView V, int level) {
+   __shared__ int stack[kStack * kWfBlock];
+   unsigned* sctl = V.ctl + 4 * level + 2;
+   const unsigned count = sctl[0];
+   TravCount tc;
+   for (;;) {
+      unsigned base = 0;
+      if (lane_id() == 0) base = atomicAdd(&sctl[1], 64u);
+      base = __shfl(base, 0);
+      if (base >= count) break;
+      const unsigned idx = base + (unsigned)lane_id();
+      if (idx < count) {
+         const float4 o = V.so[idx], d = V.sd[idx];
+         const int slot = __float_as_int(o.w);
+         const SceneView& S = V.jobs[V.ids[slot].x].scene;
+         if (!any_hit<kWfBlock, CNT>(S, mk(o), mk(d), 1e-4f, 1.0f - 1e-4f, stack + threadIdx.x, &tc)) {
+            const float4 c = V.sc[idx];
+            float4 r = V.rad[slot];
+            const V3 L = mk(r) + mk(c);
+            V.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+         }
+      }
+   }
+   if constexpr (CNT) {
+      unsigned long long a = wave_sum((unsigned long long)tc.nodes), b = wave_sum((unsigned long long)tc.prims);
+      if (lane_id() == 0) {
+         atomicAdd(&V.stats[2], a);
+         atomicAdd(&V.stats[3], b);
+      }
+   }
+}
+
+// One thread per pixel present in the round: adds that pixel's samples of the round in
+// sample order (Neumaier), updates its count and the job's counters.
+__global__ __launch_bounds__(kWfBlock) void k_wf_accumulate(WfView V, long long w0, int n) {
+   const int i = blockIdx.x * kWfBlock + threadIdx.x;
+   bool active = false;
+   int j = -1;
+   unsigned long long segs = 0, shadow = 0, prim = 0;
+   if (i < n) {
+      const long long w = w0 + i;
+      j = find_job(V, w);
+      const JobView& J = V.jobs[j];
+      const long long npix = (long long)J.width * J.height;
+      const long long l = w - V.base[j];
+      if (l < npix || w - npix < w0) {  // first sample of this pixel within the round
+         active = true;
+         const int p = (int)(l % npix);
+         const long long end = min(w0 + n, V.base[j + 1]);
+         float4 a = J.acc[p], c = J.comp[p];
+         int added = 0;
+         for (long long wk = w; wk < end; wk += npix) {
+            const int slot = (int)(wk - w0);
+            const uint32_t f = V.ids[slot].w;
+            if (f & kSkip) continue;
+            const float4 r = V.rad[slot];
+            accumulate(a, c, mk(r));
+            ++added;
+            segs += (f >> 8) & 0xFFFu;
+            shadow += f >> 20;
+         }
+         if (added) {
+            J.acc[p] = a;
+            J.comp[p] = c;
+            J.counts[p] += added;
+         }
+         prim = (unsigned long long)added;
+      }
+   }
+   // per-job counters: one atomic per wave when the wave's pixels belong to one job
+   const unsigned long long act = __ballot(active);
+   if (act == 0) return;
+   const int leader = __ffsll((long long)act) - 1;
+   const int jl = __shfl(j, leader);
+   const bool uniform = __ballot(active && j != jl) == 0;
+   if (uniform) {
+      const unsigned long long s0 = wave_sum(segs + shadow), s1 = wave_sum(shadow), s2 = wave_sum(prim);
+      if (lane_id() == leader) {
+         unsigned long long* ctr = V.jobs[jl].counters;
+         atomicAdd(&ctr[0], s0);
+         atomicAdd(&ctr[1], s1);
+         atomicAdd(&ctr[2], s2);
+      }
+   } else if (active) {
+      unsigned long long* ctr = V.jobs[j].counters;
+      atomicAdd(&ctr[0], segs + shadow);
+      atomicAdd(&ctr[1], shadow);
+      atomicAdd(&ctr[2], prim);
+   }
+}
+
+// Pass start for batch jobs with the reference adaptive test: keeps the sums before the
+// pass, which that test compares against (render_acc_kernel.cu:185-191).
+__global__ void k_wf_pass_begin(JobView J, float4* prev) {
+   const int p = blockIdx.x * blockDim.x + threadIdx.x;
+   if (p < J.width * J.height) prev[p] = J.acc[p];
+}
+
+// Pass end: the adaptive test on the pixels that received this pass's samples.
+__global__ void k_wf_pass_end(JobView J, const float4* prev, int todo) {
+   const int p = blockIdx.x * blockDim.x + threadIdx.x;
+   int conv = 0;
+   if (p < J.width * J.height) {
+      const int n = J.counts[p];
+      if (n > 0 && todo > 0) {
+         const float4 a = J.acc[p], c = J.comp[p];
+         const float4 a0 = prev ? prev[p] : a;
+         const int m = adaptive_count(J, a0, n - todo, a, c, n);
+         if (m < 0) {
+            J.counts[p] = m;
+            conv = 1;
+         }
+      }
+   }
+   const unsigned long long s = wave_sum((unsigned long long)conv);
+   if ((threadIdx.x & 63) == 0 && s) atomicAdd(&J.counters[3], s);
+}
+
+// ---------------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------------
+static rayon_status wave_reserve(rayon_ctx* ctx, int capacity, int levels, int njobs) {
+   Wave*& W = ctx->wave;
+   if (!W) W = new Wave();
+   if (!W->cus) {
+      int cus = 0;
+      RY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+      W->cus = cus > 0 ? cus : 256;
+   }
+   if (W->capacity >= capacity && W->levels >= levels && W->max_jobs >= njobs) return RAYON_OK;
+   RY_HIP(hipStreamSynchronize(ctx->stream));
+   for (auto& kv : W->graphs) {
+      (void)hipGraphExecDestroy(kv.second.exec);
+      (void)hipGraphDestroy(kv.second.graph);
+   }
+   W->graphs.clear();
+   for (auto& b : W->bufs)
+      if (b.p) (void)hipFree(b.p);
+   W->bufs.clear();
+   const int cap = std::max(capacity, W->capacity);
+   const int lv = std::max(levels, W->levels);
+   const int mj = std::max(njobs, W->max_jobs);
+   auto alloc = [&](size_t bytes, void** p) -> rayon_status {
+      hipError_t e = hipMalloc(p, bytes);
+      if (e != hipSuccess) return hip_fail(e, "hipMalloc (wavefront state)");
+      W->bufs.push_back({*p, bytes});
+      return RAYON_OK;
+   };
+   void* p;
+   rayon_status st;
+   float4** f4[] = {&W->v.o, &W->v.d, &W->v.thr, &W->v.rad, &W->v.hit, &W->v.so, &W->v.sd, &W->v.sc};
+   for (float4** f : f4) {
+      if ((st = alloc((size_t)cap * sizeof(float4), &p)) != RAYON_OK) return st;
+      *f = (float4*)p;
+   }
+   if ((st = alloc((size_t)cap * sizeof(uint4), &p)) != RAYON_OK) return st;
+   W->v.ids = (uint4*)p;
+   if ((st = alloc((size_t)cap * sizeof(int), &p)) != RAYON_OK) return st;
+   W->v.q0 = (int*)p;
+   if ((st = alloc((size_t)cap * sizeof(int), &p)) != RAYON_OK) return st;
+   W->v.q1 = (int*)p;
+   if ((st = alloc((size_t)(lv + 1) * 4 * sizeof(unsigned), &p)) != RAYON_OK) return st;
+   W->v.ctl = (unsigned*)p;
+   if ((st = alloc((size_t)mj * sizeof(JobView), &p)) != RAYON_OK) return st;
+   W->jobs_dev = (JobView*)p;
+   if ((st = alloc((size_t)(mj + 1) * sizeof(long long), &p)) != RAYON_OK) return st;
+   W->base_dev = (long long*)p;
+   if ((st = alloc(8 * sizeof(unsigned long long), &p)) != RAYON_OK) return st;
+   W->stats_dev = (unsigned long long*)p;
+   RY_HIP(launch_fill_u32((uint32_t*)W->stats_dev, 0u, 16, ctx->stream));
+   W->capacity = cap;
+   W->levels = lv;
+   W->max_jobs = mj;
+   W->v.jobs = W->jobs_dev;
+   W->v.base = W->base_dev;
+   W->v.stats = W->stats_dev;
+   return RAYON_OK;
+}
+
+void wave_destroy(rayon_ctx* ctx) {
+   Wave* W = ctx->wave;
+   if (!W) return;
+   for (auto& kv : W->graphs) {
+      (void)hipGraphExecDestroy(kv.second.exec);
+      (void)hipGraphDestroy(kv.second.graph);
+   }
+   for (auto& b : W->bufs)
+      if (b.p) (void)hipFree(b.p);
+   delete W;
+   ctx->wave = nullptr;
+}
+
+namespace {
+
+struct StageTimer {
+   Wave* W;
+   hipStream_t st;
+   int stage = 0;
+   hipEvent_t e0 = nullptr, e1 = nullptr;
+   StageTimer(Wave* w, hipStream_t s, int k) : W(w), st(s), stage(k) {
+      if (W->opt_timing) {
+         (void)hipEventCreate(&e0);
+         (void)hipEventCreate(&e1);
+         (void)hipEventRecord(e0, st);
+      }
+   }
+   ~StageTimer() {
+      if (W->opt_timing) {
+         (void)hipEventRecord(e1, st);
+         (void)hipEventSynchronize(e1);
+         float ms = 0;
+         (void)hipEventElapsedTime(&ms, e0, e1);
+         W->stage_ms[stage] += ms;
+         (void)hipEventDestroy(e0);
+         (void)hipEventDestroy(e1);
+      }
+      W->launches[stage] += 1;
+   }
+};
+
+}  // namespace
+
+// Enqueues every launch of one pass (the sequence a graph captures).
+static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jobs, const std::vector<int>& todo,
+                               long long total, int levels) {
+   Wave* W = ctx->wave;
+   hipStream_t st = ctx->stream;
+   WfView V = W->v;
+   V.njobs = (int)jobs.size();
+   V.levels = levels;
+   const int grid = W->cus * W->opt_wg_per_cu;
+   hipError_t e = hipSuccess;
+   for (size_t k = 0; k < jobs.size(); ++k) {
+      rayon_job* j = jobs[k];
+      if (todo[k] > 0 && (j->params.flags & RAYON_FLAG_ADAPTIVE_REF) && j->prev) {
+         StageTimer t(W, st, 5);
+         const int np = j->params.width * j->params.height;
+         hipLaunchKernelGGL(k_wf_pass_begin, dim3((np + 255) / 256), dim3(256), 0, st, j->view, j->prev);
+      }
+   }
+   for (long long w0 = 0; w0 < total; w0 += W->capacity) {
+      const int n = (int)std::min<long long>(W->capacity, total - w0);
+      ++W->rounds;
+      {
+         StageTimer t(W, st, 5);
+         if ((e = launch_fill_u32(V.ctl, 0u, (long long)(levels + 1) * 4, st)) != hipSuccess) return e;
+      }
+      {
+         StageTimer t(W, st, 0);
+         hipLaunchKernelGGL(k_wf_generate, dim3((n + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, st, V, w0, n);
+      }
+      for (int lv = 0; lv < levels; ++lv) {
+         {
+            StageTimer t(W, st, 1);
+            if (W->opt_count)
+               hipLaunchKernelGGL(k_wf_extend<true>, dim3(grid), dim3(kWfBlock), 0, st, V, lv);
+            else
+               hipLaunchKernelGGL(k_wf_extend<false>, dim3(grid), dim3(kWfBlock), 0, st, V, lv);
+         }
+         {
+            StageTimer t(W, st, 2);
+            if (W->opt_count)
+               hipLaunchKernelGGL(k_wf_shade<true>, dim3(grid), dim3(kWfBlock), 0, st, V, lv);
+            else
+               hipLaunchKernelGGL(k_wf_shade<false>, dim3(grid), dim3(kWfBlock), 0, st, V, lv);
+         }
+         {
+            StageTimer t(W, st, 3);
+            if (W->opt_count)
+               hipLaunchKernelGGL(k_wf_shadow<true>, dim3(grid), dim3(kWfBlock), 0, st, V, lv);
+            else
+               hipLaunchKernelGGL(k_wf_shadow<false>, dim3(grid), dim3(kWfBlock), 0, st, V, lv);
+         }
+      }
+      {
+         StageTimer t(W, st, 4);
+         hipLaunchKernelGGL(k_wf_accumulate, dim3((n + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, st, V, w0, n);
+      }
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+   }
+   for (size_t k = 0; k < jobs.size(); ++k) {
+      rayon_job* j = jobs[k];
+      if (todo[k] <= 0 || !(j->params.flags & (RAYON_FLAG_ADAPTIVE_REF | RAYON_FLAG_ADAPTIVE_STDERR))) continue;
+      StageTimer t(W, st, 5);
+      const int np = j->params.width * j->params.height;
+      hipLaunchKernelGGL(k_wf_pass_end, dim3((np + 255) / 256), dim3(256), 0, st, j->view, j->prev, todo[k]);
+   }
+   return hipGetLastError();
+}
+
+rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, int samples) {
+   if (!ctx || (n > 0 && !jobs_in) || n < 0 || samples < 0) {
+      set_error("rayon_render_batched: bad arguments");
+      return RAYON_ERR_INVALID;
+   }
+   std::vector<rayon_job*> jobs;
+   std::vector<int> todo;
+   long long total = 0;
+   int levels = 1;
+   for (int k = 0; k < n; ++k) {
+      rayon_job* j = jobs_in[k];
+      if (!j || j->ctx != ctx) {
+         set_error("rayon_render_batched: job %d is NULL or belongs to another context", k);
+         return RAYON_ERR_INVALID;
+      }
+      if (j->params.flags & RAYON_FLAG_REF_RNG) {
+         set_error("rayon_render_batched: job %d uses the sequential reference stream (RAYON_FLAG_REF_RNG), "
+                   "whose samples cannot run concurrently; render it with rayon_job_render_plain", k);
+         return RAYON_ERR_UNSUPPORTED;
+      }
+      if (j->params.max_depth > 4000) {
+         set_error("rayon_render_batched: job %d max_depth %d exceeds 4000", k, j->params.max_depth);
+         return RAYON_ERR_INVALID;
+      }
+      for (int m = 0; m < k; ++m)
+         if (jobs_in[m] == j) {
+            set_error("rayon_render_batched: job %d listed twice", k);
+            return RAYON_ERR_INVALID;
+         }
+      const int t = (int)std::max<long long>(0, std::min<long long>(samples, j->params.spp - j->samples_done));
+      jobs.push_back(j);
+      todo.push_back(t);
+      total += (long long)t * j->params.width * j->params.height;
+      levels = std::max(levels, j->params.max_depth);
+   }
+   if (total == 0) return RAYON_OK;
+   RY_HIP(hipSetDevice(ctx->device));
+   std::lock_guard<std::mutex> lk(ctx->mu);
+   if (!ctx->wave) ctx->wave = new Wave();
+   const int cap = (int)std::min<long long>(ctx->wave->opt_capacity, total);
+   rayon_status st = wave_reserve(ctx, std::max(cap, 1 << 16), levels, n);
+   if (st != RAYON_OK) return st;
+   Wave* W = ctx->wave;
+   for (rayon_job* j : jobs) {
+      if ((j->params.flags & RAYON_FLAG_ADAPTIVE_REF) && !j->prev) {
+         void* p = nullptr;
+         const size_t bytes = (size_t)j->params.width * j->params.height * sizeof(float4);
+         RY_HIP(hipMalloc(&p, bytes));
+         j->bufs.push_back({p, bytes});
+         j->prev = (float4*)p;
+      }
+   }
+   // job table (only jobs with samples to add take part in the pass)
+   std::vector<rayon_job*> active;
+   std::vector<int> atodo;
+   W->host_jobs.clear();
+   W->host_base.assign(1, 0);
+   for (size_t k = 0; k < jobs.size(); ++k) {
+      if (todo[k] <= 0) continue;
+      rayon_job* j = jobs[k];
+      fill_jobview(j);
+      j->view.sample_base = (int)j->samples_done;
+      active.push_back(j);
+      atodo.push_back(todo[k]);
+      W->host_jobs.push_back(j->view);
+      W->host_base.push_back(W->host_base.back() + (long long)todo[k] * j->params.width * j->params.height);
+   }
+   RY_HIP(hipMemcpyAsync(W->jobs_dev, W->host_jobs.data(), W->host_jobs.size() * sizeof(JobView),
+                         hipMemcpyHostToDevice, ctx->stream));
+   RY_HIP(hipMemcpyAsync(W->base_dev, W->host_base.data(), W->host_base.size() * sizeof(long long),
+                         hipMemcpyHostToDevice, ctx->stream));
+   // the table copy reads pageable host memory: finish it before the vectors change again
+   RY_HIP(hipStreamSynchronize(ctx->stream));
+   const bool use_graph = W->opt_graph && !W->opt_timing;
+   if (!use_graph) {
+      RY_HIP(enqueue_pass(ctx, active, atodo, total, levels));
+   } else {
+      // The captured launches read every job through the device job table, so a graph
+      // fits any job set with the same work per job; only the adaptive pass-begin/end
+      // kernels take a job by value, so adaptive jobs are part of the key.
+      std::vector<long long> key;
+      for (size_t k = 0; k < active.size(); ++k) {
+         const bool adaptive = (active[k]->params.flags & (RAYON_FLAG_ADAPTIVE_REF | RAYON_FLAG_ADAPTIVE_STDERR)) != 0;
+         key.push_back(adaptive ? (long long)(uintptr_t)active[k] : 0);
+         key.push_back(W->host_base[k + 1] - W->host_base[k]);
+      }
+      key.push_back(levels);
+      key.push_back(W->capacity);
+      key.push_back(W->opt_wg_per_cu);
+      key.push_back(W->opt_count);
+      auto it = W->graphs.find(key);
+      if (it == W->graphs.end()) {
+         Wave::GraphEntry ge;
+         RY_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+         hipError_t e = enqueue_pass(ctx, active, atodo, total, levels);
+         hipGraph_t g = nullptr;
+         hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+         if (e != hipSuccess) return hip_fail(e, "wavefront pass capture");
+         if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
+         ge.graph = g;
+         RY_HIP(hipGraphInstantiate(&ge.exec, g, nullptr, nullptr, 0));
+         it = W->graphs.emplace(key, ge).first;
+      }
+      RY_HIP(hipGraphLaunch(it->second.exec, ctx->stream));
+   }
+   for (size_t k = 0; k < active.size(); ++k) active[k]->samples_done += atodo[k];
+   return RAYON_OK;
+}
+
+void wave_forget_job(rayon_ctx* ctx, rayon_job* job) {
+   Wave* W = ctx ? ctx->wave : nullptr;
+   if (!W) return;
+   for (auto it = W->graphs.begin(); it != W->graphs.end();) {
+      bool uses = false;
+      for (size_t k = 0; k + 1 < it->first.size(); k += 2)
+         if (it->first[k] == (long long)(uintptr_t)job) uses = true;
+      if (uses) {
+         (void)hipGraphExecDestroy(it->second.exec);
+         (void)hipGraphDestroy(it->second.graph);
+         it = W->graphs.erase(it);
+      } else {
+         ++it;
+      }
+   }
+}
+
+rayon_status wave_set_option(rayon_ctx* ctx, int opt, long long value) {
+   if (!ctx->wave) ctx->wave = new Wave();
+   Wave* W = ctx->wave;
+   switch (opt) {
+   case RAYON_OPT_WAVE_CAPACITY:
+      if (value < (1 << 16) || value > (1ll << 28)) {
+         set_error("RAYON_OPT_WAVE_CAPACITY must be in [65536, 2^28]");
+         return RAYON_ERR_INVALID;
+      }
+      W->opt_capacity = (int)value;
+      break;
+   case RAYON_OPT_WAVE_WG_PER_CU:
+      if (value < 1 || value > 32) {
+         set_error("RAYON_OPT_WAVE_WG_PER_CU must be in [1, 32]");
+         return RAYON_ERR_INVALID;
+      }
+      W->opt_wg_per_cu = (int)value;
+      break;
+   case RAYON_OPT_GRAPH:
+      W->opt_graph = value != 0;
+      break;
+   case RAYON_OPT_STAGE_TIMING:
+      W->opt_timing = value != 0;
+      break;
+   case RAYON_OPT_COUNT_TRAVERSAL:
+      W->opt_count = value != 0;
+      break;
+   default:
+      set_error("unknown option %d", opt);
+      return RAYON_ERR_INVALID;
+   }
+   return RAYON_OK;
+}
+
+rayon_status wave_get_option(rayon_ctx* ctx, int opt, long long* value) {
+   if (!ctx->wave) ctx->wave = new Wave();
+   Wave* W = ctx->wave;
+   switch (opt) {
+   case RAYON_OPT_WAVE_CAPACITY: *value = W->opt_capacity; break;
+   case RAYON_OPT_WAVE_WG_PER_CU: *value = W->opt_wg_per_cu; break;
+   case RAYON_OPT_GRAPH: *value = W->opt_graph; break;
+   case RAYON_OPT_STAGE_TIMING: *value = W->opt_timing; break;
+   case RAYON_OPT_COUNT_TRAVERSAL: *value = W->opt_count; break;
+   default:
+      set_error("unknown option %d", opt);
+      return RAYON_ERR_INVALID;
+   }
+   return RAYON_OK;
+}
+
+rayon_status wave_stats(rayon_ctx* ctx, rayon_wave_stats* out, int reset) {
+   std::memset(out, 0, sizeof(*out));
+   Wave* W = ctx->wave;
+   if (!W) return RAYON_OK;
+   RY_HIP(hipStreamSynchronize(ctx->stream));
+   for (int k = 0; k < 6; ++k) {
+      out->stage_ms[k] = W->stage_ms[k];
+      out->launches[k] = W->launches[k];
+   }
+   out->rounds = W->rounds;
+   if (W->stats_dev) {
+      unsigned long long s[8];
+      RY_HIP(hipMemcpy(s, W->stats_dev, sizeof(s), hipMemcpyDeviceToHost));
+      out->extend_node_fetches = (int64_t)s[0];
+      out->extend_prim_fetches = (int64_t)s[1];
+      out->shadow_node_fetches = (int64_t)s[2];
+      out->shadow_prim_fetches = (int64_t)s[3];
+      out->material_fetches = (int64_t)s[4];
+      out->light_fetches = (int64_t)s[5];
+   }
+   if (reset) {
+      for (int k = 0; k < 6; ++k) {
+         W->stage_ms[k] = 0;
+         W->launches[k] = 0;
+      }
+      W->rounds = 0;
+      if (W->stats_dev) RY_HIP(launch_fill_u32((uint32_t*)W->stats_dev, 0u, 16, ctx->stream));
+   }
+   return RAYON_OK;
+}
 
 }  // namespace ry
 
-extern "C" rayon_status rayon_render_batched(rayon_ctx* ctx, rayon_job* const* jobs, int32_t n, int32_t samples) {
-   (void)ctx;
-   (void)jobs;
-   (void)n;
-   (void)samples;
-   ry::set_error("rayon_render_batched: not built yet");
-   return RAYON_ERR_UNSUPPORTED;
+extern "C" {
+
+rayon_status rayon_render_batched(rayon_ctx* ctx, rayon_job* const* jobs, int32_t n, int32_t samples) {
+   return ry::render_batched(ctx, jobs, n, samples);
 }
 
-extern "C" rayon_status rayon_load_yaml(const char* path, rayon_generated** out) {
+rayon_status rayon_ctx_set_option(rayon_ctx* ctx, int32_t opt, int64_t value) {
+   if (!ctx) {
+      ry::set_error("rayon_ctx_set_option: NULL context");
+      return RAYON_ERR_INVALID;
+   }
+   std::lock_guard<std::mutex> lk(ctx->mu);
+   return ry::wave_set_option(ctx, opt, value);
+}
+
+rayon_status rayon_ctx_get_option(rayon_ctx* ctx, int32_t opt, int64_t* value) {
+   if (!ctx || !value) {
+      ry::set_error("rayon_ctx_get_option: NULL argument");
+      return RAYON_ERR_INVALID;
+   }
+   std::lock_guard<std::mutex> lk(ctx->mu);
+   long long v = 0;
+   rayon_status s = ry::wave_get_option(ctx, opt, &v);
+   *value = v;
+   return s;
+}
+
+rayon_status rayon_ctx_wave_stats(rayon_ctx* ctx, rayon_wave_stats* out, int32_t reset) {
+   if (!ctx || !out) {
+      ry::set_error("rayon_ctx_wave_stats: NULL argument");
+      return RAYON_ERR_INVALID;
+   }
+   std::lock_guard<std::mutex> lk(ctx->mu);
+   return ry::wave_stats(ctx, out, reset);
+}
+
+rayon_status rayon_load_yaml(const char* path, rayon_generated** out) {
    (void)path;
    (void)out;
    ry::set_error("rayon_load_yaml: not built yet");
    return RAYON_ERR_UNSUPPORTED;
 }
+
+}  // extern "C"
