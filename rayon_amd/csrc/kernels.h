@@ -15,6 +15,9 @@ hipError_t launch_trace_closest(const SceneView& S, const float* o, const float*
                                 float tmax, long long* prim, float* t, hipStream_t st);
 hipError_t launch_primary(const JobView& J, int sample, float* o, float* d, hipStream_t st);
 hipError_t launch_ref_rng_init(uint32_t* p, int n, hipStream_t st);
+hipError_t launch_resolve_u8_raw(const float4* acc, const float4* comp, const int* counts, int n_uniform, int npix,
+                                 uint8_t* out, int channels, float gamma, hipStream_t st);
+hipError_t launch_heatmap(const int* counts, int npix, uint8_t* out, int channels, int max_samples, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t st);
 
 }  // namespace ry

@@ -146,6 +146,58 @@ __global__ void k_ref_rng_init(uint32_t* st, int n) {
    if (i < n) st[i] = 1234u + (uint32_t)i * 747796405u;
 }
 
+// gammaCorrectKernel (render_acc_kernel.cu:11-44) on raw buffers: sums (+ optional
+// compensation), per-pixel signed counts or one uniform count.
+__global__ void k_resolve_u8_raw(const float4* acc, const float4* comp, const int* counts, int n_uniform, int npix,
+                                 uint8_t* out, int channels, float inv_gamma) {
+   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+   if (pix >= npix) return;
+   float4 a = acc[pix];
+   if (comp) {
+      const float4 c = comp[pix];
+      a.x += c.x;
+      a.y += c.y;
+      a.z += c.z;
+   }
+   int n = counts ? abs(counts[pix]) : n_uniform;
+   if (n <= 0) n = 1;
+   const float inv = 1.0f / (float)n;
+   uint8_t* o = out + (size_t)pix * channels;
+   o[0] = (uint8_t)(256.0f * fminf(powf(fmaxf(a.x * inv, 0.0f), inv_gamma), 0.999f));
+   o[1] = (uint8_t)(256.0f * fminf(powf(fmaxf(a.y * inv, 0.0f), inv_gamma), 0.999f));
+   o[2] = (uint8_t)(256.0f * fminf(powf(fmaxf(a.z * inv, 0.0f), inv_gamma), 0.999f));
+   if (channels == 4) o[3] = 255;
+}
+
+// Sample-count heat map: |count| / max through an 8-key piecewise-linear plasma ramp
+// (the key table of render_acc_kernel.cu:50-82), 255-scaled (:110-112).
+__global__ void k_heatmap(const int* counts, int npix, uint8_t* out, int channels, int max_samples) {
+   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+   if (pix >= npix) return;
+   const float key[8][4] = {{0.000f, 0.050f, 0.030f, 0.530f}, {0.143f, 0.230f, 0.015f, 0.660f},
+                            {0.286f, 0.450f, 0.005f, 0.660f}, {0.429f, 0.650f, 0.060f, 0.540f},
+                            {0.571f, 0.820f, 0.170f, 0.360f}, {0.714f, 0.940f, 0.340f, 0.170f},
+                            {0.857f, 0.990f, 0.560f, 0.040f}, {1.000f, 0.940f, 0.975f, 0.130f}};
+   const int c = abs(counts[pix]);
+   float t = max_samples > 0 ? fminf((float)c / (float)max_samples, 1.0f) : 0.0f;
+   t = fminf(fmaxf(t, 0.0f), 1.0f);
+   float r = key[7][1], g = key[7][2], b = key[7][3];
+   for (int i = 0; i < 7; ++i) {
+      if (t <= key[i + 1][0]) {
+         const float u = (t - key[i][0]) / (key[i + 1][0] - key[i][0]);
+         r = key[i][1] + u * (key[i + 1][1] - key[i][1]);
+         g = key[i][2] + u * (key[i + 1][2] - key[i][2]);
+         b = key[i][3] + u * (key[i + 1][3] - key[i][3]);
+         break;
+      }
+   }
+   uint8_t* o = out + (size_t)pix * channels;
+   o[0] = (uint8_t)(255.0f * r);
+   o[1] = (uint8_t)(255.0f * g);
+   o[2] = (uint8_t)(255.0f * b);
+   if (channels == 4) o[3] = 255;
+}
+
 // ---- host launchers --------------------------------------------------------------
 hipError_t launch_plain(const JobView& J, int samples, hipStream_t st) {
    const int tx = (J.width + kTile - 1) / kTile, ty = (J.height + kTile - 1) / kTile;
@@ -186,6 +238,20 @@ hipError_t launch_primary(const JobView& J, int sample, float* o, float* d, hipS
 
 hipError_t launch_ref_rng_init(uint32_t* p, int n, hipStream_t st) {
    hipLaunchKernelGGL(k_ref_rng_init, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+   return hipGetLastError();
+}
+
+hipError_t launch_resolve_u8_raw(const float4* acc, const float4* comp, const int* counts, int n_uniform, int npix,
+                                 uint8_t* out, int channels, float gamma, hipStream_t st) {
+   if (npix <= 0) return hipSuccess;
+   hipLaunchKernelGGL(k_resolve_u8_raw, dim3((npix + 255) / 256), dim3(256), 0, st, acc, comp, counts, n_uniform, npix,
+                      out, channels, 1.0f / gamma);
+   return hipGetLastError();
+}
+
+hipError_t launch_heatmap(const int* counts, int npix, uint8_t* out, int channels, int max_samples, hipStream_t st) {
+   if (npix <= 0) return hipSuccess;
+   hipLaunchKernelGGL(k_heatmap, dim3((npix + 255) / 256), dim3(256), 0, st, counts, npix, out, channels, max_samples);
    return hipGetLastError();
 }
 

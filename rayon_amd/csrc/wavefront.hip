@@ -34,6 +34,7 @@
 namespace ry {
 
 constexpr int kWfBlock = 256;
+constexpr int kCtl = 8;  // control words per level
 constexpr uint32_t kSkip = 1u;        // ids.w bit: pixel converged, no path this sample
 constexpr uint32_t kPrevLambert = 2u; // ids.w bit: the last scatter was Lambertian
 // ids.w bits 8..19: extension segments, bits 20..31: shadow segments
@@ -50,7 +51,8 @@ struct WfView {
    float4* sc;    // shadow queue: contribution
    int* q0;
    int* q1;
-   unsigned* ctl;           // per level: extend count, extend head, shadow count, shadow head
+   unsigned* ctl;           // per level (stride kCtl): queue count, extend head, shade head,
+                            // shadow count, shadow head
    const JobView* jobs;
    const long long* base;   // work prefix per job (njobs + 1 entries)
    unsigned long long* stats;  // traversal statistics (optional)
@@ -154,13 +156,13 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_generate(WfView V, long long w0
 template <bool CNT>
 __global__ __launch_bounds__(kWfBlock) void k_wf_extend(WfView V, int level) {
    __shared__ int stack[kStack * kWfBlock];
-   const unsigned* ctl = V.ctl + 4 * level;
+   unsigned* ctl = V.ctl + kCtl * level;
    const unsigned count = ctl[0];
    const int* q = (level & 1) ? V.q1 : V.q0;
    TravCount tc;
    for (;;) {
       unsigned base = 0;
-      if (lane_id() == 0) base = atomicAdd((unsigned*)&ctl[1], 64u);
+      if (lane_id() == 0) base = atomicAdd(&ctl[1], 64u);
       base = __shfl(base, 0);
       if (base >= count) break;
       const unsigned idx = base + (unsigned)lane_id();
@@ -184,16 +186,16 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_extend(WfView V, int level) {
 
 template <bool CNT>
 __global__ __launch_bounds__(kWfBlock) void k_wf_shade(WfView V, int level) {
-   const unsigned* ctl = V.ctl + 4 * level;
+   unsigned* ctl = V.ctl + kCtl * level;
    const unsigned count = ctl[0];
    const int* q = (level & 1) ? V.q1 : V.q0;
    int* qn = (level & 1) ? V.q0 : V.q1;
-   unsigned* ctl_next = V.ctl + 4 * (level + 1);
-   unsigned* sctl = V.ctl + 4 * level + 2;
+   unsigned* ctl_next = V.ctl + kCtl * (level + 1);
+   unsigned* sctl = ctl + 3;
    PathCounts pc;
    for (;;) {
       unsigned base = 0;
-      if (lane_id() == 0) base = atomicAdd((unsigned*)&ctl[1], 64u);
+      if (lane_id() == 0) base = atomicAdd(&ctl[2], 64u);
       base = __shfl(base, 0);
       if (base >= count) break;
       const unsigned idx = base + (unsigned)lane_id();
@@ -246,7 +248,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shade(WfView V, int level) {
 template <bool CNT>
 __global__ __launch_bounds__(kWfBlock) void k_wf_shadow(WfView V, int level) {
    __shared__ int stack[kStack * kWfBlock];
-   unsigned* sctl = V.ctl + 4 * level + 2;
+   unsigned* sctl = V.ctl + kCtl * level + 3;
    const unsigned count = sctl[0];
    TravCount tc;
    for (;;) {
@@ -405,7 +407,7 @@ static rayon_status wave_reserve(rayon_ctx* ctx, int capacity, int levels, int n
    W->v.q0 = (int*)p;
    if ((st = alloc((size_t)cap * sizeof(int), &p)) != RAYON_OK) return st;
    W->v.q1 = (int*)p;
-   if ((st = alloc((size_t)(lv + 1) * 4 * sizeof(unsigned), &p)) != RAYON_OK) return st;
+   if ((st = alloc((size_t)(lv + 1) * kCtl * sizeof(unsigned), &p)) != RAYON_OK) return st;
    W->v.ctl = (unsigned*)p;
    if ((st = alloc((size_t)mj * sizeof(JobView), &p)) != RAYON_OK) return st;
    W->jobs_dev = (JobView*)p;
@@ -489,7 +491,7 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
       ++W->rounds;
       {
          StageTimer t(W, st, 5);
-         if ((e = launch_fill_u32(V.ctl, 0u, (long long)(levels + 1) * 4, st)) != hipSuccess) return e;
+         if ((e = launch_fill_u32(V.ctl, 0u, (long long)(levels + 1) * kCtl, st)) != hipSuccess) return e;
       }
       {
          StageTimer t(W, st, 0);

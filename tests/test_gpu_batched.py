@@ -27,16 +27,16 @@ class Scenes:
         self.ctx = ctx
         self.gens, self.scenes = {}, {}
 
-    def get(self, kind, seed, detail):
-        key = (kind, seed, detail)
+    def get(self, kind, scene_seed, detail):
+        key = (kind, scene_seed, detail)
         if key not in self.scenes:
-            g = ra.Generated.generate(kind, seed, detail=detail)
+            g = ra.Generated.generate(kind, scene_seed, detail=detail)
             self.gens[key] = g
             self.scenes[key] = self.ctx.scene(g.desc_ptr())
         return self.gens[key], self.scenes[key]
 
-    def job(self, kind, seed, detail, W, H, **kw):
-        g, sc = self.get(kind, seed, detail)
+    def job(self, kind, scene_seed, detail, W, H, **kw):
+        g, sc = self.get(kind, scene_seed, detail)
         lf, la, up, fov = g.camera()
         cam = ra.camera_frame(lf, la, up, fov, W, H)
         P = ra.job_params(W, H, cam, **kw)
