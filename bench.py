@@ -194,14 +194,17 @@ def main(argv=None) -> int:
     cj.close()
     ctx.set_option(ra.OPT.COUNT_TRAVERSAL, 0)
     ext_rays = cjs.segments - cjs.shadow_segments
-    # extend: nodes + primitives + ray in (o, d, ids: 48 B) + hit out (16 B)
-    ext_bytes = NODE_B * st_c.extend_node_fetches + PRIM_B * st_c.extend_prim_fetches + 64 * ext_rays
+    sh_rays = cjs.shadow_segments
+    # trace kernel: extension rays (nodes + primitives + ray in 48 B + hit out 16 B) and
+    # shadow rays (nodes + primitives + ray/contribution in 48 B + radiance RMW 32 B)
+    ext_bytes = (NODE_B * (st_c.extend_node_fetches + st_c.shadow_node_fetches)
+                 + PRIM_B * (st_c.extend_prim_fetches + st_c.shadow_prim_fetches) + 64 * ext_rays + 80 * sh_rays)
     ext_ms = st_t.stage_ms[1]
     ext_launches = st_t.launches[1]
     ext_achieved = ext_bytes / (ext_ms * 1e-3) / 1e9
-    stage_names = ["generate", "extend", "shade", "shadow", "accumulate", "bookkeeping"]
-    stage_ms = {n: round(st_t.stage_ms[k], 2) for k, n in enumerate(stage_names)}
-    log(f"stages (ms per frame): {stage_ms}; extend {ext_achieved:.0f} GB/s algorithmic")
+    stage_names = ["generate", "trace", "shade", "unused", "accumulate", "bookkeeping"]
+    stage_ms = {n: round(st_t.stage_ms[k], 2) for k, n in enumerate(stage_names) if n != "unused"}
+    log(f"stages (ms per frame): {stage_ms}; trace {ext_achieved:.0f} GB/s algorithmic")
 
     use_batched = a.path == "batched"
     value = batched_mrays if use_batched else plain_mrays
@@ -210,9 +213,9 @@ def main(argv=None) -> int:
     wall = b_wall if use_batched else p_wall
     if use_batched:
         roof = {"bound": "hbm", "achieved": round(ext_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ext_achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_wf_extend",
-                "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4),
-                "alg_bytes_per_ray": round(ext_bytes / max(ext_rays, 1), 1)}
+                "frac": round(ext_achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_wf_trace",
+                "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4), "launches_per_frame": int(ext_launches),
+                "alg_bytes_per_ray": round(ext_bytes / max(ext_rays + sh_rays, 1), 1)}
     else:
         roof = {"bound": "hbm", "achieved": round(plain_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(plain_achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_plain",

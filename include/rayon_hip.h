@@ -268,17 +268,20 @@ rayon_status rayon_render_batched(rayon_ctx* ctx, rayon_job* const* jobs, int32_
 
 /* Context options for the batched path. */
 enum {
-   RAYON_OPT_WAVE_CAPACITY = 0,   /* paths in flight per round (default 4M)                   */
-   RAYON_OPT_WAVE_WG_PER_CU = 1,  /* persistent workgroups per CU of the queue stages (4)     */
+   RAYON_OPT_WAVE_CAPACITY = 0,   /* paths in flight per round (default 64M, ~150 B each)     */
+   RAYON_OPT_WAVE_WG_PER_CU = 1,  /* persistent workgroups per CU of the queue stages
+                                     (0 = as many as each stage's occupancy allows)         */
    RAYON_OPT_GRAPH = 2,           /* capture each pass into a hipGraph and replay it (1)      */
    RAYON_OPT_STAGE_TIMING = 3,    /* time every stage launch with HIP events (0; disables graphs) */
-   RAYON_OPT_COUNT_TRAVERSAL = 4  /* count node/primitive fetches of the queue stages (0)     */
+   RAYON_OPT_COUNT_TRAVERSAL = 4, /* count node/primitive fetches of the queue stages (0)     */
+   RAYON_OPT_WAVE_CHUNK = 5       /* 64-ray batches a wave claims per queue atomic (4)       */
 };
 rayon_status rayon_ctx_set_option(rayon_ctx* ctx, int32_t option, int64_t value);
 rayon_status rayon_ctx_get_option(rayon_ctx* ctx, int32_t option, int64_t* value);
 
 /* Accumulated statistics of the batched path since the last reset. Stages: 0 generate,
- * 1 extend, 2 shade, 3 shadow, 4 accumulate, 5 bookkeeping. */
+ * 1 trace (extension closest hits + shadow any hits), 2 shade, 3 (unused), 4 accumulate,
+ * 5 bookkeeping. */
 typedef struct rayon_wave_stats {
    double stage_ms[6];            /* summed HIP-event time per stage (RAYON_OPT_STAGE_TIMING) */
    int64_t launches[6];

@@ -394,7 +394,7 @@ rayon_status rayon_job_reset(rayon_job* j) {
    RY_HIP(launch_fill_u32((uint32_t*)j->view.acc, 0u, (long long)npix * 4, st));
    RY_HIP(launch_fill_u32((uint32_t*)j->view.comp, 0u, (long long)npix * 4, st));
    RY_HIP(launch_fill_u32((uint32_t*)j->view.counts, 0u, (long long)npix, st));
-   RY_HIP(launch_fill_u32((uint32_t*)j->view.counters, 0u, 16, st));  // 8 x u64
+   RY_HIP(launch_fill_u32((uint32_t*)j->view.counters, 0u, 2 * 8 * kCounterSlots, st));
    if (j->view.rng) RY_HIP(launch_ref_rng_init(j->view.rng, (int)npix, st));
    j->samples_done = 0;
    return RAYON_OK;
@@ -427,7 +427,7 @@ rayon_status rayon_job_create(rayon_ctx* ctx, rayon_scene* sc, const rayon_job_p
    if ((st = dev_alloc(j->bufs, npix * sizeof(float4), &acc)) != RAYON_OK ||
        (st = dev_alloc(j->bufs, npix * sizeof(float4), &comp)) != RAYON_OK ||
        (st = dev_alloc(j->bufs, npix * sizeof(int), &counts)) != RAYON_OK ||
-       (st = dev_alloc(j->bufs, 8 * sizeof(unsigned long long), &ctr)) != RAYON_OK ||
+       (st = dev_alloc(j->bufs, kCounterSlots * 8 * sizeof(unsigned long long), &ctr)) != RAYON_OK ||
        ((p->flags & RAYON_FLAG_REF_RNG) && (st = dev_alloc(j->bufs, npix * sizeof(uint32_t), &rng)) != RAYON_OK)) {
       free_all(j->bufs);
       delete j;
@@ -528,10 +528,12 @@ rayon_status rayon_job_stats_get(rayon_ctx* ctx, rayon_job* j, rayon_job_stats* 
       set_error("rayon_job_stats_get: NULL argument");
       return RAYON_ERR_INVALID;
    }
-   unsigned long long c[8];
+   unsigned long long slots[8 * kCounterSlots], c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
    RY_HIP(hipSetDevice(ctx->device));
-   RY_HIP(hipMemcpyAsync(c, j->view.counters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+   RY_HIP(hipMemcpyAsync(slots, j->view.counters, sizeof(slots), hipMemcpyDeviceToHost, ctx->stream));
    RY_HIP(hipStreamSynchronize(ctx->stream));
+   for (int s = 0; s < kCounterSlots; ++s)
+      for (int k = 0; k < 8; ++k) c[k] += slots[8 * s + k];
    out->samples_done = j->samples_done;
    out->segments = (int64_t)c[0];
    out->shadow_segments = (int64_t)c[1];

@@ -59,11 +59,12 @@ __global__ __launch_bounds__(kBlock) void k_plain(JobView J, int samples, int ti
    unsigned long long v1 = wave_sum((unsigned long long)pc.shadow);
    unsigned long long v2 = wave_sum((unsigned long long)primary);
    unsigned long long v3 = wave_sum((unsigned long long)converged);
+   unsigned long long* ctr = J.counters + 8 * (blockIdx.x % kCounterSlots);
    if ((tid & 63) == 0) {
-      if (v0) atomicAdd(&J.counters[0], v0);
-      if (v1) atomicAdd(&J.counters[1], v1);
-      if (v2) atomicAdd(&J.counters[2], v2);
-      if (v3) atomicAdd(&J.counters[3], v3);
+      if (v0) atomicAdd(&ctr[0], v0);
+      if (v1) atomicAdd(&ctr[1], v1);
+      if (v2) atomicAdd(&ctr[2], v2);
+      if (v3) atomicAdd(&ctr[3], v3);
    }
    if constexpr (CNT) {
       unsigned long long c4 = wave_sum((unsigned long long)pc.trav.nodes);
@@ -71,10 +72,10 @@ __global__ __launch_bounds__(kBlock) void k_plain(JobView J, int samples, int ti
       unsigned long long c6 = wave_sum((unsigned long long)pc.mats);
       unsigned long long c7 = wave_sum((unsigned long long)pc.light_prims);
       if ((tid & 63) == 0) {
-         atomicAdd(&J.counters[4], c4);
-         atomicAdd(&J.counters[5], c5);
-         atomicAdd(&J.counters[6], c6);
-         atomicAdd(&J.counters[7], c7);
+         atomicAdd(&ctr[4], c4);
+         atomicAdd(&ctr[5], c5);
+         atomicAdd(&ctr[6], c6);
+         atomicAdd(&ctr[7], c7);
       }
    }
 }

@@ -27,6 +27,10 @@ struct alignas(16) int4 {
 
 namespace ry {
 
+// Job statistics are spread over this many 8-counter slots (picked by workgroup index)
+// so concurrent waves do not serialise on one address; readers sum the slots.
+constexpr int kCounterSlots = 64;
+
 // Packed reference meaning "no node" (empty scene root, empty stack).
 constexpr int kSentinel = (int)0x80000000;
 
@@ -101,7 +105,8 @@ struct JobView {
    float4* comp;       // per pixel: Neumaier compensation rgb, luminance compensation
    int* counts;        // per pixel sample count, negative = converged
    uint32_t* rng;      // per pixel sequential stream (RAYON_FLAG_REF_RNG only)
-   unsigned long long* counters;  // segments, shadow segments, primary samples, converged
+   unsigned long long* counters;  // kCounterSlots x 8: segments, shadow segments, primary
+                                  // samples, converged, node/prim/material/light fetches
    float cam_c[3], p00[3], du[3], dv[3], cu[3], cv[3];
    int width, height;
    int max_depth;

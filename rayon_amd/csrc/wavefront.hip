@@ -18,6 +18,7 @@
 // Determinism: a path's random numbers depend only on (job seed, pixel, sample); its
 // radiance is accumulated into the pixel by the accumulate stage in sample order. A job
 // therefore produces the same image alone or batched with any other jobs.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -58,6 +59,7 @@ struct WfView {
    unsigned long long* stats;  // traversal statistics (optional)
    int njobs;
    int levels;
+   int chunk;  // 64-entry batches a wave claims per queue-head atomic
 };
 
 struct Wave {
@@ -70,8 +72,15 @@ struct Wave {
    long long* base_dev = nullptr;
    unsigned long long* stats_dev = nullptr;
    // options
-   int opt_capacity = 1 << 22;
-   int opt_wg_per_cu = 4;
+   int opt_capacity = 1 << 26;
+   int opt_wg_per_cu = 0;  // 0: as many as fit (occupancy of each stage kernel)
+   int opt_chunk = 4;
+   int occ_trace = 0, occ_shade = 0;
+   struct Pending {
+      int stage;
+      hipEvent_t e0, e1;
+   };
+   std::vector<Pending> pending;  // stage events not yet read back
    int opt_graph = 1;
    int opt_timing = 0;
    int opt_count = 0;
@@ -107,6 +116,37 @@ __device__ __forceinline__ int wave_push(unsigned* counter, bool pred) {
    return pred ? (int)(base + (unsigned)__popcll(below)) : -1;
 }
 
+// Persistent-worker loop: each wave claims `chunk` batches of 64 queue entries with one
+// atomic on the queue head and runs body(index) on each; body sees indices >= count on
+// the last, partial batch and must ignore them. Returns when the queue is drained.
+template <class F>
+__device__ __forceinline__ void drain(unsigned* head, unsigned count, int chunk, F&& body) {
+   for (;;) {
+      unsigned base = 0;
+      if (lane_id() == 0) base = atomicAdd(head, 64u * (unsigned)chunk);
+      base = __shfl(base, 0);
+      if (base >= count) return;
+      for (int k = 0; k < chunk; ++k) {
+         const unsigned b = base + 64u * (unsigned)k;
+         if (b >= count) break;
+         body(b + (unsigned)lane_id());
+      }
+   }
+}
+
+// (sample, pixel) of work item l of a job; 32-bit division whenever it fits.
+__device__ __forceinline__ void split_work(long long l, unsigned npix, int& s, int& p) {
+   if (l < (1ll << 32)) {
+      const unsigned l32 = (unsigned)l, q = l32 / npix;
+      s = (int)q;
+      p = (int)(l32 - q * npix);
+   } else {
+      const long long q = l / (long long)npix;
+      s = (int)q;
+      p = (int)(l - q * (long long)npix);
+   }
+}
+
 __device__ __forceinline__ int find_job(const WfView& V, long long w) {
    int lo = 0, hi = V.njobs - 1;
    while (lo < hi) {
@@ -129,10 +169,9 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_generate(WfView V, long long w0
       const long long w = w0 + i;
       const int j = find_job(V, w);
       const JobView& J = V.jobs[j];
-      const long long npix = (long long)J.width * J.height;
-      const long long l = w - V.base[j];
-      const int s = (int)(l / npix);
-      const int p = (int)(l - (long long)s * npix);
+      const unsigned npix = (unsigned)J.width * (unsigned)J.height;
+      int s, p;
+      split_work(w - V.base[j], npix, s, p);
       const int sample = J.sample_base + s;
       uint32_t flags = 0;
       if (J.counts[p] < 0) {
@@ -153,33 +192,55 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_generate(WfView V, long long w0
    if (live) V.q0[pos] = i;
 }
 
+// Traversal stage: first the extension rays of level `ext_level` (closest hit), then the
+// shadow rays of level `shadow_level` (any hit; unoccluded contributions are added to the
+// path's radiance). Both queues were filled by the preceding shade launch and touch
+// disjoint state, so one persistent launch drains both and waves that finish their
+// extension work fill the launch's tail with shadow work. Either level may be -1.
 template <bool CNT>
-__global__ __launch_bounds__(kWfBlock) void k_wf_extend(WfView V, int level) {
+__global__ __launch_bounds__(kWfBlock) void k_wf_trace(WfView V, int ext_level, int shadow_level) {
    __shared__ int stack[kStack * kWfBlock];
-   unsigned* ctl = V.ctl + kCtl * level;
-   const unsigned count = ctl[0];
-   const int* q = (level & 1) ? V.q1 : V.q0;
-   TravCount tc;
-   for (;;) {
-      unsigned base = 0;
-      if (lane_id() == 0) base = atomicAdd(&ctl[1], 64u);
-      base = __shfl(base, 0);
-      if (base >= count) break;
-      const unsigned idx = base + (unsigned)lane_id();
-      if (idx < count) {
-         const int slot = q[idx];
-         const float4 o = V.o[slot], d = V.d[slot];
-         const SceneView& S = V.jobs[V.ids[slot].x].scene;
-         const HitMin h = closest_hit<kWfBlock, CNT>(S, mk(o), mk(d), 0.001f, 3.402823466e38f,
-                                                    stack + threadIdx.x, &tc);
-         V.hit[slot] = make_float4(h.t, __int_as_float(h.slot), h.u, h.v);
-      }
+   TravCount te, ts;
+   if (ext_level >= 0) {
+      unsigned* ctl = V.ctl + kCtl * ext_level;
+      const unsigned count = ctl[0];
+      const int* q = (ext_level & 1) ? V.q1 : V.q0;
+      drain(&ctl[1], count, V.chunk, [&](unsigned idx) {
+         if (idx < count) {
+            const int slot = q[idx];
+            const float4 o = V.o[slot], d = V.d[slot];
+            const SceneView& S = V.jobs[V.ids[slot].x].scene;
+            const HitMin h = closest_hit<kWfBlock, CNT>(S, mk(o), mk(d), 0.001f, 3.402823466e38f,
+                                                       stack + threadIdx.x, &te);
+            V.hit[slot] = make_float4(h.t, __int_as_float(h.slot), h.u, h.v);
+         }
+      });
+   }
+   if (shadow_level >= 0) {
+      unsigned* sctl = V.ctl + kCtl * shadow_level + 3;
+      const unsigned count = sctl[0];
+      drain(&sctl[1], count, V.chunk, [&](unsigned idx) {
+         if (idx < count) {
+            const float4 o = V.so[idx], d = V.sd[idx];
+            const int slot = __float_as_int(o.w);
+            const SceneView& S = V.jobs[V.ids[slot].x].scene;
+            if (!any_hit<kWfBlock, CNT>(S, mk(o), mk(d), 1e-4f, 1.0f - 1e-4f, stack + threadIdx.x, &ts)) {
+               const float4 c = V.sc[idx];
+               const float4 r = V.rad[slot];
+               const V3 L = mk(r) + mk(c);
+               V.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+            }
+         }
+      });
    }
    if constexpr (CNT) {
-      unsigned long long a = wave_sum((unsigned long long)tc.nodes), b = wave_sum((unsigned long long)tc.prims);
+      const unsigned long long a = wave_sum((unsigned long long)te.nodes), b = wave_sum((unsigned long long)te.prims);
+      const unsigned long long c = wave_sum((unsigned long long)ts.nodes), d = wave_sum((unsigned long long)ts.prims);
       if (lane_id() == 0) {
          atomicAdd(&V.stats[0], a);
          atomicAdd(&V.stats[1], b);
+         atomicAdd(&V.stats[2], c);
+         atomicAdd(&V.stats[3], d);
       }
    }
 }
@@ -193,12 +254,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shade(WfView V, int level) {
    unsigned* ctl_next = V.ctl + kCtl * (level + 1);
    unsigned* sctl = ctl + 3;
    PathCounts pc;
-   for (;;) {
-      unsigned base = 0;
-      if (lane_id() == 0) base = atomicAdd(&ctl[2], 64u);
-      base = __shfl(base, 0);
-      if (base >= count) break;
-      const unsigned idx = base + (unsigned)lane_id();
+   drain(&ctl[2], count, V.chunk, [&](unsigned idx) {
       bool cont = false, shadow = false;
       int slot = -1;
       ShadowReq sh;
@@ -235,45 +291,12 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shade(WfView V, int level) {
          V.sd[ps_pos] = make_float4(sh.d.x, sh.d.y, sh.d.z, 0.0f);
          V.sc[ps_pos] = make_float4(sh.contrib.x, sh.contrib.y, sh.contrib.z, 0.0f);
       }
-   }
+   });
    if constexpr (CNT) {
       unsigned long long a = wave_sum((unsigned long long)pc.mats), b = wave_sum((unsigned long long)pc.light_prims);
       if (lane_id() == 0) {
          atomicAdd(&V.stats[4], a);
          atomicAdd(&V.stats[5], b);
-      }
-   }
-}
-
-template <bool CNT>
-__global__ __launch_bounds__(kWfBlock) void k_wf_shadow(WfView V, int level) {
-   __shared__ int stack[kStack * kWfBlock];
-   unsigned* sctl = V.ctl + kCtl * level + 3;
-   const unsigned count = sctl[0];
-   TravCount tc;
-   for (;;) {
-      unsigned base = 0;
-      if (lane_id() == 0) base = atomicAdd(&sctl[1], 64u);
-      base = __shfl(base, 0);
-      if (base >= count) break;
-      const unsigned idx = base + (unsigned)lane_id();
-      if (idx < count) {
-         const float4 o = V.so[idx], d = V.sd[idx];
-         const int slot = __float_as_int(o.w);
-         const SceneView& S = V.jobs[V.ids[slot].x].scene;
-         if (!any_hit<kWfBlock, CNT>(S, mk(o), mk(d), 1e-4f, 1.0f - 1e-4f, stack + threadIdx.x, &tc)) {
-            const float4 c = V.sc[idx];
-            float4 r = V.rad[slot];
-            const V3 L = mk(r) + mk(c);
-            V.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
-         }
-      }
-   }
-   if constexpr (CNT) {
-      unsigned long long a = wave_sum((unsigned long long)tc.nodes), b = wave_sum((unsigned long long)tc.prims);
-      if (lane_id() == 0) {
-         atomicAdd(&V.stats[2], a);
-         atomicAdd(&V.stats[3], b);
       }
    }
 }
@@ -293,7 +316,8 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_accumulate(WfView V, long long 
       const long long l = w - V.base[j];
       if (l < npix || w - npix < w0) {  // first sample of this pixel within the round
          active = true;
-         const int p = (int)(l % npix);
+         int s, p;
+         split_work(l, (unsigned)npix, s, p);
          const long long end = min(w0 + n, V.base[j + 1]);
          float4 a = J.acc[p], c = J.comp[p];
          int added = 0;
@@ -324,13 +348,13 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_accumulate(WfView V, long long 
    if (uniform) {
       const unsigned long long s0 = wave_sum(segs + shadow), s1 = wave_sum(shadow), s2 = wave_sum(prim);
       if (lane_id() == leader) {
-         unsigned long long* ctr = V.jobs[jl].counters;
+         unsigned long long* ctr = V.jobs[jl].counters + 8 * (blockIdx.x % kCounterSlots);
          atomicAdd(&ctr[0], s0);
          atomicAdd(&ctr[1], s1);
          atomicAdd(&ctr[2], s2);
       }
    } else if (active) {
-      unsigned long long* ctr = V.jobs[j].counters;
+      unsigned long long* ctr = V.jobs[j].counters + 8 * (blockIdx.x % kCounterSlots);
       atomicAdd(&ctr[0], segs + shadow);
       atomicAdd(&ctr[1], shadow);
       atomicAdd(&ctr[2], prim);
@@ -361,7 +385,7 @@ __global__ void k_wf_pass_end(JobView J, const float4* prev, int todo) {
       }
    }
    const unsigned long long s = wave_sum((unsigned long long)conv);
-   if ((threadIdx.x & 63) == 0 && s) atomicAdd(&J.counters[3], s);
+   if ((threadIdx.x & 63) == 0 && s) atomicAdd(&J.counters[8 * (blockIdx.x % kCounterSlots) + 3], s);
 }
 
 // ---------------------------------------------------------------------------------
@@ -374,6 +398,11 @@ static rayon_status wave_reserve(rayon_ctx* ctx, int capacity, int levels, int n
       int cus = 0;
       RY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
       W->cus = cus > 0 ? cus : 256;
+      int n = 0;
+      RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_trace<false>, kWfBlock, 0));
+      W->occ_trace = n > 0 ? n : 1;
+      RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_shade<false>, kWfBlock, 0));
+      W->occ_shade = n > 0 ? n : 1;
    }
    if (W->capacity >= capacity && W->levels >= levels && W->max_jobs >= njobs) return RAYON_OK;
    RY_HIP(hipStreamSynchronize(ctx->stream));
@@ -428,6 +457,10 @@ static rayon_status wave_reserve(rayon_ctx* ctx, int capacity, int levels, int n
 void wave_destroy(rayon_ctx* ctx) {
    Wave* W = ctx->wave;
    if (!W) return;
+   for (auto& pe : W->pending) {
+      (void)hipEventDestroy(pe.e0);
+      (void)hipEventDestroy(pe.e1);
+   }
    for (auto& kv : W->graphs) {
       (void)hipGraphExecDestroy(kv.second.exec);
       (void)hipGraphDestroy(kv.second.graph);
@@ -440,31 +473,22 @@ void wave_destroy(rayon_ctx* ctx) {
 
 namespace {
 
-struct StageTimer {
-   Wave* W;
-   hipStream_t st;
-   int stage = 0;
+// Launches one stage kernel. With stage timing on, the launch goes through
+// hipExtLaunchKernelGGL with start/stop events, which the runtime stamps at the kernel's
+// own dispatch and completion; the events are read back later (rayon_ctx_wave_stats),
+// so timing adds no host synchronisation.
+template <typename K, typename... Args>
+void launch_stage(Wave* W, int stage, K kernel, dim3 grid, dim3 block, hipStream_t st, Args... args) {
    hipEvent_t e0 = nullptr, e1 = nullptr;
-   StageTimer(Wave* w, hipStream_t s, int k) : W(w), st(s), stage(k) {
-      if (W->opt_timing) {
-         (void)hipEventCreate(&e0);
-         (void)hipEventCreate(&e1);
-         (void)hipEventRecord(e0, st);
-      }
+   if (W->opt_timing && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+      hipExtLaunchKernelGGL(kernel, grid, block, 0, st, e0, e1, 0, args...);
+      W->pending.push_back({stage, e0, e1});
+   } else {
+      if (e0) (void)hipEventDestroy(e0);
+      hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
    }
-   ~StageTimer() {
-      if (W->opt_timing) {
-         (void)hipEventRecord(e1, st);
-         (void)hipEventSynchronize(e1);
-         float ms = 0;
-         (void)hipEventElapsedTime(&ms, e0, e1);
-         W->stage_ms[stage] += ms;
-         (void)hipEventDestroy(e0);
-         (void)hipEventDestroy(e1);
-      }
-      W->launches[stage] += 1;
-   }
-};
+   W->launches[stage] += 1;
+}
 
 }  // namespace
 
@@ -476,62 +500,47 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
    WfView V = W->v;
    V.njobs = (int)jobs.size();
    V.levels = levels;
-   const int grid = W->cus * W->opt_wg_per_cu;
+   V.chunk = W->opt_chunk;
+   auto grid_of = [&](int occ) { return W->cus * (W->opt_wg_per_cu > 0 ? W->opt_wg_per_cu : occ); };
+   const dim3 g_trace(grid_of(W->occ_trace)), g_shade(grid_of(W->occ_shade)), blk(kWfBlock);
+   auto trace = [&](int ext, int sh) {
+      if (W->opt_count)
+         launch_stage(W, 1, k_wf_trace<true>, g_trace, blk, st, V, ext, sh);
+      else
+         launch_stage(W, 1, k_wf_trace<false>, g_trace, blk, st, V, ext, sh);
+   };
    hipError_t e = hipSuccess;
    for (size_t k = 0; k < jobs.size(); ++k) {
       rayon_job* j = jobs[k];
       if (todo[k] > 0 && (j->params.flags & RAYON_FLAG_ADAPTIVE_REF) && j->prev) {
-         StageTimer t(W, st, 5);
          const int np = j->params.width * j->params.height;
-         hipLaunchKernelGGL(k_wf_pass_begin, dim3((np + 255) / 256), dim3(256), 0, st, j->view, j->prev);
+         launch_stage(W, 5, k_wf_pass_begin, dim3((np + 255) / 256), dim3(256), st, j->view, j->prev);
       }
    }
    for (long long w0 = 0; w0 < total; w0 += W->capacity) {
       const int n = (int)std::min<long long>(W->capacity, total - w0);
       ++W->rounds;
-      {
-         StageTimer t(W, st, 5);
-         if ((e = launch_fill_u32(V.ctl, 0u, (long long)(levels + 1) * kCtl, st)) != hipSuccess) return e;
-      }
-      {
-         StageTimer t(W, st, 0);
-         hipLaunchKernelGGL(k_wf_generate, dim3((n + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, st, V, w0, n);
-      }
+      if ((e = launch_fill_u32(V.ctl, 0u, (long long)(levels + 1) * kCtl, st)) != hipSuccess) return e;
+      W->launches[5] += 1;
+      const dim3 g_round((n + kWfBlock - 1) / kWfBlock);
+      launch_stage(W, 0, k_wf_generate, g_round, blk, st, V, w0, n);
+      trace(0, -1);
       for (int lv = 0; lv < levels; ++lv) {
-         {
-            StageTimer t(W, st, 1);
-            if (W->opt_count)
-               hipLaunchKernelGGL(k_wf_extend<true>, dim3(grid), dim3(kWfBlock), 0, st, V, lv);
-            else
-               hipLaunchKernelGGL(k_wf_extend<false>, dim3(grid), dim3(kWfBlock), 0, st, V, lv);
-         }
-         {
-            StageTimer t(W, st, 2);
-            if (W->opt_count)
-               hipLaunchKernelGGL(k_wf_shade<true>, dim3(grid), dim3(kWfBlock), 0, st, V, lv);
-            else
-               hipLaunchKernelGGL(k_wf_shade<false>, dim3(grid), dim3(kWfBlock), 0, st, V, lv);
-         }
-         {
-            StageTimer t(W, st, 3);
-            if (W->opt_count)
-               hipLaunchKernelGGL(k_wf_shadow<true>, dim3(grid), dim3(kWfBlock), 0, st, V, lv);
-            else
-               hipLaunchKernelGGL(k_wf_shadow<false>, dim3(grid), dim3(kWfBlock), 0, st, V, lv);
-         }
+         if (W->opt_count)
+            launch_stage(W, 2, k_wf_shade<true>, g_shade, blk, st, V, lv);
+         else
+            launch_stage(W, 2, k_wf_shade<false>, g_shade, blk, st, V, lv);
+         trace(lv + 1 < levels ? lv + 1 : -1, lv);
       }
-      {
-         StageTimer t(W, st, 4);
-         hipLaunchKernelGGL(k_wf_accumulate, dim3((n + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, st, V, w0, n);
-      }
+      launch_stage(W, 4, k_wf_accumulate, g_round, blk, st, V, w0, n);
       if ((e = hipGetLastError()) != hipSuccess) return e;
    }
    for (size_t k = 0; k < jobs.size(); ++k) {
       rayon_job* j = jobs[k];
       if (todo[k] <= 0 || !(j->params.flags & (RAYON_FLAG_ADAPTIVE_REF | RAYON_FLAG_ADAPTIVE_STDERR))) continue;
-      StageTimer t(W, st, 5);
       const int np = j->params.width * j->params.height;
-      hipLaunchKernelGGL(k_wf_pass_end, dim3((np + 255) / 256), dim3(256), 0, st, j->view, j->prev, todo[k]);
+      launch_stage(W, 5, k_wf_pass_end, dim3((np + 255) / 256), dim3(256), st, j->view,
+                   (const float4*)j->prev, todo[k]);
    }
    return hipGetLastError();
 }
@@ -625,6 +634,7 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
       key.push_back(levels);
       key.push_back(W->capacity);
       key.push_back(W->opt_wg_per_cu);
+      key.push_back(W->opt_chunk);
       key.push_back(W->opt_count);
       auto it = W->graphs.find(key);
       if (it == W->graphs.end()) {
@@ -674,11 +684,18 @@ rayon_status wave_set_option(rayon_ctx* ctx, int opt, long long value) {
       W->opt_capacity = (int)value;
       break;
    case RAYON_OPT_WAVE_WG_PER_CU:
-      if (value < 1 || value > 32) {
-         set_error("RAYON_OPT_WAVE_WG_PER_CU must be in [1, 32]");
+      if (value < 0 || value > 32) {
+         set_error("RAYON_OPT_WAVE_WG_PER_CU must be in [0, 32] (0: occupancy of each stage)");
          return RAYON_ERR_INVALID;
       }
       W->opt_wg_per_cu = (int)value;
+      break;
+   case RAYON_OPT_WAVE_CHUNK:
+      if (value < 1 || value > 64) {
+         set_error("RAYON_OPT_WAVE_CHUNK must be in [1, 64]");
+         return RAYON_ERR_INVALID;
+      }
+      W->opt_chunk = (int)value;
       break;
    case RAYON_OPT_GRAPH:
       W->opt_graph = value != 0;
@@ -702,6 +719,7 @@ rayon_status wave_get_option(rayon_ctx* ctx, int opt, long long* value) {
    switch (opt) {
    case RAYON_OPT_WAVE_CAPACITY: *value = W->opt_capacity; break;
    case RAYON_OPT_WAVE_WG_PER_CU: *value = W->opt_wg_per_cu; break;
+   case RAYON_OPT_WAVE_CHUNK: *value = W->opt_chunk; break;
    case RAYON_OPT_GRAPH: *value = W->opt_graph; break;
    case RAYON_OPT_STAGE_TIMING: *value = W->opt_timing; break;
    case RAYON_OPT_COUNT_TRAVERSAL: *value = W->opt_count; break;
@@ -717,6 +735,13 @@ rayon_status wave_stats(rayon_ctx* ctx, rayon_wave_stats* out, int reset) {
    Wave* W = ctx->wave;
    if (!W) return RAYON_OK;
    RY_HIP(hipStreamSynchronize(ctx->stream));
+   for (auto& pe : W->pending) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, pe.e0, pe.e1) == hipSuccess) W->stage_ms[pe.stage] += ms;
+      (void)hipEventDestroy(pe.e0);
+      (void)hipEventDestroy(pe.e1);
+   }
+   W->pending.clear();
    for (int k = 0; k < 6; ++k) {
       out->stage_ms[k] = W->stage_ms[k];
       out->launches[k] = W->launches[k];
