@@ -47,11 +47,12 @@ struct WfView {
    float4* rad;   // radiance
    uint4* ids;    // job, pixel, sample, flags
    float4* hit;   // t, primitive slot (bits), u, v
-   float4* so;    // shadow queue: origin, path slot (bits)
-   float4* sd;    // shadow queue: direction
-   float4* sc;    // shadow queue: contribution
-   int* q0;
+   float4* so;    // shadow ray of a path slot: origin
+   float4* sd;    // direction
+   float4* sc;    // contribution if unoccluded
+   int* q0;       // extension queues (path slots), alternating by level
    int* q1;
+   int* sq;       // shadow queue (path slots)
    unsigned* ctl;           // per level (stride kCtl): queue count, extend head, shade head,
                             // shadow count, shadow head
    const JobView* jobs;
@@ -162,7 +163,9 @@ __device__ __forceinline__ int find_job(const WfView& V, long long w) {
 // ---------------------------------------------------------------------------------
 // stages
 // ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(kWfBlock) void k_wf_generate(WfView V, long long w0, int n) {
+// dense: no job of the pass can skip a pixel (no adaptive test), so every slot is live
+// and the level-0 queue is the identity.
+__global__ __launch_bounds__(kWfBlock) void k_wf_generate(WfView V, long long w0, int n, int dense) {
    const int i = blockIdx.x * kWfBlock + threadIdx.x;
    bool live = false;
    if (i < n) {
@@ -187,6 +190,11 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_generate(WfView V, long long w0
          live = true;
       }
       V.ids[i] = make_uint4((uint32_t)j, (uint32_t)p, (uint32_t)sample, flags);
+   }
+   if (dense) {
+      if (i < n) V.q0[i] = i;
+      if (i == 0) V.ctl[0] = (unsigned)n;
+      return;
    }
    const int pos = wave_push(&V.ctl[0], live);
    if (live) V.q0[pos] = i;
@@ -221,11 +229,11 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_trace(WfView V, int ext_level, 
       const unsigned count = sctl[0];
       drain(&sctl[1], count, V.chunk, [&](unsigned idx) {
          if (idx < count) {
-            const float4 o = V.so[idx], d = V.sd[idx];
-            const int slot = __float_as_int(o.w);
+            const int slot = V.sq[idx];
+            const float4 o = V.so[slot], d = V.sd[slot];
             const SceneView& S = V.jobs[V.ids[slot].x].scene;
             if (!any_hit<kWfBlock, CNT>(S, mk(o), mk(d), 1e-4f, 1.0f - 1e-4f, stack + threadIdx.x, &ts)) {
-               const float4 c = V.sc[idx];
+               const float4 c = V.sc[slot];
                const float4 r = V.rad[slot];
                const V3 L = mk(r) + mk(c);
                V.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
@@ -245,56 +253,90 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_trace(WfView V, int ext_level, 
    }
 }
 
+constexpr int kMaxChunk = 16;
+
+// Appends a wave's staged queue entries with one atomic and coalesced stores.
+__device__ __forceinline__ void flush(const int* buf, int n, unsigned* counter, int* dst) {
+   if (n == 0) return;
+   unsigned b = 0;
+   if (lane_id() == 0) b = atomicAdd(counter, (unsigned)n);
+   b = __shfl(b, 0);
+   for (int i = lane_id(); i < n; i += 64) dst[b + i] = buf[i];
+}
+
+// Shade stage: one bounce (shade_step) per queued path. The surviving paths and the
+// shadow requests of each wave's claimed chunk are staged in LDS and appended to the
+// next-level and shadow queues with one atomic each per chunk.
 template <bool CNT>
 __global__ __launch_bounds__(kWfBlock) void k_wf_shade(WfView V, int level) {
+   __shared__ int stage_next[kWfBlock / 64][64 * kMaxChunk];
+   __shared__ int stage_sh[kWfBlock / 64][64 * kMaxChunk];
+   const int w = threadIdx.x >> 6, lane = lane_id();
    unsigned* ctl = V.ctl + kCtl * level;
    const unsigned count = ctl[0];
    const int* q = (level & 1) ? V.q1 : V.q0;
    int* qn = (level & 1) ? V.q0 : V.q1;
    unsigned* ctl_next = V.ctl + kCtl * (level + 1);
    unsigned* sctl = ctl + 3;
+   const int chunk = min(V.chunk, kMaxChunk);
+   const unsigned long long lt = (1ull << lane) - 1ull;
    PathCounts pc;
-   drain(&ctl[2], count, V.chunk, [&](unsigned idx) {
-      bool cont = false, shadow = false;
-      int slot = -1;
-      ShadowReq sh;
-      sh.valid = false;
-      if (idx < count) {
-         slot = q[idx];
-         uint4 id = V.ids[slot];
-         const JobView& J = V.jobs[id.x];
-         const float4 o = V.o[slot], d = V.d[slot], t = V.thr[slot], r = V.rad[slot], hf = V.hit[slot];
-         PathState ps{mk(o), mk(d), mk(t), mk(r), (id.w & kPrevLambert) != 0};
-         Rng R{__float_as_uint(o.w), false};
-         HitMin h;
-         h.t = hf.x;
-         h.slot = __float_as_int(hf.y);
-         h.u = hf.z;
-         h.v = hf.w;
-         cont = shade_step<CNT>(J, h, level, ps, R, sh, pc);
-         cont = cont && level + 1 < J.max_depth;
-         shadow = sh.valid;
-         id.w = (id.w & ~kPrevLambert) | (ps.prev_lambert ? kPrevLambert : 0u);
-         id.w += 1u << 8;
-         if (shadow) id.w += 1u << 20;
-         V.ids[slot] = id;
-         V.o[slot] = make_float4(ps.o.x, ps.o.y, ps.o.z, __uint_as_float(R.s));
-         V.d[slot] = make_float4(ps.d.x, ps.d.y, ps.d.z, 0.0f);
-         V.thr[slot] = make_float4(ps.T.x, ps.T.y, ps.T.z, 0.0f);
-         V.rad[slot] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
+   for (;;) {
+      unsigned base = 0;
+      if (lane == 0) base = atomicAdd(&ctl[2], 64u * (unsigned)chunk);
+      base = __shfl(base, 0);
+      if (base >= count) break;
+      int n_next = 0, n_sh = 0;
+      for (int k = 0; k < chunk; ++k) {
+         const unsigned bk = base + 64u * (unsigned)k;
+         if (bk >= count) break;
+         const unsigned idx = bk + (unsigned)lane;
+         bool cont = false, shadow = false;
+         int slot = -1;
+         if (idx < count) {
+            slot = q[idx];
+            uint4 id = V.ids[slot];
+            const JobView& J = V.jobs[id.x];
+            const float4 o = V.o[slot], d = V.d[slot], t = V.thr[slot], r = V.rad[slot], hf = V.hit[slot];
+            PathState ps{mk(o), mk(d), mk(t), mk(r), (id.w & kPrevLambert) != 0};
+            Rng R{__float_as_uint(o.w), false};
+            HitMin h;
+            h.t = hf.x;
+            h.slot = __float_as_int(hf.y);
+            h.u = hf.z;
+            h.v = hf.w;
+            ShadowReq sh;
+            cont = shade_step<CNT>(J, h, level, ps, R, sh, pc);
+            cont = cont && level + 1 < J.max_depth;
+            shadow = sh.valid;
+            id.w = (id.w & ~kPrevLambert) | (ps.prev_lambert ? kPrevLambert : 0u);
+            id.w += 1u << 8;
+            if (shadow) {
+               id.w += 1u << 20;
+               V.so[slot] = make_float4(sh.o.x, sh.o.y, sh.o.z, 0.0f);
+               V.sd[slot] = make_float4(sh.d.x, sh.d.y, sh.d.z, 0.0f);
+               V.sc[slot] = make_float4(sh.contrib.x, sh.contrib.y, sh.contrib.z, 0.0f);
+            }
+            V.ids[slot] = id;
+            V.o[slot] = make_float4(ps.o.x, ps.o.y, ps.o.z, __uint_as_float(R.s));
+            V.d[slot] = make_float4(ps.d.x, ps.d.y, ps.d.z, 0.0f);
+            V.thr[slot] = make_float4(ps.T.x, ps.T.y, ps.T.z, 0.0f);
+            V.rad[slot] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
+         }
+         const unsigned long long mc = __ballot(cont), ms = __ballot(shadow);
+         if (cont) stage_next[w][n_next + __popcll(mc & lt)] = slot;
+         if (shadow) stage_sh[w][n_sh + __popcll(ms & lt)] = slot;
+         n_next += __popcll(mc);
+         n_sh += __popcll(ms);
       }
-      const int pn = wave_push(&ctl_next[0], cont);
-      if (cont) qn[pn] = slot;
-      const int ps_pos = wave_push(sctl, shadow);
-      if (shadow) {
-         V.so[ps_pos] = make_float4(sh.o.x, sh.o.y, sh.o.z, __int_as_float(slot));
-         V.sd[ps_pos] = make_float4(sh.d.x, sh.d.y, sh.d.z, 0.0f);
-         V.sc[ps_pos] = make_float4(sh.contrib.x, sh.contrib.y, sh.contrib.z, 0.0f);
-      }
-   });
+      __builtin_amdgcn_wave_barrier();
+      flush(stage_next[w], n_next, &ctl_next[0], qn);
+      flush(stage_sh[w], n_sh, sctl, V.sq);
+      __builtin_amdgcn_wave_barrier();
+   }
    if constexpr (CNT) {
       unsigned long long a = wave_sum((unsigned long long)pc.mats), b = wave_sum((unsigned long long)pc.light_prims);
-      if (lane_id() == 0) {
+      if (lane == 0) {
          atomicAdd(&V.stats[4], a);
          atomicAdd(&V.stats[5], b);
       }
@@ -436,6 +478,8 @@ static rayon_status wave_reserve(rayon_ctx* ctx, int capacity, int levels, int n
    W->v.q0 = (int*)p;
    if ((st = alloc((size_t)cap * sizeof(int), &p)) != RAYON_OK) return st;
    W->v.q1 = (int*)p;
+   if ((st = alloc((size_t)cap * sizeof(int), &p)) != RAYON_OK) return st;
+   W->v.sq = (int*)p;
    if ((st = alloc((size_t)(lv + 1) * kCtl * sizeof(unsigned), &p)) != RAYON_OK) return st;
    W->v.ctl = (unsigned*)p;
    if ((st = alloc((size_t)mj * sizeof(JobView), &p)) != RAYON_OK) return st;
@@ -510,6 +554,9 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
          launch_stage(W, 1, k_wf_trace<false>, g_trace, blk, st, V, ext, sh);
    };
    hipError_t e = hipSuccess;
+   int dense = 1;
+   for (rayon_job* j : jobs)
+      if (j->params.flags & (RAYON_FLAG_ADAPTIVE_REF | RAYON_FLAG_ADAPTIVE_STDERR)) dense = 0;
    for (size_t k = 0; k < jobs.size(); ++k) {
       rayon_job* j = jobs[k];
       if (todo[k] > 0 && (j->params.flags & RAYON_FLAG_ADAPTIVE_REF) && j->prev) {
@@ -523,7 +570,7 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
       if ((e = launch_fill_u32(V.ctl, 0u, (long long)(levels + 1) * kCtl, st)) != hipSuccess) return e;
       W->launches[5] += 1;
       const dim3 g_round((n + kWfBlock - 1) / kWfBlock);
-      launch_stage(W, 0, k_wf_generate, g_round, blk, st, V, w0, n);
+      launch_stage(W, 0, k_wf_generate, g_round, blk, st, V, w0, n, dense);
       trace(0, -1);
       for (int lv = 0; lv < levels; ++lv) {
          if (W->opt_count)
@@ -691,8 +738,8 @@ rayon_status wave_set_option(rayon_ctx* ctx, int opt, long long value) {
       W->opt_wg_per_cu = (int)value;
       break;
    case RAYON_OPT_WAVE_CHUNK:
-      if (value < 1 || value > 64) {
-         set_error("RAYON_OPT_WAVE_CHUNK must be in [1, 64]");
+      if (value < 1 || value > kMaxChunk) {
+         set_error("RAYON_OPT_WAVE_CHUNK must be in [1, 16]");
          return RAYON_ERR_INVALID;
       }
       W->opt_chunk = (int)value;
