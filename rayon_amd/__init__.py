@@ -13,7 +13,7 @@ import numpy as np
 
 from .abi import (  # noqa: F401  (re-exported)
     Camera, GenParams, Geometry, JobParams, JobStats, Material, Mesh, SceneDesc, SceneInfo, SdfPrim, WaveStats,
-    MAT, GEOM, SDF, FLAG, GEN, OPT, STATUS, declare,
+    MAT, GEOM, SDF, FLAG, GEN, OPT, BVH, STATUS, declare,
 )
 
 LIB_PATH = Path(__file__).resolve().parent / "librayon_hip.so"

@@ -22,6 +22,7 @@ GEOM = _NS(SPHERE=0, RECTANGLE=1, DISPLACED_SPHERE=3, TRIANGLE=4)
 SDF = _NS(SPHERE=0, ROUNDED_BOX=1, TORUS=2)
 FLAG = _NS(REF_QUIRKS=1, REF_RNG=2, NEE=4, ADAPTIVE_REF=8, ADAPTIVE_STDERR=16, DOF=32, COUNT_TRAVERSAL=64)
 GEN = _NS(BOOK_SPHERES=0, CORNELL_ICOSPHERE=1, SDF_TERRAIN=2)
+BVH = _NS(DEFAULT=0, QUANTIZED=1)
 OPT = _NS(WAVE_CAPACITY=0, WAVE_WG_PER_CU=1, GRAPH=2, STAGE_TIMING=3, COUNT_TRAVERSAL=4, WAVE_CHUNK=5)
 
 

@@ -222,7 +222,6 @@ rayon_status rayon_ctx_sync(rayon_ctx* c) {
 }
 
 rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* d, uint32_t bvh_flags, rayon_scene** out) {
-   (void)bvh_flags;
    if (!ctx || !out) {
       set_error("rayon_scene_create: NULL argument");
       return RAYON_ERR_INVALID;
@@ -232,7 +231,7 @@ rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* d, uint3
    if (st != RAYON_OK) return st;
    auto t0 = std::chrono::steady_clock::now();
    BuildResult br;
-   if (!build_bvh(prims, br)) {
+   if (!build_bvh(prims, br, (bvh_flags & RAYON_BVH_QUANTIZED) != 0)) {
       set_error("BVH depth %d exceeds the traversal stack (%d)", br.max_depth, kMaxDepth);
       return RAYON_ERR_INVALID;
    }
@@ -318,7 +317,9 @@ rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* d, uint3
    int* dl = nullptr;
    float* da = nullptr;
    SdfRec* ds = nullptr;
+   WNode* dw = nullptr;
    if ((st = upload(sc->bufs, recs, &dp)) != RAYON_OK || (st = upload(sc->bufs, normals, &dn)) != RAYON_OK ||
+       (st = upload(sc->bufs, br.wnodes, &dw)) != RAYON_OK ||
        (st = upload(sc->bufs, br.nodes, &dnodes)) != RAYON_OK || (st = upload(sc->bufs, mats, &dm)) != RAYON_OK ||
        (st = upload(sc->bufs, lslots, &dl)) != RAYON_OK || (st = upload(sc->bufs, larea, &da)) != RAYON_OK ||
        (st = upload(sc->bufs, sdf, &ds)) != RAYON_OK) {
@@ -334,6 +335,7 @@ rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* d, uint3
    V.light_slots = dl;
    V.light_area = da;
    V.sdf = ds;
+   V.wnodes = br.wnodes.empty() ? nullptr : dw;
    V.num_lights = (int)lslots.size();
    V.num_sdf = (int)sdf.size();
    V.root = br.root;
@@ -344,7 +346,7 @@ rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* d, uint3
    rayon_scene_info& I = sc->info;
    I.num_prims = (int64_t)n;
    I.num_nodes = (int64_t)br.nodes.size();
-   I.num_qnodes = 0;
+   I.num_qnodes = (int64_t)br.wnodes.size();
    I.max_depth = br.max_depth;
    I.num_lights = V.num_lights;
    for (auto& b : sc->bufs) I.device_bytes += (int64_t)b.bytes;

@@ -232,14 +232,160 @@ int emit(const std::vector<BNode>& bn, int b, std::vector<Node2>& out) {
    return pi;
 }
 
+
+// ---- 8-wide quantized tree (compressed wide BVH of Ylitie, Karras and Laine, HPG 2017,
+// collapsed greedily from the binary tree: a wide node's children are found by
+// repeatedly opening its largest-area internal child until there are eight).
+struct WideBuilder {
+   std::vector<BNode>& bn;
+   const std::vector<int32_t>& old_order;
+   std::vector<WNode> out;
+   std::vector<int32_t> order;
+   int max_depth = 0;
+
+   WideBuilder(std::vector<BNode>& b, const std::vector<int32_t>& o) : bn(b), old_order(o) {}
+
+   static int exponent_for(double ext) {
+      // smallest e with 255 * 2^e >= ext, within the normal float range
+      int e = -126;
+      if (ext > 0) {
+         e = (int)std::ceil(std::log2(ext / 255.0));
+         while (std::ldexp(255.0, e) < ext) ++e;
+         while (e > -126 && std::ldexp(255.0, e - 1) >= ext) --e;
+      }
+      return std::min(std::max(e, -126), 127);
+   }
+
+   void emit(int wi, int b, int depth) {
+      max_depth = std::max(max_depth, depth);
+      int kids[8];
+      int nk = 0;
+      if (bn[b].count) {
+         kids[nk++] = b;
+      } else {
+         kids[nk++] = bn[b].left;
+         kids[nk++] = bn[b].right;
+      }
+      while (nk < 8) {
+         int best = -1;
+         double best_area = -1;
+         for (int k = 0; k < nk; ++k) {
+            const BNode& c = bn[kids[k]];
+            if (c.count) continue;
+            const double a = c.box.area();
+            if (a > best_area) {
+               best_area = a;
+               best = k;
+            }
+         }
+         if (best < 0) break;
+         const int c = kids[best];
+         kids[best] = bn[c].left;
+         kids[nk++] = bn[c].right;
+      }
+      Box nb;
+      for (int k = 0; k < nk; ++k) nb.grow(bn[kids[k]].box);
+      // octant slot assignment: slot bit k set <=> centroid on the negative side of axis k
+      double pc[3], d[8][3];
+      for (int a = 0; a < 3; ++a) pc[a] = 0.5 * ((double)nb.lo[a] + nb.hi[a]);
+      for (int k = 0; k < nk; ++k)
+         for (int a = 0; a < 3; ++a) d[k][a] = 0.5 * ((double)bn[kids[k]].box.lo[a] + bn[kids[k]].box.hi[a]) - pc[a];
+      struct Cand {
+         double score;
+         int kid, slot;
+      };
+      Cand cand[64];
+      int nc = 0;
+      for (int k = 0; k < nk; ++k)
+         for (int sl = 0; sl < 8; ++sl) {
+            double sc = 0;
+            for (int a = 0; a < 3; ++a) sc += ((sl >> a) & 1) ? -d[k][a] : d[k][a];
+            cand[nc++] = {sc, k, sl};
+         }
+      std::sort(cand, cand + nc, [](const Cand& x, const Cand& y) {
+         return x.score > y.score || (x.score == y.score && (x.kid < y.kid || (x.kid == y.kid && x.slot < y.slot)));
+      });
+      int slot_kid[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+      bool placed[8] = {false, false, false, false, false, false, false, false};
+      for (int c = 0; c < nc; ++c)
+         if (!placed[cand[c].kid] && slot_kid[cand[c].slot] < 0) {
+            placed[cand[c].kid] = true;
+            slot_kid[cand[c].slot] = kids[cand[c].kid];
+         }
+      WNode w;
+      std::memset(&w, 0, sizeof(w));
+      w.px = nb.lo[0];
+      w.py = nb.lo[1];
+      w.pz = nb.lo[2];
+      int e[3];
+      double scale[3];
+      for (int a = 0; a < 3; ++a) {
+         e[a] = exponent_for((double)nb.hi[a] - (double)nb.lo[a]);
+         scale[a] = std::ldexp(1.0, e[a]);
+      }
+      uint8_t q[6][8];
+      uint8_t meta[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      uint32_t imask = 0;
+      int n_int = 0;
+      for (int sl = 0; sl < 8; ++sl) {
+         const int c = slot_kid[sl];
+         if (c < 0) {  // empty slot: an inverted box no ray can hit
+            for (int a = 0; a < 3; ++a) {
+               q[a][sl] = 255;
+               q[3 + a][sl] = 0;
+            }
+            continue;
+         }
+         const BNode& cn = bn[c];
+         for (int a = 0; a < 3; ++a) {
+            const double p = (a == 0) ? w.px : (a == 1 ? w.py : w.pz);
+            double lo = std::floor(((double)cn.box.lo[a] - p) / scale[a]);
+            double hi = std::ceil(((double)cn.box.hi[a] - p) / scale[a]);
+            q[a][sl] = (uint8_t)std::min(std::max(lo, 0.0), 255.0);
+            q[3 + a][sl] = (uint8_t)std::min(std::max(hi, 0.0), 255.0);
+         }
+         if (!cn.count) {
+            imask |= 1u << sl;
+            ++n_int;
+         }
+      }
+      w.e_imask = (uint32_t)(e[0] + 127) | ((uint32_t)(e[1] + 127) << 8) | ((uint32_t)(e[2] + 127) << 16) |
+                  (imask << 24);
+      w.child_base = (uint32_t)out.size();
+      out.resize(out.size() + n_int);
+      w.prim_base = (uint32_t)order.size();
+      for (int sl = 0; sl < 8; ++sl) {
+         const int c = slot_kid[sl];
+         if (c < 0 || !bn[c].count) continue;
+         BNode& cn = bn[c];
+         const int off = (int)(order.size() - w.prim_base);
+         for (int i = cn.first; i < cn.first + cn.count; ++i) order.push_back(old_order[i]);
+         meta[sl] = (uint8_t)(off | ((cn.count - 1) << 5));
+         cn.first = (int)(w.prim_base + off);  // the binary tree follows the new order
+      }
+      std::memcpy(w.meta, meta, 8);
+      std::memcpy(w.q, q, 48);
+      out[wi] = w;
+      int k = 0;
+      for (int sl = 0; sl < 8; ++sl) {
+         const int c = slot_kid[sl];
+         if (c < 0 || bn[c].count) continue;
+         emit((int)w.child_base + k, c, depth + 1);
+         ++k;
+      }
+   }
+};
+
 }  // namespace
 
 void compute_bounds(HostPrim& p) { prim_bounds(p); }
 
-bool build_bvh(const std::vector<HostPrim>& prims, BuildResult& out) {
+bool build_bvh(const std::vector<HostPrim>& prims, BuildResult& out, bool wide) {
    const int n = (int)prims.size();
    out.order.clear();
    out.nodes.clear();
+   out.wnodes.clear();
+   out.wide_depth = 0;
    if (n == 0) {
       out.root = kSentinel;
       out.max_depth = 0;
@@ -258,6 +404,22 @@ bool build_bvh(const std::vector<HostPrim>& prims, BuildResult& out) {
    out.max_depth = b.max_depth.load();
    if (out.max_depth > kMaxDepth) return false;
    out.order = b.idx;
+   if (wide) {
+      // collapse into the 8-wide tree; this also reorders the primitives (each binary
+      // leaf stays one contiguous block) and rewrites the binary leaves to match
+      WideBuilder wb(b.nodes, b.idx);
+      wb.out.resize(1);
+      wb.order.reserve(n);
+      wb.emit(0, root, 0);
+      if (wb.max_depth < kMaxWideDepth && (int)wb.order.size() == n) {
+         out.wnodes = std::move(wb.out);
+         out.order = std::move(wb.order);
+         out.wide_depth = wb.max_depth;
+      } else {
+         // too deep for the device stack: keep the binary layout (restore leaf order)
+         return build_bvh(prims, out, false);
+      }
+   }
    const BNode& r = b.nodes[root];
    if (r.count) {
       out.root = pack_leaf(r.first, r.count);
@@ -461,6 +623,56 @@ bool check_subtree(const BuildResult& br, const std::vector<HostPrim>& prims, in
    return true;
 }
 
+// Walks the 8-wide tree: every decoded (quantized) child box must contain the exact
+// bounds of everything below it; returns those exact bounds through lo/hi.
+bool check_wide(const BuildResult& br, const std::vector<HostPrim>& prims, uint32_t ni, std::vector<int>& seen,
+                int depth, int& max_depth, double lo[3], double hi[3]) {
+   if (ni >= br.wnodes.size() || depth >= kMaxWideDepth) return false;
+   max_depth = std::max(max_depth, depth);
+   const WNode& w = br.wnodes[ni];
+   const uint8_t* q = reinterpret_cast<const uint8_t*>(w.q);
+   const uint8_t* meta = reinterpret_cast<const uint8_t*>(w.meta);
+   const double p[3] = {w.px, w.py, w.pz};
+   double sc[3];
+   for (int a = 0; a < 3; ++a) sc[a] = std::ldexp(1.0, (int)((w.e_imask >> (8 * a)) & 0xFFu) - 127);
+   const uint32_t imask = w.e_imask >> 24;
+   for (int a = 0; a < 3; ++a) {
+      lo[a] = INFINITY;
+      hi[a] = -INFINITY;
+   }
+   int k = 0;
+   for (int sl = 0; sl < 8; ++sl) {
+      if (q[sl] > q[24 + sl]) continue;  // empty slot
+      double clo[3], chi[3];
+      for (int a = 0; a < 3; ++a) {
+         clo[a] = p[a] + q[8 * a + sl] * sc[a];
+         chi[a] = p[a] + q[24 + 8 * a + sl] * sc[a];
+      }
+      double elo[3] = {INFINITY, INFINITY, INFINITY}, ehi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      if ((imask >> sl) & 1u) {
+         if (!check_wide(br, prims, w.child_base + k, seen, depth + 1, max_depth, elo, ehi)) return false;
+         ++k;
+      } else {
+         const int first = (int)w.prim_base + (meta[sl] & 31);
+         const int cnt = (meta[sl] >> 5) + 1;
+         for (int s = first; s < first + cnt; ++s) {
+            if (s < 0 || s >= (int)br.order.size() || seen[s]++) return false;
+            const HostPrim& hp = prims[br.order[s]];
+            for (int a = 0; a < 3; ++a) {
+               elo[a] = std::min(elo[a], (double)hp.lo[a]);
+               ehi[a] = std::max(ehi[a], (double)hp.hi[a]);
+            }
+         }
+      }
+      for (int a = 0; a < 3; ++a) {
+         if (elo[a] < clo[a] || ehi[a] > chi[a]) return false;
+         lo[a] = std::min(lo[a], elo[a]);
+         hi[a] = std::max(hi[a], ehi[a]);
+      }
+   }
+   return true;
+}
+
 }  // namespace
 
 }  // namespace ry
@@ -471,9 +683,20 @@ extern "C" rayon_status rayon_bvh_check(const rayon_scene_desc* d, rayon_scene_i
    rayon_status st = gather_prims(d, prims);
    if (st != RAYON_OK) return st;
    BuildResult br;
-   if (!build_bvh(prims, br)) {
+   if (!build_bvh(prims, br, true)) {
       set_error("BVH depth %d exceeds %d", br.max_depth, kMaxDepth);
       return RAYON_ERR_INVALID;
+   }
+   if (!br.wnodes.empty()) {
+      std::vector<int> wseen(prims.size(), 0);
+      int wdepth = 0;
+      double lo[3], hi[3];
+      bool wok = check_wide(br, prims, 0, wseen, 0, wdepth, lo, hi);
+      for (int v : wseen) wok = wok && v == 1;
+      if (!wok) {
+         set_error("wide BVH invariant violated (quantized containment, leaf size or slot coverage)");
+         return RAYON_ERR_INVALID;
+      }
    }
    std::vector<int> seen(prims.size(), 0);
    int depth = 0;
@@ -495,6 +718,7 @@ extern "C" rayon_status rayon_bvh_check(const rayon_scene_desc* d, rayon_scene_i
       *out = rayon_scene_info{};
       out->num_prims = (int64_t)prims.size();
       out->num_nodes = (int64_t)br.nodes.size();
+      out->num_qnodes = (int64_t)br.wnodes.size();
       out->max_depth = br.max_depth;
    }
    return RAYON_OK;

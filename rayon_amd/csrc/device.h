@@ -272,8 +272,8 @@ struct TravCount {
 };
 
 template <int STRIDE, bool CNT = false>
-__device__ inline HitMin closest_hit(const SceneView& S, V3 o, V3 d, float tmin, float tmax, int* stack,
-                                     TravCount* tc = nullptr) {
+__device__ inline HitMin closest_hit_bin(const SceneView& S, V3 o, V3 d, float tmin, float tmax, int* stack,
+                                         TravCount* tc = nullptr) {
    HitMin best;
    best.t = tmax;
    best.slot = -1;
@@ -345,8 +345,8 @@ __device__ inline HitMin closest_hit(const SceneView& S, V3 o, V3 d, float tmin,
 
 // Any hit in [tmin, tmax] by a primitive that is not an emitter (shadow rays).
 template <int STRIDE, bool CNT = false>
-__device__ inline bool any_hit(const SceneView& S, V3 o, V3 d, float tmin, float tmax, int* stack,
-                               TravCount* tc = nullptr) {
+__device__ inline bool any_hit_bin(const SceneView& S, V3 o, V3 d, float tmin, float tmax, int* stack,
+                                   TravCount* tc = nullptr) {
    int cur = S.root;
    if (cur == kSentinel) return false;
    const RayInv r = make_rayinv(o, d);
@@ -395,6 +395,227 @@ __device__ inline bool any_hit(const SceneView& S, V3 o, V3 d, float tmin, float
       cur = sp > 0 ? stack[(--sp) * STRIDE] : kSentinel;
       if (cur == kSentinel) return false;
    }
+}
+
+// Tests one primitive for the closest hit (shared by both tree layouts).
+template <bool CNT>
+__device__ __forceinline__ void test_closest(const SceneView& S, int i, V3 o, V3 d, float tmin, HitMin& best,
+                                             TravCount* tc) {
+   const PrimRec p = S.prims[i];
+   if constexpr (CNT) ++tc->prims;
+   const uint32_t info = __float_as_uint(p.a.w);
+   const uint32_t kind = info_kind(info);
+   float t = 0.0f, u = 0.0f, v = 0.0f;
+   bool hit;
+   if (kind == K_TRI)
+      hit = isect_tri(p, o, d, tmin, best.t, t, u, v);
+   else if (kind == K_SPHERE || kind == K_DISPLACED)
+      hit = isect_sphere(mk(p.a), p.b.x, o, d, tmin, best.t, t);
+   else if (kind == K_RECT)
+      hit = isect_rect(mk(p.a), mk(p.b), mk(p.c), o, d, tmin, best.t, t);
+   else
+      hit = isect_sdf(S, p, o, d, tmin, best.t, t);
+   if (hit) {
+      const int orig = __float_as_int(p.b.w);
+      if (t < best.t || orig > best.orig) {
+         best.t = t;
+         best.slot = i;
+         best.orig = orig;
+         best.u = u;
+         best.v = v;
+      }
+   }
+}
+
+template <bool CNT>
+__device__ __forceinline__ bool test_any(const SceneView& S, int i, V3 o, V3 d, float tmin, float tmax,
+                                         TravCount* tc) {
+   const PrimRec p = S.prims[i];
+   if constexpr (CNT) ++tc->prims;
+   const uint32_t info = __float_as_uint(p.a.w);
+   if (info & F_EMITTER) return false;
+   const uint32_t kind = info_kind(info);
+   float t, u, v;
+   if (kind == K_TRI) return isect_tri(p, o, d, tmin, tmax, t, u, v);
+   if (kind == K_SPHERE || kind == K_DISPLACED) return isect_sphere(mk(p.a), p.b.x, o, d, tmin, tmax, t);
+   if (kind == K_RECT) return isect_rect(mk(p.a), mk(p.b), mk(p.c), o, d, tmin, tmax, t);
+   return isect_sdf(S, p, o, d, tmin, tmax, t);
+}
+
+__device__ __forceinline__ uint32_t byte_of(const uint32_t* w, int i) { return (w[i >> 2] >> (8 * (i & 3))) & 0xFFu; }
+
+// Tests the eight quantized child boxes of a wide node. Returns the hit internal
+// children as a mask in visiting order (bit i = slot i ^ perm) and the hit leaf children
+// as a slot mask. Each child plane is q * (2^e / d) + (p - o) / d; the exit distance is
+// widened by 2^-22 relative plus 2^-21 of the largest plane offset so FP32 rounding never
+// rejects a box the exact test accepts.
+__device__ __forceinline__ void wide_children(const WNode& n, V3 o, V3 inv, uint32_t perm, float tmin, float tmax,
+                                              uint32_t& hit_int, uint32_t& hit_leaf) {
+   const uint32_t em = n.e_imask;
+   const float sx = __uint_as_float((em & 0xFFu) << 23);
+   const float sy = __uint_as_float(((em >> 8) & 0xFFu) << 23);
+   const float sz = __uint_as_float(((em >> 16) & 0xFFu) << 23);
+   const uint32_t imask = em >> 24;
+   const float ax = sx * inv.x, ay = sy * inv.y, az = sz * inv.z;
+   const float bx = (n.px - o.x) * inv.x, by = (n.py - o.y) * inv.y, bz = (n.pz - o.z) * inv.z;
+   const float slack = 4.76837158e-7f * fmaxf(fmaxf(fabsf(bx), fabsf(by)), fabsf(bz));
+   hit_int = 0;
+   hit_leaf = 0;
+#pragma unroll
+   for (int sl = 0; sl < 8; ++sl) {
+      const float lx = fmaf((float)byte_of(n.q, sl), ax, bx), hx = fmaf((float)byte_of(n.q, 24 + sl), ax, bx);
+      const float ly = fmaf((float)byte_of(n.q, 8 + sl), ay, by), hy = fmaf((float)byte_of(n.q, 32 + sl), ay, by);
+      const float lz = fmaf((float)byte_of(n.q, 16 + sl), az, bz), hz = fmaf((float)byte_of(n.q, 40 + sl), az, bz);
+      const float tn = fmaxf(fmaxf(fminf(lx, hx), fminf(ly, hy)), fmaxf(fminf(lz, hz), tmin));
+      const float tf = fminf(fminf(fmaxf(lx, hx), fmaxf(ly, hy)), fminf(fmaxf(lz, hz), tmax));
+      // an empty slot stores lo = 255 > hi = 0 on every axis: its box is inverted
+      const bool valid = byte_of(n.q, sl) <= byte_of(n.q, 24 + sl);
+      if (valid && tn <= tf * 1.00000024f + slack) {
+         if ((imask >> sl) & 1u)
+            hit_int |= 1u << (sl ^ perm);
+         else
+            hit_leaf |= 1u << sl;
+      }
+   }
+}
+
+__device__ __forceinline__ WNode load_wnode(const WNode* nodes, uint32_t i) {
+   const float4* src = reinterpret_cast<const float4*>(nodes + i);
+   WNode n;
+   float4* dst = reinterpret_cast<float4*>(&n);
+#pragma unroll
+   for (int k = 0; k < 5; ++k) dst[k] = src[k];
+   return n;
+}
+
+// Closest hit over the 8-wide quantized tree. The stack holds one entry per partly
+// visited node: its first internal child and the remaining hit mask (with the node's
+// internal-child mask, to locate a child among its internal siblings).
+template <int STRIDE, bool CNT>
+__device__ inline HitMin closest_hit_wide(const SceneView& S, V3 o, V3 d, float tmin, float tmax, int* stack,
+                                          TravCount* tc) {
+   HitMin best;
+   best.t = tmax;
+   best.slot = -1;
+   best.orig = -1;
+   best.u = best.v = 0.0f;
+   const V3 inv = mk(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
+   const uint32_t perm = 7u ^ ((d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u));
+   int sp = 0;
+   uint32_t node = 0;
+   for (;;) {
+      const WNode n = load_wnode(S.wnodes, node);
+      if constexpr (CNT) ++tc->nodes;
+      uint32_t hit_int, hit_leaf;
+      wide_children(n, o, inv, perm, tmin, best.t, hit_int, hit_leaf);
+      while (hit_leaf) {
+         const int sl = __ffs(hit_leaf) - 1;
+         hit_leaf &= hit_leaf - 1u;
+         const uint32_t m = ((sl < 4 ? n.meta[0] : n.meta[1]) >> (8 * (sl & 3))) & 0xFFu;
+         const int first = (int)n.prim_base + (int)(m & 31u);
+         const int cnt = (int)(m >> 5) + 1;
+         for (int i = first; i < first + cnt; ++i) test_closest<CNT>(S, i, o, d, tmin, best, tc);
+      }
+      const uint32_t imask = n.e_imask >> 24;
+      uint32_t base = n.child_base;
+      if (!hit_int) {  // continue with the nearest pending child of an ancestor
+         if (sp == 0) break;
+         base = (uint32_t)stack[(2 * (sp - 1)) * STRIDE];
+         const uint32_t bits = (uint32_t)stack[(2 * (sp - 1) + 1) * STRIDE];
+         hit_int = bits & 0xFFu;
+         const uint32_t im = bits >> 8;
+         const int i = __ffs(hit_int) - 1;
+         hit_int &= hit_int - 1u;
+         const uint32_t sl = (uint32_t)i ^ perm;
+         node = base + (uint32_t)__popc(im & ((1u << sl) - 1u));
+         if (hit_int)
+            stack[(2 * (sp - 1) + 1) * STRIDE] = (int)(hit_int | (im << 8));
+         else
+            --sp;
+         continue;
+      }
+      const int i = __ffs(hit_int) - 1;
+      hit_int &= hit_int - 1u;
+      const uint32_t sl = (uint32_t)i ^ perm;
+      node = base + (uint32_t)__popc(imask & ((1u << sl) - 1u));
+      if (hit_int && sp < kStack / 2) {
+         stack[(2 * sp) * STRIDE] = (int)base;
+         stack[(2 * sp + 1) * STRIDE] = (int)(hit_int | (imask << 8));
+         ++sp;
+      }
+   }
+   return best;
+}
+
+template <int STRIDE, bool CNT>
+__device__ inline bool any_hit_wide(const SceneView& S, V3 o, V3 d, float tmin, float tmax, int* stack,
+                                    TravCount* tc) {
+   const V3 inv = mk(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
+   const uint32_t perm = 7u ^ ((d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u));
+   int sp = 0;
+   uint32_t node = 0;
+   for (;;) {
+      const WNode n = load_wnode(S.wnodes, node);
+      if constexpr (CNT) ++tc->nodes;
+      uint32_t hit_int, hit_leaf;
+      wide_children(n, o, inv, perm, tmin, tmax, hit_int, hit_leaf);
+      while (hit_leaf) {
+         const int sl = __ffs(hit_leaf) - 1;
+         hit_leaf &= hit_leaf - 1u;
+         const uint32_t m = ((sl < 4 ? n.meta[0] : n.meta[1]) >> (8 * (sl & 3))) & 0xFFu;
+         const int first = (int)n.prim_base + (int)(m & 31u);
+         const int cnt = (int)(m >> 5) + 1;
+         for (int i = first; i < first + cnt; ++i)
+            if (test_any<CNT>(S, i, o, d, tmin, tmax, tc)) return true;
+      }
+      const uint32_t imask = n.e_imask >> 24;
+      uint32_t base = n.child_base;
+      if (!hit_int) {
+         if (sp == 0) return false;
+         base = (uint32_t)stack[(2 * (sp - 1)) * STRIDE];
+         const uint32_t bits = (uint32_t)stack[(2 * (sp - 1) + 1) * STRIDE];
+         hit_int = bits & 0xFFu;
+         const uint32_t im = bits >> 8;
+         const int i = __ffs(hit_int) - 1;
+         hit_int &= hit_int - 1u;
+         const uint32_t sl = (uint32_t)i ^ perm;
+         node = base + (uint32_t)__popc(im & ((1u << sl) - 1u));
+         if (hit_int)
+            stack[(2 * (sp - 1) + 1) * STRIDE] = (int)(hit_int | (im << 8));
+         else
+            --sp;
+         continue;
+      }
+      const int i = __ffs(hit_int) - 1;
+      hit_int &= hit_int - 1u;
+      const uint32_t sl = (uint32_t)i ^ perm;
+      node = base + (uint32_t)__popc(imask & ((1u << sl) - 1u));
+      if (hit_int && sp < kStack / 2) {
+         stack[(2 * sp) * STRIDE] = (int)base;
+         stack[(2 * sp + 1) * STRIDE] = (int)(hit_int | (imask << 8));
+         ++sp;
+      }
+   }
+}
+
+// Tree layouts a kernel is compiled for: the binary pair-node tree, the 8-wide quantized
+// tree, or either (chosen per ray from the scene, for batches that mix layouts).
+enum Layout : int { L_BIN = 0, L_WIDE = 1, L_ANY = 2 };
+
+template <int STRIDE, bool CNT = false, int LAYOUT = L_ANY>
+__device__ inline HitMin closest_hit(const SceneView& S, V3 o, V3 d, float tmin, float tmax, int* stack,
+                                     TravCount* tc = nullptr) {
+   if (LAYOUT == L_WIDE || (LAYOUT == L_ANY && S.wnodes))
+      return closest_hit_wide<STRIDE, CNT>(S, o, d, tmin, tmax, stack, tc);
+   return closest_hit_bin<STRIDE, CNT>(S, o, d, tmin, tmax, stack, tc);
+}
+
+template <int STRIDE, bool CNT = false, int LAYOUT = L_ANY>
+__device__ inline bool any_hit(const SceneView& S, V3 o, V3 d, float tmin, float tmax, int* stack,
+                               TravCount* tc = nullptr) {
+   if (LAYOUT == L_WIDE || (LAYOUT == L_ANY && S.wnodes))
+      return any_hit_wide<STRIDE, CNT>(S, o, d, tmin, tmax, stack, tc);
+   return any_hit_bin<STRIDE, CNT>(S, o, d, tmin, tmax, stack, tc);
 }
 
 // ------------------------------------------------------------------------------------

@@ -41,13 +41,16 @@ struct HostPrim {
 struct BuildResult {
    std::vector<int32_t> order;  // slot -> host prim index
    std::vector<Node2> nodes;
+   std::vector<WNode> wnodes;   // 8-wide quantized tree (root = 0), when requested
    int root = 0;                // packed root reference
    int max_depth = 0;
+   int wide_depth = 0;
 };
 
 // Binned-SAH build (16 bins, leaves <= 4 primitives, depth <= kMaxDepth).
 constexpr int kMaxDepth = 32;
-bool build_bvh(const std::vector<HostPrim>& prims, BuildResult& out);
+bool build_bvh(const std::vector<HostPrim>& prims, BuildResult& out, bool wide = false);
+constexpr int kMaxWideDepth = 16;  // device stack entries of the wide traversal
 
 // Flattens a description into host primitives, validates it.
 rayon_status gather_prims(const rayon_scene_desc* d, std::vector<HostPrim>& prims);

@@ -18,7 +18,7 @@ constexpr int kBlock = kTile * kTile;
 
 __device__ __forceinline__ int xcd_tile(int b, int per_xcd) { return (b & 7) * per_xcd + (b >> 3); }
 
-template <bool CNT>
+template <bool CNT, int LAYOUT>
 __global__ __launch_bounds__(kBlock) void k_plain(JobView J, int samples, int tiles_x, int ntiles, int per_xcd) {
    __shared__ int stack[kStack * kBlock];
    const int tid = threadIdx.x;
@@ -42,7 +42,7 @@ __global__ __launch_bounds__(kBlock) void k_plain(JobView J, int samples, int ti
                if (!refrng) R.s = counter_init(J.seed_lo, J.seed_hi, (uint32_t)pix, (uint32_t)(cnt + s));
                V3 o, d;
                camera_ray(J, x, y, R, o, d);
-               const V3 L = trace_path<kBlock, CNT>(J, o, d, R, stack + tid, pc);
+               const V3 L = trace_path<kBlock, CNT, LAYOUT>(J, o, d, R, stack + tid, pc);
                accumulate(a, c, L);
             }
             primary = todo;
@@ -116,7 +116,8 @@ __global__ __launch_bounds__(256) void k_trace_closest(SceneView S, const float*
    if (i >= n) return;
    const V3 oo = mk(o[3 * i], o[3 * i + 1], o[3 * i + 2]);
    const V3 dd = mk(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
-   const HitMin h = closest_hit<256>(S, oo, dd, tmin, tmax, stack + threadIdx.x);
+   const HitMin h = S.wnodes ? closest_hit<256, false, L_WIDE>(S, oo, dd, tmin, tmax, stack + threadIdx.x)
+                             : closest_hit<256, false, L_BIN>(S, oo, dd, tmin, tmax, stack + threadIdx.x);
    prim[i] = h.slot < 0 ? -1 : (long long)h.orig;
    t[i] = h.slot < 0 ? -1.0f : h.t;
 }
@@ -204,10 +205,16 @@ hipError_t launch_plain(const JobView& J, int samples, hipStream_t st) {
    const int tx = (J.width + kTile - 1) / kTile, ty = (J.height + kTile - 1) / kTile;
    const int ntiles = tx * ty;
    const int per = (ntiles + 7) / 8;
-   if (J.flags & RAYON_FLAG_COUNT_TRAVERSAL)
-      hipLaunchKernelGGL(k_plain<true>, dim3(8 * per), dim3(kBlock), 0, st, J, samples, tx, ntiles, per);
+   const bool cnt = (J.flags & RAYON_FLAG_COUNT_TRAVERSAL) != 0, wide = J.scene.wnodes != nullptr;
+   const dim3 g(8 * per), b(kBlock);
+   if (cnt && wide)
+      hipLaunchKernelGGL((k_plain<true, L_WIDE>), g, b, 0, st, J, samples, tx, ntiles, per);
+   else if (cnt)
+      hipLaunchKernelGGL((k_plain<true, L_BIN>), g, b, 0, st, J, samples, tx, ntiles, per);
+   else if (wide)
+      hipLaunchKernelGGL((k_plain<false, L_WIDE>), g, b, 0, st, J, samples, tx, ntiles, per);
    else
-      hipLaunchKernelGGL(k_plain<false>, dim3(8 * per), dim3(kBlock), 0, st, J, samples, tx, ntiles, per);
+      hipLaunchKernelGGL((k_plain<false, L_BIN>), g, b, 0, st, J, samples, tx, ntiles, per);
    return hipGetLastError();
 }
 

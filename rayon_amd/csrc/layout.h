@@ -64,6 +64,23 @@ struct alignas(64) Node2 {
 // Packed child reference for stacks: interior >= 0, leaf = ~(first << 3 | (count - 1)).
 RY_HD inline int pack_leaf(int first, int count) { return ~((first << 3) | (count - 1)); }
 
+// 8-wide compressed node, 80 B (five 16-byte loads): the compressed wide BVH layout of
+// Ylitie, Karras and Laine (HPG 2017). Child boxes are 8-bit offsets on a per-axis
+// power-of-two grid anchored at the node origin p, rounded outward (lo down, hi up), so a
+// decoded child box always contains the exact one. Internal children are stored
+// contiguously from child_base in slot order; the primitives of the leaf children follow
+// each other from prim_base. Slot s holds the child whose centroid lies in octant s of
+// the node (bit k set: negative side of axis k), so a ray visits slots in the order
+// i ^ (7 ^ octant(direction)) to go roughly front to back.
+struct alignas(16) WNode {
+   float px, py, pz;
+   uint32_t e_imask;     // ex | ey << 8 | ez << 16 (biased exponents) | imask << 24
+   uint32_t child_base;  // first internal child
+   uint32_t prim_base;   // first primitive slot of the leaf children
+   uint32_t meta[2];     // per slot: internal -> 0; leaf -> offset (bits 0..4) | (count-1) << 5
+   uint32_t q[12];       // qlo_x[8], qlo_y[8], qlo_z[8], qhi_x[8], qhi_y[8], qhi_z[8] as bytes
+};
+
 // Material record, 6 x float4.
 struct alignas(16) MatRec {
    float4 m0;  // albedo, type (bits 0..7) | pattern (bits 8..15)
@@ -88,6 +105,7 @@ struct SceneView {
    const int* light_slots;
    const float* light_area;
    const SdfRec* sdf;
+   const WNode* wnodes;  // 8-wide quantized tree, or NULL to traverse the binary pair nodes
    int num_lights;
    int num_sdf;
    int root;        // packed child reference of the root, or INT_MIN if empty

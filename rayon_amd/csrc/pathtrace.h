@@ -104,15 +104,16 @@ __device__ inline bool shade_step(const JobView& J, const HitMin& h, int bounce,
    return true;
 }
 
-template <int STRIDE, bool CNT = false>
+template <int STRIDE, bool CNT = false, int LAYOUT = L_ANY>
 __device__ inline V3 trace_path(const JobView& J, V3 o, V3 d, Rng& R, int* stack, PathCounts& pc) {
    PathState ps{o, d, mk(1.0f, 1.0f, 1.0f), mk(0.0f, 0.0f, 0.0f), false};
    ShadowReq sh;
    for (int bounce = 0; bounce < J.max_depth; ++bounce) {
       ++pc.segments;
-      const HitMin h = closest_hit<STRIDE, CNT>(J.scene, ps.o, ps.d, 0.001f, 3.402823466e38f, stack, &pc.trav);
+      const HitMin h =
+          closest_hit<STRIDE, CNT, LAYOUT>(J.scene, ps.o, ps.d, 0.001f, 3.402823466e38f, stack, &pc.trav);
       const bool cont = shade_step<CNT>(J, h, bounce, ps, R, sh, pc);
-      if (sh.valid && !any_hit<STRIDE, CNT>(J.scene, sh.o, sh.d, 1e-4f, 1.0f - 1e-4f, stack, &pc.trav))
+      if (sh.valid && !any_hit<STRIDE, CNT, LAYOUT>(J.scene, sh.o, sh.d, 1e-4f, 1.0f - 1e-4f, stack, &pc.trav))
          ps.L = ps.L + sh.contrib;
       if (!cont) break;
    }

@@ -76,7 +76,7 @@ struct Wave {
    int opt_capacity = 1 << 26;
    int opt_wg_per_cu = 0;  // 0: as many as fit (occupancy of each stage kernel)
    int opt_chunk = 4;
-   int occ_trace = 0, occ_shade = 0;
+   int occ_trace = 0, occ_trace_wide = 0, occ_shade = 0;
    struct Pending {
       int stage;
       hipEvent_t e0, e1;
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_generate(WfView V, long long w0
 // path's radiance). Both queues were filled by the preceding shade launch and touch
 // disjoint state, so one persistent launch drains both and waves that finish their
 // extension work fill the launch's tail with shadow work. Either level may be -1.
-template <bool CNT>
+template <bool CNT, int LAYOUT>
 __global__ __launch_bounds__(kWfBlock) void k_wf_trace(WfView V, int ext_level, int shadow_level) {
    __shared__ int stack[kStack * kWfBlock];
    TravCount te, ts;
@@ -218,8 +218,8 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_trace(WfView V, int ext_level, 
             const int slot = q[idx];
             const float4 o = V.o[slot], d = V.d[slot];
             const SceneView& S = V.jobs[V.ids[slot].x].scene;
-            const HitMin h = closest_hit<kWfBlock, CNT>(S, mk(o), mk(d), 0.001f, 3.402823466e38f,
-                                                       stack + threadIdx.x, &te);
+            const HitMin h = closest_hit<kWfBlock, CNT, LAYOUT>(S, mk(o), mk(d), 0.001f, 3.402823466e38f,
+                                                               stack + threadIdx.x, &te);
             V.hit[slot] = make_float4(h.t, __int_as_float(h.slot), h.u, h.v);
          }
       });
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_trace(WfView V, int ext_level, 
             const int slot = V.sq[idx];
             const float4 o = V.so[slot], d = V.sd[slot];
             const SceneView& S = V.jobs[V.ids[slot].x].scene;
-            if (!any_hit<kWfBlock, CNT>(S, mk(o), mk(d), 1e-4f, 1.0f - 1e-4f, stack + threadIdx.x, &ts)) {
+            if (!any_hit<kWfBlock, CNT, LAYOUT>(S, mk(o), mk(d), 1e-4f, 1.0f - 1e-4f, stack + threadIdx.x, &ts)) {
                const float4 c = V.sc[slot];
                const float4 r = V.rad[slot];
                const V3 L = mk(r) + mk(c);
@@ -441,8 +441,10 @@ static rayon_status wave_reserve(rayon_ctx* ctx, int capacity, int levels, int n
       RY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
       W->cus = cus > 0 ? cus : 256;
       int n = 0;
-      RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_trace<false>, kWfBlock, 0));
+      RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_trace<false, L_BIN>, kWfBlock, 0));
       W->occ_trace = n > 0 ? n : 1;
+      RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_trace<false, L_WIDE>, kWfBlock, 0));
+      W->occ_trace_wide = n > 0 ? n : 1;
       RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_shade<false>, kWfBlock, 0));
       W->occ_shade = n > 0 ? n : 1;
    }
@@ -546,12 +548,22 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
    V.levels = levels;
    V.chunk = W->opt_chunk;
    auto grid_of = [&](int occ) { return W->cus * (W->opt_wg_per_cu > 0 ? W->opt_wg_per_cu : occ); };
-   const dim3 g_trace(grid_of(W->occ_trace)), g_shade(grid_of(W->occ_shade)), blk(kWfBlock);
+   const dim3 g_shade(grid_of(W->occ_shade)), blk(kWfBlock);
+   // one layout for the whole pass when every scene shares it
+   int nwide = 0;
+   for (rayon_job* j : jobs) nwide += j->scene->view.wnodes != nullptr;
+   const int layout = nwide == 0 ? L_BIN : (nwide == (int)jobs.size() ? L_WIDE : L_ANY);
+   const dim3 g_trace(grid_of(layout == L_BIN ? W->occ_trace : std::min(W->occ_trace, W->occ_trace_wide)));
    auto trace = [&](int ext, int sh) {
-      if (W->opt_count)
-         launch_stage(W, 1, k_wf_trace<true>, g_trace, blk, st, V, ext, sh);
-      else
-         launch_stage(W, 1, k_wf_trace<false>, g_trace, blk, st, V, ext, sh);
+      if (W->opt_count) {
+         if (layout == L_BIN) launch_stage(W, 1, k_wf_trace<true, L_BIN>, g_trace, blk, st, V, ext, sh);
+         else if (layout == L_WIDE) launch_stage(W, 1, k_wf_trace<true, L_WIDE>, g_trace, blk, st, V, ext, sh);
+         else launch_stage(W, 1, k_wf_trace<true, L_ANY>, g_trace, blk, st, V, ext, sh);
+      } else {
+         if (layout == L_BIN) launch_stage(W, 1, k_wf_trace<false, L_BIN>, g_trace, blk, st, V, ext, sh);
+         else if (layout == L_WIDE) launch_stage(W, 1, k_wf_trace<false, L_WIDE>, g_trace, blk, st, V, ext, sh);
+         else launch_stage(W, 1, k_wf_trace<false, L_ANY>, g_trace, blk, st, V, ext, sh);
+      }
    };
    hipError_t e = hipSuccess;
    int dense = 1;
@@ -683,6 +695,7 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
       key.push_back(W->opt_wg_per_cu);
       key.push_back(W->opt_chunk);
       key.push_back(W->opt_count);
+      for (rayon_job* j : active) key.push_back(j->scene->view.wnodes != nullptr);
       auto it = W->graphs.find(key);
       if (it == W->graphs.end()) {
          Wave::GraphEntry ge;

@@ -86,8 +86,9 @@ def test_camera_frame_rejects_degenerate():
                                               (ra.GEN.SDF_TERRAIN, 11, 101)])
 def test_bvh_invariants(kind, seed, detail):
     g = ra.Generated.generate(kind, seed, detail=detail)
-    info = ra.bvh_check(g.desc_ptr())
+    info = ra.bvh_check(g.desc_ptr())  # also verifies the 8-wide quantized tree
     assert 0 < info.max_depth <= 32
+    assert 0 < info.num_qnodes <= info.num_nodes
     assert info.num_prims > 0 and info.num_nodes == info.num_prims - 1 or info.num_nodes > 0
 
 

@@ -195,3 +195,49 @@ def test_reference_adaptive_rule(ctx):
     assert (prev_cnt < 0).sum() > 0
     job.close()
     sc.close()
+
+
+@pytest.mark.parametrize("kind,seed,detail", [(ra.GEN.BOOK_SPHERES, 1984, 0), (ra.GEN.CORNELL_ICOSPHERE, 7, 5),
+                                              (ra.GEN.SDF_TERRAIN, 11, 101)])
+def test_quantized_wide_layout_closest_hits(ctx, kind, seed, detail):
+    """The 8-wide quantized tree (RAYON_BVH_QUANTIZED) finds the same closest primitive as
+    the FP64 oracle on random rays: its outward-rounded 8-bit boxes lose no hit."""
+    log(f"quantized layout kind={kind}")
+    g = ra.Generated.generate(kind, seed, detail=detail)
+    sc = ctx.scene(g.desc_ptr(), ra.BVH.QUANTIZED)
+    assert sc.info().num_qnodes > 0
+    rng = np.random.default_rng(3)
+    n = 100000
+    lf, la, up, fov = g.camera()
+    o = (np.array(lf, np.float32) + rng.normal(0, 1, (n, 3))).astype(np.float32)
+    d = (np.array(la, np.float32) - o + rng.normal(0, 3, (n, 3))).astype(np.float32)
+    gp, gt = sc.trace_closest(o, d)
+    osc = OracleScene(g.desc_ptr())
+    op, ot = osc.trace_closest(o, d)
+    mism = np.nonzero(gp != op)[0]
+    log(f"  hits={int((op >= 0).sum())} mismatches={len(mism)}")
+    assert (op >= 0).mean() > 0.2
+    assert len(mism) <= max(1e-5 * n, 2)
+    if len(mism):
+        assert _near_ties(osc, o[mism], d[mism], gp[mism], op[mism]).all()
+    sc.close()
+
+
+def test_quantized_layout_image_matches_binary(ctx):
+    g = ra.Generated.generate(ra.GEN.CORNELL_ICOSPHERE, 7, detail=4)
+    lf, la, up, fov = g.camera()
+    cam = ra.camera_frame(lf, la, up, fov, 96, 96)
+    imgs = []
+    for flags in (0, ra.BVH.QUANTIZED):
+        sc = ctx.scene(g.desc_ptr(), flags)
+        for batched in (False, True):
+            j = ctx.job(sc, ra.job_params(96, 96, cam, spp=32, max_depth=8, seed=21, flags=ra.FLAG.NEE))
+            if batched:
+                ctx.render_batched([j], 32)
+            else:
+                j.render_plain(32)
+            imgs.append(j.resolve_linear().astype(np.float64))
+            j.close()
+        sc.close()
+    for im in imgs[1:]:
+        assert rel_rmse(im, imgs[0]) < 1e-3
