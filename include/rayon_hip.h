@@ -318,6 +318,69 @@ rayon_status rayon_job_primary_rays(rayon_ctx* ctx, rayon_job* job, int32_t samp
                                     float* dirs);
 
 /* ---------------------------------------------------------------------------------
+ * Render server: continuous batching of many jobs on one context (no counterpart in the
+ * reference). A server thread admits queued jobs by HBM budget, gives every newly
+ * admitted job a first pass and resolves its first frame on the device, advances the
+ * other active jobs together in batched passes sized toward target_pass_ms, and retires
+ * jobs whose sample budget is spent or whose pixels all converged.
+ * ------------------------------------------------------------------------------- */
+typedef struct rayon_server rayon_server;
+
+enum {
+   RAYON_JOB_QUEUED = 0,
+   RAYON_JOB_RUNNING = 1,
+   RAYON_JOB_DONE = 2,
+   RAYON_JOB_FAILED = 3,
+   RAYON_JOB_CANCELLED = 4
+};
+
+typedef struct rayon_server_config {
+   int64_t hbm_budget_bytes;    /* accumulation + frame buffers + referenced scenes      */
+   int32_t max_active_jobs;
+   int32_t first_pass_samples;  /* samples of a job's first frame                        */
+   int32_t max_pass_samples;    /* cap of the controlled pass size                       */
+   int32_t pad_;
+   double target_pass_ms;       /* controller target for one batched pass               */
+   int64_t max_active_pixels;   /* admission stops while the active jobs hold this many
+                                   pixels, so one sample of each fits a pass            */
+} rayon_server_config;
+
+typedef struct rayon_server_job_info {
+   int32_t state;
+   int32_t pad_;
+   int64_t samples_done;
+   int64_t segments;
+   double queue_ms;             /* arrival -> admission                                  */
+   double ttff_ms;              /* arrival -> first frame resolved on the device (-1)    */
+   double ttc_ms;               /* arrival -> finished (-1)                              */
+} rayon_server_job_info;
+
+typedef struct rayon_server_stats {
+   int64_t passes, first_passes, completed, failed, queued, active, segments, used_bytes;
+   double busy_s, wall_s;
+   int32_t pass_samples;
+   int32_t pad_;
+} rayon_server_stats;
+
+void rayon_server_config_default(rayon_server_config* cfg);
+rayon_status rayon_server_create(rayon_ctx* ctx, const rayon_server_config* cfg, rayon_server** out);
+/* Thread-safe. Fails with RAYON_ERR_BUDGET if the job alone exceeds the budget. */
+rayon_status rayon_server_submit(rayon_server* s, rayon_scene* scene, const rayon_job_params* params,
+                                 uint64_t* id);
+rayon_status rayon_server_job_info_get(rayon_server* s, uint64_t id, rayon_server_job_info* out);
+/* Mean linear radiance (3 floats per pixel) of a finished, unreleased job. */
+rayon_status rayon_server_result(rayon_server* s, uint64_t id, float* rgb);
+/* The first frame of a job (gamma-2 RGBA8, width*height*4 bytes) once ttff_ms >= 0. */
+rayon_status rayon_server_first_frame(rayon_server* s, uint64_t id, uint8_t* rgba);
+rayon_status rayon_server_cancel(rayon_server* s, uint64_t id);
+/* Frees a finished job's device buffers (deferred until it finishes otherwise). */
+rayon_status rayon_server_release(rayon_server* s, uint64_t id);
+/* Waits until no job is queued or active. */
+rayon_status rayon_server_drain(rayon_server* s, double timeout_s);
+rayon_status rayon_server_stats_get(rayon_server* s, rayon_server_stats* out);
+rayon_status rayon_server_destroy(rayon_server* s);
+
+/* ---------------------------------------------------------------------------------
  * Synthetic scene generators (host) and the YAML scene loader.
  * ------------------------------------------------------------------------------- */
 typedef struct rayon_generated rayon_generated;

@@ -13,7 +13,7 @@ import numpy as np
 
 from .abi import (  # noqa: F401  (re-exported)
     Camera, GenParams, Geometry, JobParams, JobStats, Material, Mesh, SceneDesc, SceneInfo, SdfPrim, WaveStats,
-    MAT, GEOM, SDF, FLAG, GEN, OPT, BVH, STATUS, declare,
+    ServerConfig, ServerJobInfo, ServerStats, MAT, GEOM, SDF, FLAG, GEN, OPT, BVH, JOB, STATUS, declare,
 )
 
 LIB_PATH = Path(__file__).resolve().parent / "librayon_hip.so"
@@ -154,6 +154,9 @@ class Context:
         _check(lib().rayon_ctx_get_option(self._h, opt, C.byref(v)), "rayon_ctx_get_option")
         return v.value
 
+    def server(self, **cfg) -> "Server":
+        return Server(self, **cfg)
+
     def wave_stats(self, reset: bool = False) -> WaveStats:
         s = WaveStats()
         _check(lib().rayon_ctx_wave_stats(self._h, C.byref(s), int(reset)), "rayon_ctx_wave_stats")
@@ -258,3 +261,75 @@ class Job:
         if self._h:
             lib().rayon_job_destroy(self._h)
             self._h = None
+
+
+def server_config(**kw) -> ServerConfig:
+    c = ServerConfig()
+    lib().rayon_server_config_default(C.byref(c))
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise AttributeError(k)
+        setattr(c, k, v)
+    return c
+
+
+class Server:
+    """Continuous-batching render server on one context (a native thread in the library)."""
+
+    def __init__(self, ctx: Context, **cfg):
+        self.ctx = ctx
+        self.config = server_config(**cfg)
+        h = C.c_void_p()
+        _check(lib().rayon_server_create(ctx._h, C.byref(self.config), C.byref(h)), "rayon_server_create")
+        self._h = h
+        self._shape = {}
+
+    def submit(self, scene: Scene, params: JobParams) -> int:
+        i = C.c_uint64()
+        _check(lib().rayon_server_submit(self._h, scene._h, C.byref(params), C.byref(i)), "rayon_server_submit")
+        self._shape[i.value] = (params.height, params.width)
+        return i.value
+
+    def info(self, job_id: int) -> ServerJobInfo:
+        s = ServerJobInfo()
+        _check(lib().rayon_server_job_info_get(self._h, job_id, C.byref(s)), "rayon_server_job_info_get")
+        return s
+
+    def result(self, job_id: int) -> np.ndarray:
+        h, w = self._shape[job_id]
+        out = np.empty((h, w, 3), np.float32)
+        _check(lib().rayon_server_result(self._h, job_id, out.ctypes.data_as(C.POINTER(C.c_float))),
+               "rayon_server_result")
+        return out
+
+    def first_frame(self, job_id: int) -> np.ndarray:
+        h, w = self._shape[job_id]
+        out = np.empty((h, w, 4), np.uint8)
+        _check(lib().rayon_server_first_frame(self._h, job_id, out.ctypes.data_as(C.POINTER(C.c_uint8))),
+               "rayon_server_first_frame")
+        return out
+
+    def cancel(self, job_id: int):
+        _check(lib().rayon_server_cancel(self._h, job_id), "rayon_server_cancel")
+
+    def release(self, job_id: int):
+        _check(lib().rayon_server_release(self._h, job_id), "rayon_server_release")
+
+    def drain(self, timeout_s: float = 60.0):
+        _check(lib().rayon_server_drain(self._h, timeout_s), "rayon_server_drain")
+
+    def stats(self) -> ServerStats:
+        s = ServerStats()
+        _check(lib().rayon_server_stats_get(self._h, C.byref(s)), "rayon_server_stats_get")
+        return s
+
+    def close(self):
+        if self._h:
+            lib().rayon_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

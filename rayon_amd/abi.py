@@ -23,6 +23,7 @@ SDF = _NS(SPHERE=0, ROUNDED_BOX=1, TORUS=2)
 FLAG = _NS(REF_QUIRKS=1, REF_RNG=2, NEE=4, ADAPTIVE_REF=8, ADAPTIVE_STDERR=16, DOF=32, COUNT_TRAVERSAL=64)
 GEN = _NS(BOOK_SPHERES=0, CORNELL_ICOSPHERE=1, SDF_TERRAIN=2)
 BVH = _NS(DEFAULT=0, QUANTIZED=1)
+JOB = _NS(QUEUED=0, RUNNING=1, DONE=2, FAILED=3, CANCELLED=4)
 OPT = _NS(WAVE_CAPACITY=0, WAVE_WG_PER_CU=1, GRAPH=2, STAGE_TIMING=3, COUNT_TRAVERSAL=4, WAVE_CHUNK=5)
 
 
@@ -93,6 +94,24 @@ class WaveStats(C.Structure):
                 ("material_fetches", C.c_int64), ("light_fetches", C.c_int64)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_active_jobs", C.c_int32),
+                ("first_pass_samples", C.c_int32), ("max_pass_samples", C.c_int32), ("pad_", C.c_int32),
+                ("target_pass_ms", C.c_double), ("max_active_pixels", C.c_int64)]
+
+
+class ServerJobInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("pad_", C.c_int32), ("samples_done", C.c_int64), ("segments", C.c_int64),
+                ("queue_ms", C.c_double), ("ttff_ms", C.c_double), ("ttc_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("passes", C.c_int64), ("first_passes", C.c_int64), ("completed", C.c_int64),
+                ("failed", C.c_int64), ("queued", C.c_int64), ("active", C.c_int64), ("segments", C.c_int64),
+                ("used_bytes", C.c_int64), ("busy_s", C.c_double), ("wall_s", C.c_double),
+                ("pass_samples", C.c_int32), ("pad_", C.c_int32)]
+
+
 class GenParams(C.Structure):
     _fields_ = [("kind", C.c_int32), ("seed", C.c_uint64), ("detail", C.c_int32), ("count", C.c_int32)]
 
@@ -130,6 +149,17 @@ def declare(lib: C.CDLL) -> C.CDLL:
         "rayon_job_stats_get": (st, [vp, vp, P(JobStats)]),
         "rayon_trace_closest": (st, [vp, vp, P(f32), P(f32), i64, f32, f32, P(i64), P(f32)]),
         "rayon_job_primary_rays": (st, [vp, vp, i32, P(f32), P(f32)]),
+        "rayon_server_config_default": (None, [P(ServerConfig)]),
+        "rayon_server_create": (st, [vp, P(ServerConfig), P(vp)]),
+        "rayon_server_submit": (st, [vp, vp, P(JobParams), P(C.c_uint64)]),
+        "rayon_server_job_info_get": (st, [vp, C.c_uint64, P(ServerJobInfo)]),
+        "rayon_server_result": (st, [vp, C.c_uint64, P(f32)]),
+        "rayon_server_first_frame": (st, [vp, C.c_uint64, P(C.c_uint8)]),
+        "rayon_server_cancel": (st, [vp, C.c_uint64]),
+        "rayon_server_release": (st, [vp, C.c_uint64]),
+        "rayon_server_drain": (st, [vp, C.c_double]),
+        "rayon_server_stats_get": (st, [vp, P(ServerStats)]),
+        "rayon_server_destroy": (st, [vp]),
         "rayon_generate": (st, [P(GenParams), P(vp)]),
         "rayon_load_yaml": (st, [C.c_char_p, P(vp)]),
         "rayon_generated_desc": (P(SceneDesc), [vp]),

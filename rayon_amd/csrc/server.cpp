@@ -1,0 +1,467 @@
+// server.cpp — continuous-batching render server (no counterpart in the reference, whose
+// renderers run one job per process; the closest is the progressive loop of
+// renderer_cuda_progressive_host.hpp:440-920 with its FPS-driven batch controller
+// :681-686, here generalised to many concurrent jobs).
+//
+// One server thread owns the device work of a context. Each step it
+//   1. admits queued jobs while their device memory (accumulation, counts and a u8 frame,
+//      plus each newly referenced scene) fits the HBM budget and the active-job cap;
+//   2. runs one first pass for the jobs admitted in this step (first_pass_samples
+//      samples each) and resolves their first frame on the device; time-to-first-frame
+//      is stamped when that work completes;
+//   3. advances every other active job together by one batched pass whose sample count
+//      a controller steers toward target_pass_ms;
+//   4. retires jobs whose sample budget is exhausted or whose pixels have all converged
+//      (time-to-converge), and jobs that were cancelled.
+// Jobs keep their buffers until released, so results can be read after completion.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "internal.h"
+#include "kernels.h"
+
+namespace {
+
+double now_s() {
+   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct SJob {
+   uint64_t id = 0;
+   rayon_scene* scene = nullptr;
+   rayon_job_params params{};
+   rayon_job* job = nullptr;
+   uint8_t* frame = nullptr;  // device u8 RGBA of the first frame
+   int32_t state = RAYON_JOB_QUEUED;
+   double t_arrive = 0, t_admit = 0, t_first = 0, t_done = 0;
+   int64_t bytes = 0;
+   int64_t samples = 0;
+   int64_t segments = 0;
+   bool cancel = false;
+   bool released = false;
+   bool accounted = false;  // bytes (and a scene reference) are held in the budget
+};
+
+int64_t job_bytes(const rayon_job_params& p) {
+   const int64_t npix = (int64_t)p.width * p.height;
+   int64_t b = npix * (16 + 16 + 4 + 4) + 8 * 8 * ry::kCounterSlots;
+   if (p.flags & RAYON_FLAG_ADAPTIVE_REF) b += npix * 16;
+   return b;
+}
+
+}  // namespace
+
+struct rayon_server {
+   rayon_ctx* ctx = nullptr;
+   rayon_server_config cfg{};
+   std::thread th;
+   std::mutex mu;
+   std::condition_variable cv, done_cv;
+   bool stop = false;
+   uint64_t next_id = 1;
+   std::deque<SJob*> queue;
+   std::vector<SJob*> active;
+   std::unordered_map<uint64_t, SJob*> jobs;
+   std::unordered_map<rayon_scene*, int> scene_refs;
+   int64_t used_bytes = 0;
+   int64_t active_pixels = 0;
+   int pass_samples = 1;
+   // statistics
+   int64_t passes = 0, first_passes = 0, total_segments = 0, completed = 0, failed = 0;
+   double busy_s = 0, t_start = 0;
+   std::string last_error;
+
+   int64_t scene_bytes(rayon_scene* s) { return s->info.device_bytes; }
+
+   void fail_all(std::vector<SJob*>& js, const char* what) {
+      std::lock_guard<std::mutex> lk(mu);
+      last_error = std::string(what) + ": " + rayon_last_error();
+      for (SJob* j : js) {
+         j->state = RAYON_JOB_FAILED;
+         j->t_done = now_s();
+         ++failed;
+      }
+   }
+
+   // Releases device memory of a finished job and its share of the budget.
+   void free_job_locked(SJob* j) {
+      if (j->job) {
+         rayon_job_destroy(j->job);
+         j->job = nullptr;
+      }
+      if (j->accounted) {
+         j->accounted = false;
+         used_bytes -= j->bytes;
+         active_pixels -= (int64_t)j->params.width * j->params.height;
+         if (--scene_refs[j->scene] == 0) {
+            used_bytes -= scene_bytes(j->scene);
+            scene_refs.erase(j->scene);
+         }
+      }
+      if (j->frame) {
+         (void)hipFree(j->frame);
+         j->frame = nullptr;
+      }
+   }
+
+   void loop() {
+      (void)hipSetDevice(ctx->device);
+      for (;;) {
+         std::vector<SJob*> fresh, run;
+         {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return stop || !queue.empty() || !active.empty(); });
+            if (stop) return;
+            // retire cancelled jobs
+            for (auto it = active.begin(); it != active.end();) {
+               if ((*it)->cancel) {
+                  (*it)->state = RAYON_JOB_CANCELLED;
+                  (*it)->t_done = now_s();
+                  free_job_locked(*it);
+                  it = active.erase(it);
+                  done_cv.notify_all();
+               } else {
+                  ++it;
+               }
+            }
+            // admission by HBM budget and active-job cap (FIFO: a job that does not fit
+            // blocks the ones behind it, so large jobs are not starved)
+            while (!queue.empty() && (int)active.size() < cfg.max_active_jobs) {
+               SJob* j = queue.front();
+               if (j->cancel) {
+                  queue.pop_front();
+                  j->state = RAYON_JOB_CANCELLED;
+                  j->t_done = now_s();
+                  done_cv.notify_all();
+                  continue;
+               }
+               const int64_t need = j->bytes + (scene_refs.count(j->scene) ? 0 : scene_bytes(j->scene));
+               const int64_t px = (int64_t)j->params.width * j->params.height;
+               if ((used_bytes + need > cfg.hbm_budget_bytes || active_pixels + px > cfg.max_active_pixels) &&
+                   (!active.empty() || !fresh.empty()))
+                  break;
+               queue.pop_front();
+               used_bytes += need;
+               active_pixels += px;
+               scene_refs[j->scene]++;
+               j->accounted = true;
+               fresh.push_back(j);
+            }
+            for (SJob* j : active)
+               if (j->samples > 0) run.push_back(j);
+         }
+         // create device state for admitted jobs (outside the lock: allocations)
+         std::vector<SJob*> ok_fresh;
+         for (SJob* j : fresh) {
+            j->t_admit = now_s();
+            rayon_status st = rayon_job_create(ctx, j->scene, &j->params, &j->job);
+            if (st == RAYON_OK && hipMalloc(&j->frame, (size_t)j->params.width * j->params.height * 4) != hipSuccess) {
+               ry::set_error("hipMalloc of the first-frame buffer failed");
+               st = RAYON_ERR_OOM;
+            }
+            std::lock_guard<std::mutex> lk(mu);
+            if (st != RAYON_OK) {
+               last_error = rayon_last_error();
+               j->state = RAYON_JOB_FAILED;
+               j->t_done = now_s();
+               ++failed;
+               free_job_locked(j);
+               done_cv.notify_all();
+               continue;
+            }
+            j->state = RAYON_JOB_RUNNING;
+            active.push_back(j);
+            ok_fresh.push_back(j);
+         }
+         const double t0 = now_s();
+         // (2) first frames
+         if (!ok_fresh.empty()) {
+            std::vector<rayon_job*> h;
+            for (SJob* j : ok_fresh) h.push_back(j->job);
+            if (rayon_render_batched(ctx, h.data(), (int)h.size(), cfg.first_pass_samples) != RAYON_OK) {
+               fail_all(ok_fresh, "first pass");
+            } else {
+               bool ok = true;
+               for (SJob* j : ok_fresh)
+                  ok = ok && ry::launch_resolve_u8(j->job->view, j->frame, 4, 2.0f, ctx->stream) == hipSuccess;
+               hipError_t e = ok ? hipStreamSynchronize(ctx->stream) : hipErrorLaunchFailure;
+               const double t = now_s();
+               std::lock_guard<std::mutex> lk(mu);
+               ++first_passes;
+               for (SJob* j : ok_fresh) {
+                  if (e != hipSuccess) {
+                     j->state = RAYON_JOB_FAILED;
+                     ++failed;
+                  } else {
+                     j->t_first = t;
+                     j->samples = j->job->samples_done;
+                  }
+               }
+            }
+         }
+         // (3) one batched pass for the jobs that already have a frame
+         if (!run.empty()) {
+            std::vector<rayon_job*> h;
+            for (SJob* j : run) h.push_back(j->job);
+            const double tp = now_s();
+            if (rayon_render_batched(ctx, h.data(), (int)h.size(), pass_samples) != RAYON_OK ||
+                rayon_ctx_sync(ctx) != RAYON_OK) {
+               fail_all(run, "batched pass");
+            } else {
+               const double ms = (now_s() - tp) * 1e3;
+               // controller toward target_pass_ms (the ratio rule of
+               // renderer_cuda_progressive_host.hpp:681-686, clamped to [0.5, 2] per step)
+               const double ratio = std::min(2.0, std::max(0.5, cfg.target_pass_ms / std::max(ms, 0.05)));
+               pass_samples = std::min(cfg.max_pass_samples, std::max(1, (int)(pass_samples * ratio + 0.5)));
+               std::lock_guard<std::mutex> lk(mu);
+               ++passes;
+            }
+         }
+         // (4) completion
+         std::vector<SJob*> finished;
+         {
+            std::vector<SJob*> check;
+            {
+               std::lock_guard<std::mutex> lk(mu);
+               for (SJob* j : active)
+                  if (j->state == RAYON_JOB_RUNNING || j->state == RAYON_JOB_FAILED) check.push_back(j);
+            }
+            for (SJob* j : check) {
+               if (j->state == RAYON_JOB_FAILED) {
+                  finished.push_back(j);
+                  continue;
+               }
+               rayon_job_stats s;
+               if (rayon_job_stats_get(ctx, j->job, &s) != RAYON_OK) continue;
+               std::lock_guard<std::mutex> lk(mu);
+               j->samples = s.samples_done;
+               j->segments = s.segments;
+               if (s.done) finished.push_back(j);
+            }
+         }
+         {
+            std::lock_guard<std::mutex> lk(mu);
+            const double t = now_s();
+            busy_s += t - t0;
+            for (SJob* j : finished) {
+               if (j->state != RAYON_JOB_FAILED) {
+                  j->state = RAYON_JOB_DONE;
+                  ++completed;
+               }
+               j->t_done = t;
+               total_segments += j->segments;
+               active.erase(std::find(active.begin(), active.end(), j));
+               if (j->released) free_job_locked(j);
+            }
+            if (!finished.empty()) done_cv.notify_all();
+         }
+      }
+   }
+};
+
+extern "C" {
+
+void rayon_server_config_default(rayon_server_config* c) {
+   std::memset(c, 0, sizeof(*c));
+   c->hbm_budget_bytes = 200ll << 30;  // 200 GiB of the 288 GB
+   c->max_active_jobs = 512;
+   c->first_pass_samples = 1;
+   c->max_pass_samples = 64;
+   c->target_pass_ms = 20.0;
+   c->max_active_pixels = 16ll << 20;
+}
+
+rayon_status rayon_server_create(rayon_ctx* ctx, const rayon_server_config* cfg, rayon_server** out) {
+   if (!ctx || !out) {
+      ry::set_error("rayon_server_create: NULL argument");
+      return RAYON_ERR_INVALID;
+   }
+   auto* s = new rayon_server();
+   s->ctx = ctx;
+   if (cfg)
+      s->cfg = *cfg;
+   else
+      rayon_server_config_default(&s->cfg);
+   if (s->cfg.max_active_jobs < 1 || s->cfg.first_pass_samples < 1 || s->cfg.max_pass_samples < 1 ||
+       !(s->cfg.target_pass_ms > 0) || s->cfg.hbm_budget_bytes <= 0 || s->cfg.max_active_pixels <= 0) {
+      delete s;
+      ry::set_error("rayon_server_create: bad configuration");
+      return RAYON_ERR_INVALID;
+   }
+   // passes change their job set every step: replaying captured graphs would not pay
+   rayon_ctx_set_option(ctx, RAYON_OPT_GRAPH, 0);
+   s->t_start = now_s();
+   s->th = std::thread([s] { s->loop(); });
+   *out = s;
+   return RAYON_OK;
+}
+
+rayon_status rayon_server_submit(rayon_server* s, rayon_scene* scene, const rayon_job_params* p, uint64_t* id) {
+   if (!s || !scene || !p) {
+      ry::set_error("rayon_server_submit: NULL argument");
+      return RAYON_ERR_INVALID;
+   }
+   if (p->width <= 0 || p->height <= 0 || p->spp <= 0 || (p->flags & RAYON_FLAG_REF_RNG)) {
+      ry::set_error("rayon_server_submit: bad resolution or spp, or a reference-stream job (not batchable)");
+      return RAYON_ERR_INVALID;
+   }
+   auto* j = new SJob();
+   j->scene = scene;
+   j->params = *p;
+   j->bytes = job_bytes(*p) + (int64_t)p->width * p->height * 4;
+   j->t_arrive = now_s();
+   std::lock_guard<std::mutex> lk(s->mu);
+   if (j->bytes + s->scene_bytes(scene) > s->cfg.hbm_budget_bytes) {
+      delete j;
+      ry::set_error("rayon_server_submit: job needs more than the whole HBM budget");
+      return RAYON_ERR_BUDGET;
+   }
+   j->id = s->next_id++;
+   s->jobs[j->id] = j;
+   s->queue.push_back(j);
+   if (id) *id = j->id;
+   s->cv.notify_one();
+   return RAYON_OK;
+}
+
+rayon_status rayon_server_job_info_get(rayon_server* s, uint64_t id, rayon_server_job_info* out) {
+   if (!s || !out) {
+      ry::set_error("rayon_server_job_info_get: NULL argument");
+      return RAYON_ERR_INVALID;
+   }
+   std::lock_guard<std::mutex> lk(s->mu);
+   auto it = s->jobs.find(id);
+   if (it == s->jobs.end()) {
+      ry::set_error("rayon_server_job_info_get: unknown job %llu", (unsigned long long)id);
+      return RAYON_ERR_INVALID;
+   }
+   const SJob* j = it->second;
+   out->state = j->state;
+   out->samples_done = j->samples;
+   out->segments = j->segments;
+   out->ttff_ms = j->t_first > 0 ? (j->t_first - j->t_arrive) * 1e3 : -1.0;
+   out->ttc_ms = (j->state == RAYON_JOB_DONE && j->t_done > 0) ? (j->t_done - j->t_arrive) * 1e3 : -1.0;
+   out->queue_ms = j->t_admit > 0 ? (j->t_admit - j->t_arrive) * 1e3 : -1.0;
+   return RAYON_OK;
+}
+
+rayon_status rayon_server_result(rayon_server* s, uint64_t id, float* rgb) {
+   if (!s || !rgb) {
+      ry::set_error("rayon_server_result: NULL argument");
+      return RAYON_ERR_INVALID;
+   }
+   rayon_job* job = nullptr;
+   {
+      std::lock_guard<std::mutex> lk(s->mu);
+      auto it = s->jobs.find(id);
+      if (it == s->jobs.end() || it->second->state != RAYON_JOB_DONE || !it->second->job) {
+         ry::set_error("rayon_server_result: job %llu is not finished or was released", (unsigned long long)id);
+         return RAYON_ERR_INVALID;
+      }
+      job = it->second->job;
+   }
+   return rayon_job_resolve_linear(s->ctx, job, rgb);
+}
+
+rayon_status rayon_server_first_frame(rayon_server* s, uint64_t id, uint8_t* rgba) {
+   if (!s || !rgba) {
+      ry::set_error("rayon_server_first_frame: NULL argument");
+      return RAYON_ERR_INVALID;
+   }
+   std::lock_guard<std::mutex> lk(s->mu);  // keeps the frame alive while it is copied
+   auto it = s->jobs.find(id);
+   if (it == s->jobs.end() || it->second->t_first <= 0 || !it->second->frame) {
+      ry::set_error("rayon_server_first_frame: job %llu has no first frame (yet)", (unsigned long long)id);
+      return RAYON_ERR_INVALID;
+   }
+   const SJob* j = it->second;
+   (void)hipSetDevice(s->ctx->device);
+   RY_HIP(hipMemcpy(rgba, j->frame, (size_t)j->params.width * j->params.height * 4, hipMemcpyDeviceToHost));
+   return RAYON_OK;
+}
+
+rayon_status rayon_server_cancel(rayon_server* s, uint64_t id) {
+   std::lock_guard<std::mutex> lk(s->mu);
+   auto it = s->jobs.find(id);
+   if (it == s->jobs.end()) {
+      ry::set_error("rayon_server_cancel: unknown job");
+      return RAYON_ERR_INVALID;
+   }
+   it->second->cancel = true;
+   s->cv.notify_one();
+   return RAYON_OK;
+}
+
+rayon_status rayon_server_release(rayon_server* s, uint64_t id) {
+   std::lock_guard<std::mutex> lk(s->mu);
+   auto it = s->jobs.find(id);
+   if (it == s->jobs.end()) {
+      ry::set_error("rayon_server_release: unknown job");
+      return RAYON_ERR_INVALID;
+   }
+   SJob* j = it->second;
+   j->released = true;
+   if (j->state == RAYON_JOB_DONE || j->state == RAYON_JOB_FAILED || j->state == RAYON_JOB_CANCELLED) {
+      (void)hipSetDevice(s->ctx->device);
+      s->free_job_locked(j);
+   }
+   return RAYON_OK;
+}
+
+rayon_status rayon_server_drain(rayon_server* s, double timeout_s) {
+   std::unique_lock<std::mutex> lk(s->mu);
+   const bool ok = s->done_cv.wait_for(lk, std::chrono::duration<double>(timeout_s), [&] {
+      return s->queue.empty() && s->active.empty();
+   });
+   if (!ok) {
+      ry::set_error("rayon_server_drain: timed out with %zu queued and %zu active jobs", s->queue.size(),
+                    s->active.size());
+      return RAYON_ERR_INVALID;
+   }
+   return RAYON_OK;
+}
+
+rayon_status rayon_server_stats_get(rayon_server* s, rayon_server_stats* out) {
+   std::lock_guard<std::mutex> lk(s->mu);
+   out->passes = s->passes;
+   out->first_passes = s->first_passes;
+   out->completed = s->completed;
+   out->failed = s->failed;
+   out->queued = (int64_t)s->queue.size();
+   out->active = (int64_t)s->active.size();
+   out->segments = s->total_segments;
+   out->used_bytes = s->used_bytes;
+   out->busy_s = s->busy_s;
+   out->wall_s = now_s() - s->t_start;
+   out->pass_samples = s->pass_samples;
+   return RAYON_OK;
+}
+
+rayon_status rayon_server_destroy(rayon_server* s) {
+   if (!s) return RAYON_OK;
+   {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->stop = true;
+      s->cv.notify_all();
+   }
+   if (s->th.joinable()) s->th.join();
+   (void)hipSetDevice(s->ctx->device);
+   (void)rayon_ctx_sync(s->ctx);
+   for (auto& kv : s->jobs) {
+      s->free_job_locked(kv.second);
+      delete kv.second;
+   }
+   delete s;
+   return RAYON_OK;
+}
+
+}  // extern "C"
