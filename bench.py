@@ -95,9 +95,21 @@ def main(argv=None) -> int:
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--path", choices=["plain", "batched"], default="batched",
                     help="which path's throughput is `value` (both are measured)")
-    ap.add_argument("--cpu-stride", type=int, default=8, help="CPU baseline pixel subset: every Nth row/col")
+    ap.add_argument("--cpu-stride", type=int, default=2, help="CPU baseline pixel subset: every Nth row/col")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 1 GiB device-to-device 4 times (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     a = ap.parse_args(argv)
+
+    if a.copy_probe:
+        import torch
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(4):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_per_dispatch": x.numel() * 4, "dispatches": 4}), flush=True)
+        return 0
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
@@ -206,6 +218,14 @@ def main(argv=None) -> int:
     stage_ms = {n: round(st_t.stage_ms[k], 2) for k, n in enumerate(stage_names) if n != "unused"}
     log(f"stages (ms per frame): {stage_ms}; trace {ext_achieved:.0f} GB/s algorithmic")
 
+    # measured HBM bytes per traced ray (rocprofv3 FETCH_SIZE/WRITE_SIZE, scaled by a copy
+    # kernel's counts) for this exact workload, kept beside the kernel-trace summary
+    traffic = None
+    tf = Path(__file__).resolve().parent / "profiles" / "k_wf_trace_cornell_icosphere_1080p_64spp.traffic.json"
+    if tf.exists() and (W, H, spp, a.depth, a.detail) == (1920, 1080, 64, 8, 8):
+        tr = json.loads(tf.read_text())
+        traffic = round(tr["hbm_bytes_per_segment"] * (ext_rays + sh_rays) / max(ext_launches, 1))
+
     use_batched = a.path == "batched"
     value = batched_mrays if use_batched else plain_mrays
     seg_per_step = (b_segs / len(b_stats)) if use_batched else (p_segs / len(p_stats))
@@ -213,9 +233,11 @@ def main(argv=None) -> int:
     wall = b_wall if use_batched else p_wall
     if use_batched:
         roof = {"bound": "hbm", "achieved": round(ext_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ext_achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_wf_trace",
+                "frac": round(ext_achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_wf_trace",
                 "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4), "launches_per_frame": int(ext_launches),
-                "alg_bytes_per_ray": round(ext_bytes / max(ext_rays + sh_rays, 1), 1)}
+                "alg_bytes_per_ray": round(ext_bytes / max(ext_rays + sh_rays, 1), 1),
+                "alg_bytes_per_launch": round(ext_bytes / max(ext_launches, 1)),
+                "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)"}
     else:
         roof = {"bound": "hbm", "achieved": round(plain_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(plain_achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_plain",
@@ -264,7 +286,7 @@ def main(argv=None) -> int:
         t_b = time.perf_counter()
         osc = OracleScene(g.desc_ptr())
         t_build = time.perf_counter() - t_b
-        cores = os.cpu_count() or 1
+        cores = min(16, os.cpu_count() or 1)  # the host CPU share of one GPU
         t_r = time.perf_counter()
         sums, osegs = osc.render(job0.params, pixels=pix, nsamples=spp, threads=cores, camera=cam)
         t_cpu = time.perf_counter() - t_r

@@ -154,15 +154,16 @@ struct rayon_server {
                active_pixels += px;
                scene_refs[j->scene]++;
                j->accounted = true;
+               j->t_admit = now_s();
                fresh.push_back(j);
+               active.push_back(j);  // counted as active from here, so drain() waits for it
             }
-            for (SJob* j : active)
-               if (j->samples > 0) run.push_back(j);
+            for (SJob* a : active)
+               if (a->samples > 0) run.push_back(a);  // fresh jobs have no samples yet
          }
          // create device state for admitted jobs (outside the lock: allocations)
          std::vector<SJob*> ok_fresh;
          for (SJob* j : fresh) {
-            j->t_admit = now_s();
             rayon_status st = rayon_job_create(ctx, j->scene, &j->params, &j->job);
             if (st == RAYON_OK && hipMalloc(&j->frame, (size_t)j->params.width * j->params.height * 4) != hipSuccess) {
                ry::set_error("hipMalloc of the first-frame buffer failed");
@@ -175,11 +176,11 @@ struct rayon_server {
                j->t_done = now_s();
                ++failed;
                free_job_locked(j);
+               active.erase(std::find(active.begin(), active.end(), j));
                done_cv.notify_all();
                continue;
             }
             j->state = RAYON_JOB_RUNNING;
-            active.push_back(j);
             ok_fresh.push_back(j);
          }
          const double t0 = now_s();
