@@ -5,7 +5,7 @@
 //             as the reference reorders geometries, scene_description.hpp:735-777)
 //   normals : float4[3*num_prims] vertex normals, only read for smooth triangles
 //   nodes   : Node2[num_nodes], 64 B = one cache line per visited pair of children
-//   qnodes  : QNode8[...], 8-wide quantized nodes (80 B), see bvh_quant.*
+//   wnodes  : WNode[num_qnodes], 8-wide quantized nodes (80 B), built on request
 //   mats    : MatRec[num_mats], 96 B
 //   lights  : int slot + float area per emitter
 #pragma once
