@@ -97,6 +97,17 @@ def main(argv=None) -> int:
                     help="which path's throughput is `value` (both are measured)")
     ap.add_argument("--cpu-stride", type=int, default=2, help="CPU baseline pixel subset: every Nth row/col")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["cornell_1080p", "batch256", "serve"], default="cornell_1080p",
+                    help="cornell_1080p: one 1080p 64-spp mesh frame per step (default); batch256: 256 mixed "
+                         "jobs, plain one-job-per-launch vs batched passes; serve: Poisson arrivals into the server")
+    ap.add_argument("--jobs", type=int, default=256, help="batch256: number of jobs")
+    ap.add_argument("--layout", choices=["binary", "quantized"], default="binary",
+                    help="batch256/serve: BVH node layout of the scenes")
+    ap.add_argument("--pass-spp", type=int, default=32, help="batch256: samples per batched pass")
+    ap.add_argument("--lam", type=str, default="5,20,50", help="serve: arrival rates (jobs/s), comma separated")
+    ap.add_argument("--duration", type=float, default=20.0, help="serve: seconds of arrivals per rate")
+    ap.add_argument("--grace", type=float, default=20.0, help="serve: seconds to finish after arrivals stop")
+    ap.add_argument("--pool", type=int, default=4, help="serve: resident scenes per generator")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device-to-device 4 times (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     a = ap.parse_args(argv)
@@ -118,6 +129,11 @@ def main(argv=None) -> int:
 
     import rayon_amd as ra
 
+    if a.workload == "batch256":
+        return bench_batch256(a, ra)
+    if a.workload == "serve":
+        return bench_serve(a, ra)
+
     log("generating scene")
     g = ra.Generated.generate(ra.GEN.CORNELL_ICOSPHERE, 7, detail=a.detail)
     ntris = g.desc.meshes[0].num_triangles + 10
@@ -133,7 +149,8 @@ def main(argv=None) -> int:
     W, H, spp = a.width, a.height, a.spp
 
     def new_job(seed, extra=0):
-        return ctx.job(sc, ra.job_params(W, H, cam, spp=spp, max_depth=a.depth, seed=seed, flags=flags | extra))
+        return ctx.job(sc, ra.job_params(W, H, cam, spp=spp, max_depth=a.depth, seed=seed, flags=flags | extra,
+                                         background_intensity=g.background_intensity()))
 
     def timed_steps(path):
         """K fresh 64-spp renders; wall time over the region, HIP-event time per launch."""
@@ -306,6 +323,277 @@ def main(argv=None) -> int:
     sc.close()
     ctx.close()
     print(json.dumps(result), flush=True)
+    return 0
+
+
+# ---------------------------------------------------------------------------------------
+# Multi-job workloads
+# ---------------------------------------------------------------------------------------
+
+RESOLUTIONS = [(256, 256), (512, 512), (1024, 1024), (1920, 1080)]
+GEN_NAMES = ["book_spheres", "cornell_icosphere", "sdf_terrain"]
+
+
+def draw_jobs(n, seed):
+    """Job mix of the batching workload: resolution uniform over RESOLUTIONS, generator
+    uniform over the three scene generators, per-job scene seed, spp uniform in 16..256."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for j in range(n):
+        W, H = RESOLUTIONS[int(rng.integers(len(RESOLUTIONS)))]
+        out.append(dict(index=j, gen=int(rng.integers(3)), scene_seed=j + 1, W=W, H=H,
+                        spp=int(rng.integers(16, 257)), seed=1000 + j))
+    return out
+
+
+def percentile(xs, q):
+    xs = [x for x in xs]
+    if not xs:
+        return None
+    return float(np.percentile(np.array(xs, dtype=np.float64), q))
+
+
+def make_scene(ra, ctx, gen, scene_seed, layout):
+    kind = [ra.GEN.BOOK_SPHERES, ra.GEN.CORNELL_ICOSPHERE, ra.GEN.SDF_TERRAIN][gen]
+    g = ra.Generated.generate(kind, scene_seed)
+    sc = ctx.scene(g.desc_ptr(), ra.BVH.QUANTIZED if layout == "quantized" else ra.BVH.DEFAULT)
+    return g, sc
+
+
+def job_params_for(ra, g, spec, depth=8, **kw):
+    lf, la, up, fov = g.camera()
+    cam = ra.camera_frame(lf, la, up, fov, spec["W"], spec["H"])
+    flags = ra.FLAG.NEE if spec["gen"] == 1 else 0
+    flags |= kw.pop("extra_flags", 0)
+    return ra.job_params(spec["W"], spec["H"], cam, spp=spec["spp"], max_depth=depth, seed=spec["seed"],
+                         flags=flags, background_intensity=g.background_intensity(), **kw), cam
+
+
+def bench_batch256(a, ra) -> int:
+    """256 jobs of uneven size: the plain kernel one job per launch, then the batched
+    wavefront (all jobs in each hipGraph-replayed pass of --pass-spp samples, launch shape
+    from the autotuner). Same jobs, same seeds, same results; reports Mrays/s of both and
+    p50/p99 job completion latency (all jobs submitted at t = 0)."""
+    specs = draw_jobs(a.jobs, 3)
+    ctx = ra.Context(0)
+    ev = HipEvents()
+    t0 = time.perf_counter()
+    gens, scenes, params = [], [], []
+    dev_bytes = 0
+    for k, sp in enumerate(specs):
+        g, sc = make_scene(ra, ctx, sp["gen"], sp["scene_seed"], a.layout)
+        P, cam = job_params_for(ra, g, sp)
+        gens.append(g)
+        scenes.append(sc)
+        params.append((P, cam))
+        dev_bytes += sc.info().device_bytes
+        if (k + 1) % 32 == 0:
+            log(f"scenes {k + 1}/{len(specs)} ({time.perf_counter() - t0:.0f} s, {dev_bytes / 2**30:.1f} GiB)")
+    px = sum(sp["W"] * sp["H"] for sp in specs)
+    samples = sum(sp["W"] * sp["H"] * sp["spp"] for sp in specs)
+    log(f"{len(specs)} jobs, {px / 1e6:.1f} Mpixels, {samples / 1e9:.2f} G samples; scenes {dev_bytes / 2**30:.1f} GiB")
+    stream = ctx.stream
+
+    def completion(evs, e_start, done_at):
+        return [ev.elapsed_ms(e_start, evs[i]) for i in done_at]
+
+    # ---- batched: autotune on clones, then progressive passes over all unfinished jobs
+    jobs = [ctx.job(scenes[k], params[k][0]) for k in range(len(specs))]
+    tune = ctx.autotune(jobs, 2)
+    log(f"autotune: capacity {tune.capacity} chunk {tune.chunk} wg/CU {tune.wg_per_cu} "
+        f"({tune.best_ms:.1f} ms vs {tune.start_ms:.1f} ms, {tune.candidates} shapes)")
+    ctx.sync()
+    e_start = ev.create()
+    pass_events, done_at = [], [None] * len(jobs)
+    ev.record(e_start, stream)
+    tb = time.perf_counter()
+    left = list(range(len(jobs)))
+    done_spp = [0] * len(jobs)
+    while left:
+        ctx.render_batched([jobs[i] for i in left], a.pass_spp)
+        e = ev.create()
+        ev.record(e, stream)
+        pass_events.append(e)
+        nxt = []
+        for i in left:
+            done_spp[i] = min(specs[i]["spp"], done_spp[i] + a.pass_spp)
+            if done_spp[i] >= specs[i]["spp"]:
+                done_at[i] = len(pass_events) - 1
+            else:
+                nxt.append(i)
+        left = nxt
+        log(f"batched pass {len(pass_events)}: {len(left)} jobs left")
+    ctx.sync()
+    b_wall = time.perf_counter() - tb
+    b_lat = completion(pass_events, e_start, done_at)
+    b_stats = [j.stats() for j in jobs]
+    b_segs = sum(s.segments for s in b_stats)
+    check = [0, 1, 2] + list(range(3, len(jobs), max(1, len(jobs) // 8)))
+    b_imgs = {i: jobs[i].resolve_linear() for i in check}
+    for j in jobs:
+        j.close()
+    log(f"batched: {b_segs / b_wall / 1e6:.0f} Mrays/s, {b_wall:.1f} s")
+
+    # ---- plain: one launch per job, jobs in submission order
+    jobs = [ctx.job(scenes[k], params[k][0]) for k in range(len(specs))]
+    ctx.sync()
+    e_start = ev.create()
+    job_events = []
+    ev.record(e_start, stream)
+    tp = time.perf_counter()
+    for k, j in enumerate(jobs):
+        j.render_plain(specs[k]["spp"])
+        e = ev.create()
+        ev.record(e, stream)
+        job_events.append(e)
+        if (k + 1) % 32 == 0:
+            log(f"plain: {k + 1} jobs enqueued")
+    ctx.sync()
+    p_wall = time.perf_counter() - tp
+    p_lat = completion(job_events, e_start, list(range(len(jobs))))
+    p_stats = [j.stats() for j in jobs]
+    p_segs = sum(s.segments for s in p_stats)
+    worst = 0.0
+    for i in check:
+        ia, ib = jobs[i].resolve_linear(), b_imgs[i]
+        worst = max(worst, float((np.abs(ia - ib) / np.maximum(np.abs(ia), 1e-3)).max()))
+    log(f"plain: {p_segs / p_wall / 1e6:.0f} Mrays/s, {p_wall:.1f} s; batched vs plain max rel diff {worst:.2e}")
+
+    # ---- FP64 oracle on a pixel subset of one job per generator (same streams)
+    rmse = {}
+    if not a.no_cpu_baseline:
+        from oracle import OracleScene
+        for gen in range(3):
+            cand = [i for i in range(len(specs)) if specs[i]["gen"] == gen]
+            if not cand:
+                continue
+            i = min(cand, key=lambda i: specs[i]["W"] * specs[i]["H"] * specs[i]["spp"])
+            sp, (P, cam) = specs[i], params[i]
+            st = 8
+            ys, xs = np.meshgrid(np.arange(st // 2, sp["H"], st), np.arange(st // 2, sp["W"], st), indexing="ij")
+            pix = (ys * sp["W"] + xs).reshape(-1).astype(np.int64)
+            osc = OracleScene(gens[i].desc_ptr())
+            sums, _ = osc.render(P, pixels=pix, nsamples=sp["spp"], threads=min(16, os.cpu_count() or 1),
+                                 camera=cam)
+            ref = sums / sp["spp"]
+            img = jobs[i].resolve_linear().reshape(-1, 3).astype(np.float64)[pix]
+            rmse[GEN_NAMES[gen]] = round(float(np.sqrt(((img - ref) ** 2).sum() / (ref ** 2).sum())), 5)
+            log(f"oracle {GEN_NAMES[gen]} job {i}: rel RMSE {rmse[GEN_NAMES[gen]]}")
+    for j in jobs:
+        j.close()
+
+    b_mrays, p_mrays = b_segs / b_wall / 1e6, p_segs / p_wall / 1e6
+    result = {
+        "metric": METRIC, "value": round(b_mrays, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(b_wall * 1e3, 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (procedural scenes generated in-process)",
+        "config": {"workload": f"batch256: {len(specs)} jobs, resolutions uniform over 256^2/512^2/1024^2/1920x1080, "
+                               f"generators uniform over book spheres / Cornell icosphere (1.31 M tris, NEE) / "
+                               f"SDF+terrain, per-job scene seeds, spp U{{16..256}}, depth 8, {a.layout} BVH",
+                   "jobs": len(specs), "megapixels": round(px / 1e6, 2), "samples": samples,
+                   "pass_spp": a.pass_spp, "layout": a.layout},
+        "batched_mrays_per_s": round(b_mrays, 2), "plain_mrays_per_s": round(p_mrays, 2),
+        "batched_over_plain": round(b_mrays / p_mrays, 3),
+        "latency_ms": {"batched_p50": round(percentile(b_lat, 50), 1), "batched_p99": round(percentile(b_lat, 99), 1),
+                       "plain_p50": round(percentile(p_lat, 50), 1), "plain_p99": round(percentile(p_lat, 99), 1)},
+        "segments": int(b_segs), "plain_segments": int(p_segs),
+        "autotune": {"capacity": tune.capacity, "chunk": tune.chunk, "wg_per_cu": tune.wg_per_cu,
+                     "best_ms": round(tune.best_ms, 2), "start_ms": round(tune.start_ms, 2),
+                     "shapes": tune.candidates},
+        "batched_vs_plain_max_rel_diff": worst,
+        "rel_rmse_vs_oracle": rmse,
+    }
+    for sc in scenes:
+        sc.close()
+    ctx.close()
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def bench_serve(a, ra) -> int:
+    """Poisson arrivals into the render server; per rate: p50/p99 time-to-first-frame and
+    time-to-converge, completed jobs and aggregate Mrays/s. Jobs are drawn like the batching
+    workload (scenes from a resident pool of --pool seeds per generator); half have a fixed
+    spp U{16..256}, half a noise target U(0.5%, 2%) (relative std-error, 16..1024 spp)."""
+    ctx = ra.Context(0)
+    pool = {}
+    for gen in range(3):
+        for s in range(a.pool):
+            pool[(gen, s)] = make_scene(ra, ctx, gen, 100 + s, a.layout)
+    log(f"scene pool: {len(pool)} scenes")
+    out = {"metric": METRIC, "unit": "Mrays/s", "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic (procedural scenes generated in-process)",
+           "config": {"workload": f"serve: Poisson arrivals, jobs as batch256 (pool of {a.pool} scenes per "
+                                  f"generator), 50% fixed spp U{{16..256}}, 50% noise target U(0.5%,2%), "
+                                  f"{a.duration:.0f} s per rate, HBM budget 200 GiB", "layout": a.layout},
+           "rates": []}
+    best = 0.0
+    for lam in [float(x) for x in a.lam.split(",") if x]:
+        rng = np.random.default_rng(42 + int(lam))
+        arrivals, t = [], 0.0
+        while True:
+            t += rng.exponential(1.0 / lam)
+            if t >= a.duration:
+                break
+            arrivals.append(t)
+        specs = draw_jobs(len(arrivals), 7 + int(lam))
+        plist = []
+        for sp in specs:
+            g, sc = pool[(sp["gen"], int(rng.integers(a.pool)))]
+            if rng.random() < 0.5:
+                P, _ = job_params_for(ra, g, sp)
+            else:
+                sp = dict(sp, spp=1024)
+                P, _ = job_params_for(ra, g, sp, extra_flags=ra.FLAG.ADAPTIVE_STDERR, min_adaptive_samples=16,
+                                      adaptive_threshold=float(rng.uniform(0.005, 0.02)))
+            plist.append((sc, P))
+        srv = ctx.server()
+        ids = []
+        t0 = time.perf_counter()
+        next_log = 5.0
+        for (sc, P), ta in zip(plist, arrivals):
+            dt = t0 + ta - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(srv.submit(sc, P))
+            if ta >= next_log:
+                st = srv.stats()
+                log(f"lam {lam:g}: t={ta:.0f}s submitted {len(ids)} active {st.active} queued {st.queued} "
+                    f"completed {st.completed}")
+                next_log += 5.0
+        try:
+            srv.drain(a.grace)
+        except ra.RayonError:
+            pass
+        wall = time.perf_counter() - t0
+        infos = [srv.info(i) for i in ids]
+        for i, inf in zip(ids, infos):
+            if inf.state in (ra.JOB.QUEUED, ra.JOB.RUNNING):
+                srv.cancel(i)
+        srv.drain(60.0)
+        infos = [srv.info(i) for i in ids]
+        segs = sum(inf.segments for inf in infos)
+        ttff = [inf.ttff_ms if inf.ttff_ms >= 0 else float("inf") for inf in infos]
+        ttc = [inf.ttc_ms if inf.ttc_ms >= 0 else float("inf") for inf in infos]
+        st = srv.stats()
+        srv.close()
+        mrays = segs / wall / 1e6
+        best = max(best, mrays)
+        r = {"lambda": lam, "jobs": len(ids), "completed": int(sum(inf.state == ra.JOB.DONE for inf in infos)),
+             "ttff_p50_ms": percentile(ttff, 50), "ttff_p99_ms": percentile(ttff, 99),
+             "ttc_p50_ms": percentile(ttc, 50), "ttc_p99_ms": percentile(ttc, 99),
+             "mrays_per_s": round(mrays, 2), "wall_s": round(wall, 1), "passes": int(st.passes),
+             "first_passes": int(st.first_passes)}
+        for k in ("ttff_p50_ms", "ttff_p99_ms", "ttc_p50_ms", "ttc_p99_ms"):
+            r[k] = None if r[k] is None or not np.isfinite(r[k]) else round(r[k], 1)
+        log(json.dumps(r))
+        out["rates"].append(r)
+    out["value"] = round(best, 2)
+    out["steps"], out["warmup"], out["ms_per_step"] = len(out["rates"]), 0, None
+    for g, sc in pool.values():
+        sc.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
     return 0
 
 

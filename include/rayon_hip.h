@@ -317,12 +317,49 @@ rayon_status rayon_trace_closest(rayon_ctx* ctx, rayon_scene* scene, const float
 rayon_status rayon_job_primary_rays(rayon_ctx* ctx, rayon_job* job, int32_t sample, float* origins,
                                     float* dirs);
 
+/* Path a scene's jobs take in batched passes. WAVEFRONT: the staged queue pipeline
+ * (generate / trace / shade / accumulate); MEGAKERNEL: every path of a pixel in one lane,
+ * all such jobs of a pass in one launch (the plain kernel's arithmetic). AUTO: megakernel
+ * for scenes of at most 8 MiB on the device, wavefront otherwise; rayon_autotune measures
+ * both on AUTO scenes and records the faster. A job keeps the path of its first pass
+ * until it is reset. Both compute the same estimator; their FP32 rounding differs. */
+enum { RAYON_PATH_AUTO = 0, RAYON_PATH_WAVEFRONT = 1, RAYON_PATH_MEGAKERNEL = 2 };
+rayon_status rayon_scene_set_batch_path(rayon_scene* scene, int32_t path);
+/* The path the scene's new jobs take (AUTO resolved). */
+int32_t rayon_scene_batch_path(const rayon_scene* scene);
+
+/* ---------------------------------------------------------------------------------
+ * Autotuner of batched passes (no counterpart in the reference, whose launch shape is
+ * fixed at 32x8 threads per block, renderer_cuda_device.cu:95-96). On clones of the given
+ * jobs (same scenes and parameters, fresh buffers, so the jobs themselves are untouched)
+ * it (1) times each RAYON_PATH_AUTO scene's jobs on both batched paths and records the
+ * faster on the scene, then (2) times wavefront launch shapes (path-slot capacity, queue
+ * chunk, workgroups per CU) by coordinate descent, sets the fastest on the context and
+ * caches it under a signature of the workload (job count, total work, depth, scenes,
+ * flags). The shape changes only the speed of a pass, never its results.
+ * ------------------------------------------------------------------------------- */
+typedef struct rayon_tune_result {
+   int64_t capacity;     /* RAYON_OPT_WAVE_CAPACITY chosen                                */
+   int32_t chunk;        /* RAYON_OPT_WAVE_CHUNK chosen                                   */
+   int32_t wg_per_cu;    /* RAYON_OPT_WAVE_WG_PER_CU chosen (0 = occupancy)               */
+   double best_ms;       /* one pass of `samples` with the chosen shape                  */
+   double start_ms;      /* the same pass with the shape the context had before          */
+   int32_t candidates;   /* passes timed (0 when taken from the cache)                    */
+   int32_t cached;
+   int32_t mega_scenes;  /* scenes of the jobs on the megakernel path after tuning        */
+   int32_t wave_scenes;  /* scenes of the jobs on the wavefront path after tuning         */
+} rayon_tune_result;
+
+rayon_status rayon_autotune(rayon_ctx* ctx, rayon_job* const* jobs, int32_t n, int32_t samples,
+                            rayon_tune_result* out);
+
 /* ---------------------------------------------------------------------------------
  * Render server: continuous batching of many jobs on one context (no counterpart in the
  * reference). A server thread admits queued jobs by HBM budget, gives every newly
  * admitted job a first pass and resolves its first frame on the device, advances the
- * other active jobs together in batched passes sized toward target_pass_ms, and retires
- * jobs whose sample budget is spent or whose pixels all converged.
+ * oldest active jobs (up to max_pass_pixels) together in batched passes sized toward
+ * target_pass_ms, and retires jobs whose sample budget is spent or whose pixels all
+ * converged.
  * ------------------------------------------------------------------------------- */
 typedef struct rayon_server rayon_server;
 
@@ -341,8 +378,8 @@ typedef struct rayon_server_config {
    int32_t max_pass_samples;    /* cap of the controlled pass size                       */
    int32_t pad_;
    double target_pass_ms;       /* controller target for one batched pass               */
-   int64_t max_active_pixels;   /* admission stops while the active jobs hold this many
-                                   pixels, so one sample of each fits a pass            */
+   int64_t max_pass_pixels;     /* pixels a refinement pass advances: the oldest active
+                                   jobs up to this many pixels (at least one job)       */
 } rayon_server_config;
 
 typedef struct rayon_server_job_info {

@@ -13,7 +13,7 @@ import numpy as np
 
 from .abi import (  # noqa: F401  (re-exported)
     Camera, GenParams, Geometry, JobParams, JobStats, Material, Mesh, SceneDesc, SceneInfo, SdfPrim, WaveStats,
-    ServerConfig, ServerJobInfo, ServerStats, MAT, GEOM, SDF, FLAG, GEN, OPT, BVH, JOB, STATUS, declare,
+    ServerConfig, ServerJobInfo, ServerStats, TuneResult, PATH, MAT, GEOM, SDF, FLAG, GEN, OPT, BVH, JOB, STATUS, declare,
 )
 
 LIB_PATH = Path(__file__).resolve().parent / "librayon_hip.so"
@@ -154,6 +154,13 @@ class Context:
         _check(lib().rayon_ctx_get_option(self._h, opt, C.byref(v)), "rayon_ctx_get_option")
         return v.value
 
+    def autotune(self, jobs, samples: int) -> TuneResult:
+        """Times wavefront launch shapes on clones of `jobs` and keeps the fastest on this context."""
+        arr = (C.c_void_p * len(jobs))(*[j._h for j in jobs])
+        r = TuneResult()
+        _check(lib().rayon_autotune(self._h, arr, len(jobs), samples, C.byref(r)), "rayon_autotune")
+        return r
+
     def server(self, **cfg) -> "Server":
         return Server(self, **cfg)
 
@@ -182,6 +189,13 @@ class Scene:
         i = SceneInfo()
         _check(lib().rayon_scene_info_get(self._h, C.byref(i)), "rayon_scene_info_get")
         return i
+
+    def set_batch_path(self, path: int):
+        _check(lib().rayon_scene_set_batch_path(self._h, path), "rayon_scene_set_batch_path")
+
+    @property
+    def batch_path(self) -> int:
+        return lib().rayon_scene_batch_path(self._h)
 
     def prim_order(self) -> np.ndarray:
         n = self.info().num_prims

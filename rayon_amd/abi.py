@@ -24,6 +24,7 @@ FLAG = _NS(REF_QUIRKS=1, REF_RNG=2, NEE=4, ADAPTIVE_REF=8, ADAPTIVE_STDERR=16, D
 GEN = _NS(BOOK_SPHERES=0, CORNELL_ICOSPHERE=1, SDF_TERRAIN=2)
 BVH = _NS(DEFAULT=0, QUANTIZED=1)
 JOB = _NS(QUEUED=0, RUNNING=1, DONE=2, FAILED=3, CANCELLED=4)
+PATH = _NS(AUTO=0, WAVEFRONT=1, MEGAKERNEL=2)
 OPT = _NS(WAVE_CAPACITY=0, WAVE_WG_PER_CU=1, GRAPH=2, STAGE_TIMING=3, COUNT_TRAVERSAL=4, WAVE_CHUNK=5)
 
 
@@ -97,7 +98,7 @@ class WaveStats(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_active_jobs", C.c_int32),
                 ("first_pass_samples", C.c_int32), ("max_pass_samples", C.c_int32), ("pad_", C.c_int32),
-                ("target_pass_ms", C.c_double), ("max_active_pixels", C.c_int64)]
+                ("target_pass_ms", C.c_double), ("max_pass_pixels", C.c_int64)]
 
 
 class ServerJobInfo(C.Structure):
@@ -110,6 +111,12 @@ class ServerStats(C.Structure):
                 ("failed", C.c_int64), ("queued", C.c_int64), ("active", C.c_int64), ("segments", C.c_int64),
                 ("used_bytes", C.c_int64), ("busy_s", C.c_double), ("wall_s", C.c_double),
                 ("pass_samples", C.c_int32), ("pad_", C.c_int32)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("capacity", C.c_int64), ("chunk", C.c_int32), ("wg_per_cu", C.c_int32), ("best_ms", C.c_double),
+                ("start_ms", C.c_double), ("candidates", C.c_int32), ("cached", C.c_int32),
+                ("mega_scenes", C.c_int32), ("wave_scenes", C.c_int32)]
 
 
 class GenParams(C.Structure):
@@ -149,6 +156,9 @@ def declare(lib: C.CDLL) -> C.CDLL:
         "rayon_job_stats_get": (st, [vp, vp, P(JobStats)]),
         "rayon_trace_closest": (st, [vp, vp, P(f32), P(f32), i64, f32, f32, P(i64), P(f32)]),
         "rayon_job_primary_rays": (st, [vp, vp, i32, P(f32), P(f32)]),
+        "rayon_scene_set_batch_path": (st, [vp, i32]),
+        "rayon_scene_batch_path": (i32, [vp]),
+        "rayon_autotune": (st, [vp, P(vp), i32, i32, P(TuneResult)]),
         "rayon_server_config_default": (None, [P(ServerConfig)]),
         "rayon_server_create": (st, [vp, P(ServerConfig), P(vp)]),
         "rayon_server_submit": (st, [vp, vp, P(JobParams), P(C.c_uint64)]),

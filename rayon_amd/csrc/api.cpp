@@ -305,8 +305,17 @@ rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* d, uint3
    std::vector<SdfRec> sdf;
    for (int i = 0; i < d->num_sdf_prims; ++i) {
       const rayon_sdf_prim& q = d->sdf_prims[i];
-      sdf.push_back({{q.center[0], q.center[1], q.center[2], (float)q.type},
-                     {q.params[0], q.params[1], q.params[2], q.params[3]}});
+      SdfRec r{{q.center[0], q.center[1], q.center[2], -1.0f}, {0.0f, 0.0f, 0.0f, 0.0f}};
+      if (q.type == RAYON_SDF_SPHERE) {
+         r.p.w = q.params[0];
+      } else if (q.type == RAYON_SDF_ROUNDED_BOX) {
+         const float rr = q.params[3];
+         r.p = {q.params[0] - rr, q.params[1] - rr, q.params[2] - rr, rr};
+      } else {
+         r.c.w = q.params[0];
+         r.p.w = q.params[1];
+      }
+      sdf.push_back(r);
    }
    (void)hipSetDevice(ctx->device);
    SceneView& V = sc->view;
@@ -343,6 +352,9 @@ rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* d, uint3
    V.sdf_k = d->num_sdf_prims > 0 ? d->sdf_smooth_k : 0.1f;
    V.sdf_eps = d->num_sdf_prims > 0 ? d->sdf_eps : 1e-4f;
    V.sdf_steps = d->num_sdf_prims > 0 ? d->sdf_max_steps : 0;
+   V.sdf_prim = -1;
+   for (size_t i = 0; i < n; ++i)
+      if (prims[i].kind == K_SDF) V.sdf_prim = slot_of[i];
    rayon_scene_info& I = sc->info;
    I.num_prims = (int64_t)n;
    I.num_nodes = (int64_t)br.nodes.size();
@@ -399,8 +411,20 @@ rayon_status rayon_job_reset(rayon_job* j) {
    RY_HIP(launch_fill_u32((uint32_t*)j->view.counters, 0u, 2 * 8 * kCounterSlots, st));
    if (j->view.rng) RY_HIP(launch_ref_rng_init(j->view.rng, (int)npix, st));
    j->samples_done = 0;
+   j->path = -1;
    return RAYON_OK;
 }
+
+rayon_status rayon_scene_set_batch_path(rayon_scene* scene, int32_t path) {
+   if (!scene || path < RAYON_PATH_AUTO || path > RAYON_PATH_MEGAKERNEL) {
+      set_error("rayon_scene_set_batch_path: NULL scene or unknown path %d", (int)path);
+      return RAYON_ERR_INVALID;
+   }
+   scene->batch_path = path;
+   return RAYON_OK;
+}
+
+int32_t rayon_scene_batch_path(const rayon_scene* scene) { return scene ? ry::scene_batch_path(scene) : -1; }
 
 rayon_status rayon_job_create(rayon_ctx* ctx, rayon_scene* sc, const rayon_job_params* p, rayon_job** out) {
    if (!ctx || !sc || !p || !out) {

@@ -553,6 +553,16 @@ rayon_status gather_prims(const rayon_scene_desc* d, std::vector<HostPrim>& prim
       double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
       for (int i = 0; i < d->num_sdf_prims; ++i) {
          const rayon_sdf_prim& q = d->sdf_prims[i];
+         const float* a = q.params;
+         const bool ok = q.type == RAYON_SDF_SPHERE        ? a[0] > 0.0f
+                         : q.type == RAYON_SDF_ROUNDED_BOX ? (a[0] > 0.0f && a[1] > 0.0f && a[2] > 0.0f &&
+                                                              a[3] >= 0.0f && a[3] <= std::min({a[0], a[1], a[2]}))
+                         : q.type == RAYON_SDF_TORUS       ? (a[0] >= 0.0f && a[1] > 0.0f)
+                                                           : true;
+         if (!ok) {
+            set_error("SDF primitive %d: radius, half extents or rounding out of range", i);
+            return RAYON_ERR_INVALID;
+         }
          double e[3];
          if (q.type == RAYON_SDF_SPHERE) {
             e[0] = e[1] = e[2] = q.params[0];

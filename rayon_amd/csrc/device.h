@@ -116,6 +116,7 @@ struct HitMin {
    int slot;
    int orig;
    float u, v;
+   int sdf;  // slot of an SDF field left for a later marching stage (-1: none)
 };
 
 // Moller-Trumbore on (v0, e1, e2), the arithmetic of hit_triangle
@@ -207,38 +208,113 @@ __device__ __forceinline__ bool isect_rect(V3 q, V3 u, V3 v, V3 o, V3 d, float t
 }
 
 // ---- SDF field: smooth union of analytic primitives (sdf.hpp:105-143, :410-414)
+// Every primitive is evaluated by the one rounded-box formula (no per-type branch, so the
+// lanes of a wave stay together whatever primitive each one reaches): a sphere is a box of
+// zero half-extent rounded by r; a torus is the same applied to (|xz| - R, y, 0).
 __device__ __forceinline__ float sdf_prim(const SdfRec& s, V3 p) {
-   V3 o = p - mk(s.c);
-   int type = (int)s.c.w;
-   if (type == RAYON_SDF_SPHERE) return sqrtf(dot(o, o)) - s.p.x;
-   if (type == RAYON_SDF_ROUNDED_BOX) {
-      float rr = s.p.w;
-      float qx = fabsf(o.x) - (s.p.x - rr), qy = fabsf(o.y) - (s.p.y - rr), qz = fabsf(o.z) - (s.p.z - rr);
-      V3 m = mk(fmaxf(qx, 0.0f), fmaxf(qy, 0.0f), fmaxf(qz, 0.0f));
-      return sqrtf(dot(m, m)) + fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f) - rr;
-   }
-   float qx = sqrtf(o.x * o.x + o.z * o.z) - s.p.x;
-   return sqrtf(qx * qx + o.y * o.y) - s.p.y;
+   const V3 o = p - mk(s.c);
+   const float lxz = sqrtf(o.x * o.x + o.z * o.z);
+   const bool torus = s.c.w >= 0.0f;
+   const float ax = torus ? lxz - s.c.w : o.x, ay = o.y, az = torus ? 0.0f : o.z;
+   const float qx = fabsf(ax) - s.p.x, qy = fabsf(ay) - s.p.y, qz = fabsf(az) - s.p.z;
+   const float mx = fmaxf(qx, 0.0f), my = fmaxf(qy, 0.0f), mz = fmaxf(qz, 0.0f);
+   return sqrtf(mx * mx + my * my + mz * mz) + fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f) - s.p.w;
 }
 
+__device__ __forceinline__ float smooth_union(float d, float d2, float k, float ik) {
+   float h = fminf(fmaxf(0.5f + 0.5f * (d2 - d) * ik, 0.0f), 1.0f);
+   return d2 * (1.0f - h) + d * h - k * h * (1.0f - h);
+}
+
+// The whole field: primitives combined in index order.
 __device__ __forceinline__ float sdf_field(const SceneView& S, V3 p) {
    float d = sdf_prim(S.sdf[0], p);
    const float k = S.sdf_k, ik = 1.0f / S.sdf_k;
-   for (int i = 1; i < S.num_sdf; ++i) {
-      float d2 = sdf_prim(S.sdf[i], p);
-      float h = fminf(fmaxf(0.5f + 0.5f * (d2 - d) * ik, 0.0f), 1.0f);
-      d = d2 * (1.0f - h) + d * h - k * h * (1.0f - h);
+   for (int i = 1; i < S.num_sdf; ++i) d = smooth_union(d, sdf_prim(S.sdf[i], p), k, ik);
+   return d;
+}
+
+// Fields of up to 64 primitives are evaluated over the primitives near the ray only. A
+// primitive whose bounding sphere stays farther than `margin` from every point considered
+// has a distance >= margin there; in a smooth union a term that exceeds the running value
+// by at least k leaves it unchanged (h = 1), so near a surface (field value << margin) the
+// subset gives the same field as all primitives unless dozens of far primitives blend with
+// each other first. Marching steps are capped below the far primitives' distance bound.
+constexpr int kSdfMaskMax = 64;
+
+__device__ __forceinline__ float sdf_margin(const SceneView& S) { return fmaxf(10.0f * S.sdf_k, 1.0f); }
+
+__device__ __forceinline__ float sdf_bound(const SdfRec& r) {
+   return sqrtf(r.p.x * r.p.x + r.p.y * r.p.y + r.p.z * r.p.z) + r.p.w + fmaxf(r.c.w, 0.0f);
+}
+
+__device__ __forceinline__ float sdf_field_masked(const SceneView& S, V3 p, uint64_t mask) {
+   const float k = S.sdf_k, ik = 1.0f / S.sdf_k;
+   int i = __builtin_ctzll(mask);
+   float d = sdf_prim(S.sdf[i], p);
+   for (mask &= mask - 1; mask; mask &= mask - 1) {
+      i = __builtin_ctzll(mask);
+      d = smooth_union(d, sdf_prim(S.sdf[i], p), k, ik);
    }
    return d;
 }
 
+// Primitives within `margin` of the segment o + t*dn, t in [t0, t1]; `far` receives the
+// smallest distance bound of the others.
+__device__ __forceinline__ uint64_t sdf_ray_mask(const SceneView& S, V3 o, V3 dn, float t0, float t1, float margin,
+                                                 float& far) {
+   uint64_t mask = 0;
+   far = 3.4e38f;
+   for (int i = 0; i < S.num_sdf; ++i) {
+      const SdfRec r = S.sdf[i];
+      const V3 c = mk(r.c);
+      const float tc = fminf(fmaxf(dot(c - o, dn), t0), t1);
+      const V3 q = o + tc * dn - c;
+      const float dist = sqrtf(dot(q, q)) - sdf_bound(r);
+      if (dist <= margin)
+         mask |= 1ull << i;
+      else
+         far = fminf(far, dist);
+   }
+   return mask;
+}
+
+// Field at a point from the primitives within `margin` of it (all of them for large fields).
+__device__ __forceinline__ float sdf_field_near(const SceneView& S, V3 p, uint64_t mask) {
+   return S.num_sdf > kSdfMaskMax ? sdf_field(S, p) : sdf_field_masked(S, p, mask);
+}
+
+__device__ __forceinline__ uint64_t sdf_point_mask(const SceneView& S, V3 p) {
+   if (S.num_sdf > kSdfMaskMax) return ~0ull;
+   const float margin = sdf_margin(S);
+   uint64_t mask = 0;
+   for (int i = 0; i < S.num_sdf; ++i) {
+      const SdfRec r = S.sdf[i];
+      const V3 q = p - mk(r.c);
+      if (sqrtf(dot(q, q)) - sdf_bound(r) <= margin) mask |= 1ull << i;
+   }
+   return mask ? mask : 1ull;
+}
+
 // Sphere tracing from the entry of the field's box along the unit direction
-// (rayMarch, sdf.hpp:518-563). Returns the ray parameter of the hit.
-__device__ inline bool isect_sdf(const SceneView& S, const PrimRec& p, V3 o, V3 d, float tmin, float tmax, float& t) {
-   float dl = sqrtf(dot(d, d));
-   float idl = 1.0f / dl;
-   V3 dn = idl * d;
-   float t0 = tmin * dl, t1 = tmax * dl;
+// (rayMarch, sdf.hpp:518-563), split into a set-up and a resumable run so the batched path
+// can march in slices of steps and regroup the rays still marching between slices.
+struct SdfMarch {
+   float tw;     // distance marched along the unit direction
+   float tend;   // end of the segment
+   float cap;    // step cap below the distance bound of the primitives left out
+   int used;     // steps taken
+   uint64_t mask;
+};
+
+// Returns false when the ray cannot meet the field in [tmin, tmax].
+__device__ __forceinline__ bool sdf_march_begin(const SceneView& S, const PrimRec& p, V3 o, V3 d, float tmin,
+                                                float tmax, SdfMarch& m) {
+   const float dl = sqrtf(dot(d, d));
+   const V3 dn = (1.0f / dl) * d;
+   // the march (start, primitive subset, step cap) depends on tmin and the box only, not
+   // on tmax, so it takes the same steps whatever closer hit bounds it
+   float t0 = tmin * dl, tb = 3.402823466e38f;
    float lo[3] = {p.a.x, p.a.y, p.a.z}, hi[3] = {p.b.x, p.b.y, p.b.z};
    float oo[3] = {o.x, o.y, o.z}, dd[3] = {dn.x, dn.y, dn.z};
 #pragma unroll
@@ -246,20 +322,47 @@ __device__ inline bool isect_sdf(const SceneView& S, const PrimRec& p, V3 o, V3 
       float inv = safe_inv(dd[k]);
       float a = (lo[k] - oo[k]) * inv, b = (hi[k] - oo[k]) * inv;
       t0 = fmaxf(t0, fminf(a, b));
-      t1 = fminf(t1, fmaxf(a, b));
+      tb = fminf(tb, fmaxf(a, b));
    }
+   const float t1 = fminf(tb, tmax * dl);
    if (t0 > t1) return false;
-   float tw = t0;
-   for (int step = 0; step < S.sdf_steps; ++step) {
-      float dist = sdf_field(S, o + tw * dn);
-      if (dist < S.sdf_eps) {
-         t = tw * idl;
-         return true;
-      }
-      tw += dist;
-      if (tw > t1) break;
+   m.mask = ~0ull;
+   m.cap = 3.4e38f;
+   if (S.num_sdf <= kSdfMaskMax) {
+      float far;
+      m.mask = sdf_ray_mask(S, o, dn, t0, tb, sdf_margin(S), far);
+      if (!m.mask) return false;  // every primitive stays >= margin away: no surface on the segment
+      m.cap = far - 2.0f * S.sdf_k;
    }
-   return false;
+   m.tw = t0;
+   m.tend = t1;
+   m.used = 0;
+   return true;
+}
+
+// Up to `steps` more steps. Returns 1 on a hit (t = ray parameter), 0 when the march ended
+// without one, -1 when it needs more steps.
+__device__ __forceinline__ int sdf_march_run(const SceneView& S, V3 o, V3 d, SdfMarch& m, int steps, float& t) {
+   const float dl = sqrtf(dot(d, d));
+   const float idl = 1.0f / dl;
+   const V3 dn = idl * d;
+   const int stop = min(S.sdf_steps, m.used + steps);
+   for (; m.used < stop; ++m.used) {
+      const float dist = sdf_field_near(S, o + m.tw * dn, m.mask);
+      if (dist < S.sdf_eps) {
+         t = m.tw * idl;
+         return 1;
+      }
+      m.tw += fminf(dist, m.cap);
+      if (m.tw > m.tend) return 0;
+   }
+   return m.used >= S.sdf_steps ? 0 : -1;
+}
+
+__device__ inline bool isect_sdf(const SceneView& S, const PrimRec& p, V3 o, V3 d, float tmin, float tmax, float& t) {
+   SdfMarch m;
+   if (!sdf_march_begin(S, p, o, d, tmin, tmax, m)) return false;
+   return sdf_march_run(S, o, d, m, S.sdf_steps, t) == 1;
 }
 
 // Closest hit over the scene. Exact t ties go to the larger original primitive index,
@@ -271,7 +374,7 @@ struct TravCount {
    unsigned nodes = 0, prims = 0;
 };
 
-template <int STRIDE, bool CNT = false>
+template <int STRIDE, bool CNT = false, bool DEFER = false>
 __device__ inline HitMin closest_hit_bin(const SceneView& S, V3 o, V3 d, float tmin, float tmax, int* stack,
                                          TravCount* tc = nullptr) {
    HitMin best;
@@ -279,6 +382,7 @@ __device__ inline HitMin closest_hit_bin(const SceneView& S, V3 o, V3 d, float t
    best.slot = -1;
    best.orig = -1;
    best.u = best.v = 0.0f;
+   best.sdf = -1;
    int cur = S.root;
    if (cur == kSentinel) return best;
    const RayInv r = make_rayinv(o, d);
@@ -324,8 +428,12 @@ __device__ inline HitMin closest_hit_bin(const SceneView& S, V3 o, V3 d, float t
             hit = isect_sphere(mk(p.a), p.b.x, o, d, tmin, best.t, t);
          else if (kind == K_RECT)
             hit = isect_rect(mk(p.a), mk(p.b), mk(p.c), o, d, tmin, best.t, t);
-         else
+         else if constexpr (DEFER) {
+            best.sdf = i;
+            hit = false;
+         } else {
             hit = isect_sdf(S, p, o, d, tmin, best.t, t);
+         }
          if (hit) {
             const int orig = __float_as_int(p.b.w);
             if (t < best.t || orig > best.orig) {
@@ -398,7 +506,7 @@ __device__ inline bool any_hit_bin(const SceneView& S, V3 o, V3 d, float tmin, f
 }
 
 // Tests one primitive for the closest hit (shared by both tree layouts).
-template <bool CNT>
+template <bool CNT, bool DEFER = false>
 __device__ __forceinline__ void test_closest(const SceneView& S, int i, V3 o, V3 d, float tmin, HitMin& best,
                                              TravCount* tc) {
    const PrimRec p = S.prims[i];
@@ -413,8 +521,12 @@ __device__ __forceinline__ void test_closest(const SceneView& S, int i, V3 o, V3
       hit = isect_sphere(mk(p.a), p.b.x, o, d, tmin, best.t, t);
    else if (kind == K_RECT)
       hit = isect_rect(mk(p.a), mk(p.b), mk(p.c), o, d, tmin, best.t, t);
-   else
+   else if constexpr (DEFER) {
+      best.sdf = i;
+      hit = false;
+   } else {
       hit = isect_sdf(S, p, o, d, tmin, best.t, t);
+   }
    if (hit) {
       const int orig = __float_as_int(p.b.w);
       if (t < best.t || orig > best.orig) {
@@ -491,7 +603,7 @@ __device__ __forceinline__ WNode load_wnode(const WNode* nodes, uint32_t i) {
 // Closest hit over the 8-wide quantized tree. The stack holds one entry per partly
 // visited node: its first internal child and the remaining hit mask (with the node's
 // internal-child mask, to locate a child among its internal siblings).
-template <int STRIDE, bool CNT>
+template <int STRIDE, bool CNT, bool DEFER = false>
 __device__ inline HitMin closest_hit_wide(const SceneView& S, V3 o, V3 d, float tmin, float tmax, int* stack,
                                           TravCount* tc) {
    HitMin best;
@@ -499,6 +611,7 @@ __device__ inline HitMin closest_hit_wide(const SceneView& S, V3 o, V3 d, float 
    best.slot = -1;
    best.orig = -1;
    best.u = best.v = 0.0f;
+   best.sdf = -1;
    const V3 inv = mk(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
    const uint32_t perm = 7u ^ ((d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u));
    int sp = 0;
@@ -514,7 +627,7 @@ __device__ inline HitMin closest_hit_wide(const SceneView& S, V3 o, V3 d, float 
          const uint32_t m = ((sl < 4 ? n.meta[0] : n.meta[1]) >> (8 * (sl & 3))) & 0xFFu;
          const int first = (int)n.prim_base + (int)(m & 31u);
          const int cnt = (int)(m >> 5) + 1;
-         for (int i = first; i < first + cnt; ++i) test_closest<CNT>(S, i, o, d, tmin, best, tc);
+         for (int i = first; i < first + cnt; ++i) test_closest<CNT, DEFER>(S, i, o, d, tmin, best, tc);
       }
       const uint32_t imask = n.e_imask >> 24;
       uint32_t base = n.child_base;
@@ -602,12 +715,14 @@ __device__ inline bool any_hit_wide(const SceneView& S, V3 o, V3 d, float tmin, 
 // tree, or either (chosen per ray from the scene, for batches that mix layouts).
 enum Layout : int { L_BIN = 0, L_WIDE = 1, L_ANY = 2 };
 
-template <int STRIDE, bool CNT = false, int LAYOUT = L_ANY>
+// DEFER: an SDF field met by the traversal is not marched; its slot is returned in
+// HitMin::sdf for a later marching stage (which marches up to the closest other hit).
+template <int STRIDE, bool CNT = false, int LAYOUT = L_ANY, bool DEFER = false>
 __device__ inline HitMin closest_hit(const SceneView& S, V3 o, V3 d, float tmin, float tmax, int* stack,
                                      TravCount* tc = nullptr) {
    if (LAYOUT == L_WIDE || (LAYOUT == L_ANY && S.wnodes))
-      return closest_hit_wide<STRIDE, CNT>(S, o, d, tmin, tmax, stack, tc);
-   return closest_hit_bin<STRIDE, CNT>(S, o, d, tmin, tmax, stack, tc);
+      return closest_hit_wide<STRIDE, CNT, DEFER>(S, o, d, tmin, tmax, stack, tc);
+   return closest_hit_bin<STRIDE, CNT, DEFER>(S, o, d, tmin, tmax, stack, tc);
 }
 
 template <int STRIDE, bool CNT = false, int LAYOUT = L_ANY>
@@ -680,9 +795,10 @@ __device__ inline Surf surface(const SceneView& S, const HitMin& h, V3 o, V3 d) 
       s.tangent = unit(mk(p.b));
    } else if (kind == K_SDF) {  // central differences, sdf.hpp:499-507
       const float e = S.sdf_eps;
-      V3 g = mk(sdf_field(S, s.p + mk(e, 0, 0)) - sdf_field(S, s.p - mk(e, 0, 0)),
-                sdf_field(S, s.p + mk(0, e, 0)) - sdf_field(S, s.p - mk(0, e, 0)),
-                sdf_field(S, s.p + mk(0, 0, e)) - sdf_field(S, s.p - mk(0, 0, e)));
+      const uint64_t m = sdf_point_mask(S, s.p);
+      V3 g = mk(sdf_field_near(S, s.p + mk(e, 0, 0), m) - sdf_field_near(S, s.p - mk(e, 0, 0), m),
+                sdf_field_near(S, s.p + mk(0, e, 0), m) - sdf_field_near(S, s.p - mk(0, e, 0), m),
+                sdf_field_near(S, s.p + mk(0, 0, e), m) - sdf_field_near(S, s.p - mk(0, 0, e), m));
       V3 n = unit(g);
       s.front = dot(d, n) < 0.0f;
       s.n = s.front ? n : -n;

@@ -236,6 +236,7 @@ void icosphere(int levels, std::vector<double>& V, std::vector<int32_t>& F) {
 // displaced radially by r*0.05*vnoise(4*dir; seed 7). The sphere's material is a GGX
 // conductor or GGX dielectric with roughness U(0.05,0.8), drawn from seed+1.
 void gen_cornell(rayon_generated* g, uint64_t seed, int levels) {
+   g->background_intensity = 0.0f;  // the quad light is the only source
    int white = add_mat(g, mat_base(RAYON_MAT_LAMBERTIAN, 0.73f, 0.73f, 0.73f));
    int red = add_mat(g, mat_base(RAYON_MAT_LAMBERTIAN, 0.65f, 0.05f, 0.05f));
    int green = add_mat(g, mat_base(RAYON_MAT_LAMBERTIAN, 0.12f, 0.45f, 0.15f));

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -66,6 +67,9 @@ struct rayon_ctx {
    std::mutex mu;
    // wavefront state (shared by all batched launches of this context)
    ry::Wave* wave = nullptr;
+   // autotuner results by workload signature
+   std::mutex tune_mu;
+   std::map<std::vector<long long>, rayon_tune_result> tune_cache;
 };
 
 struct rayon_scene {
@@ -74,6 +78,7 @@ struct rayon_scene {
    std::vector<ry::DevBuf> bufs;
    rayon_scene_info info{};
    std::vector<int64_t> orig_of_slot;
+   int32_t batch_path = RAYON_PATH_AUTO;  // path of this scene's jobs in batched passes
 };
 
 struct rayon_job {
@@ -83,9 +88,11 @@ struct rayon_job {
    ry::JobView view{};
    std::vector<ry::DevBuf> bufs;
    int64_t samples_done = 0;
+   int32_t path = -1;       // batched path taken since the first pass (-1: not yet chosen)
    float4* prev = nullptr;  // sums at pass start (batched passes with the reference adaptive test)
 };
 
 namespace ry {
 void fill_jobview(rayon_job* j);
+int scene_batch_path(const rayon_scene* s);
 }

@@ -9,6 +9,11 @@
 namespace ry {
 
 hipError_t launch_plain(const JobView& J, int samples, hipStream_t st);
+// All tiles of `njobs` jobs in one launch (device tables: job views, tile prefix with
+// njobs + 1 entries, samples per job); layout: 0 binary, 1 wide, 2 mixed.
+hipError_t launch_multi(const JobView* jobs, const int* tile_base, const int* todo, int njobs, int total_tiles,
+                        bool count, int layout, hipStream_t st);
+int plain_tiles(int width, int height);
 hipError_t launch_resolve_u8(const JobView& J, uint8_t* out, int channels, float gamma, hipStream_t st);
 hipError_t launch_resolve_linear(const JobView& J, float* out, hipStream_t st);
 hipError_t launch_trace_closest(const SceneView& S, const float* o, const float* d, long long n, float tmin,

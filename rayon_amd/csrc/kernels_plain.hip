@@ -18,48 +18,48 @@ constexpr int kBlock = kTile * kTile;
 
 __device__ __forceinline__ int xcd_tile(int b, int per_xcd) { return (b & 7) * per_xcd + (b >> 3); }
 
+// One 16x16 tile of a job: every lane runs `samples` whole paths of its pixel in sample
+// order and folds them into the compensated sums once, then the wave's statistics go to
+// counter slot `slot`.
 template <bool CNT, int LAYOUT>
-__global__ __launch_bounds__(kBlock) void k_plain(JobView J, int samples, int tiles_x, int ntiles, int per_xcd) {
-   __shared__ int stack[kStack * kBlock];
+__device__ __forceinline__ void plain_tile(const JobView& J, int samples, int tiles_x, int tile, int slot,
+                                           int* stack) {
    const int tid = threadIdx.x;
-   const int tile = xcd_tile(blockIdx.x, per_xcd);
    PathCounts pc;
    int primary = 0, converged = 0;
-   if (tile < ntiles) {
-      const int x = (tile % tiles_x) * kTile + (tid & (kTile - 1));
-      const int y = (tile / tiles_x) * kTile + (tid / kTile);
-      if (x < J.width && y < J.height) {
-         const int pix = y * J.width + x;
-         const int cnt = J.counts[pix];
-         int todo = samples;
-         if (J.spp_target > 0) todo = min(todo, J.spp_target - cnt);
-         if (cnt >= 0 && todo > 0) {
-            const float4 a0 = J.acc[pix];
-            float4 a = a0, c = J.comp[pix];
-            const bool refrng = (J.flags & RAYON_FLAG_REF_RNG) != 0;
-            Rng R{refrng ? J.rng[pix] : 0u, refrng};
-            for (int s = 0; s < todo; ++s) {
-               if (!refrng) R.s = counter_init(J.seed_lo, J.seed_hi, (uint32_t)pix, (uint32_t)(cnt + s));
-               V3 o, d;
-               camera_ray(J, x, y, R, o, d);
-               const V3 L = trace_path<kBlock, CNT, LAYOUT>(J, o, d, R, stack + tid, pc);
-               accumulate(a, c, L);
-            }
-            primary = todo;
-            J.acc[pix] = a;
-            J.comp[pix] = c;
-            if (refrng) J.rng[pix] = R.s;
-            const int n = adaptive_count(J, a0, cnt, a, c, cnt + todo);
-            converged = n < 0 ? 1 : 0;
-            J.counts[pix] = n;
+   const int x = (tile % tiles_x) * kTile + (tid & (kTile - 1));
+   const int y = (tile / tiles_x) * kTile + (tid / kTile);
+   if (x < J.width && y < J.height) {
+      const int pix = y * J.width + x;
+      const int cnt = J.counts[pix];
+      int todo = samples;
+      if (J.spp_target > 0) todo = min(todo, J.spp_target - cnt);
+      if (cnt >= 0 && todo > 0) {
+         const float4 a0 = J.acc[pix];
+         float4 a = a0, c = J.comp[pix];
+         const bool refrng = (J.flags & RAYON_FLAG_REF_RNG) != 0;
+         Rng R{refrng ? J.rng[pix] : 0u, refrng};
+         for (int s = 0; s < todo; ++s) {
+            if (!refrng) R.s = counter_init(J.seed_lo, J.seed_hi, (uint32_t)pix, (uint32_t)(cnt + s));
+            V3 o, d;
+            camera_ray(J, x, y, R, o, d);
+            const V3 L = trace_path<kBlock, CNT, LAYOUT>(J, o, d, R, stack + tid, pc);
+            accumulate(a, c, L);
          }
+         primary = todo;
+         J.acc[pix] = a;
+         J.comp[pix] = c;
+         if (refrng) J.rng[pix] = R.s;
+         const int n = adaptive_count(J, a0, cnt, a, c, cnt + todo);
+         converged = n < 0 ? 1 : 0;
+         J.counts[pix] = n;
       }
    }
    unsigned long long v0 = wave_sum((unsigned long long)pc.segments + (unsigned long long)pc.shadow);
    unsigned long long v1 = wave_sum((unsigned long long)pc.shadow);
    unsigned long long v2 = wave_sum((unsigned long long)primary);
    unsigned long long v3 = wave_sum((unsigned long long)converged);
-   unsigned long long* ctr = J.counters + 8 * (blockIdx.x % kCounterSlots);
+   unsigned long long* ctr = J.counters + 8 * (slot % kCounterSlots);
    if ((tid & 63) == 0) {
       if (v0) atomicAdd(&ctr[0], v0);
       if (v1) atomicAdd(&ctr[1], v1);
@@ -78,6 +78,35 @@ __global__ __launch_bounds__(kBlock) void k_plain(JobView J, int samples, int ti
          atomicAdd(&ctr[7], c7);
       }
    }
+}
+
+template <bool CNT, int LAYOUT>
+__global__ __launch_bounds__(kBlock) void k_plain(JobView J, int samples, int tiles_x, int ntiles, int per_xcd) {
+   __shared__ int stack[kStack * kBlock];
+   const int tile = xcd_tile(blockIdx.x, per_xcd);
+   if (tile >= ntiles) return;
+   plain_tile<CNT, LAYOUT>(J, samples, tiles_x, tile, blockIdx.x, stack);
+}
+
+// Many jobs in one launch: the tiles of all jobs are numbered consecutively (tile_base
+// holds each job's first tile, njobs + 1 entries), and each workgroup finds its job by a
+// binary search over that prefix. The per-pixel arithmetic is the plain kernel's, so a job
+// gets the same bits from this launch as from its own.
+template <bool CNT, int LAYOUT>
+__global__ __launch_bounds__(kBlock) void k_multi(const JobView* __restrict__ jobs, const int* __restrict__ tile_base,
+                                                  const int* __restrict__ todo, int njobs, int per_xcd) {
+   __shared__ int stack[kStack * kBlock];
+   const int tile = xcd_tile(blockIdx.x, per_xcd);
+   if (tile >= tile_base[njobs]) return;
+   int lo = 0, hi = njobs - 1;
+   while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tile_base[mid] <= tile) lo = mid;
+      else hi = mid - 1;
+   }
+   const JobView& J = jobs[lo];
+   const int tiles_x = (J.width + kTile - 1) / kTile;
+   plain_tile<CNT, LAYOUT>(J, todo[lo], tiles_x, tile - tile_base[lo], blockIdx.x, stack);
 }
 
 // gammaCorrectKernel semantics (render_acc_kernel.cu:11-44) on the compensated sums.
@@ -217,6 +246,24 @@ hipError_t launch_plain(const JobView& J, int samples, hipStream_t st) {
       hipLaunchKernelGGL((k_plain<false, L_BIN>), g, b, 0, st, J, samples, tx, ntiles, per);
    return hipGetLastError();
 }
+
+hipError_t launch_multi(const JobView* jobs, const int* tile_base, const int* todo, int njobs, int total_tiles,
+                        bool count, int layout, hipStream_t st) {
+   if (njobs <= 0 || total_tiles <= 0) return hipSuccess;
+   const int per = (total_tiles + 7) / 8;
+   const dim3 g(8 * per), b(kBlock);
+   if (count)
+      hipLaunchKernelGGL((k_multi<true, L_ANY>), g, b, 0, st, jobs, tile_base, todo, njobs, per);
+   else if (layout == L_BIN)
+      hipLaunchKernelGGL((k_multi<false, L_BIN>), g, b, 0, st, jobs, tile_base, todo, njobs, per);
+   else if (layout == L_WIDE)
+      hipLaunchKernelGGL((k_multi<false, L_WIDE>), g, b, 0, st, jobs, tile_base, todo, njobs, per);
+   else
+      hipLaunchKernelGGL((k_multi<false, L_ANY>), g, b, 0, st, jobs, tile_base, todo, njobs, per);
+   return hipGetLastError();
+}
+
+int plain_tiles(int width, int height) { return ((width + kTile - 1) / kTile) * ((height + kTile - 1) / kTile); }
 
 hipError_t launch_resolve_u8(const JobView& J, uint8_t* out, int channels, float gamma, hipStream_t st) {
    const int n = J.width * J.height;

@@ -91,9 +91,14 @@ struct alignas(16) MatRec {
    float4 m5;  // pattern_color, 0
 };
 
+// SDF primitive in rounded-box form: distance = |max(q,0)| + min(max q, 0) - rounding with
+// q = |a| - half, where a = p - center, or (|xz| - R, y, 0) for a torus.
+//   sphere r        : half (0,0,0), rounding r, R = -1
+//   rounded box h,r : half h - r,   rounding r, R = -1
+//   torus R, r      : half (0,0,0), rounding r, R
 struct alignas(16) SdfRec {
-   float4 c;  // center, type
-   float4 p;  // params
+   float4 c;  // center, R (torus major radius; -1 for the other kinds)
+   float4 p;  // half extents, rounding
 };
 
 // Device view of one scene.
@@ -113,7 +118,7 @@ struct SceneView {
    float sdf_k;
    float sdf_eps;
    int sdf_steps;
-   int pad_;
+   int sdf_prim;    // slot of the SDF field primitive, or -1
 };
 
 // Per-job constants of one launch.

@@ -9,8 +9,9 @@
 //   2. runs one first pass for the jobs admitted in this step (first_pass_samples
 //      samples each) and resolves their first frame on the device; time-to-first-frame
 //      is stamped when that work completes;
-//   3. advances every other active job together by one batched pass whose sample count
-//      a controller steers toward target_pass_ms;
+//   3. advances the oldest other active jobs, up to max_pass_pixels, together by one
+//      batched pass whose sample count a controller steers toward target_pass_ms (so a
+//      step, and with it the wait for a first frame, stays short under any load);
 //   4. retires jobs whose sample budget is exhausted or whose pixels have all converged
 //      (time-to-converge), and jobs that were cancelled.
 // Jobs keep their buffers until released, so results can be read after completion.
@@ -73,7 +74,6 @@ struct rayon_server {
    std::unordered_map<uint64_t, SJob*> jobs;
    std::unordered_map<rayon_scene*, int> scene_refs;
    int64_t used_bytes = 0;
-   int64_t active_pixels = 0;
    int pass_samples = 1;
    // statistics
    int64_t passes = 0, first_passes = 0, total_segments = 0, completed = 0, failed = 0;
@@ -101,7 +101,6 @@ struct rayon_server {
       if (j->accounted) {
          j->accounted = false;
          used_bytes -= j->bytes;
-         active_pixels -= (int64_t)j->params.width * j->params.height;
          if (--scene_refs[j->scene] == 0) {
             used_bytes -= scene_bytes(j->scene);
             scene_refs.erase(j->scene);
@@ -124,6 +123,11 @@ struct rayon_server {
             // retire cancelled jobs
             for (auto it = active.begin(); it != active.end();) {
                if ((*it)->cancel) {
+                  rayon_job_stats js;
+                  if ((*it)->job && rayon_job_stats_get(ctx, (*it)->job, &js) == RAYON_OK) {
+                     (*it)->samples = js.samples_done;
+                     (*it)->segments = js.segments;
+                  }
                   (*it)->state = RAYON_JOB_CANCELLED;
                   (*it)->t_done = now_s();
                   free_job_locked(*it);
@@ -145,21 +149,24 @@ struct rayon_server {
                   continue;
                }
                const int64_t need = j->bytes + (scene_refs.count(j->scene) ? 0 : scene_bytes(j->scene));
-               const int64_t px = (int64_t)j->params.width * j->params.height;
-               if ((used_bytes + need > cfg.hbm_budget_bytes || active_pixels + px > cfg.max_active_pixels) &&
-                   (!active.empty() || !fresh.empty()))
-                  break;
+               if (used_bytes + need > cfg.hbm_budget_bytes && (!active.empty() || !fresh.empty())) break;
                queue.pop_front();
                used_bytes += need;
-               active_pixels += px;
                scene_refs[j->scene]++;
                j->accounted = true;
                j->t_admit = now_s();
                fresh.push_back(j);
                active.push_back(j);  // counted as active from here, so drain() waits for it
             }
-            for (SJob* a : active)
-               if (a->samples > 0) run.push_back(a);  // fresh jobs have no samples yet
+            // refinement set: the oldest jobs that have a first frame, up to the pixel cap
+            int64_t px = 0;
+            for (SJob* a : active) {
+               if (a->samples <= 0) continue;  // fresh jobs have no samples yet
+               const int64_t p = (int64_t)a->params.width * a->params.height;
+               if (!run.empty() && px + p > cfg.max_pass_pixels) break;
+               run.push_back(a);
+               px += p;
+            }
          }
          // create device state for admitted jobs (outside the lock: allocations)
          std::vector<SJob*> ok_fresh;
@@ -230,15 +237,18 @@ struct rayon_server {
          // (4) completion
          std::vector<SJob*> finished;
          {
-            std::vector<SJob*> check;
-            {
-               std::lock_guard<std::mutex> lk(mu);
-               for (SJob* j : active)
-                  if (j->state == RAYON_JOB_RUNNING || j->state == RAYON_JOB_FAILED) check.push_back(j);
-            }
+            // only the jobs this step touched can have finished
+            std::vector<SJob*> check = run;
+            check.insert(check.end(), ok_fresh.begin(), ok_fresh.end());
             for (SJob* j : check) {
                if (j->state == RAYON_JOB_FAILED) {
                   finished.push_back(j);
+                  continue;
+               }
+               const bool adaptive = (j->params.flags & (RAYON_FLAG_ADAPTIVE_REF | RAYON_FLAG_ADAPTIVE_STDERR)) != 0;
+               if (!adaptive && j->job->samples_done < j->params.spp) {  // not done: skip the device read
+                  std::lock_guard<std::mutex> lk(mu);
+                  j->samples = j->job->samples_done;
                   continue;
                }
                rayon_job_stats s;
@@ -274,11 +284,11 @@ extern "C" {
 void rayon_server_config_default(rayon_server_config* c) {
    std::memset(c, 0, sizeof(*c));
    c->hbm_budget_bytes = 200ll << 30;  // 200 GiB of the 288 GB
-   c->max_active_jobs = 512;
+   c->max_active_jobs = 1 << 16;
    c->first_pass_samples = 1;
    c->max_pass_samples = 64;
    c->target_pass_ms = 20.0;
-   c->max_active_pixels = 16ll << 20;
+   c->max_pass_pixels = 16ll << 20;
 }
 
 rayon_status rayon_server_create(rayon_ctx* ctx, const rayon_server_config* cfg, rayon_server** out) {
@@ -293,7 +303,7 @@ rayon_status rayon_server_create(rayon_ctx* ctx, const rayon_server_config* cfg,
    else
       rayon_server_config_default(&s->cfg);
    if (s->cfg.max_active_jobs < 1 || s->cfg.first_pass_samples < 1 || s->cfg.max_pass_samples < 1 ||
-       !(s->cfg.target_pass_ms > 0) || s->cfg.hbm_budget_bytes <= 0 || s->cfg.max_active_pixels <= 0) {
+       !(s->cfg.target_pass_ms > 0) || s->cfg.hbm_budget_bytes <= 0 || s->cfg.max_pass_pixels <= 0) {
       delete s;
       ry::set_error("rayon_server_create: bad configuration");
       return RAYON_ERR_INVALID;

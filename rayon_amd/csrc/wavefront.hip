@@ -35,6 +35,9 @@
 namespace ry {
 
 constexpr int kWfBlock = 256;
+constexpr int kSdfSlice = 16;   // marching steps per slice; rays still marching are regrouped
+constexpr int kSdfSlices = 16;  // slices per level at most (the last one runs to the end)
+constexpr long long kMegaSceneBytes = 8ll << 20;  // automatic path choice: megakernel up to this scene size
 constexpr int kCtl = 8;  // control words per level
 constexpr uint32_t kSkip = 1u;        // ids.w bit: pixel converged, no path this sample
 constexpr uint32_t kPrevLambert = 2u; // ids.w bit: the last scatter was Lambertian
@@ -55,6 +58,9 @@ struct WfView {
    int* sq;       // shadow queue (path slots)
    unsigned* ctl;           // per level (stride kCtl): queue count, extend head, shade head,
                             // shadow count, shadow head
+   int* qs0;                // SDF marching queues (path slots), alternating by slice
+   int* qs1;
+   unsigned* sctl;          // per level and slice (stride 2): marching queue count, head
    const JobView* jobs;
    const long long* base;   // work prefix per job (njobs + 1 entries)
    unsigned long long* stats;  // traversal statistics (optional)
@@ -72,11 +78,15 @@ struct Wave {
    JobView* jobs_dev = nullptr;
    long long* base_dev = nullptr;
    unsigned long long* stats_dev = nullptr;
+   // tables of the jobs that take the one-launch megakernel (k_multi)
+   JobView* mjobs_dev = nullptr;
+   int* mtile_dev = nullptr;
+   int* mtodo_dev = nullptr;
    // options
    int opt_capacity = 1 << 26;
    int opt_wg_per_cu = 0;  // 0: as many as fit (occupancy of each stage kernel)
    int opt_chunk = 4;
-   int occ_trace = 0, occ_trace_wide = 0, occ_shade = 0;
+   int occ_trace = 0, occ_trace_wide = 0, occ_shade = 0, occ_sdf = 0;
    struct Pending {
       int stage;
       hipEvent_t e0, e1;
@@ -98,6 +108,8 @@ struct Wave {
    std::map<std::vector<long long>, GraphEntry> graphs;
    std::vector<JobView> host_jobs;
    std::vector<long long> host_base;
+   std::vector<JobView> host_mjobs;
+   std::vector<int> host_mtile, host_mtodo;
 };
 
 // ---------------------------------------------------------------------------------
@@ -200,6 +212,55 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_generate(WfView V, long long w0
    if (live) V.q0[pos] = i;
 }
 
+// SDF marching stage of extension rays: slice `it` takes up to kSdfSlice steps of every
+// ray in its queue (the last slice runs to the end); rays still marching go to the next
+// slice's queue, so each slice's waves hold only live rays. A hit replaces the ray's hit
+// record (the march ends at the closest other hit, and an exact tie goes to the field,
+// which has the largest original index). The march state lives in the shadow-ray arrays
+// of the slot, free between the trace and shade stages.
+__global__ __launch_bounds__(kWfBlock) void k_wf_sdf(WfView V, int level, int it, int last) {
+   unsigned* cin = V.sctl + 2 * ((kSdfSlices + 1) * level + it);
+   unsigned* cout = cin + 2;
+   const unsigned count = cin[0];
+   const int* qin = (it & 1) ? V.qs1 : V.qs0;
+   int* qout = (it & 1) ? V.qs0 : V.qs1;
+   drain(&cin[1], count, V.chunk, [&](unsigned idx) {
+      int slot = -1;
+      bool more = false;
+      if (idx < count) {
+         slot = qin[idx];
+         const SceneView& S = V.jobs[V.ids[slot].x].scene;
+         const V3 o = mk(V.o[slot]), d = mk(V.d[slot]);
+         SdfMarch m;
+         bool live = true;
+         if (it == 0) {
+            live = sdf_march_begin(S, S.prims[S.sdf_prim], o, d, 0.001f, V.hit[slot].x, m);
+         } else {
+            const float4 a = V.so[slot], b = V.sc[slot];
+            m.tw = a.x;
+            m.tend = a.y;
+            m.cap = a.z;
+            m.used = __float_as_int(a.w);
+            m.mask = (uint64_t)__float_as_uint(b.x) | ((uint64_t)__float_as_uint(b.y) << 32);
+         }
+         if (live) {
+            float t = 0.0f;
+            const int r = sdf_march_run(S, o, d, m, last ? S.sdf_steps : kSdfSlice, t);
+            if (r == 1) {
+               V.hit[slot] = make_float4(t, __int_as_float(S.sdf_prim), 0.0f, 0.0f);
+            } else if (r < 0) {
+               more = true;
+               V.so[slot] = make_float4(m.tw, m.tend, m.cap, __int_as_float(m.used));
+               V.sc[slot] = make_float4(__uint_as_float((uint32_t)m.mask), __uint_as_float((uint32_t)(m.mask >> 32)),
+                                        0.0f, 0.0f);
+            }
+         }
+      }
+      const int pos = wave_push(&cout[0], more);
+      if (more) qout[pos] = slot;
+   });
+}
+
 // Traversal stage: first the extension rays of level `ext_level` (closest hit), then the
 // shadow rays of level `shadow_level` (any hit; unoccluded contributions are added to the
 // path's radiance). Both queues were filled by the preceding shade launch and touch
@@ -213,15 +274,21 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_trace(WfView V, int ext_level, 
       unsigned* ctl = V.ctl + kCtl * ext_level;
       const unsigned count = ctl[0];
       const int* q = (ext_level & 1) ? V.q1 : V.q0;
+      unsigned* mctl = V.sctl + 2 * (kSdfSlices + 1) * ext_level;
       drain(&ctl[1], count, V.chunk, [&](unsigned idx) {
+         int slot = -1;
+         bool march = false;
          if (idx < count) {
-            const int slot = q[idx];
+            slot = q[idx];
             const float4 o = V.o[slot], d = V.d[slot];
             const SceneView& S = V.jobs[V.ids[slot].x].scene;
-            const HitMin h = closest_hit<kWfBlock, CNT, LAYOUT>(S, mk(o), mk(d), 0.001f, 3.402823466e38f,
-                                                               stack + threadIdx.x, &te);
+            const HitMin h = closest_hit<kWfBlock, CNT, LAYOUT, true>(S, mk(o), mk(d), 0.001f, 3.402823466e38f,
+                                                                     stack + threadIdx.x, &te);
             V.hit[slot] = make_float4(h.t, __int_as_float(h.slot), h.u, h.v);
+            march = h.sdf >= 0;  // an SDF field lies on the way: march it in k_wf_sdf
          }
+         const int pos = wave_push(&mctl[0], march);
+         if (march) V.qs0[pos] = slot;
       });
    }
    if (shadow_level >= 0) {
@@ -433,6 +500,8 @@ __global__ void k_wf_pass_end(JobView J, const float4* prev, int todo) {
 // ---------------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------------
+static size_t ctl_words(int levels) { return (size_t)(levels + 1) * (kCtl + 2 * (kSdfSlices + 1)); }
+
 static rayon_status wave_reserve(rayon_ctx* ctx, int capacity, int levels, int njobs) {
    Wave*& W = ctx->wave;
    if (!W) W = new Wave();
@@ -447,6 +516,8 @@ static rayon_status wave_reserve(rayon_ctx* ctx, int capacity, int levels, int n
       W->occ_trace_wide = n > 0 ? n : 1;
       RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_shade<false>, kWfBlock, 0));
       W->occ_shade = n > 0 ? n : 1;
+      RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_sdf, kWfBlock, 0));
+      W->occ_sdf = n > 0 ? n : 1;
    }
    if (W->capacity >= capacity && W->levels >= levels && W->max_jobs >= njobs) return RAYON_OK;
    RY_HIP(hipStreamSynchronize(ctx->stream));
@@ -482,14 +553,26 @@ static rayon_status wave_reserve(rayon_ctx* ctx, int capacity, int levels, int n
    W->v.q1 = (int*)p;
    if ((st = alloc((size_t)cap * sizeof(int), &p)) != RAYON_OK) return st;
    W->v.sq = (int*)p;
-   if ((st = alloc((size_t)(lv + 1) * kCtl * sizeof(unsigned), &p)) != RAYON_OK) return st;
+   if ((st = alloc((size_t)cap * sizeof(int), &p)) != RAYON_OK) return st;
+   W->v.qs0 = (int*)p;
+   if ((st = alloc((size_t)cap * sizeof(int), &p)) != RAYON_OK) return st;
+   W->v.qs1 = (int*)p;
+   // control words: kCtl per level, then 2 per (level, marching slice), cleared together
+   if ((st = alloc(ctl_words(lv) * sizeof(unsigned), &p)) != RAYON_OK) return st;
    W->v.ctl = (unsigned*)p;
+   W->v.sctl = W->v.ctl + (size_t)(lv + 1) * kCtl;
    if ((st = alloc((size_t)mj * sizeof(JobView), &p)) != RAYON_OK) return st;
    W->jobs_dev = (JobView*)p;
    if ((st = alloc((size_t)(mj + 1) * sizeof(long long), &p)) != RAYON_OK) return st;
    W->base_dev = (long long*)p;
    if ((st = alloc(8 * sizeof(unsigned long long), &p)) != RAYON_OK) return st;
    W->stats_dev = (unsigned long long*)p;
+   if ((st = alloc((size_t)mj * sizeof(JobView), &p)) != RAYON_OK) return st;
+   W->mjobs_dev = (JobView*)p;
+   if ((st = alloc((size_t)(mj + 1) * sizeof(int), &p)) != RAYON_OK) return st;
+   W->mtile_dev = (int*)p;
+   if ((st = alloc((size_t)mj * sizeof(int), &p)) != RAYON_OK) return st;
+   W->mtodo_dev = (int*)p;
    RY_HIP(launch_fill_u32((uint32_t*)W->stats_dev, 0u, 16, ctx->stream));
    W->capacity = cap;
    W->levels = lv;
@@ -540,9 +623,26 @@ void launch_stage(Wave* W, int stage, K kernel, dim3 grid, dim3 block, hipStream
 
 // Enqueues every launch of one pass (the sequence a graph captures).
 static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jobs, const std::vector<int>& todo,
-                               long long total, int levels) {
+                               long long total, int levels, int nmega, int mega_tiles, int mega_layout) {
    Wave* W = ctx->wave;
    hipStream_t st = ctx->stream;
+   if (nmega > 0) {
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      const bool timed = W->opt_timing && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+      if (timed) (void)hipEventRecord(e0, st);
+      hipError_t e = launch_multi(W->mjobs_dev, W->mtile_dev, W->mtodo_dev, nmega, mega_tiles, W->opt_count != 0,
+                                  mega_layout, st);
+      if (e != hipSuccess) return e;
+      if (timed) {
+         (void)hipEventRecord(e1, st);
+         W->pending.push_back({3, e0, e1});
+      } else {
+         if (e0) (void)hipEventDestroy(e0);
+         if (e1) (void)hipEventDestroy(e1);
+      }
+      W->launches[3] += 1;
+   }
+   if (total <= 0) return hipSuccess;
    WfView V = W->v;
    V.njobs = (int)jobs.size();
    V.levels = levels;
@@ -565,6 +665,17 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
          else launch_stage(W, 1, k_wf_trace<false, L_ANY>, g_trace, blk, st, V, ext, sh);
       }
    };
+   // SDF marching slices after each extension trace, when a scene of the pass has a field
+   int sdf_steps = 0;
+   for (rayon_job* j : jobs)
+      if (j->scene->view.num_sdf > 0) sdf_steps = std::max(sdf_steps, j->scene->view.sdf_steps);
+   const int nslices = std::min(kSdfSlices, (sdf_steps + kSdfSlice - 1) / kSdfSlice);
+   const dim3 g_sdf(grid_of(W->occ_sdf));
+   auto march = [&](int level) {
+      if (level < 0) return;
+      for (int it = 0; it < nslices; ++it)
+         launch_stage(W, 1, k_wf_sdf, g_sdf, blk, st, V, level, it, it == nslices - 1 ? 1 : 0);
+   };
    hipError_t e = hipSuccess;
    int dense = 1;
    for (rayon_job* j : jobs)
@@ -579,17 +690,19 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
    for (long long w0 = 0; w0 < total; w0 += W->capacity) {
       const int n = (int)std::min<long long>(W->capacity, total - w0);
       ++W->rounds;
-      if ((e = launch_fill_u32(V.ctl, 0u, (long long)(levels + 1) * kCtl, st)) != hipSuccess) return e;
+      if ((e = launch_fill_u32(V.ctl, 0u, (long long)ctl_words(W->levels), st)) != hipSuccess) return e;
       W->launches[5] += 1;
       const dim3 g_round((n + kWfBlock - 1) / kWfBlock);
       launch_stage(W, 0, k_wf_generate, g_round, blk, st, V, w0, n, dense);
       trace(0, -1);
+      march(0);
       for (int lv = 0; lv < levels; ++lv) {
          if (W->opt_count)
             launch_stage(W, 2, k_wf_shade<true>, g_shade, blk, st, V, lv);
          else
             launch_stage(W, 2, k_wf_shade<false>, g_shade, blk, st, V, lv);
          trace(lv + 1 < levels ? lv + 1 : -1, lv);
+         march(lv + 1 < levels ? lv + 1 : -1);
       }
       launch_stage(W, 4, k_wf_accumulate, g_round, blk, st, V, w0, n);
       if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -604,6 +717,14 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
    return hipGetLastError();
 }
 
+// Path of a scene's jobs in batched passes: its explicit choice, else the megakernel for
+// scenes small enough to stay in the L2 caches (there a whole path per lane wastes no
+// bandwidth on queue state) and the wavefront otherwise.
+int scene_batch_path(const rayon_scene* s) {
+   if (s->batch_path == RAYON_PATH_WAVEFRONT || s->batch_path == RAYON_PATH_MEGAKERNEL) return s->batch_path;
+   return s->info.device_bytes <= kMegaSceneBytes ? RAYON_PATH_MEGAKERNEL : RAYON_PATH_WAVEFRONT;
+}
+
 rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, int samples) {
    if (!ctx || (n > 0 && !jobs_in) || n < 0 || samples < 0) {
       set_error("rayon_render_batched: bad arguments");
@@ -611,8 +732,6 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
    }
    std::vector<rayon_job*> jobs;
    std::vector<int> todo;
-   long long total = 0;
-   int levels = 1;
    for (int k = 0; k < n; ++k) {
       rayon_job* j = jobs_in[k];
       if (!j || j->ctx != ctx) {
@@ -636,19 +755,32 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
       const int t = (int)std::max<long long>(0, std::min<long long>(samples, j->params.spp - j->samples_done));
       jobs.push_back(j);
       todo.push_back(t);
-      total += (long long)t * j->params.width * j->params.height;
-      levels = std::max(levels, j->params.max_depth);
    }
-   if (total == 0) return RAYON_OK;
+   // each job keeps the path it took in its first pass (the two paths round differently)
+   for (rayon_job* j : jobs)
+      if (j->path < 0) j->path = scene_batch_path(j->scene);
+   long long total = 0;
+   int levels = 1, nmega = 0;
+   for (size_t k = 0; k < jobs.size(); ++k) {
+      if (todo[k] <= 0) continue;
+      if (jobs[k]->path == RAYON_PATH_MEGAKERNEL) {
+         ++nmega;
+         continue;
+      }
+      total += (long long)todo[k] * jobs[k]->params.width * jobs[k]->params.height;
+      levels = std::max(levels, jobs[k]->params.max_depth);
+   }
+   if (total == 0 && nmega == 0) return RAYON_OK;
    RY_HIP(hipSetDevice(ctx->device));
    std::lock_guard<std::mutex> lk(ctx->mu);
    if (!ctx->wave) ctx->wave = new Wave();
-   const int cap = (int)std::min<long long>(ctx->wave->opt_capacity, total);
+   const int cap = (int)std::min<long long>(ctx->wave->opt_capacity, std::max(total, 1ll));
    rayon_status st = wave_reserve(ctx, std::max(cap, 1 << 16), levels, n);
    if (st != RAYON_OK) return st;
    Wave* W = ctx->wave;
-   for (rayon_job* j : jobs) {
-      if ((j->params.flags & RAYON_FLAG_ADAPTIVE_REF) && !j->prev) {
+   for (size_t k = 0; k < jobs.size(); ++k) {
+      rayon_job* j = jobs[k];
+      if (todo[k] > 0 && j->path != RAYON_PATH_MEGAKERNEL && (j->params.flags & RAYON_FLAG_ADAPTIVE_REF) && !j->prev) {
          void* p = nullptr;
          const size_t bytes = (size_t)j->params.width * j->params.height * sizeof(float4);
          RY_HIP(hipMalloc(&p, bytes));
@@ -656,34 +788,59 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
          j->prev = (float4*)p;
       }
    }
-   // job table (only jobs with samples to add take part in the pass)
-   std::vector<rayon_job*> active;
-   std::vector<int> atodo;
+   // job tables (only jobs with samples to add take part in the pass)
+   std::vector<rayon_job*> active, mactive;
+   std::vector<int> atodo, mtodo;
    W->host_jobs.clear();
    W->host_base.assign(1, 0);
+   W->host_mjobs.clear();
+   W->host_mtile.assign(1, 0);
+   W->host_mtodo.clear();
+   int mwide = 0;
    for (size_t k = 0; k < jobs.size(); ++k) {
       if (todo[k] <= 0) continue;
       rayon_job* j = jobs[k];
       fill_jobview(j);
       j->view.sample_base = (int)j->samples_done;
+      if (j->path == RAYON_PATH_MEGAKERNEL) {
+         mactive.push_back(j);
+         mtodo.push_back(todo[k]);
+         mwide += j->scene->view.wnodes != nullptr;
+         W->host_mjobs.push_back(j->view);
+         W->host_mtile.push_back(W->host_mtile.back() + plain_tiles(j->params.width, j->params.height));
+         W->host_mtodo.push_back(todo[k]);
+         continue;
+      }
       active.push_back(j);
       atodo.push_back(todo[k]);
       W->host_jobs.push_back(j->view);
       W->host_base.push_back(W->host_base.back() + (long long)todo[k] * j->params.width * j->params.height);
    }
-   RY_HIP(hipMemcpyAsync(W->jobs_dev, W->host_jobs.data(), W->host_jobs.size() * sizeof(JobView),
-                         hipMemcpyHostToDevice, ctx->stream));
-   RY_HIP(hipMemcpyAsync(W->base_dev, W->host_base.data(), W->host_base.size() * sizeof(long long),
-                         hipMemcpyHostToDevice, ctx->stream));
-   // the table copy reads pageable host memory: finish it before the vectors change again
+   const int mlayout = mwide == 0 ? L_BIN : (mwide == nmega ? L_WIDE : L_ANY);
+   if (!active.empty()) {
+      RY_HIP(hipMemcpyAsync(W->jobs_dev, W->host_jobs.data(), W->host_jobs.size() * sizeof(JobView),
+                            hipMemcpyHostToDevice, ctx->stream));
+      RY_HIP(hipMemcpyAsync(W->base_dev, W->host_base.data(), W->host_base.size() * sizeof(long long),
+                            hipMemcpyHostToDevice, ctx->stream));
+   }
+   if (nmega > 0) {
+      RY_HIP(hipMemcpyAsync(W->mjobs_dev, W->host_mjobs.data(), W->host_mjobs.size() * sizeof(JobView),
+                            hipMemcpyHostToDevice, ctx->stream));
+      RY_HIP(hipMemcpyAsync(W->mtile_dev, W->host_mtile.data(), W->host_mtile.size() * sizeof(int),
+                            hipMemcpyHostToDevice, ctx->stream));
+      RY_HIP(hipMemcpyAsync(W->mtodo_dev, W->host_mtodo.data(), W->host_mtodo.size() * sizeof(int),
+                            hipMemcpyHostToDevice, ctx->stream));
+   }
+   // the table copies read pageable host memory: finish them before the vectors change again
    RY_HIP(hipStreamSynchronize(ctx->stream));
+   const int mtiles = W->host_mtile.back();
    const bool use_graph = W->opt_graph && !W->opt_timing;
    if (!use_graph) {
-      RY_HIP(enqueue_pass(ctx, active, atodo, total, levels));
+      RY_HIP(enqueue_pass(ctx, active, atodo, total, levels, nmega, mtiles, mlayout));
    } else {
-      // The captured launches read every job through the device job table, so a graph
-      // fits any job set with the same work per job; only the adaptive pass-begin/end
-      // kernels take a job by value, so adaptive jobs are part of the key.
+      // The captured launches read every job through the device tables, so a graph fits
+      // any job set with the same work per job; only the adaptive pass-begin/end kernels
+      // take a job by value, so adaptive jobs are part of the key.
       std::vector<long long> key;
       for (size_t k = 0; k < active.size(); ++k) {
          const bool adaptive = (active[k]->params.flags & (RAYON_FLAG_ADAPTIVE_REF | RAYON_FLAG_ADAPTIVE_STDERR)) != 0;
@@ -696,11 +853,19 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
       key.push_back(W->opt_chunk);
       key.push_back(W->opt_count);
       for (rayon_job* j : active) key.push_back(j->scene->view.wnodes != nullptr);
+      int sdf_steps = 0;  // the marching slices the pass enqueues
+      for (rayon_job* j : active)
+         if (j->scene->view.num_sdf > 0) sdf_steps = std::max(sdf_steps, j->scene->view.sdf_steps);
+      key.push_back(sdf_steps);
+      key.push_back(-1);
+      key.push_back(nmega);
+      key.push_back(mtiles);
+      key.push_back(mlayout);
       auto it = W->graphs.find(key);
       if (it == W->graphs.end()) {
          Wave::GraphEntry ge;
          RY_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-         hipError_t e = enqueue_pass(ctx, active, atodo, total, levels);
+         hipError_t e = enqueue_pass(ctx, active, atodo, total, levels, nmega, mtiles, mlayout);
          hipGraph_t g = nullptr;
          hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
          if (e != hipSuccess) return hip_fail(e, "wavefront pass capture");
@@ -711,6 +876,7 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
       }
       RY_HIP(hipGraphLaunch(it->second.exec, ctx->stream));
    }
+   for (size_t k = 0; k < mactive.size(); ++k) mactive[k]->samples_done += mtodo[k];
    for (size_t k = 0; k < active.size(); ++k) active[k]->samples_done += atodo[k];
    return RAYON_OK;
 }

@@ -97,6 +97,7 @@ def test_alone_equals_in_mix_bit_exact(ctx, scenes):
 
 def test_rounds_and_graphs_do_not_change_results(ctx, scenes):
     log("capacity / graph invariance")
+    scenes.get(*BOOK)[1].set_batch_path(ra.PATH.WAVEFRONT)
     base, *_ = scenes.job(*BOOK, 120, 80, spp=12, seed=4, max_depth=16)
     ctx.set_option(ra.OPT.GRAPH, 0)
     ctx.render_batched([base], 12)
@@ -114,7 +115,68 @@ def test_rounds_and_graphs_do_not_change_results(ctx, scenes):
     finally:
         ctx.set_option(ra.OPT.WAVE_CAPACITY, old)
         ctx.set_option(ra.OPT.GRAPH, 1)
+        scenes.get(*BOOK)[1].set_batch_path(ra.PATH.AUTO)
     base.close()
+
+
+def test_graph_of_same_shape_other_scene(ctx, scenes):
+    """A pass whose work sizes match an earlier pass on another kind of scene (here an SDF
+    field after a sphere field) must not reuse that pass's graph: results with graphs
+    equal the eager ones."""
+    log("graph key covers the scene's stages")
+    try:
+        for sc in (BOOK, SDF):
+            scenes.get(*sc)[1].set_batch_path(ra.PATH.WAVEFRONT)
+        ctx.set_option(ra.OPT.GRAPH, 1)
+        b, *_ = scenes.job(*BOOK, 64, 48, spp=4, seed=31, max_depth=8)
+        ctx.render_batched([b], 4)
+        g, *_ = scenes.job(*SDF, 64, 48, spp=4, seed=32, max_depth=8)
+        ctx.render_batched([g], 4)
+        ctx.set_option(ra.OPT.GRAPH, 0)
+        e, *_ = scenes.job(*SDF, 64, 48, spp=4, seed=32, max_depth=8)
+        ctx.render_batched([e], 4)
+        np.testing.assert_array_equal(g.resolve_linear(), e.resolve_linear())
+        assert g.stats().segments == e.stats().segments
+        for j in (b, g, e):
+            j.close()
+    finally:
+        ctx.set_option(ra.OPT.GRAPH, 1)
+        for sc in (BOOK, SDF):
+            scenes.get(*sc)[1].set_batch_path(ra.PATH.AUTO)
+
+
+def test_megakernel_path_equals_plain_bit_exact(ctx, scenes):
+    """Jobs on the megakernel path share one launch per pass and get the plain kernel's
+    bits; the same jobs on the wavefront path agree to 1e-5 relative."""
+    log("megakernel path")
+    specs = [(BOOK, 96, 54, dict(spp=12, seed=21, max_depth=50)),
+             (SDF, 64, 48, dict(spp=7, seed=22, max_depth=8)),
+             (CORN, 40, 30, dict(spp=9, seed=23, max_depth=8, flags=ra.FLAG.NEE)),
+             (BOOK, 17, 33, dict(spp=5, seed=24, max_depth=16))]
+    try:
+        for sc in (BOOK, SDF, CORN):
+            scenes.get(*sc)[1].set_batch_path(ra.PATH.MEGAKERNEL)
+        plain = [scenes.job(*sc, W, H, **kw)[0] for sc, W, H, kw in specs]
+        for j, (_, _, _, kw) in zip(plain, specs):
+            j.render_plain(kw["spp"])
+        mega = [scenes.job(*sc, W, H, **kw)[0] for sc, W, H, kw in specs]
+        ctx.render_batched(mega, 4)   # two passes: 4 samples, then the rest
+        ctx.render_batched(mega, 64)
+        for a, b in zip(plain, mega):
+            np.testing.assert_array_equal(b.resolve_linear(), a.resolve_linear())
+            assert a.stats().segments == b.stats().segments
+        for sc in (BOOK, SDF, CORN):
+            scenes.get(*sc)[1].set_batch_path(ra.PATH.WAVEFRONT)
+        wave = [scenes.job(*sc, W, H, **kw)[0] for sc, W, H, kw in specs]
+        ctx.render_batched(wave, 64)
+        for a, b in zip(plain, wave):
+            ia, ib = a.resolve_linear(), b.resolve_linear()
+            assert (np.abs(ia - ib) / np.maximum(np.abs(ia), 1e-3)).max() <= 1e-5
+        for j in plain + mega + wave:
+            j.close()
+    finally:
+        for sc in (BOOK, SDF, CORN):
+            scenes.get(*sc)[1].set_batch_path(ra.PATH.AUTO)
 
 
 def test_graph_replay_repeats_eager_result(ctx, scenes):
@@ -140,8 +202,10 @@ def test_graph_replay_repeats_eager_result(ctx, scenes):
 
 def test_batched_against_oracle_sdf_terrain(ctx, scenes):
     log("sdf + terrain batched vs oracle")
+    scenes.get(*SDF)[1].set_batch_path(ra.PATH.WAVEFRONT)
     j, P, cam, g = scenes.job(*SDF, 64, 40, spp=32, seed=6, max_depth=8)
     ctx.render_batched([j], 32)
+    scenes.get(*SDF)[1].set_batch_path(ra.PATH.AUTO)
     img = j.resolve_linear().reshape(-1, 3).astype(np.float64)
     sums, segs = OracleScene(g.desc_ptr()).render(P, nsamples=32, camera=cam)
     ref = sums / 32
@@ -175,3 +239,44 @@ def test_reference_stream_jobs_are_rejected(ctx, scenes):
     with pytest.raises(ra.RayonError, match="REF_RNG"):
         ctx.render_batched([j], 4)
     j.close()
+
+
+def test_autotune_keeps_results_and_jobs(ctx, scenes):
+    """The autotuner times paths and shapes on clones (the jobs stay untouched), records a
+    path per scene, caches its shape choice per workload, and the tuned shape renders the
+    same bits as the default one."""
+    log("autotune")
+    specs = [(BOOK, 128, 72, dict(spp=8, seed=1, max_depth=8)),
+             (CORN, 96, 64, dict(spp=8, seed=2, max_depth=8, flags=ra.FLAG.NEE)),
+             (SDF, 64, 48, dict(spp=8, seed=3, max_depth=8))]
+    opt_keys = (ra.OPT.WAVE_CAPACITY, ra.OPT.WAVE_CHUNK, ra.OPT.WAVE_WG_PER_CU)
+    default = {o: ctx.get_option(o) for o in opt_keys}
+    try:
+        jobs = [scenes.job(*sc, W, H, **kw)[0] for sc, W, H, kw in specs]
+        r = ctx.autotune(jobs, 2)
+        log(f"  chose capacity {r.capacity} chunk {r.chunk} wg {r.wg_per_cu}: {r.best_ms:.2f} ms "
+            f"(start {r.start_ms:.2f} ms, {r.candidates} passes timed; {r.mega_scenes} megakernel / "
+            f"{r.wave_scenes} wavefront scenes)")
+        assert r.candidates >= 5 and r.cached == 0 and r.best_ms <= r.start_ms
+        assert r.mega_scenes + r.wave_scenes == 3
+        assert all(j.stats().samples_done == 0 for j in jobs)
+        assert ctx.get_option(ra.OPT.WAVE_CHUNK) == r.chunk
+        again = ctx.autotune(jobs, 2)
+        assert again.cached == 1 and again.chunk == r.chunk and again.capacity == r.capacity
+        tuned = {o: ctx.get_option(o) for o in opt_keys}
+        for o, v in default.items():
+            ctx.set_option(o, v)
+        refs = [scenes.job(*sc, W, H, **kw)[0] for sc, W, H, kw in specs]
+        ctx.render_batched(refs, 8)
+        for o, v in tuned.items():
+            ctx.set_option(o, v)
+        ctx.render_batched(jobs, 8)
+        for j, ref in zip(jobs, refs):
+            np.testing.assert_array_equal(j.resolve_linear(), ref.resolve_linear())
+            j.close()
+            ref.close()
+    finally:
+        for o, v in default.items():
+            ctx.set_option(o, v)
+        for sc in (BOOK, SDF, CORN):
+            scenes.get(*sc)[1].set_batch_path(ra.PATH.AUTO)

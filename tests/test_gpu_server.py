@@ -88,7 +88,7 @@ def test_served_jobs_equal_alone_renders(world):
             srv.result(ids[0])
 
 
-def test_admission_by_budget_and_pixel_cap(world):
+def test_admission_by_budget(world):
     """Jobs that do not all fit the budget run in admission waves and still finish with the
     same results; a job larger than the whole budget is refused at submission."""
     ctx, gens, scenes = world
@@ -97,7 +97,7 @@ def test_admission_by_budget_and_pixel_cap(world):
     img = ref.resolve_linear()
     ref.close()
     scene_bytes = scenes[key].info().device_bytes
-    with ctx.server(hbm_budget_bytes=scene_bytes + 2 * W * H * 48, max_active_pixels=2 * W * H) as srv:
+    with ctx.server(hbm_budget_bytes=scene_bytes + 2 * W * H * 48, max_pass_pixels=W * H) as srv:
         ids = [srv.submit(scenes[key], params(gens, key, W, H, **kw)) for _ in range(5)]
         with pytest.raises(ra.RayonError, match="BUDGET"):
             srv.submit(scenes[key], params(gens, key, 640, 480, spp=4))
