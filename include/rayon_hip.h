@@ -357,9 +357,9 @@ rayon_status rayon_autotune(rayon_ctx* ctx, rayon_job* const* jobs, int32_t n, i
  * Render server: continuous batching of many jobs on one context (no counterpart in the
  * reference). A server thread admits queued jobs by HBM budget, gives every newly
  * admitted job a first pass and resolves its first frame on the device, advances the
- * oldest active jobs (up to max_pass_pixels) together in batched passes sized toward
- * target_pass_ms, and retires jobs whose sample budget is spent or whose pixels all
- * converged.
+ * oldest active jobs together in batched passes whose work (pixels x samples) a
+ * controller sizes toward target_pass_ms, and retires jobs whose sample budget is spent
+ * or whose pixels all converged.
  * ------------------------------------------------------------------------------- */
 typedef struct rayon_server rayon_server;
 
@@ -378,8 +378,8 @@ typedef struct rayon_server_config {
    int32_t max_pass_samples;    /* cap of the controlled pass size                       */
    int32_t pad_;
    double target_pass_ms;       /* controller target for one batched pass               */
-   int64_t max_pass_pixels;     /* pixels a refinement pass advances: the oldest active
-                                   jobs up to this many pixels (at least one job)       */
+   int64_t max_pass_pixels;     /* cap of the pixels a refinement pass advances (the
+                                   oldest active jobs; at least one job)                */
 } rayon_server_config;
 
 typedef struct rayon_server_job_info {
@@ -436,7 +436,21 @@ typedef struct rayon_gen_params {
 } rayon_gen_params;
 
 rayon_status rayon_generate(const rayon_gen_params* params, rayon_generated** out);
+/* Scene file loader (Scene::loadSceneFromYAML, yaml_scene_loader.cc:505-576, with its OBJ
+ * loader obj_loader.hpp:34-171): the same keys, defaults, limits and fall-backs, and the same
+ * geometry list (one triangle per OBJ face triangle, in file order). Errors: RAYON_ERR_IO
+ * (file), RAYON_ERR_PARSE (a number that does not parse, no materials, no geometry). */
 rayon_status rayon_load_yaml(const char* path, rayon_generated** out);
+enum { RAYON_YAML_LIFT_CAPS = 1 << 0 };  /* read past 100 materials / 1000 geometry entries */
+rayon_status rayon_load_yaml_ex(const char* path, uint32_t flags, rayon_generated** out);
+/* Writes a scene file that rayon_load_yaml reads back to the same description, camera and
+ * settings (the role of saveSceneToYAML, yaml_scene_loader.cc:578-744, without its loss:
+ * every number is written with 9 significant digits and triangles are kept). */
+rayon_status rayon_save_yaml(const rayon_generated* g, const char* path);
+/* Settings of a loaded scene file (defaults for generated scenes). */
+rayon_status rayon_generated_settings(const rayon_generated* g, float background_color[3],
+                                      float* background_intensity, float* ambient_light, int32_t* use_bvh,
+                                      int32_t* adaptive_sampling);
 /* The description stays valid until rayon_generated_free. */
 const rayon_scene_desc* rayon_generated_desc(const rayon_generated* g);
 /* Camera from the generator or the YAML camera section: look_from, look_at, vup, vfov. */

@@ -1,0 +1,16 @@
+#!/bin/bash
+# Builds the reference's scene-file loader from its own sources, where they lie under
+# /root/reference, into oracle/_ref/ (git-ignored). Test infrastructure only: the tests
+# compare rayon_load_yaml against it. Exits 0 without building when the sources are absent.
+set -e
+REF=${RAYON_REFERENCE:-/root/reference}
+HERE=$(cd "$(dirname "$0")" && pwd)
+SRC=$REF/src/rayon
+if [ ! -f "$SRC/scenes/yaml_scene_loader.cc" ]; then
+  echo "reference sources not found under $REF: oracle/_ref not built"
+  exit 0
+fi
+mkdir -p "$HERE/_ref"
+g++ -std=c++17 -O1 -w -I"$SRC/scenes" -I"$SRC/data_structures" -I"$SRC/cpu_renderers/utils" -I"$SRC" \
+    "$SRC/scenes/yaml_scene_loader.cc" "$HERE/ref_yaml_dump.cpp" -o "$HERE/_ref/ref_yaml_dump"
+echo "$HERE/_ref/ref_yaml_dump"

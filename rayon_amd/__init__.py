@@ -13,7 +13,7 @@ import numpy as np
 
 from .abi import (  # noqa: F401  (re-exported)
     Camera, GenParams, Geometry, JobParams, JobStats, Material, Mesh, SceneDesc, SceneInfo, SdfPrim, WaveStats,
-    ServerConfig, ServerJobInfo, ServerStats, TuneResult, PATH, MAT, GEOM, SDF, FLAG, GEN, OPT, BVH, JOB, STATUS, declare,
+    ServerConfig, ServerJobInfo, ServerStats, TuneResult, PATH, YAML, MAT, GEOM, SDF, FLAG, GEN, OPT, BVH, JOB, STATUS, declare,
 )
 
 LIB_PATH = Path(__file__).resolve().parent / "librayon_hip.so"
@@ -82,10 +82,22 @@ class Generated:
         return cls(h)
 
     @classmethod
-    def load_yaml(cls, path: str) -> "Generated":
+    def load_yaml(cls, path: str, flags: int = 0) -> "Generated":
         h = C.c_void_p()
-        _check(lib().rayon_load_yaml(str(path).encode(), C.byref(h)), "rayon_load_yaml")
+        _check(lib().rayon_load_yaml_ex(str(path).encode(), flags, C.byref(h)), "rayon_load_yaml")
         return cls(h)
+
+    def save_yaml(self, path: str):
+        _check(lib().rayon_save_yaml(self._h, str(path).encode()), "rayon_save_yaml")
+
+    def settings(self) -> dict:
+        col = (C.c_float * 3)()
+        bi, amb = C.c_float(), C.c_float()
+        bvh, ada = C.c_int32(), C.c_int32()
+        _check(lib().rayon_generated_settings(self._h, col, C.byref(bi), C.byref(amb), C.byref(bvh), C.byref(ada)),
+               "rayon_generated_settings")
+        return dict(background_color=list(col), background_intensity=bi.value, ambient_light=amb.value,
+                    use_bvh=bool(bvh.value), adaptive_sampling=bool(ada.value))
 
     @property
     def desc(self) -> SceneDesc:

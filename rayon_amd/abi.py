@@ -23,6 +23,7 @@ SDF = _NS(SPHERE=0, ROUNDED_BOX=1, TORUS=2)
 FLAG = _NS(REF_QUIRKS=1, REF_RNG=2, NEE=4, ADAPTIVE_REF=8, ADAPTIVE_STDERR=16, DOF=32, COUNT_TRAVERSAL=64)
 GEN = _NS(BOOK_SPHERES=0, CORNELL_ICOSPHERE=1, SDF_TERRAIN=2)
 BVH = _NS(DEFAULT=0, QUANTIZED=1)
+YAML = _NS(LIFT_CAPS=1)
 JOB = _NS(QUEUED=0, RUNNING=1, DONE=2, FAILED=3, CANCELLED=4)
 PATH = _NS(AUTO=0, WAVEFRONT=1, MEGAKERNEL=2)
 OPT = _NS(WAVE_CAPACITY=0, WAVE_WG_PER_CU=1, GRAPH=2, STAGE_TIMING=3, COUNT_TRAVERSAL=4, WAVE_CHUNK=5)
@@ -172,6 +173,9 @@ def declare(lib: C.CDLL) -> C.CDLL:
         "rayon_server_destroy": (st, [vp]),
         "rayon_generate": (st, [P(GenParams), P(vp)]),
         "rayon_load_yaml": (st, [C.c_char_p, P(vp)]),
+        "rayon_load_yaml_ex": (st, [C.c_char_p, C.c_uint32, P(vp)]),
+        "rayon_save_yaml": (st, [vp, C.c_char_p]),
+        "rayon_generated_settings": (st, [vp, P(f32), P(f32), P(f32), P(i32), P(i32)]),
         "rayon_generated_desc": (P(SceneDesc), [vp]),
         "rayon_generated_camera": (st, [vp, P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_double)]),
         "rayon_generated_background_intensity": (f32, [vp]),

@@ -71,6 +71,12 @@ def build_oracle() -> Path:
     return ORACLE_LIB
 
 
+def build_reference() -> None:
+    """The reference's own scene-file loader, compiled from its sources where they lie (only
+    where they exist, i.e. not on a machine without the reference), into oracle/_ref/."""
+    _run(["bash", str(ORACLE_DIR / "build_ref.sh")])
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--product", action="store_true")
@@ -82,6 +88,7 @@ def main(argv=None) -> int:
         print(build_product(a.j))
     if a.oracle or both:
         print(build_oracle())
+        build_reference()
     return 0
 
 

@@ -374,7 +374,7 @@ struct TravCount {
    unsigned nodes = 0, prims = 0;
 };
 
-template <int STRIDE, bool CNT = false, bool DEFER = false>
+template <int STRIDE, bool CNT = false, bool DEFER = false, int STACK = kStack>
 __device__ inline HitMin closest_hit_bin(const SceneView& S, V3 o, V3 d, float tmin, float tmax, int* stack,
                                          TravCount* tc = nullptr) {
    HitMin best;
@@ -402,7 +402,7 @@ __device__ inline HitMin closest_hit_bin(const SceneView& S, V3 o, V3 d, float t
                nr = r1;
                fr = r0;
             }
-            if (sp < kStack) stack[(sp++) * STRIDE] = fr;
+            if (sp < STACK) stack[(sp++) * STRIDE] = fr;
             cur = nr;
          } else if (h0) {
             cur = r0;
@@ -452,7 +452,7 @@ __device__ inline HitMin closest_hit_bin(const SceneView& S, V3 o, V3 d, float t
 }
 
 // Any hit in [tmin, tmax] by a primitive that is not an emitter (shadow rays).
-template <int STRIDE, bool CNT = false>
+template <int STRIDE, bool CNT = false, int STACK = kStack>
 __device__ inline bool any_hit_bin(const SceneView& S, V3 o, V3 d, float tmin, float tmax, int* stack,
                                    TravCount* tc = nullptr) {
    int cur = S.root;
@@ -469,7 +469,7 @@ __device__ inline bool any_hit_bin(const SceneView& S, V3 o, V3 d, float tmin, f
          int r0 = n.meta.z ? pack_leaf(n.meta.x, n.meta.z) : n.meta.x;
          int r1 = n.meta.w ? pack_leaf(n.meta.y, n.meta.w) : n.meta.y;
          if (h0 && h1) {
-            if (sp < kStack) stack[(sp++) * STRIDE] = r1;
+            if (sp < STACK) stack[(sp++) * STRIDE] = r1;
             cur = r0;
          } else if (h0) {
             cur = r0;
@@ -717,20 +717,22 @@ enum Layout : int { L_BIN = 0, L_WIDE = 1, L_ANY = 2 };
 
 // DEFER: an SDF field met by the traversal is not marched; its slot is returned in
 // HitMin::sdf for a later marching stage (which marches up to the closest other hit).
-template <int STRIDE, bool CNT = false, int LAYOUT = L_ANY, bool DEFER = false>
+// STACK: LDS entries per lane the caller provides; binary trees need at most their depth,
+// the wide traversal always takes kStack.
+template <int STRIDE, bool CNT = false, int LAYOUT = L_ANY, bool DEFER = false, int STACK = kStack>
 __device__ inline HitMin closest_hit(const SceneView& S, V3 o, V3 d, float tmin, float tmax, int* stack,
                                      TravCount* tc = nullptr) {
    if (LAYOUT == L_WIDE || (LAYOUT == L_ANY && S.wnodes))
       return closest_hit_wide<STRIDE, CNT, DEFER>(S, o, d, tmin, tmax, stack, tc);
-   return closest_hit_bin<STRIDE, CNT, DEFER>(S, o, d, tmin, tmax, stack, tc);
+   return closest_hit_bin<STRIDE, CNT, DEFER, STACK>(S, o, d, tmin, tmax, stack, tc);
 }
 
-template <int STRIDE, bool CNT = false, int LAYOUT = L_ANY>
+template <int STRIDE, bool CNT = false, int LAYOUT = L_ANY, int STACK = kStack>
 __device__ inline bool any_hit(const SceneView& S, V3 o, V3 d, float tmin, float tmax, int* stack,
                                TravCount* tc = nullptr) {
    if (LAYOUT == L_WIDE || (LAYOUT == L_ANY && S.wnodes))
       return any_hit_wide<STRIDE, CNT>(S, o, d, tmin, tmax, stack, tc);
-   return any_hit_bin<STRIDE, CNT>(S, o, d, tmin, tmax, stack, tc);
+   return any_hit_bin<STRIDE, CNT, STACK>(S, o, d, tmin, tmax, stack, tc);
 }
 
 // ------------------------------------------------------------------------------------

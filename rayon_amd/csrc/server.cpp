@@ -9,9 +9,11 @@
 //   2. runs one first pass for the jobs admitted in this step (first_pass_samples
 //      samples each) and resolves their first frame on the device; time-to-first-frame
 //      is stamped when that work completes;
-//   3. advances the oldest other active jobs, up to max_pass_pixels, together by one
-//      batched pass whose sample count a controller steers toward target_pass_ms (so a
-//      step, and with it the wait for a first frame, stays short under any load);
+//   3. advances the oldest other active jobs together by one batched pass whose work
+//      (pixels x samples) a controller steers toward target_pass_ms: the jobs are taken
+//      oldest first until their pixels reach the work budget (at least one job, at most
+//      max_pass_pixels), and the sample count fills the budget (so a step, and with it
+//      the wait for a first frame, stays short under any load);
 //   4. retires jobs whose sample budget is exhausted or whose pixels have all converged
 //      (time-to-converge), and jobs that were cancelled.
 // Jobs keep their buffers until released, so results can be read after completion.
@@ -75,6 +77,7 @@ struct rayon_server {
    std::unordered_map<rayon_scene*, int> scene_refs;
    int64_t used_bytes = 0;
    int pass_samples = 1;
+   double pass_work = 4.0 * (1 << 20);  // pixel-samples of the next refinement pass
    // statistics
    int64_t passes = 0, first_passes = 0, total_segments = 0, completed = 0, failed = 0;
    double busy_s = 0, t_start = 0;
@@ -158,15 +161,18 @@ struct rayon_server {
                fresh.push_back(j);
                active.push_back(j);  // counted as active from here, so drain() waits for it
             }
-            // refinement set: the oldest jobs that have a first frame, up to the pixel cap
+            // refinement set: the oldest jobs that have a first frame, until their pixels
+            // reach the work budget or the pixel cap
             int64_t px = 0;
             for (SJob* a : active) {
                if (a->samples <= 0) continue;  // fresh jobs have no samples yet
                const int64_t p = (int64_t)a->params.width * a->params.height;
-               if (!run.empty() && px + p > cfg.max_pass_pixels) break;
+               if (!run.empty() && (px + p > cfg.max_pass_pixels || (double)px >= pass_work)) break;
                run.push_back(a);
                px += p;
             }
+            if (px > 0)
+               pass_samples = (int)std::min<double>(cfg.max_pass_samples, std::max(1.0, pass_work / (double)px));
          }
          // create device state for admitted jobs (outside the lock: allocations)
          std::vector<SJob*> ok_fresh;
@@ -229,7 +235,9 @@ struct rayon_server {
                // controller toward target_pass_ms (the ratio rule of
                // renderer_cuda_progressive_host.hpp:681-686, clamped to [0.5, 2] per step)
                const double ratio = std::min(2.0, std::max(0.5, cfg.target_pass_ms / std::max(ms, 0.05)));
-               pass_samples = std::min(cfg.max_pass_samples, std::max(1, (int)(pass_samples * ratio + 0.5)));
+               double done = 0;
+               for (SJob* j : run) done += (double)j->params.width * j->params.height;
+               pass_work = std::min(std::max(done * pass_samples * ratio, 65536.0), 1e12);
                std::lock_guard<std::mutex> lk(mu);
                ++passes;
             }
@@ -287,7 +295,7 @@ void rayon_server_config_default(rayon_server_config* c) {
    c->max_active_jobs = 1 << 16;
    c->first_pass_samples = 1;
    c->max_pass_samples = 64;
-   c->target_pass_ms = 20.0;
+   c->target_pass_ms = 15.0;
    c->max_pass_pixels = 16ll << 20;
 }
 

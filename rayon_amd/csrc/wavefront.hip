@@ -35,6 +35,7 @@
 namespace ry {
 
 constexpr int kWfBlock = 256;
+constexpr int kShortStack = 24;  // LDS stack of binary trees up to depth 23 (more waves per CU)
 constexpr int kSdfSlice = 16;   // marching steps per slice; rays still marching are regrouped
 constexpr int kSdfSlices = 16;  // slices per level at most (the last one runs to the end)
 constexpr long long kMegaSceneBytes = 8ll << 20;  // automatic path choice: megakernel up to this scene size
@@ -86,7 +87,7 @@ struct Wave {
    int opt_capacity = 1 << 26;
    int opt_wg_per_cu = 0;  // 0: as many as fit (occupancy of each stage kernel)
    int opt_chunk = 4;
-   int occ_trace = 0, occ_trace_wide = 0, occ_shade = 0, occ_sdf = 0;
+   int occ_trace = 0, occ_trace_wide = 0, occ_trace_short = 0, occ_shade = 0, occ_sdf = 0;
    struct Pending {
       int stage;
       hipEvent_t e0, e1;
@@ -266,9 +267,9 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_sdf(WfView V, int level, int it
 // path's radiance). Both queues were filled by the preceding shade launch and touch
 // disjoint state, so one persistent launch drains both and waves that finish their
 // extension work fill the launch's tail with shadow work. Either level may be -1.
-template <bool CNT, int LAYOUT>
+template <bool CNT, int LAYOUT, int STACK = kStack>
 __global__ __launch_bounds__(kWfBlock) void k_wf_trace(WfView V, int ext_level, int shadow_level) {
-   __shared__ int stack[kStack * kWfBlock];
+   __shared__ int stack[STACK * kWfBlock];
    TravCount te, ts;
    if (ext_level >= 0) {
       unsigned* ctl = V.ctl + kCtl * ext_level;
@@ -282,8 +283,8 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_trace(WfView V, int ext_level, 
             slot = q[idx];
             const float4 o = V.o[slot], d = V.d[slot];
             const SceneView& S = V.jobs[V.ids[slot].x].scene;
-            const HitMin h = closest_hit<kWfBlock, CNT, LAYOUT, true>(S, mk(o), mk(d), 0.001f, 3.402823466e38f,
-                                                                     stack + threadIdx.x, &te);
+            const HitMin h = closest_hit<kWfBlock, CNT, LAYOUT, true, STACK>(S, mk(o), mk(d), 0.001f,
+                                                                            3.402823466e38f, stack + threadIdx.x, &te);
             V.hit[slot] = make_float4(h.t, __int_as_float(h.slot), h.u, h.v);
             march = h.sdf >= 0;  // an SDF field lies on the way: march it in k_wf_sdf
          }
@@ -299,7 +300,8 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_trace(WfView V, int ext_level, 
             const int slot = V.sq[idx];
             const float4 o = V.so[slot], d = V.sd[slot];
             const SceneView& S = V.jobs[V.ids[slot].x].scene;
-            if (!any_hit<kWfBlock, CNT, LAYOUT>(S, mk(o), mk(d), 1e-4f, 1.0f - 1e-4f, stack + threadIdx.x, &ts)) {
+            if (!any_hit<kWfBlock, CNT, LAYOUT, STACK>(S, mk(o), mk(d), 1e-4f, 1.0f - 1e-4f, stack + threadIdx.x,
+                                                       &ts)) {
                const float4 c = V.sc[slot];
                const float4 r = V.rad[slot];
                const V3 L = mk(r) + mk(c);
@@ -335,7 +337,7 @@ __device__ __forceinline__ void flush(const int* buf, int n, unsigned* counter, 
 // shadow requests of each wave's claimed chunk are staged in LDS and appended to the
 // next-level and shadow queues with one atomic each per chunk.
 template <bool CNT>
-__global__ __launch_bounds__(kWfBlock) void k_wf_shade(WfView V, int level) {
+__global__ __launch_bounds__(kWfBlock, 4) void k_wf_shade(WfView V, int level) {
    __shared__ int stage_next[kWfBlock / 64][64 * kMaxChunk];
    __shared__ int stage_sh[kWfBlock / 64][64 * kMaxChunk];
    const int w = threadIdx.x >> 6, lane = lane_id();
@@ -514,6 +516,8 @@ static rayon_status wave_reserve(rayon_ctx* ctx, int capacity, int levels, int n
       W->occ_trace = n > 0 ? n : 1;
       RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_trace<false, L_WIDE>, kWfBlock, 0));
       W->occ_trace_wide = n > 0 ? n : 1;
+      RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_trace<false, L_BIN, kShortStack>, kWfBlock, 0));
+      W->occ_trace_short = n > 0 ? n : 1;
       RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_shade<false>, kWfBlock, 0));
       W->occ_shade = n > 0 ? n : 1;
       RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_sdf, kWfBlock, 0));
@@ -653,14 +657,21 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
    int nwide = 0;
    for (rayon_job* j : jobs) nwide += j->scene->view.wnodes != nullptr;
    const int layout = nwide == 0 ? L_BIN : (nwide == (int)jobs.size() ? L_WIDE : L_ANY);
-   const dim3 g_trace(grid_of(layout == L_BIN ? W->occ_trace : std::min(W->occ_trace, W->occ_trace_wide)));
+   // binary trees no deeper than the short stack take the kernel with the smaller LDS stack
+   int depth = 0;
+   for (rayon_job* j : jobs) depth = std::max(depth, j->scene->info.max_depth);
+   const bool short_stack = layout == L_BIN && depth < kShortStack;
+   const dim3 g_trace(grid_of(short_stack        ? W->occ_trace_short
+                              : layout == L_BIN ? W->occ_trace
+                                                : std::min(W->occ_trace, W->occ_trace_wide)));
    auto trace = [&](int ext, int sh) {
       if (W->opt_count) {
          if (layout == L_BIN) launch_stage(W, 1, k_wf_trace<true, L_BIN>, g_trace, blk, st, V, ext, sh);
          else if (layout == L_WIDE) launch_stage(W, 1, k_wf_trace<true, L_WIDE>, g_trace, blk, st, V, ext, sh);
          else launch_stage(W, 1, k_wf_trace<true, L_ANY>, g_trace, blk, st, V, ext, sh);
       } else {
-         if (layout == L_BIN) launch_stage(W, 1, k_wf_trace<false, L_BIN>, g_trace, blk, st, V, ext, sh);
+         if (short_stack) launch_stage(W, 1, k_wf_trace<false, L_BIN, kShortStack>, g_trace, blk, st, V, ext, sh);
+         else if (layout == L_BIN) launch_stage(W, 1, k_wf_trace<false, L_BIN>, g_trace, blk, st, V, ext, sh);
          else if (layout == L_WIDE) launch_stage(W, 1, k_wf_trace<false, L_WIDE>, g_trace, blk, st, V, ext, sh);
          else launch_stage(W, 1, k_wf_trace<false, L_ANY>, g_trace, blk, st, V, ext, sh);
       }
@@ -857,6 +868,9 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
       for (rayon_job* j : active)
          if (j->scene->view.num_sdf > 0) sdf_steps = std::max(sdf_steps, j->scene->view.sdf_steps);
       key.push_back(sdf_steps);
+      int depth = 0;  // picks the trace kernel's stack size
+      for (rayon_job* j : active) depth = std::max(depth, j->scene->info.max_depth);
+      key.push_back(depth < kShortStack);
       key.push_back(-1);
       key.push_back(nmega);
       key.push_back(mtiles);
@@ -1030,13 +1044,6 @@ rayon_status rayon_ctx_wave_stats(rayon_ctx* ctx, rayon_wave_stats* out, int32_t
    }
    std::lock_guard<std::mutex> lk(ctx->mu);
    return ry::wave_stats(ctx, out, reset);
-}
-
-rayon_status rayon_load_yaml(const char* path, rayon_generated** out) {
-   (void)path;
-   (void)out;
-   ry::set_error("rayon_load_yaml: not built yet");
-   return RAYON_ERR_UNSUPPORTED;
 }
 
 }  // extern "C"
