@@ -446,8 +446,9 @@ def bench_batch256(a, ra) -> int:
         e = ev.create()
         ev.record(e, stream)
         job_events.append(e)
-        if (k + 1) % 32 == 0:
-            log(f"plain: {k + 1} jobs enqueued")
+        if (k + 1) % 16 == 0:
+            ctx.sync()  # keeps the log moving; each job is one long launch anyway
+            log(f"plain: {k + 1} jobs done, {time.perf_counter() - tp:.0f} s")
     ctx.sync()
     p_wall = time.perf_counter() - tp
     p_lat = completion(job_events, e_start, list(range(len(jobs))))
@@ -579,12 +580,24 @@ def bench_serve(a, ra) -> int:
         srv.close()
         mrays = segs / wall / 1e6
         best = max(best, mrays)
+        ttc_done = [x for x in ttc if np.isfinite(x)]
         r = {"lambda": lam, "jobs": len(ids), "completed": int(sum(inf.state == ra.JOB.DONE for inf in infos)),
+             "offered_msamples_per_s": None,
              "ttff_p50_ms": percentile(ttff, 50), "ttff_p99_ms": percentile(ttff, 99),
              "ttc_p50_ms": percentile(ttc, 50), "ttc_p99_ms": percentile(ttc, 99),
+             "ttc_p50_ms_completed": percentile(ttc_done, 50), "ttc_p99_ms_completed": percentile(ttc_done, 99),
              "mrays_per_s": round(mrays, 2), "wall_s": round(wall, 1), "passes": int(st.passes),
-             "first_passes": int(st.first_passes)}
-        for k in ("ttff_p50_ms", "ttff_p99_ms", "ttc_p50_ms", "ttc_p99_ms"):
+             "first_passes": int(st.first_passes),
+             "refine_s": round(st.pass_ms / 1e3, 2), "first_pass_s": round(st.first_ms / 1e3, 2),
+             "stats_read_s": round(st.check_ms / 1e3, 2),
+             "refine_msamples_per_s": round(st.pass_paths / max(st.pass_ms, 1e-9) / 1e3, 1),
+             "segments_by_generator": {GEN_NAMES[gi]: int(sum(inf.segments for inf, (sc, P), sp in
+                                                             zip(infos, plist, specs) if sp["gen"] == gi))
+                                       for gi in range(3)}}
+        # offered load: the sample budget (pixels x spp cap) of the arrivals per second of the run
+        r["offered_msamples_per_s"] = round(sum(P.width * P.height * P.spp for _, P in plist) / max(wall, 1e-9) / 1e6, 1)
+        for k in ("ttff_p50_ms", "ttff_p99_ms", "ttc_p50_ms", "ttc_p99_ms", "ttc_p50_ms_completed",
+                  "ttc_p99_ms_completed"):
             r[k] = None if r[k] is None or not np.isfinite(r[k]) else round(r[k], 1)
         log(json.dumps(r))
         out["rates"].append(r)

@@ -264,9 +264,13 @@ rayon_status rayon_job_reset(rayon_job* job);
  * Asynchronous on the context stream. */
 rayon_status rayon_job_render_plain(rayon_ctx* ctx, rayon_job* job, int32_t samples);
 
-/* Batched wavefront path: one progressive pass over `n` jobs, each advanced by
- * `samples` samples (clamped to its budget), with generate / extend / shade / shadow /
- * accumulate stages over shared device queues. Asynchronous on the context stream. */
+/* Batched path: one progressive pass over `n` jobs, each advanced by `samples` samples
+ * (clamped to its budget). Jobs of wavefront scenes go through generate / extend / shade /
+ * shadow / accumulate stages over shared device queues, jobs of megakernel scenes share
+ * one launch (rayon_scene_set_batch_path). Asynchronous on the context's stream. The jobs
+ * may belong to any context of the same device (the caller orders their use across
+ * streams); such passes are not captured in graphs. A job must not be in two passes at
+ * once. */
 rayon_status rayon_render_batched(rayon_ctx* ctx, rayon_job* const* jobs, int32_t n, int32_t samples);
 
 /* Context options for the batched path. */
@@ -354,9 +358,52 @@ rayon_status rayon_autotune(rayon_ctx* ctx, rayon_job* const* jobs, int32_t n, i
                             rayon_tune_result* out);
 
 /* ---------------------------------------------------------------------------------
+ * Headless progressive renderer of one job: the render loop of the reference's
+ * interactive mode (RendererCUDAProgressive, renderer_cuda_progressive_host.hpp:440-920)
+ * without a window. Each step renders one batch sized by its frame-time controller
+ * (:681-686), optionally at its progressive depth schedule (:853-876), and resolves the
+ * display image on a second stream into pinned host memory while the caller goes on.
+ * ------------------------------------------------------------------------------- */
+typedef struct rayon_progressive rayon_progressive;
+
+typedef struct rayon_progressive_config {
+   double target_fps;       /* frame-time target of the batch controller (60)               */
+   int32_t max_batch;       /* cap of samples per batch (the reference's slider, 64)        */
+   int32_t first_batch;     /* samples of the first batch (4)                               */
+   int32_t adaptive_depth;  /* 1: shallow paths first (depth 4 .. the job's own)           */
+   int32_t channels;        /* display channels, 3 or 4 (4)                                  */
+   float gamma;             /* display gamma (2)                                             */
+   int32_t pad_;
+} rayon_progressive_config;
+
+typedef struct rayon_progressive_frame {
+   int64_t samples;         /* samples per pixel after this step                           */
+   int64_t segments;        /* ray segments traced so far                                  */
+   double frame_ms;         /* wall time of the batch                                      */
+   double convergence;      /* percent of pixels converged (adaptive jobs), from device counters */
+   int32_t batch;           /* samples rendered by this step                               */
+   int32_t next_batch;      /* the controller's next batch                                 */
+   int32_t depth;           /* max depth used by this step                                 */
+   int32_t done;            /* sample budget spent or every pixel converged                */
+} rayon_progressive_frame;
+
+void rayon_progressive_config_default(rayon_progressive_config* cfg);
+rayon_status rayon_progressive_create(rayon_ctx* ctx, rayon_scene* scene, const rayon_job_params* params,
+                                      const rayon_progressive_config* cfg, rayon_progressive** out);
+rayon_status rayon_progressive_step(rayon_progressive* p, rayon_progressive_frame* frame);
+/* Latest display image (width*height*channels bytes) and its samples per pixel. */
+rayon_status rayon_progressive_display(rayon_progressive* p, uint8_t* out, int64_t* samples);
+/* Camera change: new camera (or NULL to keep it), sums and counts cleared, first batch again. */
+rayon_status rayon_progressive_reset(rayon_progressive* p, const rayon_camera* camera);
+/* The job being rendered (for resolves and statistics); owned by the renderer. */
+rayon_job* rayon_progressive_job(rayon_progressive* p);
+rayon_status rayon_progressive_destroy(rayon_progressive* p);
+
+/* ---------------------------------------------------------------------------------
  * Render server: continuous batching of many jobs on one context (no counterpart in the
- * reference). A server thread admits queued jobs by HBM budget, gives every newly
- * admitted job a first pass and resolves its first frame on the device, advances the
+ * reference). A server thread admits queued jobs by HBM budget, renders every newly
+ * admitted job's first frame (on a stream of its own, overlapping the pass in flight)
+ * and resolves it on the device, advances the
  * oldest active jobs together in batched passes whose work (pixels x samples) a
  * controller sizes toward target_pass_ms, and retires jobs whose sample budget is spent
  * or whose pixels all converged.
@@ -374,7 +421,8 @@ enum {
 typedef struct rayon_server_config {
    int64_t hbm_budget_bytes;    /* accumulation + frame buffers + referenced scenes      */
    int32_t max_active_jobs;
-   int32_t first_pass_samples;  /* samples of a job's first frame                        */
+   int32_t first_pass_samples;  /* samples of a job's first frame (a plain-kernel preview
+                                   of its first samples, outside its sums)              */
    int32_t max_pass_samples;    /* cap of the controlled pass size                       */
    int32_t pad_;
    double target_pass_ms;       /* controller target for one batched pass               */
@@ -395,8 +443,12 @@ typedef struct rayon_server_job_info {
 typedef struct rayon_server_stats {
    int64_t passes, first_passes, completed, failed, queued, active, segments, used_bytes;
    double busy_s, wall_s;
-   int32_t pass_samples;
-   int32_t pad_;
+   int32_t pass_samples;        /* samples of the last refinement pass                    */
+   int32_t pass_jobs;           /* jobs of the last refinement pass                       */
+   double pass_ms;              /* summed wall time of refinement passes (start to done)  */
+   double first_ms;             /* summed wall time of first passes                       */
+   double check_ms;             /* summed time reading job statistics after passes        */
+   double pass_paths;           /* summed pixel-samples of refinement passes              */
 } rayon_server_stats;
 
 void rayon_server_config_default(rayon_server_config* cfg);

@@ -13,7 +13,7 @@ import numpy as np
 
 from .abi import (  # noqa: F401  (re-exported)
     Camera, GenParams, Geometry, JobParams, JobStats, Material, Mesh, SceneDesc, SceneInfo, SdfPrim, WaveStats,
-    ServerConfig, ServerJobInfo, ServerStats, TuneResult, PATH, YAML, MAT, GEOM, SDF, FLAG, GEN, OPT, BVH, JOB, STATUS, declare,
+    ServerConfig, ServerJobInfo, ServerStats, TuneResult, ProgressiveConfig, ProgressiveFrame, PATH, YAML, MAT, GEOM, SDF, FLAG, GEN, OPT, BVH, JOB, STATUS, declare,
 )
 
 LIB_PATH = Path(__file__).resolve().parent / "librayon_hip.so"
@@ -172,6 +172,9 @@ class Context:
         r = TuneResult()
         _check(lib().rayon_autotune(self._h, arr, len(jobs), samples, C.byref(r)), "rayon_autotune")
         return r
+
+    def progressive(self, scene: "Scene", params: JobParams, **cfg) -> "Progressive":
+        return Progressive(self, scene, params, **cfg)
 
     def server(self, **cfg) -> "Server":
         return Server(self, **cfg)
@@ -359,3 +362,45 @@ class Server:
 
     def __exit__(self, *a):
         self.close()
+
+
+class Progressive:
+    """Headless progressive renderer of one job (frame-time batch controller, optional
+    progressive depth, display resolved on a second stream into pinned memory)."""
+
+    def __init__(self, ctx: Context, scene: Scene, params: JobParams, **cfg):
+        self.ctx, self.params = ctx, params
+        c = ProgressiveConfig()
+        lib().rayon_progressive_config_default(C.byref(c))
+        for k, v in cfg.items():
+            if not hasattr(c, k):
+                raise AttributeError(k)
+            setattr(c, k, v)
+        self.config = c
+        h = C.c_void_p()
+        _check(lib().rayon_progressive_create(ctx._h, scene._h, C.byref(params), C.byref(c), C.byref(h)),
+               "rayon_progressive_create")
+        self._h = h
+        self.job = Job(ctx, scene, C.c_void_p(lib().rayon_progressive_job(h)), params)
+
+    def step(self) -> ProgressiveFrame:
+        f = ProgressiveFrame()
+        _check(lib().rayon_progressive_step(self._h, C.byref(f)), "rayon_progressive_step")
+        return f
+
+    def display(self):
+        out = np.empty((self.params.height, self.params.width, self.config.channels), np.uint8)
+        n = C.c_int64()
+        _check(lib().rayon_progressive_display(self._h, out.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(n)),
+               "rayon_progressive_display")
+        return out, n.value
+
+    def reset(self, camera: Camera = None):
+        _check(lib().rayon_progressive_reset(self._h, C.byref(camera) if camera is not None else None),
+               "rayon_progressive_reset")
+
+    def close(self):
+        if self._h:
+            self.job._h = None  # owned by the renderer
+            lib().rayon_progressive_destroy(self._h)
+            self._h = None

@@ -111,13 +111,25 @@ class ServerStats(C.Structure):
     _fields_ = [("passes", C.c_int64), ("first_passes", C.c_int64), ("completed", C.c_int64),
                 ("failed", C.c_int64), ("queued", C.c_int64), ("active", C.c_int64), ("segments", C.c_int64),
                 ("used_bytes", C.c_int64), ("busy_s", C.c_double), ("wall_s", C.c_double),
-                ("pass_samples", C.c_int32), ("pad_", C.c_int32)]
+                ("pass_samples", C.c_int32), ("pass_jobs", C.c_int32), ("pass_ms", C.c_double),
+                ("first_ms", C.c_double), ("check_ms", C.c_double), ("pass_paths", C.c_double)]
 
 
 class TuneResult(C.Structure):
     _fields_ = [("capacity", C.c_int64), ("chunk", C.c_int32), ("wg_per_cu", C.c_int32), ("best_ms", C.c_double),
                 ("start_ms", C.c_double), ("candidates", C.c_int32), ("cached", C.c_int32),
                 ("mega_scenes", C.c_int32), ("wave_scenes", C.c_int32)]
+
+
+class ProgressiveConfig(C.Structure):
+    _fields_ = [("target_fps", C.c_double), ("max_batch", C.c_int32), ("first_batch", C.c_int32),
+                ("adaptive_depth", C.c_int32), ("channels", C.c_int32), ("gamma", C.c_float), ("pad_", C.c_int32)]
+
+
+class ProgressiveFrame(C.Structure):
+    _fields_ = [("samples", C.c_int64), ("segments", C.c_int64), ("frame_ms", C.c_double),
+                ("convergence", C.c_double), ("batch", C.c_int32), ("next_batch", C.c_int32),
+                ("depth", C.c_int32), ("done", C.c_int32)]
 
 
 class GenParams(C.Structure):
@@ -159,6 +171,13 @@ def declare(lib: C.CDLL) -> C.CDLL:
         "rayon_job_primary_rays": (st, [vp, vp, i32, P(f32), P(f32)]),
         "rayon_scene_set_batch_path": (st, [vp, i32]),
         "rayon_scene_batch_path": (i32, [vp]),
+        "rayon_progressive_config_default": (None, [P(ProgressiveConfig)]),
+        "rayon_progressive_create": (st, [vp, vp, P(JobParams), P(ProgressiveConfig), P(vp)]),
+        "rayon_progressive_step": (st, [vp, P(ProgressiveFrame)]),
+        "rayon_progressive_display": (st, [vp, P(C.c_uint8), P(i64)]),
+        "rayon_progressive_reset": (st, [vp, P(Camera)]),
+        "rayon_progressive_job": (vp, [vp]),
+        "rayon_progressive_destroy": (st, [vp]),
         "rayon_autotune": (st, [vp, P(vp), i32, i32, P(TuneResult)]),
         "rayon_server_config_default": (None, [P(ServerConfig)]),
         "rayon_server_create": (st, [vp, P(ServerConfig), P(vp)]),

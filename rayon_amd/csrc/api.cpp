@@ -52,6 +52,26 @@ static void free_all(std::vector<DevBuf>& bufs) {
    bufs.clear();
 }
 
+// Job buffers come from the device's stream-ordered pool: a server creates and retires
+// jobs continuously, and pool allocations neither map fresh pages each time nor make a
+// free wait for the whole device.
+static rayon_status job_alloc(rayon_ctx* ctx, std::vector<DevBuf>& bufs, size_t bytes, void** out) {
+   void* p = nullptr;
+   if (bytes) {
+      hipError_t e = hipMallocAsync(&p, bytes, ctx->stream);
+      if (e != hipSuccess) return hip_fail(e, "hipMallocAsync");
+   }
+   bufs.push_back({p, bytes});
+   *out = p;
+   return RAYON_OK;
+}
+
+static void job_free_all(rayon_ctx* ctx, std::vector<DevBuf>& bufs) {
+   for (auto& b : bufs)
+      if (b.p) (void)hipFreeAsync(b.p, ctx->stream);
+   bufs.clear();
+}
+
 template <class T>
 static rayon_status upload(std::vector<DevBuf>& bufs, const std::vector<T>& host, T** out) {
    void* p = nullptr;
@@ -100,6 +120,8 @@ void ry::fill_jobview(rayon_job* j) {
    J.adaptive_threshold = P.adaptive_threshold;
    J.spp_target = P.spp;
    J.sample_base = (int)j->samples_done;
+   J.live = nullptr;
+   J.nlive = -1;
 }
 
 extern "C" {
@@ -194,6 +216,12 @@ rayon_status rayon_ctx_create(int32_t device, rayon_ctx** out) {
    if (e != hipSuccess) {
       delete c;
       return hip_fail(e, "hipStreamCreateWithFlags");
+   }
+   // keep memory freed back to the device's pool for the next jobs
+   hipMemPool_t pool = nullptr;
+   if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+      uint64_t keep = UINT64_MAX;
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
    }
    *out = c;
    return RAYON_OK;
@@ -450,12 +478,12 @@ rayon_status rayon_job_create(rayon_ctx* ctx, rayon_scene* sc, const rayon_job_p
    const size_t npix = (size_t)p->width * p->height;
    void *acc, *comp, *counts, *rng = nullptr, *ctr;
    rayon_status st;
-   if ((st = dev_alloc(j->bufs, npix * sizeof(float4), &acc)) != RAYON_OK ||
-       (st = dev_alloc(j->bufs, npix * sizeof(float4), &comp)) != RAYON_OK ||
-       (st = dev_alloc(j->bufs, npix * sizeof(int), &counts)) != RAYON_OK ||
-       (st = dev_alloc(j->bufs, kCounterSlots * 8 * sizeof(unsigned long long), &ctr)) != RAYON_OK ||
-       ((p->flags & RAYON_FLAG_REF_RNG) && (st = dev_alloc(j->bufs, npix * sizeof(uint32_t), &rng)) != RAYON_OK)) {
-      free_all(j->bufs);
+   if ((st = job_alloc(ctx, j->pool_bufs, npix * sizeof(float4), &acc)) != RAYON_OK ||
+       (st = job_alloc(ctx, j->pool_bufs, npix * sizeof(float4), &comp)) != RAYON_OK ||
+       (st = job_alloc(ctx, j->pool_bufs, npix * sizeof(int), &counts)) != RAYON_OK ||
+       (st = job_alloc(ctx, j->pool_bufs, kCounterSlots * 8 * sizeof(unsigned long long), &ctr)) != RAYON_OK ||
+       ((p->flags & RAYON_FLAG_REF_RNG) && (st = job_alloc(ctx, j->pool_bufs, npix * sizeof(uint32_t), &rng)) != RAYON_OK)) {
+      job_free_all(ctx, j->pool_bufs);
       delete j;
       return st;
    }
@@ -466,8 +494,10 @@ rayon_status rayon_job_create(rayon_ctx* ctx, rayon_scene* sc, const rayon_job_p
    j->view.counters = (unsigned long long*)ctr;
    fill_jobview(j);
    st = rayon_job_reset(j);
+   // the buffers are valid for every stream (and host copies) once the clears have run
+   if (st == RAYON_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) st = hip_fail(hipGetLastError(), "job set-up");
    if (st != RAYON_OK) {
-      free_all(j->bufs);
+      job_free_all(ctx, j->pool_bufs);
       delete j;
       return st;
    }
@@ -484,6 +514,7 @@ rayon_status rayon_job_destroy(rayon_job* j) {
       wave_forget_job(j->ctx, j);
    }
    free_all(j->bufs);
+   job_free_all(j->ctx, j->pool_bufs);
    delete j;
    return RAYON_OK;
 }

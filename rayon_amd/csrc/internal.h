@@ -86,10 +86,13 @@ struct rayon_job {
    rayon_scene* scene = nullptr;
    rayon_job_params params{};
    ry::JobView view{};
-   std::vector<ry::DevBuf> bufs;
+   std::vector<ry::DevBuf> bufs;       // hipMalloc
+   std::vector<ry::DevBuf> pool_bufs;  // stream-ordered pool (accumulation, counts, counters)
    int64_t samples_done = 0;
    int32_t path = -1;       // batched path taken since the first pass (-1: not yet chosen)
    float4* prev = nullptr;  // sums at pass start (batched passes with the reference adaptive test)
+   int* live = nullptr;      // pixels still sampling (batched passes of adaptive jobs)
+   unsigned* live_n = nullptr;
 };
 
 namespace ry {

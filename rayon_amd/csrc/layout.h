@@ -141,6 +141,9 @@ struct JobView {
    float adaptive_threshold;
    int sample_base;    // index of the first sample of this pass
    int spp_target;
+   const int* live;    // batched passes of adaptive jobs: the pixels still sampling, or NULL
+   int nlive;          // entries of `live` (-1: every pixel, in pixel order)
+   int pad_;
 };
 
 }  // namespace ry

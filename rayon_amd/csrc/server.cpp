@@ -6,8 +6,9 @@
 // One server thread owns the device work of a context. Each step it
 //   1. admits queued jobs while their device memory (accumulation, counts and a u8 frame,
 //      plus each newly referenced scene) fits the HBM budget and the active-job cap;
-//   2. runs one first pass for the jobs admitted in this step (first_pass_samples
-//      samples each) and resolves their first frame on the device; time-to-first-frame
+//   2. renders a first frame for each job admitted in this step: first_pass_samples
+//      samples of the job by the one-launch plain kernel into a scratch buffer, resolved
+//      on the device (a preview that does not enter the job's sums); time-to-first-frame
 //      is stamped when that work completes;
 //   3. advances the oldest other active jobs together by one batched pass whose work
 //      (pixels x samples) a controller steers toward target_pass_ms: the jobs are taken
@@ -49,6 +50,7 @@ struct SJob {
    int64_t bytes = 0;
    int64_t samples = 0;
    int64_t segments = 0;
+   int64_t live_px = 0;  // pixels still sampling (all, until an adaptive job reports converged ones)
    bool cancel = false;
    bool released = false;
    bool accounted = false;  // bytes (and a scene reference) are held in the budget
@@ -65,6 +67,7 @@ int64_t job_bytes(const rayon_job_params& p) {
 
 struct rayon_server {
    rayon_ctx* ctx = nullptr;
+   rayon_ctx* first_ctx = nullptr;  // first passes, on a stream and wavefront state of their own
    rayon_server_config cfg{};
    std::thread th;
    std::mutex mu;
@@ -81,6 +84,8 @@ struct rayon_server {
    // statistics
    int64_t passes = 0, first_passes = 0, total_segments = 0, completed = 0, failed = 0;
    double busy_s = 0, t_start = 0;
+   double pass_ms_sum = 0, first_ms_sum = 0, check_ms_sum = 0, pass_paths_sum = 0;
+   int pass_jobs = 0;
    std::string last_error;
 
    int64_t scene_bytes(rayon_scene* s) { return s->info.device_bytes; }
@@ -110,30 +115,87 @@ struct rayon_server {
          }
       }
       if (j->frame) {
-         (void)hipFree(j->frame);
+         (void)hipFreeAsync(j->frame, first_ctx->stream);
          j->frame = nullptr;
       }
    }
 
+   // Marks finished jobs DONE/FAILED and takes them out of the active set.
+   void finish_locked(const std::vector<SJob*>& finished) {
+      const double t = now_s();
+      for (SJob* j : finished) {
+         if (j->state != RAYON_JOB_FAILED) {
+            j->state = RAYON_JOB_DONE;
+            ++completed;
+         }
+         j->t_done = t;
+         total_segments += j->segments;
+         active.erase(std::find(active.begin(), active.end(), j));
+         if (j->released) free_job_locked(j);
+      }
+      if (!finished.empty()) done_cv.notify_all();
+   }
+
+   // Reads the statistics of jobs a pass on `c` touched (the pass has completed) and
+   // returns those that are done. Fixed-budget jobs that have samples left need no read.
+   std::vector<SJob*> check(rayon_ctx* c, const std::vector<SJob*>& touched) {
+      std::vector<SJob*> finished;
+      for (SJob* j : touched) {
+         if (j->state == RAYON_JOB_FAILED) {
+            finished.push_back(j);
+            continue;
+         }
+         const bool adaptive = (j->params.flags & (RAYON_FLAG_ADAPTIVE_REF | RAYON_FLAG_ADAPTIVE_STDERR)) != 0;
+         if (!adaptive && j->job->samples_done < j->params.spp) {
+            std::lock_guard<std::mutex> lk(mu);
+            j->samples = j->job->samples_done;
+            continue;
+         }
+         rayon_job_stats s;
+         if (rayon_job_stats_get(c, j->job, &s) != RAYON_OK) continue;
+         std::lock_guard<std::mutex> lk(mu);
+         j->samples = s.samples_done;
+         j->segments = s.segments;
+         j->live_px = std::max<int64_t>(1, (int64_t)j->params.width * j->params.height - s.converged_pixels);
+         if (s.done) finished.push_back(j);
+      }
+      return finished;
+   }
+
+   // The server thread. First frames run on a context of their own (first_ctx), so they
+   // overlap the refinement pass in flight on the main context instead of waiting for it:
+   //   - admit queued jobs, create them, run their first pass and first-frame resolve on
+   //     first_ctx, wait for that alone, stamp time-to-first-frame;
+   //   - when no refinement pass is in flight, start one over the oldest ready jobs;
+   //   - when the pass in flight has completed, check its jobs and steer the next size.
    void loop() {
       (void)hipSetDevice(ctx->device);
+      std::vector<SJob*> in_flight;
+      bool flying = false;
+      double t_pass = 0;
+      hipEvent_t pass_done = nullptr;
+      (void)hipEventCreateWithFlags(&pass_done, hipEventDisableTiming);
       for (;;) {
-         std::vector<SJob*> fresh, run;
+         std::vector<SJob*> fresh;
          {
             std::unique_lock<std::mutex> lk(mu);
-            cv.wait(lk, [&] { return stop || !queue.empty() || !active.empty(); });
-            if (stop) return;
-            // retire cancelled jobs
+            if (flying)  // wake up for arrivals, or soon to look at the pass in flight
+               cv.wait_for(lk, std::chrono::microseconds(200), [&] { return stop || !queue.empty(); });
+            else
+               cv.wait(lk, [&] { return stop || !queue.empty() || !active.empty(); });
+            if (stop) break;
+            // retire cancelled jobs that no pass is using
             for (auto it = active.begin(); it != active.end();) {
-               if ((*it)->cancel) {
+               SJob* j = *it;
+               if (j->cancel && std::find(in_flight.begin(), in_flight.end(), j) == in_flight.end()) {
                   rayon_job_stats js;
-                  if ((*it)->job && rayon_job_stats_get(ctx, (*it)->job, &js) == RAYON_OK) {
-                     (*it)->samples = js.samples_done;
-                     (*it)->segments = js.segments;
+                  if (j->job && rayon_job_stats_get(ctx, j->job, &js) == RAYON_OK) {
+                     j->samples = js.samples_done;
+                     j->segments = js.segments;
                   }
-                  (*it)->state = RAYON_JOB_CANCELLED;
-                  (*it)->t_done = now_s();
-                  free_job_locked(*it);
+                  j->state = RAYON_JOB_CANCELLED;
+                  j->t_done = now_s();
+                  free_job_locked(j);
                   it = active.erase(it);
                   done_cv.notify_all();
                } else {
@@ -161,129 +223,127 @@ struct rayon_server {
                fresh.push_back(j);
                active.push_back(j);  // counted as active from here, so drain() waits for it
             }
-            // refinement set: the oldest jobs that have a first frame, until their pixels
-            // reach the work budget or the pixel cap
-            int64_t px = 0;
-            for (SJob* a : active) {
-               if (a->samples <= 0) continue;  // fresh jobs have no samples yet
-               const int64_t p = (int64_t)a->params.width * a->params.height;
-               if (!run.empty() && (px + p > cfg.max_pass_pixels || (double)px >= pass_work)) break;
-               run.push_back(a);
-               px += p;
-            }
-            if (px > 0)
-               pass_samples = (int)std::min<double>(cfg.max_pass_samples, std::max(1.0, pass_work / (double)px));
          }
-         // create device state for admitted jobs (outside the lock: allocations)
-         std::vector<SJob*> ok_fresh;
-         for (SJob* j : fresh) {
-            rayon_status st = rayon_job_create(ctx, j->scene, &j->params, &j->job);
-            if (st == RAYON_OK && hipMalloc(&j->frame, (size_t)j->params.width * j->params.height * 4) != hipSuccess) {
-               ry::set_error("hipMalloc of the first-frame buffer failed");
-               st = RAYON_ERR_OOM;
+         // (1) new jobs: buffers, first pass and first frame on first_ctx
+         if (!fresh.empty()) {
+            const double t0 = now_s();
+            std::vector<SJob*> ok_fresh;
+            for (SJob* j : fresh) {
+               // created on first_ctx: its allocations and clears must not queue behind
+               // the refinement pass in flight on the main stream
+               rayon_status st = rayon_job_create(first_ctx, j->scene, &j->params, &j->job);
+               if (st == RAYON_OK && hipMallocAsync((void**)&j->frame, (size_t)j->params.width * j->params.height * 4,
+                                                    first_ctx->stream) != hipSuccess) {
+                  ry::set_error("allocation of the first-frame buffer failed");
+                  st = RAYON_ERR_OOM;
+               }
+               std::lock_guard<std::mutex> lk(mu);
+               if (st != RAYON_OK) {
+                  last_error = rayon_last_error();
+                  j->state = RAYON_JOB_FAILED;
+                  j->t_done = now_s();
+                  ++failed;
+                  free_job_locked(j);
+                  active.erase(std::find(active.begin(), active.end(), j));
+                  done_cv.notify_all();
+                  continue;
+               }
+               j->state = RAYON_JOB_RUNNING;
+               j->live_px = (int64_t)j->params.width * j->params.height;
+               ok_fresh.push_back(j);
             }
-            std::lock_guard<std::mutex> lk(mu);
-            if (st != RAYON_OK) {
-               last_error = rayon_last_error();
-               j->state = RAYON_JOB_FAILED;
-               j->t_done = now_s();
-               ++failed;
-               free_job_locked(j);
-               active.erase(std::find(active.begin(), active.end(), j));
-               done_cv.notify_all();
-               continue;
-            }
-            j->state = RAYON_JOB_RUNNING;
-            ok_fresh.push_back(j);
-         }
-         const double t0 = now_s();
-         // (2) first frames
-         if (!ok_fresh.empty()) {
-            std::vector<rayon_job*> h;
-            for (SJob* j : ok_fresh) h.push_back(j->job);
-            if (rayon_render_batched(ctx, h.data(), (int)h.size(), cfg.first_pass_samples) != RAYON_OK) {
-               fail_all(ok_fresh, "first pass");
-            } else {
+            if (!ok_fresh.empty()) {
+               // The first frame is a preview: the job's first samples rendered by the
+               // one-launch plain kernel into a scratch job and resolved to RGBA8. It does
+               // not enter the job's sums (its samples arrive with the refinement passes),
+               // so a served job's result equals the job rendered alone.
                bool ok = true;
-               for (SJob* j : ok_fresh)
-                  ok = ok && ry::launch_resolve_u8(j->job->view, j->frame, 4, 2.0f, ctx->stream) == hipSuccess;
-               hipError_t e = ok ? hipStreamSynchronize(ctx->stream) : hipErrorLaunchFailure;
+               for (SJob* j : ok_fresh) {
+                  rayon_job_params pp = j->params;
+                  pp.spp = cfg.first_pass_samples;
+                  pp.flags &= ~(uint32_t)(RAYON_FLAG_ADAPTIVE_REF | RAYON_FLAG_ADAPTIVE_STDERR);
+                  rayon_job* pv = nullptr;
+                  ok = ok && rayon_job_create(first_ctx, j->scene, &pp, &pv) == RAYON_OK;
+                  ok = ok && rayon_job_render_plain(first_ctx, pv, cfg.first_pass_samples) == RAYON_OK;
+                  ok = ok && ry::launch_resolve_u8(pv->view, j->frame, 4, 2.0f, first_ctx->stream) == hipSuccess;
+                  if (pv) rayon_job_destroy(pv);  // waits for the resolve
+               }
                const double t = now_s();
                std::lock_guard<std::mutex> lk(mu);
                ++first_passes;
+               busy_s += t - t0;
+               first_ms_sum += (t - t0) * 1e3;
                for (SJob* j : ok_fresh) {
-                  if (e != hipSuccess) {
+                  if (!ok) {
                      j->state = RAYON_JOB_FAILED;
                      ++failed;
                   } else {
                      j->t_first = t;
-                     j->samples = j->job->samples_done;
                   }
                }
+               std::vector<SJob*> failed_now;
+               for (SJob* j : ok_fresh)
+                  if (j->state == RAYON_JOB_FAILED) failed_now.push_back(j);
+               finish_locked(failed_now);
             }
          }
-         // (3) one batched pass for the jobs that already have a frame
-         if (!run.empty()) {
-            std::vector<rayon_job*> h;
-            for (SJob* j : run) h.push_back(j->job);
-            const double tp = now_s();
-            if (rayon_render_batched(ctx, h.data(), (int)h.size(), pass_samples) != RAYON_OK ||
-                rayon_ctx_sync(ctx) != RAYON_OK) {
-               fail_all(run, "batched pass");
-            } else {
-               const double ms = (now_s() - tp) * 1e3;
-               // controller toward target_pass_ms (the ratio rule of
-               // renderer_cuda_progressive_host.hpp:681-686, clamped to [0.5, 2] per step)
-               const double ratio = std::min(2.0, std::max(0.5, cfg.target_pass_ms / std::max(ms, 0.05)));
-               double done = 0;
-               for (SJob* j : run) done += (double)j->params.width * j->params.height;
-               pass_work = std::min(std::max(done * pass_samples * ratio, 65536.0), 1e12);
-               std::lock_guard<std::mutex> lk(mu);
-               ++passes;
-            }
-         }
-         // (4) completion
-         std::vector<SJob*> finished;
-         {
-            // only the jobs this step touched can have finished
-            std::vector<SJob*> check = run;
-            check.insert(check.end(), ok_fresh.begin(), ok_fresh.end());
-            for (SJob* j : check) {
-               if (j->state == RAYON_JOB_FAILED) {
-                  finished.push_back(j);
-                  continue;
-               }
-               const bool adaptive = (j->params.flags & (RAYON_FLAG_ADAPTIVE_REF | RAYON_FLAG_ADAPTIVE_STDERR)) != 0;
-               if (!adaptive && j->job->samples_done < j->params.spp) {  // not done: skip the device read
-                  std::lock_guard<std::mutex> lk(mu);
-                  j->samples = j->job->samples_done;
-                  continue;
-               }
-               rayon_job_stats s;
-               if (rayon_job_stats_get(ctx, j->job, &s) != RAYON_OK) continue;
-               std::lock_guard<std::mutex> lk(mu);
-               j->samples = s.samples_done;
-               j->segments = s.segments;
-               if (s.done) finished.push_back(j);
-            }
-         }
-         {
+         // (2) the refinement pass in flight, once complete
+         if (flying && hipEventQuery(pass_done) == hipSuccess) {
+            const double ms = (now_s() - t_pass) * 1e3;
+            // controller toward target_pass_ms (the ratio rule of
+            // renderer_cuda_progressive_host.hpp:681-686, clamped to [0.5, 2] per step)
+            const double ratio = std::min(2.0, std::max(0.5, cfg.target_pass_ms / std::max(ms, 0.05)));
+            double done = 0;
+            for (SJob* j : in_flight) done += (double)j->live_px;
+            pass_work = std::min(std::max(done * pass_samples * ratio, 65536.0), 1e12);
+            const double tc = now_s();
+            std::vector<SJob*> finished = check(ctx, in_flight);
             std::lock_guard<std::mutex> lk(mu);
-            const double t = now_s();
-            busy_s += t - t0;
-            for (SJob* j : finished) {
-               if (j->state != RAYON_JOB_FAILED) {
-                  j->state = RAYON_JOB_DONE;
-                  ++completed;
+            ++passes;
+            busy_s += ms * 1e-3;
+            pass_ms_sum += ms;
+            check_ms_sum += (now_s() - tc) * 1e3;
+            pass_paths_sum += done * pass_samples;
+            pass_jobs = (int)in_flight.size();
+            finish_locked(finished);
+            in_flight.clear();
+            flying = false;
+         }
+         // (3) start the next refinement pass over the oldest jobs that have a frame
+         if (!flying) {
+            std::vector<SJob*> run;
+            {
+               std::lock_guard<std::mutex> lk(mu);
+               int64_t px = 0;
+               for (SJob* a : active) {
+                  if (a->state != RAYON_JOB_RUNNING || a->t_first <= 0 || a->cancel) continue;
+                  const int64_t p = a->live_px;
+                  if (!run.empty() && (px + p > cfg.max_pass_pixels || (double)px >= pass_work)) break;
+                  run.push_back(a);
+                  px += p;
                }
-               j->t_done = t;
-               total_segments += j->segments;
-               active.erase(std::find(active.begin(), active.end(), j));
-               if (j->released) free_job_locked(j);
+               if (px > 0)
+                  pass_samples = (int)std::min<double>(cfg.max_pass_samples, std::max(1.0, pass_work / (double)px));
             }
-            if (!finished.empty()) done_cv.notify_all();
+            if (!run.empty()) {
+               std::vector<rayon_job*> h;
+               for (SJob* j : run) h.push_back(j->job);
+               t_pass = now_s();
+               if (rayon_render_batched(ctx, h.data(), (int)h.size(), pass_samples) != RAYON_OK ||
+                   hipEventRecord(pass_done, ctx->stream) != hipSuccess) {
+                  fail_all(run, "batched pass");
+                  (void)rayon_ctx_sync(ctx);
+                  std::lock_guard<std::mutex> lk(mu);
+                  finish_locked(run);
+               } else {
+                  in_flight = run;
+                  flying = true;
+               }
+            }
          }
       }
+      (void)rayon_ctx_sync(ctx);
+      (void)hipEventDestroy(pass_done);
    }
 };
 
@@ -295,7 +355,7 @@ void rayon_server_config_default(rayon_server_config* c) {
    c->max_active_jobs = 1 << 16;
    c->first_pass_samples = 1;
    c->max_pass_samples = 64;
-   c->target_pass_ms = 15.0;
+   c->target_pass_ms = 80.0;
    c->max_pass_pixels = 16ll << 20;
 }
 
@@ -316,8 +376,13 @@ rayon_status rayon_server_create(rayon_ctx* ctx, const rayon_server_config* cfg,
       ry::set_error("rayon_server_create: bad configuration");
       return RAYON_ERR_INVALID;
    }
+   if (rayon_ctx_create(ctx->device, &s->first_ctx) != RAYON_OK) {
+      delete s;
+      return RAYON_ERR_HIP;
+   }
    // passes change their job set every step: replaying captured graphs would not pay
    rayon_ctx_set_option(ctx, RAYON_OPT_GRAPH, 0);
+   rayon_ctx_set_option(s->first_ctx, RAYON_OPT_GRAPH, 0);
    s->t_start = now_s();
    s->th = std::thread([s] { s->loop(); });
    *out = s;
@@ -462,6 +527,11 @@ rayon_status rayon_server_stats_get(rayon_server* s, rayon_server_stats* out) {
    out->busy_s = s->busy_s;
    out->wall_s = now_s() - s->t_start;
    out->pass_samples = s->pass_samples;
+   out->pass_jobs = s->pass_jobs;
+   out->pass_ms = s->pass_ms_sum;
+   out->first_ms = s->first_ms_sum;
+   out->check_ms = s->check_ms_sum;
+   out->pass_paths = s->pass_paths_sum;
    return RAYON_OK;
 }
 
@@ -479,6 +549,8 @@ rayon_status rayon_server_destroy(rayon_server* s) {
       s->free_job_locked(kv.second);
       delete kv.second;
    }
+   (void)rayon_ctx_sync(s->first_ctx);
+   rayon_ctx_destroy(s->first_ctx);
    delete s;
    return RAYON_OK;
 }

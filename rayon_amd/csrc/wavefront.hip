@@ -36,8 +36,8 @@ namespace ry {
 
 constexpr int kWfBlock = 256;
 constexpr int kShortStack = 24;  // LDS stack of binary trees up to depth 23 (more waves per CU)
-constexpr int kSdfSlice = 16;   // marching steps per slice; rays still marching are regrouped
-constexpr int kSdfSlices = 16;  // slices per level at most (the last one runs to the end)
+constexpr int kSdfSlice = 8;   // steps of the first marching slice; slice k takes kSdfSlice << k
+constexpr int kSdfSlices = 8;  // slices per level at most (the last one runs to the end)
 constexpr long long kMegaSceneBytes = 8ll << 20;  // automatic path choice: megakernel up to this scene size
 constexpr int kCtl = 8;  // control words per level
 constexpr uint32_t kSkip = 1u;        // ids.w bit: pixel converged, no path this sample
@@ -74,7 +74,8 @@ struct Wave {
    int capacity = 0;
    int levels = 0;
    int max_jobs = 0;
-   std::vector<DevBuf> bufs;
+   int round = 0;  // path slots of one round of the current pass (<= capacity)
+   std::vector<DevBuf> bufs, bufs_slots, bufs_ctl, bufs_tables;
    WfView v{};
    JobView* jobs_dev = nullptr;
    long long* base_dev = nullptr;
@@ -185,9 +186,10 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_generate(WfView V, long long w0
       const long long w = w0 + i;
       const int j = find_job(V, w);
       const JobView& J = V.jobs[j];
-      const unsigned npix = (unsigned)J.width * (unsigned)J.height;
+      const unsigned units = J.nlive >= 0 ? (unsigned)J.nlive : (unsigned)J.width * (unsigned)J.height;
       int s, p;
-      split_work(w - V.base[j], npix, s, p);
+      split_work(w - V.base[j], units, s, p);
+      if (J.nlive >= 0) p = J.live[p];
       const int sample = J.sample_base + s;
       uint32_t flags = 0;
       if (J.counts[p] < 0) {
@@ -213,8 +215,8 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_generate(WfView V, long long w0
    if (live) V.q0[pos] = i;
 }
 
-// SDF marching stage of extension rays: slice `it` takes up to kSdfSlice steps of every
-// ray in its queue (the last slice runs to the end); rays still marching go to the next
+// SDF marching stage of extension rays: slice `it` takes up to kSdfSlice << it steps of
+// every ray in its queue (the last slice runs to the end); rays still marching go to the next
 // slice's queue, so each slice's waves hold only live rays. A hit replaces the ray's hit
 // record (the march ends at the closest other hit, and an exact tie goes to the field,
 // which has the largest original index). The march state lives in the shadow-ray arrays
@@ -246,7 +248,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_sdf(WfView V, int level, int it
          }
          if (live) {
             float t = 0.0f;
-            const int r = sdf_march_run(S, o, d, m, last ? S.sdf_steps : kSdfSlice, t);
+            const int r = sdf_march_run(S, o, d, m, last ? S.sdf_steps : (kSdfSlice << it), t);
             if (r == 1) {
                V.hit[slot] = make_float4(t, __int_as_float(S.sdf_prim), 0.0f, 0.0f);
             } else if (r < 0) {
@@ -423,16 +425,17 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_accumulate(WfView V, long long 
       const long long w = w0 + i;
       j = find_job(V, w);
       const JobView& J = V.jobs[j];
-      const long long npix = (long long)J.width * J.height;
+      const long long units = J.nlive >= 0 ? (long long)J.nlive : (long long)J.width * J.height;
       const long long l = w - V.base[j];
-      if (l < npix || w - npix < w0) {  // first sample of this pixel within the round
+      if (l < units || w - units < w0) {  // first sample of this pixel within the round
          active = true;
          int s, p;
-         split_work(l, (unsigned)npix, s, p);
+         split_work(l, (unsigned)units, s, p);
+         if (J.nlive >= 0) p = J.live[p];
          const long long end = min(w0 + n, V.base[j + 1]);
          float4 a = J.acc[p], c = J.comp[p];
          int added = 0;
-         for (long long wk = w; wk < end; wk += npix) {
+         for (long long wk = w; wk < end; wk += units) {
             const int slot = (int)(wk - w0);
             const uint32_t f = V.ids[slot].w;
             if (f & kSkip) continue;
@@ -499,6 +502,16 @@ __global__ void k_wf_pass_end(JobView J, const float4* prev, int todo) {
    if ((threadIdx.x & 63) == 0 && s) atomicAdd(&J.counters[8 * (blockIdx.x % kCounterSlots) + 3], s);
 }
 
+// The pixels of an adaptive job that still sample (count >= 0), in any order; each pass's
+// work then covers these only. The order does not reach the results: every pixel still
+// folds its samples in sample order, and streams depend on (pixel, sample) only.
+__global__ void k_wf_live(JobView J, int* live, unsigned* n) {
+   const int p = blockIdx.x * blockDim.x + threadIdx.x;
+   const bool on = p < J.width * J.height && J.counts[p] >= 0;
+   const int pos = wave_push(n, on);
+   if (on) live[pos] = p;
+}
+
 // ---------------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------------
@@ -524,63 +537,74 @@ static rayon_status wave_reserve(rayon_ctx* ctx, int capacity, int levels, int n
       W->occ_sdf = n > 0 ? n : 1;
    }
    if (W->capacity >= capacity && W->levels >= levels && W->max_jobs >= njobs) return RAYON_OK;
+   // Three groups grow independently and geometrically (path slots, control words, job
+   // tables), so a growing job count or pass size does not reallocate gigabytes of slot
+   // state each time. Captured graphs hold the old pointers: drop them.
    RY_HIP(hipStreamSynchronize(ctx->stream));
    for (auto& kv : W->graphs) {
       (void)hipGraphExecDestroy(kv.second.exec);
       (void)hipGraphDestroy(kv.second.graph);
    }
    W->graphs.clear();
-   for (auto& b : W->bufs)
-      if (b.p) (void)hipFree(b.p);
-   W->bufs.clear();
-   const int cap = std::max(capacity, W->capacity);
-   const int lv = std::max(levels, W->levels);
-   const int mj = std::max(njobs, W->max_jobs);
-   auto alloc = [&](size_t bytes, void** p) -> rayon_status {
+   auto release = [](std::vector<DevBuf>& g) {
+      for (auto& b : g)
+         if (b.p) (void)hipFree(b.p);
+      g.clear();
+   };
+   auto alloc = [&](std::vector<DevBuf>& g, size_t bytes, void** p) -> rayon_status {
       hipError_t e = hipMalloc(p, bytes);
       if (e != hipSuccess) return hip_fail(e, "hipMalloc (wavefront state)");
-      W->bufs.push_back({*p, bytes});
+      g.push_back({*p, bytes});
       return RAYON_OK;
    };
    void* p;
    rayon_status st;
-   float4** f4[] = {&W->v.o, &W->v.d, &W->v.thr, &W->v.rad, &W->v.hit, &W->v.so, &W->v.sd, &W->v.sc};
-   for (float4** f : f4) {
-      if ((st = alloc((size_t)cap * sizeof(float4), &p)) != RAYON_OK) return st;
-      *f = (float4*)p;
+   if (!W->stats_dev) {
+      if ((st = alloc(W->bufs, 8 * sizeof(unsigned long long), &p)) != RAYON_OK) return st;
+      W->stats_dev = (unsigned long long*)p;
+      RY_HIP(launch_fill_u32((uint32_t*)W->stats_dev, 0u, 16, ctx->stream));
    }
-   if ((st = alloc((size_t)cap * sizeof(uint4), &p)) != RAYON_OK) return st;
-   W->v.ids = (uint4*)p;
-   if ((st = alloc((size_t)cap * sizeof(int), &p)) != RAYON_OK) return st;
-   W->v.q0 = (int*)p;
-   if ((st = alloc((size_t)cap * sizeof(int), &p)) != RAYON_OK) return st;
-   W->v.q1 = (int*)p;
-   if ((st = alloc((size_t)cap * sizeof(int), &p)) != RAYON_OK) return st;
-   W->v.sq = (int*)p;
-   if ((st = alloc((size_t)cap * sizeof(int), &p)) != RAYON_OK) return st;
-   W->v.qs0 = (int*)p;
-   if ((st = alloc((size_t)cap * sizeof(int), &p)) != RAYON_OK) return st;
-   W->v.qs1 = (int*)p;
-   // control words: kCtl per level, then 2 per (level, marching slice), cleared together
-   if ((st = alloc(ctl_words(lv) * sizeof(unsigned), &p)) != RAYON_OK) return st;
-   W->v.ctl = (unsigned*)p;
-   W->v.sctl = W->v.ctl + (size_t)(lv + 1) * kCtl;
-   if ((st = alloc((size_t)mj * sizeof(JobView), &p)) != RAYON_OK) return st;
-   W->jobs_dev = (JobView*)p;
-   if ((st = alloc((size_t)(mj + 1) * sizeof(long long), &p)) != RAYON_OK) return st;
-   W->base_dev = (long long*)p;
-   if ((st = alloc(8 * sizeof(unsigned long long), &p)) != RAYON_OK) return st;
-   W->stats_dev = (unsigned long long*)p;
-   if ((st = alloc((size_t)mj * sizeof(JobView), &p)) != RAYON_OK) return st;
-   W->mjobs_dev = (JobView*)p;
-   if ((st = alloc((size_t)(mj + 1) * sizeof(int), &p)) != RAYON_OK) return st;
-   W->mtile_dev = (int*)p;
-   if ((st = alloc((size_t)mj * sizeof(int), &p)) != RAYON_OK) return st;
-   W->mtodo_dev = (int*)p;
-   RY_HIP(launch_fill_u32((uint32_t*)W->stats_dev, 0u, 16, ctx->stream));
-   W->capacity = cap;
-   W->levels = lv;
-   W->max_jobs = mj;
+   if (capacity > W->capacity) {
+      release(W->bufs_slots);
+      const int cap = (int)std::max<long long>(capacity, std::min<long long>(2ll * W->capacity, W->opt_capacity));
+      float4** f4[] = {&W->v.o, &W->v.d, &W->v.thr, &W->v.rad, &W->v.hit, &W->v.so, &W->v.sd, &W->v.sc};
+      for (float4** f : f4) {
+         if ((st = alloc(W->bufs_slots, (size_t)cap * sizeof(float4), &p)) != RAYON_OK) return st;
+         *f = (float4*)p;
+      }
+      if ((st = alloc(W->bufs_slots, (size_t)cap * sizeof(uint4), &p)) != RAYON_OK) return st;
+      W->v.ids = (uint4*)p;
+      int** qs[] = {&W->v.q0, &W->v.q1, &W->v.sq, &W->v.qs0, &W->v.qs1};
+      for (int** q : qs) {
+         if ((st = alloc(W->bufs_slots, (size_t)cap * sizeof(int), &p)) != RAYON_OK) return st;
+         *q = (int*)p;
+      }
+      W->capacity = cap;
+   }
+   if (levels > W->levels) {
+      release(W->bufs_ctl);
+      const int lv = std::max(levels, std::max(2 * W->levels, 16));
+      // control words: kCtl per level, then 2 per (level, marching slice), cleared together
+      if ((st = alloc(W->bufs_ctl, ctl_words(lv) * sizeof(unsigned), &p)) != RAYON_OK) return st;
+      W->v.ctl = (unsigned*)p;
+      W->v.sctl = W->v.ctl + (size_t)(lv + 1) * kCtl;
+      W->levels = lv;
+   }
+   if (njobs > W->max_jobs) {
+      release(W->bufs_tables);
+      const int mj = std::max(njobs, std::max(2 * W->max_jobs, 64));
+      if ((st = alloc(W->bufs_tables, (size_t)mj * sizeof(JobView), &p)) != RAYON_OK) return st;
+      W->jobs_dev = (JobView*)p;
+      if ((st = alloc(W->bufs_tables, (size_t)(mj + 1) * sizeof(long long), &p)) != RAYON_OK) return st;
+      W->base_dev = (long long*)p;
+      if ((st = alloc(W->bufs_tables, (size_t)mj * sizeof(JobView), &p)) != RAYON_OK) return st;
+      W->mjobs_dev = (JobView*)p;
+      if ((st = alloc(W->bufs_tables, (size_t)(mj + 1) * sizeof(int), &p)) != RAYON_OK) return st;
+      W->mtile_dev = (int*)p;
+      if ((st = alloc(W->bufs_tables, (size_t)mj * sizeof(int), &p)) != RAYON_OK) return st;
+      W->mtodo_dev = (int*)p;
+      W->max_jobs = mj;
+   }
    W->v.jobs = W->jobs_dev;
    W->v.base = W->base_dev;
    W->v.stats = W->stats_dev;
@@ -598,8 +622,9 @@ void wave_destroy(rayon_ctx* ctx) {
       (void)hipGraphExecDestroy(kv.second.exec);
       (void)hipGraphDestroy(kv.second.graph);
    }
-   for (auto& b : W->bufs)
-      if (b.p) (void)hipFree(b.p);
+   for (auto* g : {&W->bufs, &W->bufs_slots, &W->bufs_ctl, &W->bufs_tables})
+      for (auto& b : *g)
+         if (b.p) (void)hipFree(b.p);
    delete W;
    ctx->wave = nullptr;
 }
@@ -651,7 +676,13 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
    V.njobs = (int)jobs.size();
    V.levels = levels;
    V.chunk = W->opt_chunk;
-   auto grid_of = [&](int occ) { return W->cus * (W->opt_wg_per_cu > 0 ? W->opt_wg_per_cu : occ); };
+   // persistent grids: the resident workgroups, but no more than one lane per path of a
+   // round (small passes then launch and drain quickly)
+   const long long per_round = std::min<long long>(std::max(total, 1ll), W->round);
+   const int needed = (int)std::max<long long>(1, (per_round + kWfBlock - 1) / kWfBlock);
+   auto grid_of = [&](int occ) {
+      return std::min(needed, W->cus * (W->opt_wg_per_cu > 0 ? W->opt_wg_per_cu : occ));
+   };
    const dim3 g_shade(grid_of(W->occ_shade)), blk(kWfBlock);
    // one layout for the whole pass when every scene shares it
    int nwide = 0;
@@ -680,7 +711,8 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
    int sdf_steps = 0;
    for (rayon_job* j : jobs)
       if (j->scene->view.num_sdf > 0) sdf_steps = std::max(sdf_steps, j->scene->view.sdf_steps);
-   const int nslices = std::min(kSdfSlices, (sdf_steps + kSdfSlice - 1) / kSdfSlice);
+   int nslices = 0;
+   for (int covered = 0; covered < sdf_steps && nslices < kSdfSlices; ++nslices) covered += kSdfSlice << nslices;
    const dim3 g_sdf(grid_of(W->occ_sdf));
    auto march = [&](int level) {
       if (level < 0) return;
@@ -698,8 +730,8 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
          launch_stage(W, 5, k_wf_pass_begin, dim3((np + 255) / 256), dim3(256), st, j->view, j->prev);
       }
    }
-   for (long long w0 = 0; w0 < total; w0 += W->capacity) {
-      const int n = (int)std::min<long long>(W->capacity, total - w0);
+   for (long long w0 = 0; w0 < total; w0 += W->round) {
+      const int n = (int)std::min<long long>(W->round, total - w0);
       ++W->rounds;
       if ((e = launch_fill_u32(V.ctl, 0u, (long long)ctl_words(W->levels), st)) != hipSuccess) return e;
       W->launches[5] += 1;
@@ -745,8 +777,8 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
    std::vector<int> todo;
    for (int k = 0; k < n; ++k) {
       rayon_job* j = jobs_in[k];
-      if (!j || j->ctx != ctx) {
-         set_error("rayon_render_batched: job %d is NULL or belongs to another context", k);
+      if (!j || j->ctx->device != ctx->device) {
+         set_error("rayon_render_batched: job %d is NULL or lives on another device", k);
          return RAYON_ERR_INVALID;
       }
       if (j->params.flags & RAYON_FLAG_REF_RNG) {
@@ -770,6 +802,44 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
    // each job keeps the path it took in its first pass (the two paths round differently)
    for (rayon_job* j : jobs)
       if (j->path < 0) j->path = scene_batch_path(j->scene);
+   RY_HIP(hipSetDevice(ctx->device));
+   std::lock_guard<std::mutex> lk(ctx->mu);
+   // Adaptive jobs on the wavefront whose pixels may have converged: list the pixels still
+   // sampling, so the pass's work covers those only (units of work per sample).
+   std::vector<long long> units(jobs.size());
+   std::vector<size_t> listed;
+   for (size_t k = 0; k < jobs.size(); ++k) {
+      rayon_job* j = jobs[k];
+      units[k] = (long long)j->params.width * j->params.height;
+      const bool adaptive = (j->params.flags & (RAYON_FLAG_ADAPTIVE_REF | RAYON_FLAG_ADAPTIVE_STDERR)) != 0;
+      if (todo[k] <= 0 || j->path == RAYON_PATH_MEGAKERNEL || !adaptive ||
+          j->samples_done < std::max(1, j->params.min_adaptive_samples))
+         continue;
+      if (!j->live) {
+         void* p = nullptr;
+         const size_t bytes = (size_t)units[k] * sizeof(int) + 256;
+         RY_HIP(hipMalloc(&p, bytes));
+         j->bufs.push_back({p, bytes});
+         j->live_n = (unsigned*)p;
+         j->live = (int*)((char*)p + 256);
+      }
+      RY_HIP(launch_fill_u32(j->live_n, 0u, 1, ctx->stream));
+      hipLaunchKernelGGL(k_wf_live, dim3((unsigned)((units[k] + 255) / 256)), dim3(256), 0, ctx->stream, j->view,
+                         j->live, j->live_n);
+      RY_HIP(hipGetLastError());
+      listed.push_back(k);
+   }
+   if (!listed.empty()) {
+      std::vector<unsigned> counts(listed.size());
+      for (size_t i = 0; i < listed.size(); ++i)
+         RY_HIP(hipMemcpyAsync(&counts[i], jobs[listed[i]]->live_n, sizeof(unsigned), hipMemcpyDeviceToHost,
+                               ctx->stream));
+      RY_HIP(hipStreamSynchronize(ctx->stream));
+      for (size_t i = 0; i < listed.size(); ++i) {
+         units[listed[i]] = counts[i];
+         if (counts[i] == 0) todo[listed[i]] = 0;  // every pixel has converged
+      }
+   }
    long long total = 0;
    int levels = 1, nmega = 0;
    for (size_t k = 0; k < jobs.size(); ++k) {
@@ -778,17 +848,16 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
          ++nmega;
          continue;
       }
-      total += (long long)todo[k] * jobs[k]->params.width * jobs[k]->params.height;
+      total += (long long)todo[k] * units[k];
       levels = std::max(levels, jobs[k]->params.max_depth);
    }
    if (total == 0 && nmega == 0) return RAYON_OK;
-   RY_HIP(hipSetDevice(ctx->device));
-   std::lock_guard<std::mutex> lk(ctx->mu);
    if (!ctx->wave) ctx->wave = new Wave();
-   const int cap = (int)std::min<long long>(ctx->wave->opt_capacity, std::max(total, 1ll));
-   rayon_status st = wave_reserve(ctx, std::max(cap, 1 << 16), levels, n);
+   const int cap = std::max((int)std::min<long long>(ctx->wave->opt_capacity, std::max(total, 1ll)), 1 << 16);
+   rayon_status st = wave_reserve(ctx, cap, levels, n);
    if (st != RAYON_OK) return st;
    Wave* W = ctx->wave;
+   W->round = cap;  // rounds follow the capacity option, whatever has been allocated
    for (size_t k = 0; k < jobs.size(); ++k) {
       rayon_job* j = jobs[k];
       if (todo[k] > 0 && j->path != RAYON_PATH_MEGAKERNEL && (j->params.flags & RAYON_FLAG_ADAPTIVE_REF) && !j->prev) {
@@ -822,10 +891,14 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
          W->host_mtodo.push_back(todo[k]);
          continue;
       }
+      if (std::find(listed.begin(), listed.end(), k) != listed.end()) {
+         j->view.live = j->live;
+         j->view.nlive = (int)units[k];
+      }
       active.push_back(j);
       atodo.push_back(todo[k]);
       W->host_jobs.push_back(j->view);
-      W->host_base.push_back(W->host_base.back() + (long long)todo[k] * j->params.width * j->params.height);
+      W->host_base.push_back(W->host_base.back() + (long long)todo[k] * units[k]);
    }
    const int mlayout = mwide == 0 ? L_BIN : (mwide == nmega ? L_WIDE : L_ANY);
    if (!active.empty()) {
@@ -845,7 +918,12 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
    // the table copies read pageable host memory: finish them before the vectors change again
    RY_HIP(hipStreamSynchronize(ctx->stream));
    const int mtiles = W->host_mtile.back();
-   const bool use_graph = W->opt_graph && !W->opt_timing;
+   // a graph keyed by adaptive jobs must be forgotten with them, which only their own
+   // context's job destruction does: passes over other contexts' jobs run eagerly
+   bool foreign = false;
+   for (rayon_job* j : jobs) foreign = foreign || j->ctx != ctx;
+   // passes over pixel lists change size every time: not worth a graph each
+   const bool use_graph = W->opt_graph && !W->opt_timing && !foreign && listed.empty();
    if (!use_graph) {
       RY_HIP(enqueue_pass(ctx, active, atodo, total, levels, nmega, mtiles, mlayout));
    } else {
@@ -859,7 +937,7 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
          key.push_back(W->host_base[k + 1] - W->host_base[k]);
       }
       key.push_back(levels);
-      key.push_back(W->capacity);
+      key.push_back(W->round);
       key.push_back(W->opt_wg_per_cu);
       key.push_back(W->opt_chunk);
       key.push_back(W->opt_count);
@@ -876,6 +954,14 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
       key.push_back(mtiles);
       key.push_back(mlayout);
       auto it = W->graphs.find(key);
+      if (it == W->graphs.end() && W->graphs.size() >= 64) {  // bounded cache: start over
+         for (auto& kv : W->graphs) {
+            (void)hipGraphExecDestroy(kv.second.exec);
+            (void)hipGraphDestroy(kv.second.graph);
+         }
+         W->graphs.clear();
+         it = W->graphs.end();
+      }
       if (it == W->graphs.end()) {
          Wave::GraphEntry ge;
          RY_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));

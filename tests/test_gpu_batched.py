@@ -234,6 +234,35 @@ def test_adaptive_stderr_batched_matches_plain(ctx, scenes):
     b.close()
 
 
+def test_adaptive_on_the_wavefront_compacts_live_pixels(ctx, scenes):
+    """Adaptive jobs on the wavefront sample only the pixels that have not converged (a
+    pixel list per pass); their per-pixel counts and sums agree with the plain kernel's
+    (at most 0.1% of pixels differ in count: a convergence decision can flip on the last
+    bits of the two paths' rounding)."""
+    log("adaptive on the wavefront")
+    kw = dict(spp=96, seed=5, max_depth=8, flags=ra.FLAG.ADAPTIVE_STDERR, min_adaptive_samples=16,
+              adaptive_threshold=0.05)
+    try:
+        scenes.get(*BOOK)[1].set_batch_path(ra.PATH.WAVEFRONT)
+        a, *_ = scenes.job(*BOOK, 96, 64, **kw)
+        b, *_ = scenes.job(*BOOK, 96, 64, **kw)
+        for _ in range(6):
+            a.render_plain(16)
+            ctx.render_batched([b], 16)
+        ca, cb = a.sample_counts(), b.sample_counts()
+        log(f"  converged plain {(ca < 0).mean():.3f} batched {(cb < 0).mean():.3f} agree {np.mean(ca == cb):.4f}")
+        assert (cb < 0).mean() > 0.2 and np.mean(ca == cb) > 0.999
+        assert b.stats().converged_pixels == int((cb < 0).sum())
+        same = ca == cb
+        ia, ib = a.resolve_linear()[same], b.resolve_linear()[same]
+        assert (np.abs(ia - ib) / np.maximum(np.abs(ia), 1e-3)).max() <= 1e-4
+        assert b.stats().primary_samples == int(np.abs(cb).sum())
+        a.close()
+        b.close()
+    finally:
+        scenes.get(*BOOK)[1].set_batch_path(ra.PATH.AUTO)
+
+
 def test_reference_stream_jobs_are_rejected(ctx, scenes):
     j, *_ = scenes.job(*BOOK, 16, 16, spp=4, flags=ra.FLAG.REF_RNG)
     with pytest.raises(ra.RayonError, match="REF_RNG"):

@@ -4,7 +4,7 @@ Tolerances: a job served among others must equal the same job rendered alone wit
 rayon_render_batched bit for bit (the random streams depend only on seed, pixel and
 sample index, and each pixel accumulates its samples in sample order, whatever the
 pass sizes the server's controller picks); the first frame equals a one-sample render
-of the job resolved to RGBA8 with gamma 2, byte for byte.
+of the job by the plain kernel resolved to RGBA8 with gamma 2, byte for byte.
 """
 import sys
 
@@ -63,7 +63,8 @@ def test_served_jobs_equal_alone_renders(world):
         j = alone(ctx, scenes, gens, key, W, H, kw)
         refs.append(j.resolve_linear())
         j.close()
-        f = alone(ctx, scenes, gens, key, W, H, kw, samples=1)
+        f = ctx.job(scenes[key], params(gens, key, W, H, **dict(kw, spp=1)))
+        f.render_plain(1)
         firsts.append(f.resolve_u8(4, 2.0))
         f.close()
     log("alone renders done")
