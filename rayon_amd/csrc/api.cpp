@@ -455,6 +455,14 @@ rayon_status rayon_scene_set_batch_path(rayon_scene* scene, int32_t path) {
 int32_t rayon_scene_batch_path(const rayon_scene* scene) { return scene ? ry::scene_batch_path(scene) : -1; }
 
 rayon_status rayon_job_create(rayon_ctx* ctx, rayon_scene* sc, const rayon_job_params* p, rayon_job** out) {
+   return ry::job_create(ctx, sc, p, out, true);
+}
+
+}  // extern "C"
+
+// sync = false leaves the buffers' allocation and clears queued on the context's stream:
+// only for callers that order every later use after that stream.
+rayon_status ry::job_create(rayon_ctx* ctx, rayon_scene* sc, const rayon_job_params* p, rayon_job** out, bool sync) {
    if (!ctx || !sc || !p || !out) {
       set_error("rayon_job_create: NULL argument");
       return RAYON_ERR_INVALID;
@@ -495,7 +503,8 @@ rayon_status rayon_job_create(rayon_ctx* ctx, rayon_scene* sc, const rayon_job_p
    fill_jobview(j);
    st = rayon_job_reset(j);
    // the buffers are valid for every stream (and host copies) once the clears have run
-   if (st == RAYON_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) st = hip_fail(hipGetLastError(), "job set-up");
+   if (st == RAYON_OK && sync && hipStreamSynchronize(ctx->stream) != hipSuccess)
+      st = hip_fail(hipGetLastError(), "job set-up");
    if (st != RAYON_OK) {
       job_free_all(ctx, j->pool_bufs);
       delete j;
@@ -504,6 +513,8 @@ rayon_status rayon_job_create(rayon_ctx* ctx, rayon_scene* sc, const rayon_job_p
    *out = j;
    return RAYON_OK;
 }
+
+extern "C" {
 
 rayon_status rayon_job_destroy(rayon_job* j) {
    if (!j) return RAYON_OK;

@@ -7,9 +7,9 @@
 //   1. admits queued jobs while their device memory (accumulation, counts and a u8 frame,
 //      plus each newly referenced scene) fits the HBM budget and the active-job cap;
 //   2. renders a first frame for each job admitted in this step: first_pass_samples
-//      samples of the job by the one-launch plain kernel into a scratch buffer, resolved
-//      on the device (a preview that does not enter the job's sums); time-to-first-frame
-//      is stamped when that work completes;
+//      samples of the job by the plain kernel's arithmetic (all new jobs in one launch)
+//      into scratch buffers, resolved on the device (a preview that does not enter the
+//      job's sums); time-to-first-frame is stamped when that work completes;
 //   3. advances the oldest other active jobs together by one batched pass whose work
 //      (pixels x samples) a controller steers toward target_pass_ms: the jobs are taken
 //      oldest first until their pixels reach the work budget (at least one job, at most
@@ -231,7 +231,8 @@ struct rayon_server {
             for (SJob* j : fresh) {
                // created on first_ctx: its allocations and clears must not queue behind
                // the refinement pass in flight on the main stream
-               rayon_status st = rayon_job_create(first_ctx, j->scene, &j->params, &j->job);
+               // (no wait here: the previews' synchronisation below orders every later use)
+               rayon_status st = ry::job_create(first_ctx, j->scene, &j->params, &j->job, false);
                if (st == RAYON_OK && hipMallocAsync((void**)&j->frame, (size_t)j->params.width * j->params.height * 4,
                                                     first_ctx->stream) != hipSuccess) {
                   ry::set_error("allocation of the first-frame buffer failed");
@@ -257,17 +258,25 @@ struct rayon_server {
                // one-launch plain kernel into a scratch job and resolved to RGBA8. It does
                // not enter the job's sums (its samples arrive with the refinement passes),
                // so a served job's result equals the job rendered alone.
+               // All new jobs' previews share one launch (the multi-job form of the plain
+               // kernel, which gives the plain kernel's bits).
                bool ok = true;
+               std::vector<rayon_job*> pvs;
                for (SJob* j : ok_fresh) {
                   rayon_job_params pp = j->params;
                   pp.spp = cfg.first_pass_samples;
                   pp.flags &= ~(uint32_t)(RAYON_FLAG_ADAPTIVE_REF | RAYON_FLAG_ADAPTIVE_STDERR);
                   rayon_job* pv = nullptr;
-                  ok = ok && rayon_job_create(first_ctx, j->scene, &pp, &pv) == RAYON_OK;
-                  ok = ok && rayon_job_render_plain(first_ctx, pv, cfg.first_pass_samples) == RAYON_OK;
-                  ok = ok && ry::launch_resolve_u8(pv->view, j->frame, 4, 2.0f, first_ctx->stream) == hipSuccess;
-                  if (pv) rayon_job_destroy(pv);  // waits for the resolve
+                  ok = ok && ry::job_create(first_ctx, j->scene, &pp, &pv, false) == RAYON_OK;
+                  if (!pv) break;
+                  pv->path = RAYON_PATH_MEGAKERNEL;
+                  pvs.push_back(pv);
                }
+               ok = ok && rayon_render_batched(first_ctx, pvs.data(), (int)pvs.size(), cfg.first_pass_samples) == RAYON_OK;
+               for (size_t k = 0; ok && k < pvs.size(); ++k)
+                  ok = ry::launch_resolve_u8(pvs[k]->view, ok_fresh[k]->frame, 4, 2.0f, first_ctx->stream) == hipSuccess;
+               ok = ok && hipStreamSynchronize(first_ctx->stream) == hipSuccess;
+               for (rayon_job* pv : pvs) rayon_job_destroy(pv);
                const double t = now_s();
                std::lock_guard<std::mutex> lk(mu);
                ++first_passes;
