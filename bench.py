@@ -97,6 +97,9 @@ def main(argv=None) -> int:
                     help="which path's throughput is `value` (both are measured)")
     ap.add_argument("--cpu-stride", type=int, default=2, help="CPU baseline pixel subset: every Nth row/col")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--capacity", type=int, default=0, help="path slots per wavefront round (0: library default)")
+    ap.add_argument("--chunk", type=int, default=0, help="queue entries (x64) a trace wave claims (0: default)")
+    ap.add_argument("--shade-chunk", type=int, default=0, help="queue entries (x64) a shade wave claims (0: default)")
     ap.add_argument("--workload", choices=["cornell_1080p", "batch256", "serve"], default="cornell_1080p",
                     help="cornell_1080p: one 1080p 64-spp mesh frame per step (default); batch256: 256 mixed "
                          "jobs, plain one-job-per-launch vs batched passes; serve: Poisson arrivals into the server")
@@ -138,6 +141,12 @@ def main(argv=None) -> int:
     g = ra.Generated.generate(ra.GEN.CORNELL_ICOSPHERE, 7, detail=a.detail)
     ntris = g.desc.meshes[0].num_triangles + 10
     ctx = ra.Context(0)
+    if a.capacity:
+        ctx.set_option(ra.OPT.WAVE_CAPACITY, a.capacity)
+    if a.chunk:
+        ctx.set_option(ra.OPT.WAVE_CHUNK, a.chunk)
+    if a.shade_chunk:
+        ctx.set_option(ra.OPT.SHADE_CHUNK, a.shade_chunk)
     t0 = time.perf_counter()
     sc = ctx.scene(g.desc_ptr())
     info = sc.info()
@@ -498,7 +507,8 @@ def bench_batch256(a, ra) -> int:
         "latency_ms": {"batched_p50": round(percentile(b_lat, 50), 1), "batched_p99": round(percentile(b_lat, 99), 1),
                        "plain_p50": round(percentile(p_lat, 50), 1), "plain_p99": round(percentile(p_lat, 99), 1)},
         "segments": int(b_segs), "plain_segments": int(p_segs),
-        "autotune": {"capacity": tune.capacity, "chunk": tune.chunk, "wg_per_cu": tune.wg_per_cu,
+        "autotune": {"capacity": tune.capacity, "chunk": tune.chunk, "shade_chunk": tune.shade_chunk,
+                     "wg_per_cu": tune.wg_per_cu, "mega_scenes": tune.mega_scenes, "wave_scenes": tune.wave_scenes,
                      "best_ms": round(tune.best_ms, 2), "start_ms": round(tune.start_ms, 2),
                      "shapes": tune.candidates},
         "batched_vs_plain_max_rel_diff": worst,

@@ -275,13 +275,15 @@ rayon_status rayon_render_batched(rayon_ctx* ctx, rayon_job* const* jobs, int32_
 
 /* Context options for the batched path. */
 enum {
-   RAYON_OPT_WAVE_CAPACITY = 0,   /* paths in flight per round (default 64M, ~150 B each)     */
+   RAYON_OPT_WAVE_CAPACITY = 0,   /* paths in flight per round (default 128M, 164 B each,
+                                     allocated as passes need them)                         */
    RAYON_OPT_WAVE_WG_PER_CU = 1,  /* persistent workgroups per CU of the queue stages
                                      (0 = as many as each stage's occupancy allows)         */
    RAYON_OPT_GRAPH = 2,           /* capture each pass into a hipGraph and replay it (1)      */
    RAYON_OPT_STAGE_TIMING = 3,    /* time every stage launch with HIP events (0; disables graphs) */
    RAYON_OPT_COUNT_TRAVERSAL = 4, /* count node/primitive fetches of the queue stages (0)     */
-   RAYON_OPT_WAVE_CHUNK = 5       /* 64-ray batches a wave claims per queue atomic (4)       */
+   RAYON_OPT_WAVE_CHUNK = 5,      /* 64-ray batches a trace wave claims per queue atomic (4) */
+   RAYON_OPT_SHADE_CHUNK = 6      /* 64-hit batches a shade wave claims per queue atomic (16) */
 };
 rayon_status rayon_ctx_set_option(rayon_ctx* ctx, int32_t option, int64_t value);
 rayon_status rayon_ctx_get_option(rayon_ctx* ctx, int32_t option, int64_t* value);
@@ -338,7 +340,7 @@ int32_t rayon_scene_batch_path(const rayon_scene* scene);
  * jobs (same scenes and parameters, fresh buffers, so the jobs themselves are untouched)
  * it (1) times each RAYON_PATH_AUTO scene's jobs on both batched paths and records the
  * faster on the scene, then (2) times wavefront launch shapes (path-slot capacity, queue
- * chunk, workgroups per CU) by coordinate descent, sets the fastest on the context and
+ * chunk, shade chunk, workgroups per CU) by coordinate descent, sets the fastest on the context and
  * caches it under a signature of the workload (job count, total work, depth, scenes,
  * flags). The shape changes only the speed of a pass, never its results.
  * ------------------------------------------------------------------------------- */
@@ -352,6 +354,8 @@ typedef struct rayon_tune_result {
    int32_t cached;
    int32_t mega_scenes;  /* scenes of the jobs on the megakernel path after tuning        */
    int32_t wave_scenes;  /* scenes of the jobs on the wavefront path after tuning         */
+   int32_t shade_chunk;  /* RAYON_OPT_SHADE_CHUNK chosen                                  */
+   int32_t pad_;
 } rayon_tune_result;
 
 rayon_status rayon_autotune(rayon_ctx* ctx, rayon_job* const* jobs, int32_t n, int32_t samples,

@@ -25,13 +25,17 @@ struct Shape {
    long long capacity;
    int chunk;
    int wg;
-   bool operator==(const Shape& o) const { return capacity == o.capacity && chunk == o.chunk && wg == o.wg; }
+   int shade_chunk;
+   bool operator==(const Shape& o) const {
+      return capacity == o.capacity && chunk == o.chunk && wg == o.wg && shade_chunk == o.shade_chunk;
+   }
 };
 
 rayon_status apply(rayon_ctx* ctx, const Shape& s) {
    rayon_status st = rayon_ctx_set_option(ctx, RAYON_OPT_WAVE_CAPACITY, s.capacity);
    if (st == RAYON_OK) st = rayon_ctx_set_option(ctx, RAYON_OPT_WAVE_CHUNK, s.chunk);
    if (st == RAYON_OK) st = rayon_ctx_set_option(ctx, RAYON_OPT_WAVE_WG_PER_CU, s.wg);
+   if (st == RAYON_OK) st = rayon_ctx_set_option(ctx, RAYON_OPT_SHADE_CHUNK, s.shade_chunk);
    return st;
 }
 
@@ -93,7 +97,7 @@ extern "C" rayon_status rayon_autotune(rayon_ctx* ctx, rayon_job* const* jobs, i
          *out = it->second;
          out->candidates = 0;
          out->cached = 1;
-         return apply(ctx, {out->capacity, out->chunk, out->wg_per_cu});
+         return apply(ctx, {out->capacity, out->chunk, out->wg_per_cu, out->shade_chunk});
       }
    }
    Shape start{};
@@ -105,6 +109,8 @@ extern "C" rayon_status rayon_autotune(rayon_ctx* ctx, rayon_job* const* jobs, i
       start.chunk = (int)v;
       if (st == RAYON_OK) st = rayon_ctx_get_option(ctx, RAYON_OPT_WAVE_WG_PER_CU, &v);
       start.wg = (int)v;
+      if (st == RAYON_OK) st = rayon_ctx_get_option(ctx, RAYON_OPT_SHADE_CHUNK, &v);
+      start.shade_chunk = (int)v;
       if (st != RAYON_OK) return st;
    }
    std::vector<rayon_job*> clones;
@@ -163,19 +169,21 @@ extern "C" rayon_status rayon_autotune(rayon_ctx* ctx, rayon_job* const* jobs, i
    std::vector<Shape> tried = {start};
    rayon_status st = measure(start, &start_ms);
    best_ms = start_ms;
-   // coordinate descent: queue chunk, then workers per CU, then capacity
+   // coordinate descent: trace chunk, shade chunk, workers per CU, then capacity
    const long long round_cap = std::min<long long>(1ll << 28, std::max<long long>(1 << 16, work));
    const std::vector<std::vector<long long>> axes = {
       {1, 2, 4, 8, 16},
+      {2, 4, 8, 16},
       {0, 1, 2, 4, 8},
       {1ll << 22, 1ll << 24, 1ll << 25, 1ll << 26, 1ll << 27},
    };
-   for (int ax = 0; ax < 3 && st == RAYON_OK; ++ax) {
+   for (int ax = 0; ax < 4 && st == RAYON_OK; ++ax) {
       for (long long v : axes[ax]) {
          Shape s = best;
          if (ax == 0) s.chunk = (int)v;
-         if (ax == 1) s.wg = (int)v;
-         if (ax == 2) {
+         if (ax == 1) s.shade_chunk = (int)v;
+         if (ax == 2) s.wg = (int)v;
+         if (ax == 3) {
             // capacities above the work of one pass behave alike: try the exact fit once
             s.capacity = std::min(v, round_cap);
             if (s.capacity < (1 << 16)) s.capacity = 1 << 16;
@@ -202,6 +210,7 @@ extern "C" rayon_status rayon_autotune(rayon_ctx* ctx, rayon_job* const* jobs, i
    r.capacity = best.capacity;
    r.chunk = best.chunk;
    r.wg_per_cu = best.wg;
+   r.shade_chunk = best.shade_chunk;
    r.best_ms = best_ms;
    r.start_ms = start_ms;
    r.candidates = timed;

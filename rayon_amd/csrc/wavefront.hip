@@ -67,7 +67,8 @@ struct WfView {
    unsigned long long* stats;  // traversal statistics (optional)
    int njobs;
    int levels;
-   int chunk;  // 64-entry batches a wave claims per queue-head atomic
+   int chunk;  // 64-entry batches a wave claims per queue-head atomic (trace, marching)
+   int shade_chunk;  // the same for the shade stage
 };
 
 struct Wave {
@@ -85,9 +86,10 @@ struct Wave {
    int* mtile_dev = nullptr;
    int* mtodo_dev = nullptr;
    // options
-   int opt_capacity = 1 << 26;
+   int opt_capacity = 1 << 27;
    int opt_wg_per_cu = 0;  // 0: as many as fit (occupancy of each stage kernel)
    int opt_chunk = 4;
+   int opt_shade_chunk = 16;
    int occ_trace = 0, occ_trace_wide = 0, occ_trace_short = 0, occ_shade = 0, occ_sdf = 0;
    struct Pending {
       int stage;
@@ -349,7 +351,7 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_wf_shade(WfView V, int level) {
    int* qn = (level & 1) ? V.q0 : V.q1;
    unsigned* ctl_next = V.ctl + kCtl * (level + 1);
    unsigned* sctl = ctl + 3;
-   const int chunk = min(V.chunk, kMaxChunk);
+   const int chunk = min(V.shade_chunk, kMaxChunk);
    const unsigned long long lt = (1ull << lane) - 1ull;
    PathCounts pc;
    for (;;) {
@@ -368,8 +370,10 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_wf_shade(WfView V, int level) {
             slot = q[idx];
             uint4 id = V.ids[slot];
             const JobView& J = V.jobs[id.x];
-            const float4 o = V.o[slot], d = V.d[slot], t = V.thr[slot], r = V.rad[slot], hf = V.hit[slot];
-            PathState ps{mk(o), mk(d), mk(t), mk(r), (id.w & kPrevLambert) != 0};
+            const float4 o = V.o[slot], d = V.d[slot], t = V.thr[slot], hf = V.hit[slot];
+            // the step's radiance starts at zero and is added to the slot's only if it
+            // adds any (one term per step: rad + (0 + T*e) is rad + T*e bit for bit)
+            PathState ps{mk(o), mk(d), mk(t), mk(0.0f, 0.0f, 0.0f), (id.w & kPrevLambert) != 0};
             Rng R{__float_as_uint(o.w), false};
             HitMin h;
             h.t = hf.x;
@@ -389,10 +393,16 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_wf_shade(WfView V, int level) {
                V.sc[slot] = make_float4(sh.contrib.x, sh.contrib.y, sh.contrib.z, 0.0f);
             }
             V.ids[slot] = id;
-            V.o[slot] = make_float4(ps.o.x, ps.o.y, ps.o.z, __uint_as_float(R.s));
-            V.d[slot] = make_float4(ps.d.x, ps.d.y, ps.d.z, 0.0f);
-            V.thr[slot] = make_float4(ps.T.x, ps.T.y, ps.T.z, 0.0f);
-            V.rad[slot] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
+            if (cont) {  // a finished path needs only its radiance and ids
+               V.o[slot] = make_float4(ps.o.x, ps.o.y, ps.o.z, __uint_as_float(R.s));
+               V.d[slot] = make_float4(ps.d.x, ps.d.y, ps.d.z, 0.0f);
+               V.thr[slot] = make_float4(ps.T.x, ps.T.y, ps.T.z, 0.0f);
+            }
+            if (ps.L.x != 0.0f || ps.L.y != 0.0f || ps.L.z != 0.0f) {
+               const float4 r = V.rad[slot];
+               const V3 L = mk(r) + ps.L;
+               V.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+            }
          }
          const unsigned long long mc = __ballot(cont), ms = __ballot(shadow);
          if (cont) stage_next[w][n_next + __popcll(mc & lt)] = slot;
@@ -676,6 +686,7 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
    V.njobs = (int)jobs.size();
    V.levels = levels;
    V.chunk = W->opt_chunk;
+   V.shade_chunk = W->opt_shade_chunk;
    // persistent grids: the resident workgroups, but no more than one lane per path of a
    // round (small passes then launch and drain quickly)
    const long long per_round = std::min<long long>(std::max(total, 1ll), W->round);
@@ -940,6 +951,7 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
       key.push_back(W->round);
       key.push_back(W->opt_wg_per_cu);
       key.push_back(W->opt_chunk);
+      key.push_back(W->opt_shade_chunk);
       key.push_back(W->opt_count);
       for (rayon_job* j : active) key.push_back(j->scene->view.wnodes != nullptr);
       int sdf_steps = 0;  // the marching slices the pass enqueues
@@ -1023,6 +1035,13 @@ rayon_status wave_set_option(rayon_ctx* ctx, int opt, long long value) {
       }
       W->opt_chunk = (int)value;
       break;
+   case RAYON_OPT_SHADE_CHUNK:
+      if (value < 1 || value > kMaxChunk) {
+         set_error("RAYON_OPT_SHADE_CHUNK must be in [1, 16]");
+         return RAYON_ERR_INVALID;
+      }
+      W->opt_shade_chunk = (int)value;
+      break;
    case RAYON_OPT_GRAPH:
       W->opt_graph = value != 0;
       break;
@@ -1046,6 +1065,7 @@ rayon_status wave_get_option(rayon_ctx* ctx, int opt, long long* value) {
    case RAYON_OPT_WAVE_CAPACITY: *value = W->opt_capacity; break;
    case RAYON_OPT_WAVE_WG_PER_CU: *value = W->opt_wg_per_cu; break;
    case RAYON_OPT_WAVE_CHUNK: *value = W->opt_chunk; break;
+   case RAYON_OPT_SHADE_CHUNK: *value = W->opt_shade_chunk; break;
    case RAYON_OPT_GRAPH: *value = W->opt_graph; break;
    case RAYON_OPT_STAGE_TIMING: *value = W->opt_timing; break;
    case RAYON_OPT_COUNT_TRAVERSAL: *value = W->opt_count; break;

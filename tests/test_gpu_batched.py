@@ -278,7 +278,7 @@ def test_autotune_keeps_results_and_jobs(ctx, scenes):
     specs = [(BOOK, 128, 72, dict(spp=8, seed=1, max_depth=8)),
              (CORN, 96, 64, dict(spp=8, seed=2, max_depth=8, flags=ra.FLAG.NEE)),
              (SDF, 64, 48, dict(spp=8, seed=3, max_depth=8))]
-    opt_keys = (ra.OPT.WAVE_CAPACITY, ra.OPT.WAVE_CHUNK, ra.OPT.WAVE_WG_PER_CU)
+    opt_keys = (ra.OPT.WAVE_CAPACITY, ra.OPT.WAVE_CHUNK, ra.OPT.WAVE_WG_PER_CU, ra.OPT.SHADE_CHUNK)
     default = {o: ctx.get_option(o) for o in opt_keys}
     try:
         jobs = [scenes.job(*sc, W, H, **kw)[0] for sc, W, H, kw in specs]
