@@ -240,7 +240,7 @@ def main(argv=None) -> int:
     ext_ms = st_t.stage_ms[1]
     ext_launches = st_t.launches[1]
     ext_achieved = ext_bytes / (ext_ms * 1e-3) / 1e9
-    stage_names = ["generate", "trace", "shade", "unused", "accumulate", "bookkeeping"]
+    stage_names = ["generate", "trace", "shade", "megakernel", "accumulate", "bookkeeping", "sdf_march", "unused"]
     stage_ms = {n: round(st_t.stage_ms[k], 2) for k, n in enumerate(stage_names) if n != "unused"}
     log(f"stages (ms per frame): {stage_ms}; trace {ext_achieved:.0f} GB/s algorithmic")
 

@@ -289,11 +289,11 @@ rayon_status rayon_ctx_set_option(rayon_ctx* ctx, int32_t option, int64_t value)
 rayon_status rayon_ctx_get_option(rayon_ctx* ctx, int32_t option, int64_t* value);
 
 /* Accumulated statistics of the batched path since the last reset. Stages: 0 generate,
- * 1 trace (extension closest hits + shadow any hits), 2 shade, 3 (unused), 4 accumulate,
- * 5 bookkeeping. */
+ * 1 trace (extension closest hits + shadow any hits), 2 shade, 3 megakernel (jobs of
+ * megakernel scenes), 4 accumulate, 5 bookkeeping, 6 SDF marching, 7 (unused). */
 typedef struct rayon_wave_stats {
-   double stage_ms[6];            /* summed HIP-event time per stage (RAYON_OPT_STAGE_TIMING) */
-   int64_t launches[6];
+   double stage_ms[8];            /* summed HIP-event time per stage (RAYON_OPT_STAGE_TIMING) */
+   int64_t launches[8];
    int64_t rounds;
    int64_t extend_node_fetches;   /* RAYON_OPT_COUNT_TRAVERSAL                               */
    int64_t extend_prim_fetches;

@@ -90,7 +90,7 @@ class JobStats(C.Structure):
 
 
 class WaveStats(C.Structure):
-    _fields_ = [("stage_ms", C.c_double * 6), ("launches", C.c_int64 * 6), ("rounds", C.c_int64),
+    _fields_ = [("stage_ms", C.c_double * 8), ("launches", C.c_int64 * 8), ("rounds", C.c_int64),
                 ("extend_node_fetches", C.c_int64), ("extend_prim_fetches", C.c_int64),
                 ("shadow_node_fetches", C.c_int64), ("shadow_prim_fetches", C.c_int64),
                 ("material_fetches", C.c_int64), ("light_fetches", C.c_int64)]

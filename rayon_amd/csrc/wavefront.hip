@@ -101,8 +101,8 @@ struct Wave {
    int opt_count = 0;
    int cus = 0;
    // timing
-   double stage_ms[6] = {0, 0, 0, 0, 0, 0};
-   long long launches[6] = {0, 0, 0, 0, 0, 0};
+   double stage_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+   long long launches[8] = {0, 0, 0, 0, 0, 0, 0, 0};
    long long rounds = 0;
    // graph cache
    struct GraphEntry {
@@ -728,7 +728,7 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
    auto march = [&](int level) {
       if (level < 0) return;
       for (int it = 0; it < nslices; ++it)
-         launch_stage(W, 1, k_wf_sdf, g_sdf, blk, st, V, level, it, it == nslices - 1 ? 1 : 0);
+         launch_stage(W, 6, k_wf_sdf, g_sdf, blk, st, V, level, it, it == nslices - 1 ? 1 : 0);
    };
    hipError_t e = hipSuccess;
    int dense = 1;
@@ -1088,7 +1088,7 @@ rayon_status wave_stats(rayon_ctx* ctx, rayon_wave_stats* out, int reset) {
       (void)hipEventDestroy(pe.e1);
    }
    W->pending.clear();
-   for (int k = 0; k < 6; ++k) {
+   for (int k = 0; k < 8; ++k) {
       out->stage_ms[k] = W->stage_ms[k];
       out->launches[k] = W->launches[k];
    }
@@ -1104,7 +1104,7 @@ rayon_status wave_stats(rayon_ctx* ctx, rayon_wave_stats* out, int reset) {
       out->light_fetches = (int64_t)s[5];
    }
    if (reset) {
-      for (int k = 0; k < 6; ++k) {
+      for (int k = 0; k < 8; ++k) {
          W->stage_ms[k] = 0;
          W->launches[k] = 0;
       }
