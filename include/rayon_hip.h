@@ -301,6 +301,11 @@ typedef struct rayon_wave_stats {
    int64_t shadow_prim_fetches;
    int64_t material_fetches;
    int64_t light_fetches;
+   int64_t sdf_marches;           /* rays that started an SDF march (RAYON_OPT_COUNT_TRAVERSAL) */
+   int64_t sdf_steps;             /* field evaluations (sphere-tracing steps)                */
+   int64_t sdf_prim_evals;        /* primitive distance evaluations within those steps       */
+   int64_t sdf_wave_slots;        /* lane-steps the waves spent (64 x the wave's longest
+                                     lane per batch): sdf_steps / sdf_wave_slots = lane use */
 } rayon_wave_stats;
 rayon_status rayon_ctx_wave_stats(rayon_ctx* ctx, rayon_wave_stats* out, int32_t reset);
 

@@ -93,7 +93,8 @@ class WaveStats(C.Structure):
     _fields_ = [("stage_ms", C.c_double * 8), ("launches", C.c_int64 * 8), ("rounds", C.c_int64),
                 ("extend_node_fetches", C.c_int64), ("extend_prim_fetches", C.c_int64),
                 ("shadow_node_fetches", C.c_int64), ("shadow_prim_fetches", C.c_int64),
-                ("material_fetches", C.c_int64), ("light_fetches", C.c_int64)]
+                ("material_fetches", C.c_int64), ("light_fetches", C.c_int64), ("sdf_marches", C.c_int64),
+                ("sdf_steps", C.c_int64), ("sdf_prim_evals", C.c_int64), ("sdf_wave_slots", C.c_int64)]
 
 
 class ServerConfig(C.Structure):

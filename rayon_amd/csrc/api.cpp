@@ -331,6 +331,7 @@ rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* d, uint3
       larea.push_back((float)area);
    }
    std::vector<SdfRec> sdf;
+   std::vector<float4> sdf_sph;
    for (int i = 0; i < d->num_sdf_prims; ++i) {
       const rayon_sdf_prim& q = d->sdf_prims[i];
       SdfRec r{{q.center[0], q.center[1], q.center[2], -1.0f}, {0.0f, 0.0f, 0.0f, 0.0f}};
@@ -344,6 +345,10 @@ rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* d, uint3
          r.p.w = q.params[1];
       }
       sdf.push_back(r);
+      // bounding sphere: |half extents| + rounding + torus major radius, rounded up
+      const double br = std::sqrt((double)r.p.x * r.p.x + (double)r.p.y * r.p.y + (double)r.p.z * r.p.z) +
+                        (double)r.p.w + std::max(0.0, (double)r.c.w);
+      sdf_sph.push_back({r.c.x, r.c.y, r.c.z, std::nextafter((float)br, 3.4e38f)});
    }
    (void)hipSetDevice(ctx->device);
    SceneView& V = sc->view;
@@ -354,12 +359,13 @@ rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* d, uint3
    int* dl = nullptr;
    float* da = nullptr;
    SdfRec* ds = nullptr;
+   float4* dsph = nullptr;
    WNode* dw = nullptr;
    if ((st = upload(sc->bufs, recs, &dp)) != RAYON_OK || (st = upload(sc->bufs, normals, &dn)) != RAYON_OK ||
        (st = upload(sc->bufs, br.wnodes, &dw)) != RAYON_OK ||
        (st = upload(sc->bufs, br.nodes, &dnodes)) != RAYON_OK || (st = upload(sc->bufs, mats, &dm)) != RAYON_OK ||
        (st = upload(sc->bufs, lslots, &dl)) != RAYON_OK || (st = upload(sc->bufs, larea, &da)) != RAYON_OK ||
-       (st = upload(sc->bufs, sdf, &ds)) != RAYON_OK) {
+       (st = upload(sc->bufs, sdf, &ds)) != RAYON_OK || (st = upload(sc->bufs, sdf_sph, &dsph)) != RAYON_OK) {
       free_all(sc->bufs);
       delete sc;
       return st;
@@ -372,6 +378,7 @@ rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* d, uint3
    V.light_slots = dl;
    V.light_area = da;
    V.sdf = ds;
+   V.sdf_sph = dsph;
    V.wnodes = br.wnodes.empty() ? nullptr : dw;
    V.num_lights = (int)lslots.size();
    V.num_sdf = (int)sdf.size();

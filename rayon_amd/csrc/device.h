@@ -36,6 +36,39 @@ __device__ __forceinline__ V3 unit(V3 a) { return (1.0f / sqrtf(dot(a, a))) * a;
 __device__ __forceinline__ float maxc(V3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
 __device__ __forceinline__ V3 ld3(const float* p) { return {p[0], p[1], p[2]}; }
 
+// Scene arrays are device allocations, but their pointers are read from memory (the job
+// table), so the compiler can only type them as generic addresses and emits flat loads.
+// Flat loads count against the LDS counter too: every wait for a node or record also
+// waits for the LDS traversal stack, and vice versa. Scene records are read with this
+// load through the global address space instead (16-byte vectors where the record allows).
+typedef float gvec4 __attribute__((ext_vector_type(4)));
+template <class T>
+__device__ __forceinline__ T gload(const T* p) {
+   T v;
+   if constexpr (sizeof(T) % 16 == 0 && alignof(T) >= 16) {
+      constexpr int n = (int)(sizeof(T) / 16);
+      const __attribute__((address_space(1))) gvec4* s = (const __attribute__((address_space(1))) gvec4*)p;
+      gvec4 w[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) w[k] = s[k];
+      __builtin_memcpy(&v, w, sizeof(T));
+   } else {
+      static_assert(sizeof(T) % 4 == 0, "scene records are whole 32-bit words");
+      constexpr int n = (int)(sizeof(T) / 4);
+      const __attribute__((address_space(1))) uint32_t* s = (const __attribute__((address_space(1))) uint32_t*)p;
+      uint32_t w[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) w[k] = s[k];
+      __builtin_memcpy(&v, w, sizeof(T));
+   }
+   return v;
+}
+
+// Square root by the hardware instruction alone (v_sqrt_f32, within 1 ulp). sqrtf adds a
+// correction sequence for correct rounding (about 15 instructions); the SDF distance
+// evaluation, which runs two square roots per primitive per marching step, uses this one.
+__device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // sin/cos of 2*pi*u for u in [0,1]: v_sin_f32 / v_cos_f32 take revolutions.
 __device__ __forceinline__ float sin2pi(float u) { return __builtin_amdgcn_sinf(u); }
 __device__ __forceinline__ float cos2pi(float u) { return __builtin_amdgcn_cosf(u); }
@@ -213,12 +246,12 @@ __device__ __forceinline__ bool isect_rect(V3 q, V3 u, V3 v, V3 o, V3 d, float t
 // zero half-extent rounded by r; a torus is the same applied to (|xz| - R, y, 0).
 __device__ __forceinline__ float sdf_prim(const SdfRec& s, V3 p) {
    const V3 o = p - mk(s.c);
-   const float lxz = sqrtf(o.x * o.x + o.z * o.z);
+   const float lxz = hw_sqrt(o.x * o.x + o.z * o.z);
    const bool torus = s.c.w >= 0.0f;
    const float ax = torus ? lxz - s.c.w : o.x, ay = o.y, az = torus ? 0.0f : o.z;
    const float qx = fabsf(ax) - s.p.x, qy = fabsf(ay) - s.p.y, qz = fabsf(az) - s.p.z;
    const float mx = fmaxf(qx, 0.0f), my = fmaxf(qy, 0.0f), mz = fmaxf(qz, 0.0f);
-   return sqrtf(mx * mx + my * my + mz * mz) + fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f) - s.p.w;
+   return hw_sqrt(mx * mx + my * my + mz * mz) + fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f) - s.p.w;
 }
 
 __device__ __forceinline__ float smooth_union(float d, float d2, float k, float ik) {
@@ -228,9 +261,9 @@ __device__ __forceinline__ float smooth_union(float d, float d2, float k, float 
 
 // The whole field: primitives combined in index order.
 __device__ __forceinline__ float sdf_field(const SceneView& S, V3 p) {
-   float d = sdf_prim(S.sdf[0], p);
+   float d = sdf_prim(gload(S.sdf), p);
    const float k = S.sdf_k, ik = 1.0f / S.sdf_k;
-   for (int i = 1; i < S.num_sdf; ++i) d = smooth_union(d, sdf_prim(S.sdf[i], p), k, ik);
+   for (int i = 1; i < S.num_sdf; ++i) d = smooth_union(d, sdf_prim(gload(S.sdf + i), p), k, ik);
    return d;
 }
 
@@ -244,37 +277,34 @@ constexpr int kSdfMaskMax = 64;
 
 __device__ __forceinline__ float sdf_margin(const SceneView& S) { return fmaxf(10.0f * S.sdf_k, 1.0f); }
 
-__device__ __forceinline__ float sdf_bound(const SdfRec& r) {
-   return sqrtf(r.p.x * r.p.x + r.p.y * r.p.y + r.p.z * r.p.z) + r.p.w + fmaxf(r.c.w, 0.0f);
-}
-
 __device__ __forceinline__ float sdf_field_masked(const SceneView& S, V3 p, uint64_t mask) {
    const float k = S.sdf_k, ik = 1.0f / S.sdf_k;
    int i = __builtin_ctzll(mask);
-   float d = sdf_prim(S.sdf[i], p);
+   float d = sdf_prim(gload(S.sdf + i), p);
    for (mask &= mask - 1; mask; mask &= mask - 1) {
       i = __builtin_ctzll(mask);
-      d = smooth_union(d, sdf_prim(S.sdf[i], p), k, ik);
+      d = smooth_union(d, sdf_prim(gload(S.sdf + i), p), k, ik);
    }
    return d;
 }
 
 // Primitives within `margin` of the segment o + t*dn, t in [t0, t1]; `far` receives the
-// smallest distance bound of the others.
+// smallest distance bound of the others. Each primitive is tested through its bounding
+// sphere (S.sdf_sph: centre, radius), comparing squared distances with the sphere grown
+// by the margin, so only the distance bounds of the excluded primitives take a root.
 __device__ __forceinline__ uint64_t sdf_ray_mask(const SceneView& S, V3 o, V3 dn, float t0, float t1, float margin,
                                                  float& far) {
    uint64_t mask = 0;
    far = 3.4e38f;
    for (int i = 0; i < S.num_sdf; ++i) {
-      const SdfRec r = S.sdf[i];
-      const V3 c = mk(r.c);
-      const float tc = fminf(fmaxf(dot(c - o, dn), t0), t1);
-      const V3 q = o + tc * dn - c;
-      const float dist = sqrtf(dot(q, q)) - sdf_bound(r);
-      if (dist <= margin)
-         mask |= 1ull << i;
-      else
-         far = fminf(far, dist);
+      const float4 b = gload(S.sdf_sph + i);
+      const V3 oc = mk(b) - o;
+      const float tc = fminf(fmaxf(dot(oc, dn), t0), t1);
+      const V3 q = tc * dn - oc;
+      const float d2 = dot(q, q), r = margin + b.w;
+      const bool in = d2 <= r * r;
+      mask |= in ? 1ull << i : 0ull;
+      far = fminf(far, in ? far : hw_sqrt(d2) - b.w);
    }
    return mask;
 }
@@ -289,9 +319,10 @@ __device__ __forceinline__ uint64_t sdf_point_mask(const SceneView& S, V3 p) {
    const float margin = sdf_margin(S);
    uint64_t mask = 0;
    for (int i = 0; i < S.num_sdf; ++i) {
-      const SdfRec r = S.sdf[i];
-      const V3 q = p - mk(r.c);
-      if (sqrtf(dot(q, q)) - sdf_bound(r) <= margin) mask |= 1ull << i;
+      const float4 b = gload(S.sdf_sph + i);
+      const V3 q = p - mk(b);
+      const float r = margin + b.w;
+      mask |= dot(q, q) <= r * r ? 1ull << i : 0ull;
    }
    return mask ? mask : 1ull;
 }
@@ -389,7 +420,7 @@ __device__ inline HitMin closest_hit_bin(const SceneView& S, V3 o, V3 d, float t
    int sp = 0;
    for (;;) {
       while (cur >= 0) {
-         const Node2 n = S.nodes[cur];
+         const Node2 n = gload(S.nodes + cur);
          if constexpr (CNT) ++tc->nodes;
          float tn0, tn1;
          bool h0, h1;
@@ -416,7 +447,7 @@ __device__ inline HitMin closest_hit_bin(const SceneView& S, V3 o, V3 d, float t
       const int packed = ~cur;
       const int first = packed >> 3, cnt = (packed & 7) + 1;
       for (int i = first; i < first + cnt; ++i) {
-         const PrimRec p = S.prims[i];
+         const PrimRec p = gload(S.prims + i);
          if constexpr (CNT) ++tc->prims;
          const uint32_t info = __float_as_uint(p.a.w);
          const uint32_t kind = info_kind(info);
@@ -461,7 +492,7 @@ __device__ inline bool any_hit_bin(const SceneView& S, V3 o, V3 d, float tmin, f
    int sp = 0;
    for (;;) {
       while (cur >= 0) {
-         const Node2 n = S.nodes[cur];
+         const Node2 n = gload(S.nodes + cur);
          if constexpr (CNT) ++tc->nodes;
          float tn0, tn1;
          bool h0, h1;
@@ -483,7 +514,7 @@ __device__ inline bool any_hit_bin(const SceneView& S, V3 o, V3 d, float tmin, f
       const int packed = ~cur;
       const int first = packed >> 3, cnt = (packed & 7) + 1;
       for (int i = first; i < first + cnt; ++i) {
-         const PrimRec p = S.prims[i];
+         const PrimRec p = gload(S.prims + i);
          if constexpr (CNT) ++tc->prims;
          const uint32_t info = __float_as_uint(p.a.w);
          if (info & F_EMITTER) continue;
@@ -509,7 +540,7 @@ __device__ inline bool any_hit_bin(const SceneView& S, V3 o, V3 d, float tmin, f
 template <bool CNT, bool DEFER = false>
 __device__ __forceinline__ void test_closest(const SceneView& S, int i, V3 o, V3 d, float tmin, HitMin& best,
                                              TravCount* tc) {
-   const PrimRec p = S.prims[i];
+   const PrimRec p = gload(S.prims + i);
    if constexpr (CNT) ++tc->prims;
    const uint32_t info = __float_as_uint(p.a.w);
    const uint32_t kind = info_kind(info);
@@ -542,7 +573,7 @@ __device__ __forceinline__ void test_closest(const SceneView& S, int i, V3 o, V3
 template <bool CNT>
 __device__ __forceinline__ bool test_any(const SceneView& S, int i, V3 o, V3 d, float tmin, float tmax,
                                          TravCount* tc) {
-   const PrimRec p = S.prims[i];
+   const PrimRec p = gload(S.prims + i);
    if constexpr (CNT) ++tc->prims;
    const uint32_t info = __float_as_uint(p.a.w);
    if (info & F_EMITTER) return false;
@@ -592,12 +623,7 @@ __device__ __forceinline__ void wide_children(const WNode& n, V3 o, V3 inv, uint
 }
 
 __device__ __forceinline__ WNode load_wnode(const WNode* nodes, uint32_t i) {
-   const float4* src = reinterpret_cast<const float4*>(nodes + i);
-   WNode n;
-   float4* dst = reinterpret_cast<float4*>(&n);
-#pragma unroll
-   for (int k = 0; k < 5; ++k) dst[k] = src[k];
-   return n;
+   return gload(nodes + i);
 }
 
 // Closest hit over the 8-wide quantized tree. The stack holds one entry per partly
@@ -769,7 +795,7 @@ __device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
 }
 
 __device__ inline Surf surface(const SceneView& S, const HitMin& h, V3 o, V3 d) {
-   const PrimRec p = S.prims[h.slot];
+   const PrimRec p = gload(S.prims + h.slot);
    Surf s;
    s.info = __float_as_uint(p.a.w);
    const uint32_t kind = info_kind(s.info);
@@ -784,7 +810,7 @@ __device__ inline Surf surface(const SceneView& S, const HitMin& h, V3 o, V3 d) 
       if (smooth) {
          const float4* nn = S.normals + 3 * h.slot;
          float w = 1.0f - h.u - h.v;
-         sn = unit(w * mk(nn[0]) + h.u * mk(nn[1]) + h.v * mk(nn[2]));
+         sn = unit(w * mk(gload(nn)) + h.u * mk(gload(nn + 1)) + h.v * mk(gload(nn + 2)));
          if (dot(sn, geo) < 0.0f) sn = -sn;
       }
       s.n = s.front ? sn : -sn;

@@ -110,6 +110,7 @@ struct SceneView {
    const int* light_slots;
    const float* light_area;
    const SdfRec* sdf;
+   const float4* sdf_sph;  // per SDF primitive: bounding sphere centre and radius
    const WNode* wnodes;  // 8-wide quantized tree, or NULL to traverse the binary pair nodes
    int num_lights;
    int num_sdf;

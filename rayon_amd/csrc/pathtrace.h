@@ -41,8 +41,8 @@ __device__ inline void nee_request(const JobView& J, const Surf& s, V3 color, V3
    const SceneView& S = J.scene;
    float ul = R.next(), ua = R.next(), ub = R.next();
    int li = min((int)(ul * (float)S.num_lights), S.num_lights - 1);
-   const int slot = S.light_slots[li];
-   const PrimRec lp = S.prims[slot];
+   const int slot = gload(S.light_slots + li);
+   const PrimRec lp = gload(S.prims + slot);
    if constexpr (CNT) ++pc.light_prims;
    V3 q, nl;
    bool one_sided;
@@ -57,9 +57,9 @@ __device__ inline void nee_request(const JobView& J, const Surf& s, V3 color, V3
    sh.valid = cs > 0.0f && cl > 0.0f;
    if (sh.valid) {
       ++pc.shadow;
-      const MatRec LM = S.mats[info_mat(__float_as_uint(lp.a.w))];
+      const MatRec LM = gload(S.mats + info_mat(__float_as_uint(lp.a.w)));
       if constexpr (CNT) ++pc.mats;
-      float g = cs * cl / d2 * S.light_area[li] * (float)S.num_lights * (1.0f / kPiF);
+      float g = cs * cl / d2 * gload(S.light_area + li) * (float)S.num_lights * (1.0f / kPiF);
       sh.contrib = mul_rn(T, (g * J.light_intensity) * (color * mk(LM.m1)));
    }
 }
@@ -84,7 +84,7 @@ __device__ inline bool shade_step(const JobView& J, const HitMin& h, int bounce,
       ps.o = s.p + 0.01f * ps.d;
       return true;
    }
-   const MatRec M = S.mats[info_mat(s.info)];
+   const MatRec M = gload(S.mats + info_mat(s.info));
    if constexpr (CNT) ++pc.mats;
    const V3 color = surface_color(M, s.p, s.center);
    Scatter sc;
@@ -159,6 +159,11 @@ __device__ __forceinline__ int adaptive_count(const JobView& J, float4 a_before,
 
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+   return v;
+}
+
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+   for (int off = 32; off > 0; off >>= 1) v = max(v, (unsigned)__shfl_xor((int)v, off));
    return v;
 }
 

@@ -136,11 +136,20 @@ __device__ __forceinline__ int wave_push(unsigned* counter, bool pred) {
 // Persistent-worker loop: each wave claims `chunk` batches of 64 queue entries with one
 // atomic on the queue head and runs body(index) on each; body sees indices >= count on
 // the last, partial batch and must ignore them. Returns when the queue is drained.
+// All the waves of a launch contend for the one head counter, so a wave does not claim
+// when it can tell the queue is already drained: an empty queue, a head already past the
+// end (a plain read, which can only lag behind the true head), or a claim of its own that
+// reached the end.
 template <class F>
 __device__ __forceinline__ void drain(unsigned* head, unsigned count, int chunk, F&& body) {
+   if (count == 0) return;
+   unsigned seen = 0;
+   if (lane_id() == 0) seen = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+   if (__shfl(seen, 0) >= count) return;
+   const unsigned step = 64u * (unsigned)chunk;
    for (;;) {
       unsigned base = 0;
-      if (lane_id() == 0) base = atomicAdd(head, 64u * (unsigned)chunk);
+      if (lane_id() == 0) base = atomicAdd(head, step);
       base = __shfl(base, 0);
       if (base >= count) return;
       for (int k = 0; k < chunk; ++k) {
@@ -148,6 +157,7 @@ __device__ __forceinline__ void drain(unsigned* head, unsigned count, int chunk,
          if (b >= count) break;
          body(b + (unsigned)lane_id());
       }
+      if (base + step >= count) return;
    }
 }
 
@@ -223,7 +233,9 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_generate(WfView V, long long w0
 // record (the march ends at the closest other hit, and an exact tie goes to the field,
 // which has the largest original index). The march state lives in the shadow-ray arrays
 // of the slot, free between the trace and shade stages.
+template <bool CNT>
 __global__ __launch_bounds__(kWfBlock) void k_wf_sdf(WfView V, int level, int it, int last) {
+   unsigned long long n_begin = 0, n_steps = 0, n_evals = 0, n_slots = 0;
    unsigned* cin = V.sctl + 2 * ((kSdfSlices + 1) * level + it);
    unsigned* cout = cin + 2;
    const unsigned count = cin[0];
@@ -232,6 +244,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_sdf(WfView V, int level, int it
    drain(&cin[1], count, V.chunk, [&](unsigned idx) {
       int slot = -1;
       bool more = false;
+      unsigned taken = 0;
       if (idx < count) {
          slot = qin[idx];
          const SceneView& S = V.jobs[V.ids[slot].x].scene;
@@ -239,7 +252,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_sdf(WfView V, int level, int it
          SdfMarch m;
          bool live = true;
          if (it == 0) {
-            live = sdf_march_begin(S, S.prims[S.sdf_prim], o, d, 0.001f, V.hit[slot].x, m);
+            live = sdf_march_begin(S, gload(S.prims + S.sdf_prim), o, d, 0.001f, V.hit[slot].x, m);
          } else {
             const float4 a = V.so[slot], b = V.sc[slot];
             m.tw = a.x;
@@ -250,7 +263,15 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_sdf(WfView V, int level, int it
          }
          if (live) {
             float t = 0.0f;
+            const int used0 = m.used;
             const int r = sdf_march_run(S, o, d, m, last ? S.sdf_steps : (kSdfSlice << it), t);
+            if constexpr (CNT) {
+               // steps evaluated: the loop's count, plus the step that returned from inside it
+               taken = (unsigned)(m.used - used0 + (r >= 0 && m.used < S.sdf_steps ? 1 : 0));
+               n_begin += it == 0;
+               n_steps += taken;
+               n_evals += (unsigned long long)taken * (S.num_sdf > kSdfMaskMax ? S.num_sdf : __popcll(m.mask));
+            }
             if (r == 1) {
                V.hit[slot] = make_float4(t, __int_as_float(S.sdf_prim), 0.0f, 0.0f);
             } else if (r < 0) {
@@ -261,9 +282,19 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_sdf(WfView V, int level, int it
             }
          }
       }
+      if constexpr (CNT) n_slots += 64ull * wave_max_u32(taken);  // lane-steps the wave's SIMD spent
       const int pos = wave_push(&cout[0], more);
       if (more) qout[pos] = slot;
    });
+   if constexpr (CNT) {
+      const unsigned long long a = wave_sum(n_begin), b = wave_sum(n_steps), c = wave_sum(n_evals);
+      if (lane_id() == 0) {
+         atomicAdd(&V.stats[8], a);
+         atomicAdd(&V.stats[9], b);
+         atomicAdd(&V.stats[10], c);
+         atomicAdd(&V.stats[11], n_slots);
+      }
+   }
 }
 
 // Traversal stage: first the extension rays of level `ext_level` (closest hit), then the
@@ -354,11 +385,15 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_wf_shade(WfView V, int level) {
    const int chunk = min(V.shade_chunk, kMaxChunk);
    const unsigned long long lt = (1ull << lane) - 1ull;
    PathCounts pc;
-   for (;;) {
+   unsigned seen = count;  // skip the claims of an empty or drained queue, as drain() does
+   if (count != 0 && lane == 0) seen = __hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+   bool more = __shfl(seen, 0) < count;
+   while (more) {
       unsigned base = 0;
       if (lane == 0) base = atomicAdd(&ctl[2], 64u * (unsigned)chunk);
       base = __shfl(base, 0);
       if (base >= count) break;
+      more = base + 64u * (unsigned)chunk < count;
       int n_next = 0, n_sh = 0;
       for (int k = 0; k < chunk; ++k) {
          const unsigned bk = base + 64u * (unsigned)k;
@@ -543,7 +578,7 @@ static rayon_status wave_reserve(rayon_ctx* ctx, int capacity, int levels, int n
       W->occ_trace_short = n > 0 ? n : 1;
       RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_shade<false>, kWfBlock, 0));
       W->occ_shade = n > 0 ? n : 1;
-      RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_sdf, kWfBlock, 0));
+      RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_sdf<false>, kWfBlock, 0));
       W->occ_sdf = n > 0 ? n : 1;
    }
    if (W->capacity >= capacity && W->levels >= levels && W->max_jobs >= njobs) return RAYON_OK;
@@ -570,9 +605,9 @@ static rayon_status wave_reserve(rayon_ctx* ctx, int capacity, int levels, int n
    void* p;
    rayon_status st;
    if (!W->stats_dev) {
-      if ((st = alloc(W->bufs, 8 * sizeof(unsigned long long), &p)) != RAYON_OK) return st;
+      if ((st = alloc(W->bufs, 16 * sizeof(unsigned long long), &p)) != RAYON_OK) return st;
       W->stats_dev = (unsigned long long*)p;
-      RY_HIP(launch_fill_u32((uint32_t*)W->stats_dev, 0u, 16, ctx->stream));
+      RY_HIP(launch_fill_u32((uint32_t*)W->stats_dev, 0u, 32, ctx->stream));
    }
    if (capacity > W->capacity) {
       release(W->bufs_slots);
@@ -728,7 +763,8 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
    auto march = [&](int level) {
       if (level < 0) return;
       for (int it = 0; it < nslices; ++it)
-         launch_stage(W, 6, k_wf_sdf, g_sdf, blk, st, V, level, it, it == nslices - 1 ? 1 : 0);
+         if (W->opt_count) launch_stage(W, 6, k_wf_sdf<true>, g_sdf, blk, st, V, level, it, it == nslices - 1 ? 1 : 0);
+         else launch_stage(W, 6, k_wf_sdf<false>, g_sdf, blk, st, V, level, it, it == nslices - 1 ? 1 : 0);
    };
    hipError_t e = hipSuccess;
    int dense = 1;
@@ -1094,7 +1130,7 @@ rayon_status wave_stats(rayon_ctx* ctx, rayon_wave_stats* out, int reset) {
    }
    out->rounds = W->rounds;
    if (W->stats_dev) {
-      unsigned long long s[8];
+      unsigned long long s[16];
       RY_HIP(hipMemcpy(s, W->stats_dev, sizeof(s), hipMemcpyDeviceToHost));
       out->extend_node_fetches = (int64_t)s[0];
       out->extend_prim_fetches = (int64_t)s[1];
@@ -1102,6 +1138,10 @@ rayon_status wave_stats(rayon_ctx* ctx, rayon_wave_stats* out, int reset) {
       out->shadow_prim_fetches = (int64_t)s[3];
       out->material_fetches = (int64_t)s[4];
       out->light_fetches = (int64_t)s[5];
+      out->sdf_marches = (int64_t)s[8];
+      out->sdf_steps = (int64_t)s[9];
+      out->sdf_prim_evals = (int64_t)s[10];
+      out->sdf_wave_slots = (int64_t)s[11];
    }
    if (reset) {
       for (int k = 0; k < 8; ++k) {
@@ -1109,7 +1149,7 @@ rayon_status wave_stats(rayon_ctx* ctx, rayon_wave_stats* out, int reset) {
          W->launches[k] = 0;
       }
       W->rounds = 0;
-      if (W->stats_dev) RY_HIP(launch_fill_u32((uint32_t*)W->stats_dev, 0u, 16, ctx->stream));
+      if (W->stats_dev) RY_HIP(launch_fill_u32((uint32_t*)W->stats_dev, 0u, 32, ctx->stream));
    }
    return RAYON_OK;
 }
