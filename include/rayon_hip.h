@@ -512,6 +512,36 @@ rayon_status rayon_save_yaml(const rayon_generated* g, const char* path);
 rayon_status rayon_generated_settings(const rayon_generated* g, float background_color[3],
                                       float* background_intensity, float* ambient_light, int32_t* use_bvh,
                                       int32_t* adaptive_sampling);
+/* ---- Output files of a finished render -------------------------------------------------
+ * 8-bit image (channels 1 gray, 2 gray+alpha, 3 RGB, 4 RGBA; rows top to bottom, tightly
+ * packed) written as a PNG file, creating missing parent directories. Replaces
+ * FileUtils::writeImage (utils.hpp:34-48, stb with RGB) and ensureDirectoryExists
+ * (utils.hpp:22-31). Errors: RAYON_ERR_INVALID (sizes), RAYON_ERR_IO (file). */
+rayon_status rayon_write_png(const char* path, const uint8_t* pixels, int32_t width, int32_t height,
+                             int32_t channels);
+/* Resolves a job's image (gamma-corrected RGB, as rayon_job_resolve_u8) and writes it as a
+ * PNG: the reference's output of a render (main.cc:468-471). */
+rayon_status rayon_job_write_png(rayon_ctx* ctx, rayon_job* job, const char* path, float gamma);
+
+/* Statistics of one render, the fields of FileUtils::writeRenderStats (utils.hpp:118-172). */
+typedef struct rayon_render_stats {
+   int32_t samples_per_pixel;
+   int32_t width;
+   int32_t height;
+   int32_t max_depth;
+   int64_t rays_traced;
+   int64_t image_size_bytes;  /* -1: the size of the file at image_path (0 if it is absent) */
+   int64_t render_time_ns;
+} rayon_render_stats;
+/* Writes the statistics next to the image: image_path with its extension replaced by .txt
+ * and by .json, with the keys, order and number formats of writeRenderStats
+ * (utils.hpp:118-172): rays_per_second rounded to the nearest integer (0 for a zero
+ * duration), render_time_ms truncated, render_time_pretty as rayon_format_duration. */
+rayon_status rayon_write_render_stats(const char* image_path, const rayon_render_stats* stats);
+/* Duration text of formatDuration (utils.hpp:90-116): "<m>m <s>s" from one minute, "<s>s"
+ * from 10 s, "<s.ss>s" from 1 s, "<ms>ms" below. Writes at most buf_len bytes with the NUL. */
+rayon_status rayon_format_duration(int64_t duration_ns, char* buf, int32_t buf_len);
+
 /* The description stays valid until rayon_generated_free. */
 const rayon_scene_desc* rayon_generated_desc(const rayon_generated* g);
 /* Camera from the generator or the YAML camera section: look_from, look_at, vup, vfov. */

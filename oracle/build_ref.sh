@@ -1,7 +1,7 @@
 #!/bin/bash
-# Builds the reference's scene-file loader from its own sources, where they lie under
-# /root/reference, into oracle/_ref/ (git-ignored). Test infrastructure only: the tests
-# compare rayon_load_yaml against it. Exits 0 without building when the sources are absent.
+# Builds the reference's scene-file loader and its output code from their own sources, where
+# they lie under /root/reference, into oracle/_ref/ (git-ignored). Test infrastructure only:
+# the tests compare rayon_load_yaml and the PNG/statistics writers against them. Exits 0 without building when the sources are absent.
 set -e
 REF=${RAYON_REFERENCE:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)
@@ -14,3 +14,7 @@ mkdir -p "$HERE/_ref"
 g++ -std=c++17 -O1 -w -I"$SRC/scenes" -I"$SRC/data_structures" -I"$SRC/cpu_renderers/utils" -I"$SRC" \
     "$SRC/scenes/yaml_scene_loader.cc" "$HERE/ref_yaml_dump.cpp" -o "$HERE/_ref/ref_yaml_dump"
 echo "$HERE/_ref/ref_yaml_dump"
+# the output code of the command-line renderer (utils.hpp, with the stb writer it vendors)
+g++ -std=c++17 -O1 -w -I"$SRC" -I"$SRC/camera" -I"$SRC/data_structures" -I"$SRC/cpu_renderers/utils" -I"$REF/src/external" \
+    "$HERE/ref_output_dump.cpp" -o "$HERE/_ref/ref_output_dump"
+echo "$HERE/_ref/ref_output_dump"

@@ -12,7 +12,8 @@ from pathlib import Path
 import numpy as np
 
 from .abi import (  # noqa: F401  (re-exported)
-    Camera, GenParams, Geometry, JobParams, JobStats, Material, Mesh, SceneDesc, SceneInfo, SdfPrim, WaveStats,
+    Camera, GenParams, Geometry, JobParams, JobStats, Material, Mesh, RenderStats, SceneDesc, SceneInfo, SdfPrim,
+    WaveStats,
     ServerConfig, ServerJobInfo, ServerStats, TuneResult, ProgressiveConfig, ProgressiveFrame, PATH, YAML, MAT, GEOM, SDF, FLAG, GEN, OPT, BVH, JOB, STATUS, declare,
 )
 
@@ -66,6 +67,31 @@ def job_params(width: int, height: int, camera: Camera, *, spp: int = 64, max_de
             raise AttributeError(k)
         setattr(p, k, v)
     return p
+
+
+def write_png(path: str, image: np.ndarray):
+    """Writes an 8-bit image (H x W, or H x W x C with C in 1..4) as a PNG file."""
+    a = np.ascontiguousarray(image, dtype=np.uint8)
+    if a.ndim == 2:
+        a = a[:, :, None]
+    if a.ndim != 3:
+        raise ValueError("image must be H x W or H x W x C")
+    h, w, c = a.shape
+    _check(lib().rayon_write_png(str(path).encode(), a.ctypes.data_as(C.POINTER(C.c_uint8)), w, h, c),
+           "rayon_write_png")
+
+
+def write_render_stats(image_path: str, *, samples_per_pixel: int, width: int, height: int, max_depth: int,
+                       rays_traced: int, render_time_ns: int, image_size_bytes: int = -1):
+    """Writes the .txt and .json statistics of a render next to its image."""
+    s = RenderStats(samples_per_pixel, width, height, max_depth, rays_traced, image_size_bytes, render_time_ns)
+    _check(lib().rayon_write_render_stats(str(image_path).encode(), C.byref(s)), "rayon_write_render_stats")
+
+
+def format_duration(duration_ns: int) -> str:
+    buf = C.create_string_buffer(64)
+    _check(lib().rayon_format_duration(int(duration_ns), buf, 64), "rayon_format_duration")
+    return buf.value.decode()
 
 
 class Generated:
@@ -265,6 +291,11 @@ class Job:
         _check(lib().rayon_job_resolve_u8(self.ctx._h, self._h, out.ctypes.data_as(C.POINTER(C.c_uint8)),
                                           channels, gamma), "rayon_job_resolve_u8")
         return out
+
+    def write_png(self, path: str, gamma: float = 2.0):
+        """Resolves the image (gamma-corrected RGB) and writes it as a PNG file."""
+        _check(lib().rayon_job_write_png(self.ctx._h, self._h, str(path).encode(), gamma),
+               "rayon_job_write_png")
 
     def sample_counts(self) -> np.ndarray:
         h, w = self.shape

@@ -89,6 +89,12 @@ class JobStats(C.Structure):
                 ("material_fetches", C.c_int64), ("light_fetches", C.c_int64)]
 
 
+class RenderStats(C.Structure):
+    _fields_ = [("samples_per_pixel", C.c_int32), ("width", C.c_int32), ("height", C.c_int32),
+                ("max_depth", C.c_int32), ("rays_traced", C.c_int64), ("image_size_bytes", C.c_int64),
+                ("render_time_ns", C.c_int64)]
+
+
 class WaveStats(C.Structure):
     _fields_ = [("stage_ms", C.c_double * 8), ("launches", C.c_int64 * 8), ("rounds", C.c_int64),
                 ("extend_node_fetches", C.c_int64), ("extend_prim_fetches", C.c_int64),
@@ -201,6 +207,10 @@ def declare(lib: C.CDLL) -> C.CDLL:
         "rayon_generated_camera": (st, [vp, P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_double)]),
         "rayon_generated_background_intensity": (f32, [vp]),
         "rayon_generated_free": (None, [vp]),
+        "rayon_write_png": (st, [C.c_char_p, P(C.c_uint8), i32, i32, i32]),
+        "rayon_job_write_png": (st, [vp, vp, C.c_char_p, f32]),
+        "rayon_write_render_stats": (st, [C.c_char_p, P(RenderStats)]),
+        "rayon_format_duration": (st, [i64, C.c_char_p, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
