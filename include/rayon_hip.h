@@ -134,13 +134,19 @@ typedef struct rayon_scene_desc {
 } rayon_scene_desc;
 
 /* BVH build options. Default: binary tree stored as FP32 pair nodes (64 B per pair of
- * children). RAYON_BVH_QUANTIZED also collapses it into an 8-wide tree of 80 B nodes
- * with 8-bit child boxes rounded outward, and traversal uses that tree instead (2.4x
- * fewer node bytes per ray; more box-test arithmetic per node). */
+ * children), built on the host by binned SAH. RAYON_BVH_QUANTIZED also collapses it into
+ * an 8-wide tree of 80 B nodes with 8-bit child boxes rounded outward, and traversal uses
+ * that tree instead (2.4x fewer node bytes per ray; more box-test arithmetic per node).
+ * RAYON_BVH_DEVICE builds the binary tree on the GPU instead (Morton-ordered linear BVH:
+ * milliseconds for a million triangles, at some traversal cost against SAH); images are
+ * identical to the host tree's. It falls back to the host build if the tree would be
+ * deeper than the traversal stack; rayon_scene_info.builder says which one ran. */
 enum {
    RAYON_BVH_DEFAULT = 0,
-   RAYON_BVH_QUANTIZED = 1 << 0
+   RAYON_BVH_QUANTIZED = 1 << 0,
+   RAYON_BVH_DEVICE = 1 << 1
 };
+enum { RAYON_BVH_BUILDER_HOST = 0, RAYON_BVH_BUILDER_DEVICE = 1 };
 
 typedef struct rayon_scene_info {
    int64_t num_prims;
@@ -149,8 +155,10 @@ typedef struct rayon_scene_info {
    int32_t max_depth;         /* binary tree depth (root = 0)                            */
    int32_t num_lights;
    int64_t device_bytes;      /* device memory held by the scene                         */
-   double build_ms;           /* host BVH build time                                     */
-   double upload_ms;
+   double build_ms;           /* BVH build time (host, or device including its sort)      */
+   double upload_ms;          /* host-to-device copies of the scene                      */
+   int32_t builder;           /* RAYON_BVH_BUILDER_HOST or RAYON_BVH_BUILDER_DEVICE        */
+   int32_t pad_;
 } rayon_scene_info;
 
 /* ---------------------------------------------------------------------------------
@@ -248,6 +256,10 @@ rayon_status rayon_scene_info_get(const rayon_scene* scene, rayon_scene_info* ou
  * invariants (every primitive in exactly one leaf, leaves of 1..4, every child box
  * containing what lies below it, depth within the traversal stack). Fills counts. */
 rayon_status rayon_bvh_check(const rayon_scene_desc* desc, rayon_scene_info* out);
+/* The same invariants for a created scene's binary tree (either builder), read back from
+ * the device and checked against the description it was created from; the depth of its
+ * deepest leaf through `depth`. */
+rayon_status rayon_scene_check(const rayon_scene* scene, const rayon_scene_desc* desc, int32_t* depth);
 /* Index (into rayon_scene_desc order: geometries first, then mesh triangles in order)
  * of the primitive stored at BVH slot `slot`; for host-side checks. */
 rayon_status rayon_scene_prim_order(const rayon_scene* scene, int64_t* out_orig_ids, int64_t n);

@@ -238,6 +238,12 @@ class Scene:
     def batch_path(self) -> int:
         return lib().rayon_scene_batch_path(self._h)
 
+    def check(self, desc_ptr) -> int:
+        """Verifies the scene's binary tree against its description; returns the tree depth."""
+        depth = C.c_int32(0)
+        _check(lib().rayon_scene_check(self._h, desc_ptr, C.byref(depth)), "rayon_scene_check")
+        return depth.value
+
     def prim_order(self) -> np.ndarray:
         n = self.info().num_prims
         out = np.empty(n, np.int64)

@@ -22,7 +22,7 @@ GEOM = _NS(SPHERE=0, RECTANGLE=1, DISPLACED_SPHERE=3, TRIANGLE=4)
 SDF = _NS(SPHERE=0, ROUNDED_BOX=1, TORUS=2)
 FLAG = _NS(REF_QUIRKS=1, REF_RNG=2, NEE=4, ADAPTIVE_REF=8, ADAPTIVE_STDERR=16, DOF=32, COUNT_TRAVERSAL=64)
 GEN = _NS(BOOK_SPHERES=0, CORNELL_ICOSPHERE=1, SDF_TERRAIN=2)
-BVH = _NS(DEFAULT=0, QUANTIZED=1)
+BVH = _NS(DEFAULT=0, QUANTIZED=1, DEVICE=2, BUILDER_HOST=0, BUILDER_DEVICE=1)
 YAML = _NS(LIFT_CAPS=1)
 JOB = _NS(QUEUED=0, RUNNING=1, DONE=2, FAILED=3, CANCELLED=4)
 PATH = _NS(AUTO=0, WAVEFRONT=1, MEGAKERNEL=2)
@@ -64,7 +64,7 @@ class SceneDesc(C.Structure):
 class SceneInfo(C.Structure):
     _fields_ = [("num_prims", C.c_int64), ("num_nodes", C.c_int64), ("num_qnodes", C.c_int64),
                 ("max_depth", C.c_int32), ("num_lights", C.c_int32), ("device_bytes", C.c_int64),
-                ("build_ms", C.c_double), ("upload_ms", C.c_double)]
+                ("build_ms", C.c_double), ("upload_ms", C.c_double), ("builder", C.c_int32), ("pad_", C.c_int32)]
 
 
 class Camera(C.Structure):
@@ -161,6 +161,7 @@ def declare(lib: C.CDLL) -> C.CDLL:
         "rayon_scene_create": (st, [vp, P(SceneDesc), C.c_uint32, P(vp)]),
         "rayon_scene_info_get": (st, [vp, P(SceneInfo)]),
         "rayon_bvh_check": (st, [P(SceneDesc), P(SceneInfo)]),
+        "rayon_scene_check": (st, [vp, P(SceneDesc), P(i32)]),
         "rayon_scene_prim_order": (st, [vp, P(i64), i64]),
         "rayon_scene_destroy": (st, [vp]),
         "rayon_job_create": (st, [vp, vp, P(JobParams), P(vp)]),

@@ -249,56 +249,138 @@ rayon_status rayon_ctx_sync(rayon_ctx* c) {
    return RAYON_OK;
 }
 
+// Record (and vertex normals, when `nrm` is given) of one primitive, as layout.h defines
+// them: the packing shared by the host and the device builds.
+static PrimRec pack_rec(const HostPrim& p, float4* nrm) {
+   const uint32_t info = (p.kind << 28) | p.flags | p.mat;
+   PrimRec r;
+   r.a = {p.a[0], p.a[1], p.a[2], bits_as_float(info)};
+   const float of = bits_as_float((uint32_t)p.orig);
+   if (p.kind == K_TRI) {
+      r.b = {p.b[0] - p.a[0], p.b[1] - p.a[1], p.b[2] - p.a[2], of};
+      r.c = {p.c[0] - p.a[0], p.c[1] - p.a[1], p.c[2] - p.a[2], 0.0f};
+      if (nrm && (p.flags & F_NORMALS)) {
+         for (int v = 0; v < 3; ++v) nrm[v] = {p.n[3 * v], p.n[3 * v + 1], p.n[3 * v + 2], 0.0f};
+      }
+   } else if (p.kind == K_SPHERE || p.kind == K_DISPLACED) {
+      r.b = {p.b[0], 0.0f, 0.0f, of};
+      r.c = {0, 0, 0, 0};
+   } else if (p.kind == K_RECT) {
+      r.b = {p.b[0], p.b[1], p.b[2], of};
+      r.c = {p.c[0], p.c[1], p.c[2], 0.0f};
+   } else {  // SDF field: bounds
+      r.a = {p.lo[0], p.lo[1], p.lo[2], bits_as_float(info)};
+      r.b = {p.hi[0], p.hi[1], p.hi[2], of};
+      r.c = {0, 0, 0, 0};
+   }
+   return r;
+}
+
+static double ms_between(std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+   return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
 rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* d, uint32_t bvh_flags, rayon_scene** out) {
    if (!ctx || !out) {
       set_error("rayon_scene_create: NULL argument");
       return RAYON_ERR_INVALID;
    }
+   if ((bvh_flags & RAYON_BVH_DEVICE) && (bvh_flags & RAYON_BVH_QUANTIZED)) {
+      set_error("rayon_scene_create: RAYON_BVH_DEVICE builds the binary layout; the quantized layout is host-built");
+      return RAYON_ERR_UNSUPPORTED;
+   }
    std::vector<HostPrim> prims;
    rayon_status st = gather_prims(d, prims);
    if (st != RAYON_OK) return st;
-   auto t0 = std::chrono::steady_clock::now();
-   BuildResult br;
-   if (!build_bvh(prims, br, (bvh_flags & RAYON_BVH_QUANTIZED) != 0)) {
-      set_error("BVH depth %d exceeds the traversal stack (%d)", br.max_depth, kMaxDepth);
-      return RAYON_ERR_INVALID;
-   }
-   auto t1 = std::chrono::steady_clock::now();
+   using clock = std::chrono::steady_clock;
    const size_t n = prims.size();
-   std::vector<PrimRec> recs(n);
-   std::vector<float4> normals;
    bool any_normals = false;
    for (auto& p : prims) any_normals |= (p.flags & F_NORMALS) != 0;
-   if (any_normals) normals.assign(3 * n, float4{0, 0, 0, 0});
+   RY_HIP(hipSetDevice(ctx->device));
    auto* sc = new rayon_scene();
    sc->ctx = ctx;
+   auto fail = [&](rayon_status s) {
+      free_all(sc->bufs);
+      delete sc;
+      return s;
+   };
+   BuildResult br;
+   PrimRec* dp = nullptr;
+   float4* dn = nullptr;
+   Node2* dnodes = nullptr;
+   WNode* dw = nullptr;
+   int64_t num_nodes = 0;
+   double build_ms = 0.0, upload_ms = 0.0;
+   int32_t builder = RAYON_BVH_BUILDER_HOST;
+   if ((bvh_flags & RAYON_BVH_DEVICE) && n > 0) {
+      // records and boxes go up in host order; the device sorts them into slot order
+      const auto u0 = clock::now();
+      std::vector<PrimRec> recs(n);
+      std::vector<float4> normals, lo(n), hi(n);
+      if (any_normals) normals.assign(3 * n, float4{0, 0, 0, 0});
+      for (size_t i = 0; i < n; ++i) {
+         const HostPrim& p = prims[i];
+         recs[i] = pack_rec(p, any_normals ? &normals[3 * i] : nullptr);
+         lo[i] = {p.lo[0], p.lo[1], p.lo[2], 0.0f};
+         hi[i] = {p.hi[0], p.hi[1], p.hi[2], 0.0f};
+      }
+      std::vector<DevBuf> tmp;
+      PrimRec* sr = nullptr;
+      float4 *sn = nullptr, *slo = nullptr, *shi = nullptr;
+      if ((st = upload(tmp, recs, &sr)) != RAYON_OK || (st = upload(tmp, normals, &sn)) != RAYON_OK ||
+          (st = upload(tmp, lo, &slo)) != RAYON_OK || (st = upload(tmp, hi, &shi)) != RAYON_OK) {
+         free_all(tmp);
+         return fail(st);
+      }
+      const auto u1 = clock::now();
+      DeviceTree dt;
+      st = build_bvh_device(ctx, slo, shi, sr, any_normals ? sn : nullptr, (int)n, dt);
+      const auto u2 = clock::now();
+      free_all(tmp);
+      if (st != RAYON_OK) return fail(st);
+      if (dt.max_depth > kMaxDepth) {
+         dt.release();  // deeper than the traversal stack: the host build below instead
+      } else {
+         if (dt.nodes) sc->bufs.push_back({dt.nodes, sizeof(Node2) * (size_t)dt.num_nodes});
+         sc->bufs.push_back({dt.recs, sizeof(PrimRec) * n});
+         if (dt.normals) sc->bufs.push_back({dt.normals, sizeof(float4) * 3 * n});
+         dp = dt.recs;
+         dn = dt.normals;
+         dnodes = dt.nodes;
+         num_nodes = dt.num_nodes;
+         br.order = std::move(dt.order);
+         br.root = dt.root;
+         br.max_depth = dt.max_depth;
+         builder = RAYON_BVH_BUILDER_DEVICE;
+         upload_ms = ms_between(u0, u1);
+         build_ms = ms_between(u1, u2);
+      }
+   }
+   if (builder == RAYON_BVH_BUILDER_HOST) {
+      const auto t0 = clock::now();
+      if (!build_bvh(prims, br, (bvh_flags & RAYON_BVH_QUANTIZED) != 0)) {
+         set_error("BVH depth %d exceeds the traversal stack (%d)", br.max_depth, kMaxDepth);
+         return fail(RAYON_ERR_INVALID);
+      }
+      const auto t1 = clock::now();
+      std::vector<PrimRec> recs(n);
+      std::vector<float4> normals;
+      if (any_normals) normals.assign(3 * n, float4{0, 0, 0, 0});
+      for (size_t s = 0; s < n; ++s) recs[s] = pack_rec(prims[br.order[s]], any_normals ? &normals[3 * s] : nullptr);
+      if ((st = upload(sc->bufs, recs, &dp)) != RAYON_OK || (st = upload(sc->bufs, normals, &dn)) != RAYON_OK ||
+          (st = upload(sc->bufs, br.wnodes, &dw)) != RAYON_OK || (st = upload(sc->bufs, br.nodes, &dnodes)) != RAYON_OK)
+         return fail(st);
+      num_nodes = (int64_t)br.nodes.size();
+      build_ms = ms_between(t0, t1);
+      upload_ms = ms_between(t1, clock::now());
+   }
+   const auto c0 = clock::now();
    sc->orig_of_slot.resize(n);
+   sc->order = br.order;
    std::vector<int> slot_of(n);
    for (size_t s = 0; s < n; ++s) {
-      const HostPrim& p = prims[br.order[s]];
       slot_of[br.order[s]] = (int)s;
-      sc->orig_of_slot[s] = p.orig;
-      const uint32_t info = (p.kind << 28) | p.flags | p.mat;
-      PrimRec& r = recs[s];
-      r.a = {p.a[0], p.a[1], p.a[2], bits_as_float(info)};
-      const float of = bits_as_float((uint32_t)p.orig);
-      if (p.kind == K_TRI) {
-         r.b = {p.b[0] - p.a[0], p.b[1] - p.a[1], p.b[2] - p.a[2], of};
-         r.c = {p.c[0] - p.a[0], p.c[1] - p.a[1], p.c[2] - p.a[2], 0.0f};
-         if (p.flags & F_NORMALS) {
-            for (int v = 0; v < 3; ++v) normals[3 * s + v] = {p.n[3 * v], p.n[3 * v + 1], p.n[3 * v + 2], 0.0f};
-         }
-      } else if (p.kind == K_SPHERE || p.kind == K_DISPLACED) {
-         r.b = {p.b[0], 0.0f, 0.0f, of};
-         r.c = {0, 0, 0, 0};
-      } else if (p.kind == K_RECT) {
-         r.b = {p.b[0], p.b[1], p.b[2], of};
-         r.c = {p.c[0], p.c[1], p.c[2], 0.0f};
-      } else {  // SDF field: bounds
-         r.a = {p.lo[0], p.lo[1], p.lo[2], bits_as_float(info)};
-         r.b = {p.hi[0], p.hi[1], p.hi[2], of};
-         r.c = {0, 0, 0, 0};
-      }
+      sc->orig_of_slot[s] = prims[br.order[s]].orig;
    }
    std::vector<MatRec> mats(d->num_materials);
    for (int i = 0; i < d->num_materials; ++i) mats[i] = pack_material(d->materials[i]);
@@ -346,31 +428,21 @@ rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* d, uint3
       }
       sdf.push_back(r);
       // bounding sphere: |half extents| + rounding + torus major radius, rounded up
-      const double br = std::sqrt((double)r.p.x * r.p.x + (double)r.p.y * r.p.y + (double)r.p.z * r.p.z) +
-                        (double)r.p.w + std::max(0.0, (double)r.c.w);
-      sdf_sph.push_back({r.c.x, r.c.y, r.c.z, std::nextafter((float)br, 3.4e38f)});
+      const double rad = std::sqrt((double)r.p.x * r.p.x + (double)r.p.y * r.p.y + (double)r.p.z * r.p.z) +
+                         (double)r.p.w + std::max(0.0, (double)r.c.w);
+      sdf_sph.push_back({r.c.x, r.c.y, r.c.z, std::nextafter((float)rad, 3.4e38f)});
    }
-   (void)hipSetDevice(ctx->device);
    SceneView& V = sc->view;
-   PrimRec* dp = nullptr;
-   float4* dn = nullptr;
-   Node2* dnodes = nullptr;
    MatRec* dm = nullptr;
    int* dl = nullptr;
    float* da = nullptr;
    SdfRec* ds = nullptr;
    float4* dsph = nullptr;
-   WNode* dw = nullptr;
-   if ((st = upload(sc->bufs, recs, &dp)) != RAYON_OK || (st = upload(sc->bufs, normals, &dn)) != RAYON_OK ||
-       (st = upload(sc->bufs, br.wnodes, &dw)) != RAYON_OK ||
-       (st = upload(sc->bufs, br.nodes, &dnodes)) != RAYON_OK || (st = upload(sc->bufs, mats, &dm)) != RAYON_OK ||
-       (st = upload(sc->bufs, lslots, &dl)) != RAYON_OK || (st = upload(sc->bufs, larea, &da)) != RAYON_OK ||
-       (st = upload(sc->bufs, sdf, &ds)) != RAYON_OK || (st = upload(sc->bufs, sdf_sph, &dsph)) != RAYON_OK) {
-      free_all(sc->bufs);
-      delete sc;
-      return st;
-   }
-   auto t2 = std::chrono::steady_clock::now();
+   if ((st = upload(sc->bufs, mats, &dm)) != RAYON_OK || (st = upload(sc->bufs, lslots, &dl)) != RAYON_OK ||
+       (st = upload(sc->bufs, larea, &da)) != RAYON_OK || (st = upload(sc->bufs, sdf, &ds)) != RAYON_OK ||
+       (st = upload(sc->bufs, sdf_sph, &dsph)) != RAYON_OK)
+      return fail(st);
+   upload_ms += ms_between(c0, clock::now());
    V.prims = dp;
    V.normals = dn;
    V.nodes = dnodes;
@@ -392,13 +464,14 @@ rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* d, uint3
       if (prims[i].kind == K_SDF) V.sdf_prim = slot_of[i];
    rayon_scene_info& I = sc->info;
    I.num_prims = (int64_t)n;
-   I.num_nodes = (int64_t)br.nodes.size();
+   I.num_nodes = num_nodes;
    I.num_qnodes = (int64_t)br.wnodes.size();
    I.max_depth = br.max_depth;
    I.num_lights = V.num_lights;
    for (auto& b : sc->bufs) I.device_bytes += (int64_t)b.bytes;
-   I.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-   I.upload_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+   I.build_ms = build_ms;
+   I.upload_ms = upload_ms;
+   I.builder = builder;
    *out = sc;
    return RAYON_OK;
 }

@@ -683,9 +683,61 @@ bool check_wide(const BuildResult& br, const std::vector<HostPrim>& prims, uint3
    return true;
 }
 
+// Binary tree from br.root: containment, leaf sizes, every slot exactly once; the depth of
+// the deepest leaf through `depth`.
+bool check_binary(const BuildResult& br, const std::vector<HostPrim>& prims, int& depth) {
+   std::vector<int> seen(prims.size(), 0);
+   depth = 0;
+   bool ok;
+   if (prims.empty()) {
+      ok = true;
+   } else if (br.root < 0) {
+      const int packed = ~br.root;
+      ok = check_subtree(br, prims, packed >> 3, (packed & 7) + 1, nullptr, seen, 0, depth);
+   } else {
+      ok = check_subtree(br, prims, 0, 0, nullptr, seen, 0, depth);
+   }
+   for (int s : seen) ok = ok && s == 1;
+   return ok;
+}
+
 }  // namespace
 
 }  // namespace ry
+
+extern "C" rayon_status rayon_scene_check(const rayon_scene* sc, const rayon_scene_desc* d, int32_t* depth) {
+   using namespace ry;
+   if (!sc || !d) {
+      set_error("rayon_scene_check: NULL argument");
+      return RAYON_ERR_INVALID;
+   }
+   std::vector<HostPrim> prims;
+   rayon_status st = gather_prims(d, prims);
+   if (st != RAYON_OK) return st;
+   if (prims.size() != sc->order.size()) {
+      set_error("rayon_scene_check: the description has %zu primitives, the scene %zu", prims.size(),
+                sc->order.size());
+      return RAYON_ERR_INVALID;
+   }
+   if (sc->view.wnodes) {
+      set_error("rayon_scene_check: checks the binary layout only");
+      return RAYON_ERR_UNSUPPORTED;
+   }
+   BuildResult br;
+   br.order = sc->order;
+   br.root = sc->view.root;
+   br.nodes.resize((size_t)sc->info.num_nodes);
+   RY_HIP(hipSetDevice(sc->ctx->device));
+   if (!br.nodes.empty())
+      RY_HIP(hipMemcpy(br.nodes.data(), sc->view.nodes, br.nodes.size() * sizeof(Node2), hipMemcpyDeviceToHost));
+   int dep = 0;
+   if (!check_binary(br, prims, dep)) {
+      set_error("BVH invariant violated (containment, leaf size or slot coverage)");
+      return RAYON_ERR_INVALID;
+   }
+   if (depth) *depth = dep;
+   return RAYON_OK;
+}
 
 extern "C" rayon_status rayon_bvh_check(const rayon_scene_desc* d, rayon_scene_info* out) {
    using namespace ry;
@@ -708,19 +760,8 @@ extern "C" rayon_status rayon_bvh_check(const rayon_scene_desc* d, rayon_scene_i
          return RAYON_ERR_INVALID;
       }
    }
-   std::vector<int> seen(prims.size(), 0);
    int depth = 0;
-   bool ok;
-   if (prims.empty()) {
-      ok = true;
-   } else if (br.root < 0) {
-      const int packed = ~br.root;
-      ok = check_subtree(br, prims, packed >> 3, (packed & 7) + 1, nullptr, seen, 0, depth);
-   } else {
-      ok = check_subtree(br, prims, 0, 0, nullptr, seen, 0, depth);
-   }
-   for (int s : seen) ok = ok && s == 1;
-   if (!ok) {
+   if (!check_binary(br, prims, depth)) {
       set_error("BVH invariant violated (containment, leaf size or slot coverage)");
       return RAYON_ERR_INVALID;
    }

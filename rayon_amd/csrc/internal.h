@@ -53,6 +53,28 @@ constexpr int kMaxDepth = 32;
 bool build_bvh(const std::vector<HostPrim>& prims, BuildResult& out, bool wide = false);
 constexpr int kMaxWideDepth = 16;  // device stack entries of the wide traversal
 
+// Linear BVH built on the device (bvh_device.hip) over primitive boxes and records in host
+// order. The outputs are plain device allocations that the scene takes over.
+struct DeviceTree {
+   Node2* nodes = nullptr;      // pair nodes (root = 0), or none when the root is a leaf
+   PrimRec* recs = nullptr;     // records in slot (leaf) order
+   float4* normals = nullptr;   // vertex normals in slot order (when given)
+   std::vector<int32_t> order;  // slot -> host prim index
+   int root = 0;                // packed root reference
+   int max_depth = 0;
+   int64_t num_nodes = 0;
+   void release() {
+      if (nodes) (void)hipFree(nodes);
+      if (recs) (void)hipFree(recs);
+      if (normals) (void)hipFree(normals);
+      nodes = nullptr;
+      recs = nullptr;
+      normals = nullptr;
+   }
+};
+rayon_status build_bvh_device(rayon_ctx* ctx, const float4* d_lo, const float4* d_hi, const PrimRec* d_recs,
+                              const float4* d_normals, int n, DeviceTree& out);
+
 // Flattens a description into host primitives, validates it.
 rayon_status gather_prims(const rayon_scene_desc* d, std::vector<HostPrim>& prims);
 MatRec pack_material(const rayon_material& m);
@@ -78,6 +100,7 @@ struct rayon_scene {
    std::vector<ry::DevBuf> bufs;
    rayon_scene_info info{};
    std::vector<int64_t> orig_of_slot;
+   std::vector<int32_t> order;  // slot -> primitive of the gathered description
    int32_t batch_path = RAYON_PATH_AUTO;  // path of this scene's jobs in batched passes
 };
 
