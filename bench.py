@@ -149,9 +149,19 @@ def main(argv=None) -> int:
         ctx.set_option(ra.OPT.SHADE_CHUNK, a.shade_chunk)
     t0 = time.perf_counter()
     sc = ctx.scene(g.desc_ptr())
+    create_ms = (time.perf_counter() - t0) * 1e3
     info = sc.info()
     log(f"scene: {info.num_prims} prims, {info.num_nodes} pair nodes, depth {info.max_depth}, "
         f"build {info.build_ms:.0f} ms, upload {info.upload_ms:.0f} ms, {info.device_bytes / 2**20:.1f} MiB")
+    # the same scene with the tree built on the device (RAYON_BVH_DEVICE): creation time
+    # here, one frame's throughput after the main measurement
+    ctx.sync()
+    t0 = time.perf_counter()
+    dsc = ctx.scene(g.desc_ptr(), ra.BVH.DEVICE)
+    dev_create_ms = (time.perf_counter() - t0) * 1e3
+    dinfo = dsc.info()
+    log(f"device-built tree: {dinfo.num_nodes} pair nodes, depth {dinfo.max_depth}, build {dinfo.build_ms:.1f} ms, "
+        f"upload {dinfo.upload_ms:.1f} ms, create {dev_create_ms:.0f} ms (host build: create {create_ms:.0f} ms)")
     lf, la, up, fov = g.camera()
     cam = ra.camera_frame(lf, la, up, fov, a.width, a.height)
     flags = ra.FLAG.NEE
@@ -252,6 +262,25 @@ def main(argv=None) -> int:
         tr = json.loads(tf.read_text())
         traffic = round(tr["hbm_bytes_per_segment"] * (ext_rays + sh_rays) / max(ext_launches, 1))
 
+    # one batched frame over the device-built tree (same job, identical image)
+    dj = ctx.job(dsc, ra.job_params(W, H, cam, spp=spp, max_depth=a.depth, seed=1, flags=flags,
+                                    background_intensity=g.background_intensity()))
+    ctx.render_batched([dj], spp)  # warm-up frame: the timed one replays its captured pass
+    dj.close()
+    dj = ctx.job(dsc, ra.job_params(W, H, cam, spp=spp, max_depth=a.depth, seed=1, flags=flags,
+                                    background_intensity=g.background_intensity()))
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.render_batched([dj], spp)
+    ctx.sync()
+    d_wall = time.perf_counter() - t0
+    dev_tree = {"build_ms": round(dinfo.build_ms, 2), "upload_ms": round(dinfo.upload_ms, 1),
+                "scene_create_ms": round(dev_create_ms, 1), "pair_nodes": int(dinfo.num_nodes),
+                "depth": int(dinfo.max_depth), "builder": "device" if dinfo.builder == ra.BVH.BUILDER_DEVICE else "host",
+                "batched_mrays_per_s": round(dj.stats().segments / d_wall / 1e6, 2)}
+    log(f"device-built tree: {dev_tree['batched_mrays_per_s']} Mrays/s batched")
+    dj.close()
+    dsc.close()
     use_batched = a.path == "batched"
     value = batched_mrays if use_batched else plain_mrays
     seg_per_step = (b_segs / len(b_stats)) if use_batched else (p_segs / len(p_stats))
@@ -294,7 +323,9 @@ def main(argv=None) -> int:
         "segments_per_step": int(seg_per_step),
         "stage_ms_per_frame": stage_ms,
         "bvh": {"build_ms": round(info.build_ms, 1), "pair_nodes": int(info.num_nodes),
-                "depth": int(info.max_depth), "device_MiB": round(info.device_bytes / 2**20, 1)},
+                "depth": int(info.max_depth), "device_MiB": round(info.device_bytes / 2**20, 1),
+                "scene_create_ms": round(create_ms, 1),
+                "device_build": dev_tree},
         "roofline": roof,
         "roofline_plain": {"kernel": "k_plain", "achieved": round(plain_achieved, 1),
                            "frac": round(plain_achieved / HBM_PEAK_GBS, 4), "avg_launch_ms": round(p_avg_ms, 3)},
