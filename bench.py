@@ -100,6 +100,7 @@ def main(argv=None) -> int:
     ap.add_argument("--capacity", type=int, default=0, help="path slots per wavefront round (0: library default)")
     ap.add_argument("--chunk", type=int, default=0, help="queue entries (x64) a trace wave claims (0: default)")
     ap.add_argument("--shade-chunk", type=int, default=0, help="queue entries (x64) a shade wave claims (0: default)")
+    ap.add_argument("--refill", type=int, default=0, help="idle lanes at which a trace wave takes new rays (0: default)")
     ap.add_argument("--workload", choices=["cornell_1080p", "batch256", "serve"], default="cornell_1080p",
                     help="cornell_1080p: one 1080p 64-spp mesh frame per step (default); batch256: 256 mixed "
                          "jobs, plain one-job-per-launch vs batched passes; serve: Poisson arrivals into the server")
@@ -147,6 +148,8 @@ def main(argv=None) -> int:
         ctx.set_option(ra.OPT.WAVE_CHUNK, a.chunk)
     if a.shade_chunk:
         ctx.set_option(ra.OPT.SHADE_CHUNK, a.shade_chunk)
+    if a.refill:
+        ctx.set_option(ra.OPT.TRACE_REFILL, a.refill)
     t0 = time.perf_counter()
     sc = ctx.scene(g.desc_ptr())
     create_ms = (time.perf_counter() - t0) * 1e3
@@ -539,6 +542,7 @@ def bench_batch256(a, ra) -> int:
                        "plain_p50": round(percentile(p_lat, 50), 1), "plain_p99": round(percentile(p_lat, 99), 1)},
         "segments": int(b_segs), "plain_segments": int(p_segs),
         "autotune": {"capacity": tune.capacity, "chunk": tune.chunk, "shade_chunk": tune.shade_chunk,
+                     "refill": tune.refill,
                      "wg_per_cu": tune.wg_per_cu, "mega_scenes": tune.mega_scenes, "wave_scenes": tune.wave_scenes,
                      "best_ms": round(tune.best_ms, 2), "start_ms": round(tune.start_ms, 2),
                      "shapes": tune.candidates},

@@ -295,7 +295,9 @@ enum {
    RAYON_OPT_STAGE_TIMING = 3,    /* time every stage launch with HIP events (0; disables graphs) */
    RAYON_OPT_COUNT_TRAVERSAL = 4, /* count node/primitive fetches of the queue stages (0)     */
    RAYON_OPT_WAVE_CHUNK = 5,      /* 64-ray batches a trace wave claims per queue atomic (4) */
-   RAYON_OPT_SHADE_CHUNK = 6      /* 64-hit batches a shade wave claims per queue atomic (16) */
+   RAYON_OPT_SHADE_CHUNK = 6,     /* 64-hit batches a shade wave claims per queue atomic (16) */
+   RAYON_OPT_TRACE_REFILL = 7     /* idle lanes (of 64) at which a binary-tree trace wave takes
+                                     new rays from the queue (16); 64 traces whole batches     */
 };
 rayon_status rayon_ctx_set_option(rayon_ctx* ctx, int32_t option, int64_t value);
 rayon_status rayon_ctx_get_option(rayon_ctx* ctx, int32_t option, int64_t* value);
@@ -372,7 +374,7 @@ typedef struct rayon_tune_result {
    int32_t mega_scenes;  /* scenes of the jobs on the megakernel path after tuning        */
    int32_t wave_scenes;  /* scenes of the jobs on the wavefront path after tuning         */
    int32_t shade_chunk;  /* RAYON_OPT_SHADE_CHUNK chosen                                  */
-   int32_t pad_;
+   int32_t refill;       /* RAYON_OPT_TRACE_REFILL chosen                                 */
 } rayon_tune_result;
 
 rayon_status rayon_autotune(rayon_ctx* ctx, rayon_job* const* jobs, int32_t n, int32_t samples,

@@ -26,7 +26,8 @@ BVH = _NS(DEFAULT=0, QUANTIZED=1, DEVICE=2, BUILDER_HOST=0, BUILDER_DEVICE=1)
 YAML = _NS(LIFT_CAPS=1)
 JOB = _NS(QUEUED=0, RUNNING=1, DONE=2, FAILED=3, CANCELLED=4)
 PATH = _NS(AUTO=0, WAVEFRONT=1, MEGAKERNEL=2)
-OPT = _NS(WAVE_CAPACITY=0, WAVE_WG_PER_CU=1, GRAPH=2, STAGE_TIMING=3, COUNT_TRAVERSAL=4, WAVE_CHUNK=5, SHADE_CHUNK=6)
+OPT = _NS(WAVE_CAPACITY=0, WAVE_WG_PER_CU=1, GRAPH=2, STAGE_TIMING=3, COUNT_TRAVERSAL=4, WAVE_CHUNK=5, SHADE_CHUNK=6,
+          TRACE_REFILL=7)
 
 
 class Material(C.Structure):
@@ -126,7 +127,7 @@ class TuneResult(C.Structure):
     _fields_ = [("capacity", C.c_int64), ("chunk", C.c_int32), ("wg_per_cu", C.c_int32), ("best_ms", C.c_double),
                 ("start_ms", C.c_double), ("candidates", C.c_int32), ("cached", C.c_int32),
                 ("mega_scenes", C.c_int32), ("wave_scenes", C.c_int32), ("shade_chunk", C.c_int32),
-                ("pad_", C.c_int32)]
+                ("refill", C.c_int32)]
 
 
 class ProgressiveConfig(C.Structure):

@@ -26,8 +26,10 @@ struct Shape {
    int chunk;
    int wg;
    int shade_chunk;
+   int refill;
    bool operator==(const Shape& o) const {
-      return capacity == o.capacity && chunk == o.chunk && wg == o.wg && shade_chunk == o.shade_chunk;
+      return capacity == o.capacity && chunk == o.chunk && wg == o.wg && shade_chunk == o.shade_chunk &&
+             refill == o.refill;
    }
 };
 
@@ -36,6 +38,7 @@ rayon_status apply(rayon_ctx* ctx, const Shape& s) {
    if (st == RAYON_OK) st = rayon_ctx_set_option(ctx, RAYON_OPT_WAVE_CHUNK, s.chunk);
    if (st == RAYON_OK) st = rayon_ctx_set_option(ctx, RAYON_OPT_WAVE_WG_PER_CU, s.wg);
    if (st == RAYON_OK) st = rayon_ctx_set_option(ctx, RAYON_OPT_SHADE_CHUNK, s.shade_chunk);
+   if (st == RAYON_OK) st = rayon_ctx_set_option(ctx, RAYON_OPT_TRACE_REFILL, s.refill);
    return st;
 }
 
@@ -97,7 +100,7 @@ extern "C" rayon_status rayon_autotune(rayon_ctx* ctx, rayon_job* const* jobs, i
          *out = it->second;
          out->candidates = 0;
          out->cached = 1;
-         return apply(ctx, {out->capacity, out->chunk, out->wg_per_cu, out->shade_chunk});
+         return apply(ctx, {out->capacity, out->chunk, out->wg_per_cu, out->shade_chunk, out->refill});
       }
    }
    Shape start{};
@@ -111,6 +114,8 @@ extern "C" rayon_status rayon_autotune(rayon_ctx* ctx, rayon_job* const* jobs, i
       start.wg = (int)v;
       if (st == RAYON_OK) st = rayon_ctx_get_option(ctx, RAYON_OPT_SHADE_CHUNK, &v);
       start.shade_chunk = (int)v;
+      if (st == RAYON_OK) st = rayon_ctx_get_option(ctx, RAYON_OPT_TRACE_REFILL, &v);
+      start.refill = (int)v;
       if (st != RAYON_OK) return st;
    }
    std::vector<rayon_job*> clones;
@@ -169,20 +174,23 @@ extern "C" rayon_status rayon_autotune(rayon_ctx* ctx, rayon_job* const* jobs, i
    std::vector<Shape> tried = {start};
    rayon_status st = measure(start, &start_ms);
    best_ms = start_ms;
-   // coordinate descent: trace chunk, shade chunk, workers per CU, then capacity
+   // coordinate descent: trace chunk, shade chunk, workers per CU, capacity, then the
+   // trace waves' refill threshold
    const long long round_cap = std::min<long long>(1ll << 28, std::max<long long>(1 << 16, work));
    const std::vector<std::vector<long long>> axes = {
       {1, 2, 4, 8, 16},
       {2, 4, 8, 16},
       {0, 1, 2, 4, 8},
       {1ll << 22, 1ll << 24, 1ll << 25, 1ll << 26, 1ll << 27},
+      {8, 16, 32, 64},
    };
-   for (int ax = 0; ax < 4 && st == RAYON_OK; ++ax) {
+   for (int ax = 0; ax < 5 && st == RAYON_OK; ++ax) {
       for (long long v : axes[ax]) {
          Shape s = best;
          if (ax == 0) s.chunk = (int)v;
          if (ax == 1) s.shade_chunk = (int)v;
          if (ax == 2) s.wg = (int)v;
+         if (ax == 4) s.refill = (int)v;
          if (ax == 3) {
             // capacities above the work of one pass behave alike: try the exact fit once
             s.capacity = std::min(v, round_cap);
@@ -211,6 +219,7 @@ extern "C" rayon_status rayon_autotune(rayon_ctx* ctx, rayon_job* const* jobs, i
    r.chunk = best.chunk;
    r.wg_per_cu = best.wg;
    r.shade_chunk = best.shade_chunk;
+   r.refill = best.refill;
    r.best_ms = best_ms;
    r.start_ms = start_ms;
    r.candidates = timed;

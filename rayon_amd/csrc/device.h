@@ -11,7 +11,7 @@
 
 namespace ry {
 
-constexpr int kStack = 32;            // LDS traversal stack entries per lane
+constexpr int kStack = 32;            // traversal stack entries per lane (binary tree: 31 in LDS + 1 register)
 constexpr float kPiF = 3.14159265358979f;
 
 // ------------------------------------------------------------------------------------
@@ -405,6 +405,32 @@ struct TravCount {
    unsigned nodes = 0, prims = 0;
 };
 
+// Traversal stack of the binary tree: `CAP` entries, the top one in a register and the
+// others in LDS (entry i at lds[i * STRIDE]), so a pop hands over the next node at once and
+// the LDS read that refills the top overlaps the node fetch. A push beyond CAP is dropped;
+// the builders bound the tree depth (the most entries a traversal can hold) by the capacity
+// of the kernel that traverses it, so that never happens.
+template <int STRIDE, int CAP>
+struct BinStack {
+   int* lds;
+   int n = 0;  // entries held, the top one in `top`
+   int top = kSentinel;
+   __device__ __forceinline__ explicit BinStack(int* p) : lds(p) {}
+   __device__ __forceinline__ void push(int x) {
+      if (n >= CAP) return;
+      if (n > 0) lds[(n - 1) * STRIDE] = top;
+      top = x;
+      ++n;
+   }
+   __device__ __forceinline__ int pop() {
+      if (n == 0) return kSentinel;
+      const int x = top;
+      --n;
+      if (n > 0) top = lds[(n - 1) * STRIDE];
+      return x;
+   }
+};
+
 template <int STRIDE, bool CNT = false, bool DEFER = false, int STACK = kStack>
 __device__ inline HitMin closest_hit_bin(const SceneView& S, V3 o, V3 d, float tmin, float tmax, int* stack,
                                          TravCount* tc = nullptr) {
@@ -417,7 +443,7 @@ __device__ inline HitMin closest_hit_bin(const SceneView& S, V3 o, V3 d, float t
    int cur = S.root;
    if (cur == kSentinel) return best;
    const RayInv r = make_rayinv(o, d);
-   int sp = 0;
+   BinStack<STRIDE, STACK> st(stack);
    for (;;) {
       while (cur >= 0) {
          const Node2 n = gload(S.nodes + cur);
@@ -433,14 +459,14 @@ __device__ inline HitMin closest_hit_bin(const SceneView& S, V3 o, V3 d, float t
                nr = r1;
                fr = r0;
             }
-            if (sp < STACK) stack[(sp++) * STRIDE] = fr;
+            st.push(fr);
             cur = nr;
          } else if (h0) {
             cur = r0;
          } else if (h1) {
             cur = r1;
          } else {
-            cur = sp > 0 ? stack[(--sp) * STRIDE] : kSentinel;
+            cur = st.pop();
          }
       }
       if (cur == kSentinel) break;
@@ -476,7 +502,7 @@ __device__ inline HitMin closest_hit_bin(const SceneView& S, V3 o, V3 d, float t
             }
          }
       }
-      cur = sp > 0 ? stack[(--sp) * STRIDE] : kSentinel;
+      cur = st.pop();
       if (cur == kSentinel) break;
    }
    return best;
@@ -489,7 +515,7 @@ __device__ inline bool any_hit_bin(const SceneView& S, V3 o, V3 d, float tmin, f
    int cur = S.root;
    if (cur == kSentinel) return false;
    const RayInv r = make_rayinv(o, d);
-   int sp = 0;
+   BinStack<STRIDE, STACK> st(stack);
    for (;;) {
       while (cur >= 0) {
          const Node2 n = gload(S.nodes + cur);
@@ -500,14 +526,14 @@ __device__ inline bool any_hit_bin(const SceneView& S, V3 o, V3 d, float tmin, f
          int r0 = n.meta.z ? pack_leaf(n.meta.x, n.meta.z) : n.meta.x;
          int r1 = n.meta.w ? pack_leaf(n.meta.y, n.meta.w) : n.meta.y;
          if (h0 && h1) {
-            if (sp < STACK) stack[(sp++) * STRIDE] = r1;
+            st.push(r1);
             cur = r0;
          } else if (h0) {
             cur = r0;
          } else if (h1) {
             cur = r1;
          } else {
-            cur = sp > 0 ? stack[(--sp) * STRIDE] : kSentinel;
+            cur = st.pop();
          }
       }
       if (cur == kSentinel) return false;
@@ -531,7 +557,7 @@ __device__ inline bool any_hit_bin(const SceneView& S, V3 o, V3 d, float tmin, f
             hit = isect_sdf(S, p, o, d, tmin, tmax, t);
          if (hit) return true;
       }
-      cur = sp > 0 ? stack[(--sp) * STRIDE] : kSentinel;
+      cur = st.pop();
       if (cur == kSentinel) return false;
    }
 }

@@ -35,7 +35,9 @@
 namespace ry {
 
 constexpr int kWfBlock = 256;
-constexpr int kShortStack = 24;  // LDS stack of binary trees up to depth 23 (more waves per CU)
+// Binary trees up to 23 levels deep take a 23-entry stack (22 LDS entries per lane + the
+// register top), which fits 7 workgroups of the trace kernel per CU instead of 5.
+constexpr int kShortStack = 23;
 constexpr int kSdfSlice = 8;   // steps of the first marching slice; slice k takes kSdfSlice << k
 constexpr int kSdfSlices = 8;  // slices per level at most (the last one runs to the end)
 constexpr long long kMegaSceneBytes = 8ll << 20;  // automatic path choice: megakernel up to this scene size
@@ -69,6 +71,8 @@ struct WfView {
    int levels;
    int chunk;  // 64-entry batches a wave claims per queue-head atomic (trace, marching)
    int shade_chunk;  // the same for the shade stage
+   int shadow_refill;  // shadow rays with lane refill (passes without SDF fields)
+   int refill;         // idle lanes (of 64) at which a trace wave takes new rays
 };
 
 struct Wave {
@@ -90,6 +94,7 @@ struct Wave {
    int opt_wg_per_cu = 0;  // 0: as many as fit (occupancy of each stage kernel)
    int opt_chunk = 4;
    int opt_shade_chunk = 16;
+   int opt_refill = 16;
    int occ_trace = 0, occ_trace_wide = 0, occ_trace_short = 0, occ_shade = 0, occ_sdf = 0;
    struct Pending {
       int stage;
@@ -132,6 +137,41 @@ __device__ __forceinline__ int wave_push(unsigned* counter, bool pred) {
    const unsigned long long below = m & ((1ull << lane_id()) - 1ull);
    return pred ? (int)(base + (unsigned)__popcll(below)) : -1;
 }
+
+constexpr int kMarchRes = 128;  // marching-queue positions a trace wave reserves at a time
+// Positions beyond the marches themselves that the slice-0 marching queue must hold: every
+// wave of a trace launch may leave at most kMarchRes - 1 reserved positions unfilled
+// (at most 32 workgroups of 4 waves per CU).
+constexpr long long kMarchHeadroomPerCu = 32ll * (kWfBlock / 64) * kMarchRes;
+
+// Appends to a marching queue through positions the wave reserves kMarchRes at a time, so
+// the queue's head counter, shared by every wave of a launch, takes one atomic per
+// reservation instead of one per push. close() marks the positions the wave reserved and
+// did not fill with -1; the consumer skips them.
+struct QueueRes {
+   unsigned next = 0, end = 0;  // wave-uniform
+   __device__ __forceinline__ void push(unsigned* head, int* q, bool pred, int value) {
+      const unsigned long long m = __ballot(pred);
+      if (m == 0) return;
+      const unsigned k = (unsigned)__popcll(m);
+      const unsigned rank = (unsigned)__popcll(m & ((1ull << lane_id()) - 1ull));
+      const unsigned have = end - next;
+      if (pred && rank < have) q[next + rank] = value;
+      if (k > have) {
+         unsigned base = 0;
+         if (lane_id() == 0) base = atomicAdd(head, (unsigned)kMarchRes);
+         base = __shfl(base, 0);
+         if (pred && rank >= have) q[base + (rank - have)] = value;
+         next = base + (k - have);
+         end = base + (unsigned)kMarchRes;
+      } else {
+         next += k;
+      }
+   }
+   __device__ __forceinline__ void close(int* q) {
+      for (unsigned i = next + (unsigned)lane_id(); i < end; i += 64) q[i] = -1;
+   }
+};
 
 // Persistent-worker loop: each wave claims `chunk` batches of 64 queue entries with one
 // atomic on the queue head and runs body(index) on each; body sees indices >= count on
@@ -241,12 +281,13 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_sdf(WfView V, int level, int it
    const unsigned count = cin[0];
    const int* qin = (it & 1) ? V.qs1 : V.qs0;
    int* qout = (it & 1) ? V.qs0 : V.qs1;
+   QueueRes res;  // rays still marching after this slice
    drain(&cin[1], count, V.chunk, [&](unsigned idx) {
       int slot = -1;
       bool more = false;
       unsigned taken = 0;
-      if (idx < count) {
-         slot = qin[idx];
+      if (idx < count) slot = qin[idx];  // -1: a position a trace wave reserved and left unused
+      if (slot >= 0) {
          const SceneView& S = V.jobs[V.ids[slot].x].scene;
          const V3 o = mk(V.o[slot]), d = mk(V.d[slot]);
          SdfMarch m;
@@ -283,9 +324,9 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_sdf(WfView V, int level, int it
          }
       }
       if constexpr (CNT) n_slots += 64ull * wave_max_u32(taken);  // lane-steps the wave's SIMD spent
-      const int pos = wave_push(&cout[0], more);
-      if (more) qout[pos] = slot;
+      res.push(&cout[0], qout, more, slot);
    });
+   res.close(qout);
    if constexpr (CNT) {
       const unsigned long long a = wave_sum(n_begin), b = wave_sum(n_steps), c = wave_sum(n_evals);
       if (lane_id() == 0) {
@@ -297,15 +338,289 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_sdf(WfView V, int level, int it
    }
 }
 
+// ---- Binary-tree traversal with lane refill (dynamic ray fetch, after Aila and Laine,
+// "Understanding the efficiency of ray traversal on GPUs", HPG 2009). A wave keeps a pool of
+// queue entries it has claimed; whenever at least V.refill of its lanes have no ray, those
+// lanes take the next entries of the pool (claiming another chunk when it runs out), so a
+// wave is not held by the slowest of 64 rays. Each lane runs the same per-ray traversal as
+// closest_hit_bin / any_hit_bin (same node order, same tie rule), one leaf per iteration,
+// so every ray's result is what the run-to-completion traversal returns.
+
+struct WavePool {
+   unsigned next = 0, end = 0;  // wave-uniform: entries claimed and not yet handed out
+   bool drained = false;        // wave-uniform: the queue has no unclaimed entries left
+
+   // Hands entries to the idle lanes (in lane order) when at least `at` lanes are idle;
+   // returns the lane's new queue index, or -1.
+   __device__ __forceinline__ long long refill(unsigned* head, unsigned count, unsigned step, bool idle_lane,
+                                               int at) {
+      const unsigned long long idle = __ballot(idle_lane);
+      const int nidle = __popcll(idle);
+      if (nidle < at) return -1;
+      if (next == end && !drained) {
+         unsigned base = 0;
+         if (lane_id() == 0) base = atomicAdd(head, step);
+         base = __shfl(base, 0);
+         if (base >= count) {
+            drained = true;
+         } else {
+            next = base;
+            end = min(base + step, count);
+            drained = end == count;  // no entries beyond this chunk
+         }
+      }
+      const unsigned avail = end - next;
+      const unsigned rank = (unsigned)__popcll(idle & ((1ull << lane_id()) - 1ull));
+      const long long mine = (idle_lane && rank < avail) ? (long long)(next + rank) : -1;
+      next += min(avail, (unsigned)nidle);
+      return mine;
+   }
+};
+
+// One wave's state at the start of a drain: an empty queue or one already drained by
+// other waves hands out nothing.
+__device__ __forceinline__ WavePool start_pool(unsigned* head, unsigned count) {
+   WavePool p;
+   unsigned seen = count;
+   if (count != 0 && lane_id() == 0) seen = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+   p.drained = __shfl(seen, 0) >= count;
+   return p;
+}
+
+// Extension rays of one level: closest hit (SDF fields deferred to k_wf_sdf).
+template <bool CNT, int STACK>
+__device__ void ext_rays_bin(const WfView& V, int level, int* lds, TravCount& tc) {
+   unsigned* ctl = V.ctl + kCtl * level;
+   const unsigned count = ctl[0];
+   const int* q = (level & 1) ? V.q1 : V.q0;
+   unsigned* mctl = V.sctl + 2 * (kSdfSlices + 1) * level;
+   const unsigned step = 64u * (unsigned)V.chunk;
+   WavePool pool = start_pool(&ctl[1], count);
+   int slot = -1, cur = kSentinel;
+   // slot of a finished ray whose SDF march is still to be queued. The lanes' marches are
+   // queued together before each refill, into positions the wave reserved in the marching
+   // queue kMarchRes at a time (one atomic per reservation, not per refill): the queue head
+   // is shared by every wave of the launch. Positions reserved but not filled by the wave's
+   // end hold -1, which k_wf_sdf skips.
+   int pend = -1;
+   QueueRes res;
+   auto queue_marches = [&]() {
+      res.push(&mctl[0], V.qs0, pend >= 0, pend);
+      pend = -1;
+   };
+   const Node2* nodes = nullptr;
+   const PrimRec* prims = nullptr;
+   RayInv r;
+   HitMin best;
+   BinStack<kWfBlock, STACK> st(lds);
+   for (;;) {
+      if (!pool.drained || pool.next != pool.end) {
+         if (__popcll(__ballot(slot < 0)) >= V.refill) queue_marches();
+         const long long idx = pool.refill(&ctl[1], count, step, slot < 0, V.refill);
+         if (idx >= 0) {
+            slot = q[idx];
+            const SceneView& S = V.jobs[V.ids[slot].x].scene;
+            nodes = S.nodes;
+            prims = S.prims;
+            cur = S.root;
+            r = make_rayinv(mk(V.o[slot]), mk(V.d[slot]));
+            best.t = 3.402823466e38f;
+            best.slot = -1;
+            best.orig = -1;
+            best.u = best.v = 0.0f;
+            best.sdf = -1;
+            st.n = 0;
+            st.top = kSentinel;
+         }
+      }
+      if (__ballot(slot >= 0) == 0) break;
+      if (slot >= 0) {
+         bool finished;
+         while (cur >= 0) {
+            const Node2 n = gload(nodes + cur);
+            if constexpr (CNT) ++tc.nodes;
+            float tn0, tn1;
+            bool h0, h1;
+            box2(r, n, 0.001f, best.t, tn0, tn1, h0, h1);
+            const int r0 = n.meta.z ? pack_leaf(n.meta.x, n.meta.z) : n.meta.x;
+            const int r1 = n.meta.w ? pack_leaf(n.meta.y, n.meta.w) : n.meta.y;
+            if (h0 && h1) {
+               const bool swap = tn1 < tn0;
+               st.push(swap ? r0 : r1);
+               cur = swap ? r1 : r0;
+            } else if (h0) {
+               cur = r0;
+            } else if (h1) {
+               cur = r1;
+            } else {
+               cur = st.pop();
+            }
+         }
+         if (cur != kSentinel) {  // one leaf
+            const int packed = ~cur;
+            const int first = packed >> 3, cnt = (packed & 7) + 1;
+            for (int i = first; i < first + cnt; ++i) {
+               const PrimRec p = gload(prims + i);
+               if constexpr (CNT) ++tc.prims;
+               const uint32_t kind = info_kind(__float_as_uint(p.a.w));
+               float t = 0.0f, u = 0.0f, v = 0.0f;
+               bool hit = false;
+               if (kind == K_TRI)
+                  hit = isect_tri(p, r.o, r.d, 0.001f, best.t, t, u, v);
+               else if (kind == K_SPHERE || kind == K_DISPLACED)
+                  hit = isect_sphere(mk(p.a), p.b.x, r.o, r.d, 0.001f, best.t, t);
+               else if (kind == K_RECT)
+                  hit = isect_rect(mk(p.a), mk(p.b), mk(p.c), r.o, r.d, 0.001f, best.t, t);
+               else
+                  best.sdf = i;  // marched later by k_wf_sdf
+               if (hit) {
+                  const int orig = __float_as_int(p.b.w);
+                  if (t < best.t || orig > best.orig) {
+                     best.t = t;
+                     best.slot = i;
+                     best.orig = orig;
+                     best.u = u;
+                     best.v = v;
+                  }
+               }
+            }
+            cur = st.pop();
+         }
+         finished = cur == kSentinel;
+         if (finished) {
+            V.hit[slot] = make_float4(best.t, __int_as_float(best.slot), best.u, best.v);
+            if (best.sdf >= 0) pend = slot;  // an SDF field lies on the way
+            slot = -1;
+         }
+      }
+   }
+   queue_marches();
+   res.close(V.qs0);
+}
+
+// Shadow rays of one level: any non-emitter hit; unoccluded contributions are added to the
+// path's radiance.
+template <bool CNT, int STACK>
+__device__ void shadow_rays_bin(const WfView& V, int level, int* lds, TravCount& tc) {
+   unsigned* sctl = V.ctl + kCtl * level + 3;
+   const unsigned count = sctl[0];
+   const unsigned step = 64u * (unsigned)V.chunk;
+   WavePool pool = start_pool(&sctl[1], count);
+   int slot = -1, cur = kSentinel;
+   const SceneView* S = nullptr;
+   RayInv r;
+   BinStack<kWfBlock, STACK> st(lds);
+   constexpr float tmin = 1e-4f, tmax = 1.0f - 1e-4f;
+   for (;;) {
+      if (!pool.drained || pool.next != pool.end) {
+         const long long idx = pool.refill(&sctl[1], count, step, slot < 0, V.refill);
+         if (idx >= 0) {
+            slot = V.sq[idx];
+            S = &V.jobs[V.ids[slot].x].scene;
+            cur = S->root;
+            r = make_rayinv(mk(V.so[slot]), mk(V.sd[slot]));
+            st.n = 0;
+            st.top = kSentinel;
+         }
+      }
+      if (__ballot(slot >= 0) == 0) break;
+      if (slot >= 0) {
+         bool occluded = false;
+         while (cur >= 0) {
+            const Node2 n = gload(S->nodes + cur);
+            if constexpr (CNT) ++tc.nodes;
+            float tn0, tn1;
+            bool h0, h1;
+            box2(r, n, tmin, tmax, tn0, tn1, h0, h1);
+            const int r0 = n.meta.z ? pack_leaf(n.meta.x, n.meta.z) : n.meta.x;
+            const int r1 = n.meta.w ? pack_leaf(n.meta.y, n.meta.w) : n.meta.y;
+            if (h0 && h1) {
+               st.push(r1);
+               cur = r0;
+            } else if (h0) {
+               cur = r0;
+            } else if (h1) {
+               cur = r1;
+            } else {
+               cur = st.pop();
+            }
+         }
+         if (cur != kSentinel) {  // one leaf
+            const int packed = ~cur;
+            const int first = packed >> 3, cnt = (packed & 7) + 1;
+            for (int i = first; i < first + cnt && !occluded; ++i) {
+               const PrimRec p = gload(S->prims + i);
+               if constexpr (CNT) ++tc.prims;
+               const uint32_t info = __float_as_uint(p.a.w);
+               if (info & F_EMITTER) continue;
+               const uint32_t kind = info_kind(info);
+               float t, u, v;
+               if (kind == K_TRI)
+                  occluded = isect_tri(p, r.o, r.d, tmin, tmax, t, u, v);
+               else if (kind == K_SPHERE || kind == K_DISPLACED)
+                  occluded = isect_sphere(mk(p.a), p.b.x, r.o, r.d, tmin, tmax, t);
+               else if (kind == K_RECT)
+                  occluded = isect_rect(mk(p.a), mk(p.b), mk(p.c), r.o, r.d, tmin, tmax, t);
+               else
+                  occluded = isect_sdf(*S, p, r.o, r.d, tmin, tmax, t);
+            }
+            cur = occluded ? kSentinel : st.pop();
+         }
+         if (cur == kSentinel) {
+            if (!occluded) {
+               const float4 c = V.sc[slot];
+               const float4 rd = V.rad[slot];
+               const V3 L = mk(rd) + mk(c);
+               V.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+            }
+            slot = -1;
+         }
+      }
+   }
+}
+
+// Shadow rays of one level, each wave tracing batches of 64 rays to completion.
+template <bool CNT, int LAYOUT, int STACK>
+__device__ void shadow_rays_drain(const WfView& V, int level, int* lds, TravCount& ts) {
+   unsigned* sctl = V.ctl + kCtl * level + 3;
+   const unsigned count = sctl[0];
+   drain(&sctl[1], count, V.chunk, [&](unsigned idx) {
+      if (idx < count) {
+         const int slot = V.sq[idx];
+         const float4 o = V.so[slot], d = V.sd[slot];
+         const SceneView& S = V.jobs[V.ids[slot].x].scene;
+         if (!any_hit<kWfBlock, CNT, LAYOUT, STACK>(S, mk(o), mk(d), 1e-4f, 1.0f - 1e-4f, lds, &ts)) {
+            const float4 c = V.sc[slot];
+            const float4 r = V.rad[slot];
+            const V3 L = mk(r) + mk(c);
+            V.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+         }
+      }
+   });
+}
+
 // Traversal stage: first the extension rays of level `ext_level` (closest hit), then the
 // shadow rays of level `shadow_level` (any hit; unoccluded contributions are added to the
 // path's radiance). Both queues were filled by the preceding shade launch and touch
 // disjoint state, so one persistent launch drains both and waves that finish their
 // extension work fill the launch's tail with shadow work. Either level may be -1.
 template <bool CNT, int LAYOUT, int STACK = kStack>
-__global__ __launch_bounds__(kWfBlock) void k_wf_trace(WfView V, int ext_level, int shadow_level) {
-   __shared__ int stack[STACK * kWfBlock];
+__global__ __launch_bounds__(kWfBlock, (LAYOUT == L_BIN && STACK == kShortStack) ? 7 : 1) void k_wf_trace(
+    WfView V, int ext_level, int shadow_level) {
+   // binary traversal: STACK - 1 LDS entries per lane; the wide one keeps kStack / 2 pairs
+   __shared__ int stack[(LAYOUT == L_BIN ? STACK - 1 : kStack) * kWfBlock];
    TravCount te, ts;
+   if constexpr (LAYOUT == L_BIN) {
+      if (ext_level >= 0) ext_rays_bin<CNT, STACK>(V, ext_level, stack + threadIdx.x, te);
+      if (shadow_level >= 0) {
+         // a shadow ray across an SDF field marches it inline; lanes refilled at different
+         // times would march one by one, so passes with fields keep whole batches together
+         if (V.shadow_refill)
+            shadow_rays_bin<CNT, STACK>(V, shadow_level, stack + threadIdx.x, ts);
+         else
+            shadow_rays_drain<CNT, LAYOUT, STACK>(V, shadow_level, stack + threadIdx.x, ts);
+      }
+   } else {
    if (ext_level >= 0) {
       unsigned* ctl = V.ctl + kCtl * ext_level;
       const unsigned count = ctl[0];
@@ -327,23 +642,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_trace(WfView V, int ext_level, 
          if (march) V.qs0[pos] = slot;
       });
    }
-   if (shadow_level >= 0) {
-      unsigned* sctl = V.ctl + kCtl * shadow_level + 3;
-      const unsigned count = sctl[0];
-      drain(&sctl[1], count, V.chunk, [&](unsigned idx) {
-         if (idx < count) {
-            const int slot = V.sq[idx];
-            const float4 o = V.so[slot], d = V.sd[slot];
-            const SceneView& S = V.jobs[V.ids[slot].x].scene;
-            if (!any_hit<kWfBlock, CNT, LAYOUT, STACK>(S, mk(o), mk(d), 1e-4f, 1.0f - 1e-4f, stack + threadIdx.x,
-                                                       &ts)) {
-               const float4 c = V.sc[slot];
-               const float4 r = V.rad[slot];
-               const V3 L = mk(r) + mk(c);
-               V.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
-            }
-         }
-      });
+   if (shadow_level >= 0) shadow_rays_drain<CNT, LAYOUT, STACK>(V, shadow_level, stack + threadIdx.x, ts);
    }
    if constexpr (CNT) {
       const unsigned long long a = wave_sum((unsigned long long)te.nodes), b = wave_sum((unsigned long long)te.prims);
@@ -621,7 +920,10 @@ static rayon_status wave_reserve(rayon_ctx* ctx, int capacity, int levels, int n
       W->v.ids = (uint4*)p;
       int** qs[] = {&W->v.q0, &W->v.q1, &W->v.sq, &W->v.qs0, &W->v.qs1};
       for (int** q : qs) {
-         if ((st = alloc(W->bufs_slots, (size_t)cap * sizeof(int), &p)) != RAYON_OK) return st;
+         // the marching queues also hold the unused tails of the waves' reservations
+         const bool marching = q == &W->v.qs0 || q == &W->v.qs1;
+         const size_t n = (size_t)cap + (marching ? (size_t)(kMarchHeadroomPerCu * W->cus) : 0);
+         if ((st = alloc(W->bufs_slots, n * sizeof(int), &p)) != RAYON_OK) return st;
          *q = (int*)p;
       }
       W->capacity = cap;
@@ -722,6 +1024,10 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
    V.levels = levels;
    V.chunk = W->opt_chunk;
    V.shade_chunk = W->opt_shade_chunk;
+   V.refill = W->opt_refill;
+   V.shadow_refill = 1;
+   for (rayon_job* j : jobs)
+      if (j->scene->view.num_sdf > 0) V.shadow_refill = 0;
    // persistent grids: the resident workgroups, but no more than one lane per path of a
    // round (small passes then launch and drain quickly)
    const long long per_round = std::min<long long>(std::max(total, 1ll), W->round);
@@ -737,7 +1043,7 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
    // binary trees no deeper than the short stack take the kernel with the smaller LDS stack
    int depth = 0;
    for (rayon_job* j : jobs) depth = std::max(depth, j->scene->info.max_depth);
-   const bool short_stack = layout == L_BIN && depth < kShortStack;
+   const bool short_stack = layout == L_BIN && depth <= kShortStack;
    const dim3 g_trace(grid_of(short_stack        ? W->occ_trace_short
                               : layout == L_BIN ? W->occ_trace
                                                 : std::min(W->occ_trace, W->occ_trace_wide)));
@@ -988,6 +1294,7 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
       key.push_back(W->opt_wg_per_cu);
       key.push_back(W->opt_chunk);
       key.push_back(W->opt_shade_chunk);
+      key.push_back(W->opt_refill);
       key.push_back(W->opt_count);
       for (rayon_job* j : active) key.push_back(j->scene->view.wnodes != nullptr);
       int sdf_steps = 0;  // the marching slices the pass enqueues
@@ -996,7 +1303,7 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
       key.push_back(sdf_steps);
       int depth = 0;  // picks the trace kernel's stack size
       for (rayon_job* j : active) depth = std::max(depth, j->scene->info.max_depth);
-      key.push_back(depth < kShortStack);
+      key.push_back(depth <= kShortStack);
       key.push_back(-1);
       key.push_back(nmega);
       key.push_back(mtiles);
@@ -1078,6 +1385,13 @@ rayon_status wave_set_option(rayon_ctx* ctx, int opt, long long value) {
       }
       W->opt_shade_chunk = (int)value;
       break;
+   case RAYON_OPT_TRACE_REFILL:
+      if (value < 1 || value > 64) {
+         set_error("RAYON_OPT_TRACE_REFILL must be in [1, 64]");
+         return RAYON_ERR_INVALID;
+      }
+      W->opt_refill = (int)value;
+      break;
    case RAYON_OPT_GRAPH:
       W->opt_graph = value != 0;
       break;
@@ -1102,6 +1416,7 @@ rayon_status wave_get_option(rayon_ctx* ctx, int opt, long long* value) {
    case RAYON_OPT_WAVE_WG_PER_CU: *value = W->opt_wg_per_cu; break;
    case RAYON_OPT_WAVE_CHUNK: *value = W->opt_chunk; break;
    case RAYON_OPT_SHADE_CHUNK: *value = W->opt_shade_chunk; break;
+   case RAYON_OPT_TRACE_REFILL: *value = W->opt_refill; break;
    case RAYON_OPT_GRAPH: *value = W->opt_graph; break;
    case RAYON_OPT_STAGE_TIMING: *value = W->opt_timing; break;
    case RAYON_OPT_COUNT_TRAVERSAL: *value = W->opt_count; break;
