@@ -108,6 +108,8 @@ def main(argv=None) -> int:
     ap.add_argument("--layout", choices=["binary", "quantized"], default="binary",
                     help="batch256/serve: BVH node layout of the scenes")
     ap.add_argument("--pass-spp", type=int, default=32, help="batch256: samples per batched pass")
+    ap.add_argument("--batch-path", choices=["auto", "wavefront", "megakernel"], default="auto",
+                    help="batch256: batched path of every scene (auto: measured per scene by the autotuner)")
     ap.add_argument("--lam", type=str, default="5,20,50", help="serve: arrival rates (jobs/s), comma separated")
     ap.add_argument("--duration", type=float, default=20.0, help="serve: seconds of arrivals per rate")
     ap.add_argument("--grace", type=float, default=20.0, help="serve: seconds to finish after arrivals stop")
@@ -441,6 +443,9 @@ def bench_batch256(a, ra) -> int:
         return [ev.elapsed_ms(e_start, evs[i]) for i in done_at]
 
     # ---- batched: autotune on clones, then progressive passes over all unfinished jobs
+    if a.batch_path != "auto":
+        for sc in scenes:
+            sc.set_batch_path(ra.PATH.WAVEFRONT if a.batch_path == "wavefront" else ra.PATH.MEGAKERNEL)
     jobs = [ctx.job(scenes[k], params[k][0]) for k in range(len(specs))]
     tune = ctx.autotune(jobs, 2)
     log(f"autotune: capacity {tune.capacity} chunk {tune.chunk} wg/CU {tune.wg_per_cu} "

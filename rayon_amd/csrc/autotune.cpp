@@ -133,8 +133,10 @@ extern "C" rayon_status rayon_autotune(rayon_ctx* ctx, rayon_job* const* jobs, i
       clones.push_back(c);
    }
    int timed = 0;
-   // batched path per scene left on AUTO: time both paths on that scene's jobs, with
-   // enough samples for about 4 M paths, and keep the faster one on the scene
+   // batched path per scene left on AUTO: time both paths on that scene's jobs at two
+   // sample counts (about 4 M and 8 M paths) and compare the added time per added sample.
+   // In the real pass every job shares the wavefront's fixed cost per level, so the
+   // marginal cost, not the cost of a pass of one scene alone, decides.
    {
       std::vector<rayon_scene*> done;
       for (int k = 0; k < n; ++k) {
@@ -150,16 +152,20 @@ extern "C" rayon_status rayon_autotune(rayon_ctx* ctx, rayon_job* const* jobs, i
                px += (long long)jobs[m]->params.width * jobs[m]->params.height;
                spp = std::min(spp, jobs[m]->params.spp);
             }
-         const int probe = (int)std::max<long long>(1, std::min<long long>({64, spp, (4ll << 20) / std::max(px, 1ll) + 1}));
-         double t_wave = 0, t_mega = 0;
-         rayon_status st = time_pass(ctx, sub, probe, &t_wave, RAYON_PATH_WAVEFRONT);
-         if (st == RAYON_OK) st = time_pass(ctx, sub, probe, &t_mega, RAYON_PATH_MEGAKERNEL);
-         timed += 2;
+         // the larger probe (2x) must stay within the jobs' sample budgets
+         const int probe = (int)std::max<long long>(
+             1, std::min<long long>({64, std::max(1, spp / 2), (4ll << 20) / std::max(px, 1ll) + 1}));
+         double w1 = 0, w2 = 0, m1 = 0, m2 = 0;
+         rayon_status st = time_pass(ctx, sub, probe, &w1, RAYON_PATH_WAVEFRONT);
+         if (st == RAYON_OK) st = time_pass(ctx, sub, 2 * probe, &w2, RAYON_PATH_WAVEFRONT);
+         if (st == RAYON_OK) st = time_pass(ctx, sub, probe, &m1, RAYON_PATH_MEGAKERNEL);
+         if (st == RAYON_OK) st = time_pass(ctx, sub, 2 * probe, &m2, RAYON_PATH_MEGAKERNEL);
+         timed += 4;
          if (st != RAYON_OK) {
             cleanup();
             return st;
          }
-         sc->batch_path = t_mega < t_wave ? RAYON_PATH_MEGAKERNEL : RAYON_PATH_WAVEFRONT;
+         sc->batch_path = (m2 - m1) < (w2 - w1) ? RAYON_PATH_MEGAKERNEL : RAYON_PATH_WAVEFRONT;
       }
       for (rayon_job* c : clones) c->path = -1;
    }
