@@ -95,7 +95,7 @@ struct Wave {
    int opt_chunk = 4;
    int opt_shade_chunk = 16;
    int opt_refill = 16;
-   int occ_trace = 0, occ_trace_wide = 0, occ_trace_short = 0, occ_shade = 0, occ_sdf = 0;
+   int occ_trace = 0, occ_trace_wide = 0, occ_trace_short = 0, occ_shade = 0, occ_sdf = 0, occ_sdf_rest = 0;
    struct Pending {
       int stage;
       hipEvent_t e0, e1;
@@ -385,6 +385,69 @@ __device__ __forceinline__ WavePool start_pool(unsigned* head, unsigned count) {
    if (count != 0 && lane_id() == 0) seen = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
    p.drained = __shfl(seen, 0) >= count;
    return p;
+}
+
+// SDF marching after the first slice: one persistent launch per level marches every ray
+// that slice 0 left unfinished to its end, from the state slice 0 saved, in bursts of
+// kSdfBurst steps; lanes whose march ended take the next rays of the queue (WavePool), so
+// a few long grazing marches do not hold whole waves, and no further slices are launched.
+constexpr int kSdfBurst = 4;
+
+template <bool CNT>
+__global__ __launch_bounds__(kWfBlock) void k_wf_sdf_rest(WfView V, int level) {
+   unsigned long long n_steps = 0, n_evals = 0, n_slots = 0;
+   unsigned* cin = V.sctl + 2 * ((kSdfSlices + 1) * level + 1);
+   const unsigned count = cin[0];
+   const unsigned step = 64u * (unsigned)V.chunk;
+   WavePool pool = start_pool(&cin[1], count);
+   int slot = -1;
+   const SceneView* S = nullptr;
+   V3 o, d;
+   SdfMarch m;
+   for (;;) {
+      if (!pool.drained || pool.next != pool.end) {
+         const long long idx = pool.refill(&cin[1], count, step, slot < 0, V.refill);
+         const int s = idx >= 0 ? V.qs1[idx] : -1;  // -1 also marks unused reserved positions
+         if (s >= 0) {
+            slot = s;
+            S = &V.jobs[V.ids[slot].x].scene;
+            o = mk(V.o[slot]);
+            d = mk(V.d[slot]);
+            const float4 a = V.so[slot], b = V.sc[slot];
+            m.tw = a.x;
+            m.tend = a.y;
+            m.cap = a.z;
+            m.used = __float_as_int(a.w);
+            m.mask = (uint64_t)__float_as_uint(b.x) | ((uint64_t)__float_as_uint(b.y) << 32);
+         }
+      }
+      if (__ballot(slot >= 0) == 0) {
+         if (pool.drained && pool.next == pool.end) break;
+         continue;
+      }
+      unsigned taken = 0;
+      if (slot >= 0) {
+         float t = 0.0f;
+         const int used0 = m.used;
+         const int r = sdf_march_run(*S, o, d, m, kSdfBurst, t);
+         if constexpr (CNT) {
+            taken = (unsigned)(m.used - used0 + (r >= 0 && m.used < S->sdf_steps ? 1 : 0));
+            n_steps += taken;
+            n_evals += (unsigned long long)taken * (S->num_sdf > kSdfMaskMax ? S->num_sdf : __popcll(m.mask));
+         }
+         if (r == 1) V.hit[slot] = make_float4(t, __int_as_float(S->sdf_prim), 0.0f, 0.0f);
+         if (r >= 0) slot = -1;
+      }
+      if constexpr (CNT) n_slots += 64ull * wave_max_u32(taken);
+   }
+   if constexpr (CNT) {
+      const unsigned long long b = wave_sum(n_steps), c = wave_sum(n_evals);
+      if (lane_id() == 0) {
+         atomicAdd(&V.stats[9], b);
+         atomicAdd(&V.stats[10], c);
+         atomicAdd(&V.stats[11], n_slots);
+      }
+   }
 }
 
 // Extension rays of one level: closest hit (SDF fields deferred to k_wf_sdf).
@@ -879,6 +942,8 @@ static rayon_status wave_reserve(rayon_ctx* ctx, int capacity, int levels, int n
       W->occ_shade = n > 0 ? n : 1;
       RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_sdf<false>, kWfBlock, 0));
       W->occ_sdf = n > 0 ? n : 1;
+      RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_sdf_rest<false>, kWfBlock, 0));
+      W->occ_sdf_rest = n > 0 ? n : 1;
    }
    if (W->capacity >= capacity && W->levels >= levels && W->max_jobs >= njobs) return RAYON_OK;
    // Three groups grow independently and geometrically (path slots, control words, job
@@ -1063,14 +1128,18 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
    int sdf_steps = 0;
    for (rayon_job* j : jobs)
       if (j->scene->view.num_sdf > 0) sdf_steps = std::max(sdf_steps, j->scene->view.sdf_steps);
-   int nslices = 0;
-   for (int covered = 0; covered < sdf_steps && nslices < kSdfSlices; ++nslices) covered += kSdfSlice << nslices;
-   const dim3 g_sdf(grid_of(W->occ_sdf));
+   // slice 0 (set-up and the first kSdfSlice steps of every march, all lanes busy), then one
+   // refilling launch that finishes the longer marches
+   const int nslices = sdf_steps <= 0 ? 0 : (sdf_steps <= kSdfSlice ? 1 : 2);
+   const dim3 g_sdf(grid_of(W->occ_sdf)), g_sdf_rest(grid_of(W->occ_sdf_rest));
    auto march = [&](int level) {
-      if (level < 0) return;
-      for (int it = 0; it < nslices; ++it)
-         if (W->opt_count) launch_stage(W, 6, k_wf_sdf<true>, g_sdf, blk, st, V, level, it, it == nslices - 1 ? 1 : 0);
-         else launch_stage(W, 6, k_wf_sdf<false>, g_sdf, blk, st, V, level, it, it == nslices - 1 ? 1 : 0);
+      if (level < 0 || nslices == 0) return;
+      const int last = nslices == 1 ? 1 : 0;
+      if (W->opt_count) launch_stage(W, 6, k_wf_sdf<true>, g_sdf, blk, st, V, level, 0, last);
+      else launch_stage(W, 6, k_wf_sdf<false>, g_sdf, blk, st, V, level, 0, last);
+      if (nslices < 2) return;
+      if (W->opt_count) launch_stage(W, 6, k_wf_sdf_rest<true>, g_sdf_rest, blk, st, V, level);
+      else launch_stage(W, 6, k_wf_sdf_rest<false>, g_sdf_rest, blk, st, V, level);
    };
    hipError_t e = hipSuccess;
    int dense = 1;
