@@ -1106,7 +1106,8 @@ oracle_scene* oracle_scene_create(const rayon_scene_desc* d) {
          p.lo = v3(std::min(p.lo.x, c.x - ex), std::min(p.lo.y, c.y - ey), std::min(p.lo.z, c.z - ez));
          p.hi = v3(std::max(p.hi.x, c.x + ex), std::max(p.hi.y, c.y + ey), std::max(p.hi.z, c.z + ez));
       }
-      const double pad = 0.25 * d->sdf_smooth_k + 2 * d->sdf_eps;
+      // the surface of an n-way smooth union lies within (n - 1) k / 4 of the primitives
+      const double pad = 0.25 * d->sdf_smooth_k * std::max(1, d->num_sdf_prims - 1) + 2 * d->sdf_eps;
       p.lo = p.lo - v3(pad, pad, pad);
       p.hi = p.hi + v3(pad, pad, pad);
       p.id = id++;

@@ -582,7 +582,10 @@ rayon_status gather_prims(const rayon_scene_desc* d, std::vector<HostPrim>& prim
             hi[k] = std::max(hi[k], (double)q.center[k] + e[k]);
          }
       }
-      const double pad = 0.25 * d->sdf_smooth_k + 2.0 * d->sdf_eps;
+      // Each smooth-union step lowers the running value by at most k/4 below the smaller of
+      // its two inputs, so the field of n primitives is at least (distance to the nearest
+      // primitive) - (n - 1) k / 4: its surface lies within that much of the primitives' boxes.
+      const double pad = 0.25 * d->sdf_smooth_k * std::max(1, d->num_sdf_prims - 1) + 2.0 * d->sdf_eps;
       for (int k = 0; k < 3; ++k) {
          p.lo[k] = round_down(lo[k] - pad);
          p.hi[k] = round_up(hi[k] + pad);
