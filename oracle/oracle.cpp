@@ -1273,6 +1273,30 @@ uint8_t oracle_gamma_u8(double sum, int64_t n, double gamma) {
    return (uint8_t)(256.0 * v);
 }
 
+// Surface colour of a material at point p of a sphere centred at `center` (the dots pattern).
+void oracle_surface_color(const rayon_material* m, const double p[3], const double center[3], double out[3]) {
+   const V3 c = surface_color(*m, dv(p), dv(center));
+   out[0] = c.x;
+   out[1] = c.y;
+   out[2] = c.z;
+}
+
+// Golf-ball sphere hit: t, displaced point and shading normal; returns 0 on a miss.
+int32_t oracle_golf_hit(const double c[3], double rad, const double o[3], const double d[3], double* t, double p[3],
+                        double n[3]) {
+   Ray r{dv(o), dv(d)};
+   Hit h;
+   if (!isect_golf(dv(c), rad, r, 0.001, 1e300, h)) return 0;
+   *t = h.t;
+   p[0] = h.p.x;
+   p[1] = h.p.y;
+   p[2] = h.p.z;
+   n[0] = h.normal.x;
+   n[1] = h.normal.y;
+   n[2] = h.normal.z;
+   return 1;
+}
+
 int32_t oracle_scatter_probe(const rayon_material* m, uint32_t flags, float fuzz, float glass_ior, float light,
                              const double dir[3], const double n[3], int32_t front, const double tangent[3],
                              const float* rnd, int32_t nrnd, double od[3], double at[3], double em[3],

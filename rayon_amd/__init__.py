@@ -14,7 +14,7 @@ import numpy as np
 from .abi import (  # noqa: F401  (re-exported)
     Camera, GenParams, Geometry, JobParams, JobStats, Material, Mesh, RenderStats, SceneDesc, SceneInfo, SdfPrim,
     WaveStats,
-    ServerConfig, ServerJobInfo, ServerStats, TuneResult, ProgressiveConfig, ProgressiveFrame, PATH, YAML, MAT, GEOM, SDF, FLAG, GEN, OPT, BVH, JOB, STATUS, declare,
+    ServerConfig, ServerJobInfo, ServerStats, TuneResult, ProgressiveConfig, ProgressiveFrame, PATH, YAML, MAT, GEOM, SDF, PATTERN, FLAG, GEN, OPT, BVH, JOB, STATUS, declare,
 )
 
 LIB_PATH = Path(__file__).resolve().parent / "librayon_hip.so"

@@ -20,6 +20,7 @@ MAT = _NS(LAMBERTIAN=0, METAL=1, MIRROR=2, ROUGH_MIRROR=3, GLASS=4, DIELECTRIC=5
           GGX_CONDUCTOR=13, GGX_DIELECTRIC=14)
 GEOM = _NS(SPHERE=0, RECTANGLE=1, DISPLACED_SPHERE=3, TRIANGLE=4)
 SDF = _NS(SPHERE=0, ROUNDED_BOX=1, TORUS=2)
+PATTERN = _NS(NONE=0, FIBONACCI_DOTS=1)
 FLAG = _NS(REF_QUIRKS=1, REF_RNG=2, NEE=4, ADAPTIVE_REF=8, ADAPTIVE_STDERR=16, DOF=32, COUNT_TRAVERSAL=64)
 GEN = _NS(BOOK_SPHERES=0, CORNELL_ICOSPHERE=1, SDF_TERRAIN=2)
 BVH = _NS(DEFAULT=0, QUANTIZED=1, DEVICE=2, BUILDER_HOST=0, BUILDER_DEVICE=1)

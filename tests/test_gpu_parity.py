@@ -332,3 +332,48 @@ def test_yaml_scene_renders_like_oracle(ctx, tmp_path):
         assert err <= 0.01
         job.close()
     sc.close()
+
+
+def test_material_gallery_renders_like_oracle(ctx):
+    """Thin film, clear coat, the golf-ball sphere and the Fibonacci-dots pattern (whose
+    oracle functions are pinned by closed forms in test_oracle_materials.py) render within
+    1% relative RMSE of the oracle on the same streams, plain and batched."""
+    import ctypes as C
+
+    from test_oracle import make_desc, mat, rect, sphere
+
+    log("material gallery")
+    mats = [
+        mat(ra.MAT.LAMBERTIAN, (0.6, 0.6, 0.55)),
+        mat(ra.MAT.LIGHT, emission=(5.0, 4.8, 4.5)),
+        mat(ra.MAT.THIN_FILM, (1, 1, 1), refractive_index=1.0, film_ior=1.33, film_thickness=420.0),
+        mat(ra.MAT.CLEAR_COAT, (0.7, 0.1, 0.1), refractive_index=1.5, roughness=0.15),
+        mat(ra.MAT.LAMBERTIAN, (0.85, 0.85, 0.85)),
+        mat(ra.MAT.LAMBERTIAN, (0.2, 0.4, 0.8), pattern=ra.PATTERN.FIBONACCI_DOTS, pattern_color=(0.9, 0.8, 0.1),
+            pattern_param1=40.0, pattern_param2=0.2),
+    ]
+    golf = sphere((0.5, 0.45, 0.2), 0.45, 4)
+    golf.type = ra.GEOM.DISPLACED_SPHERE
+    geoms = [rect((-4, 0, -4), (8, 0, 0), (0, 0, 8), 0), rect((-1, 3, -1), (2, 0, 0), (0, 0, 2), 1),
+             sphere((-1.5, 0.5, 0), 0.5, 2), sphere((-0.5, 0.45, -0.4), 0.45, 3), golf,
+             sphere((1.5, 0.5, -0.2), 0.5, 5)]
+    desc, keep = make_desc(mats, geoms)
+    dp = C.pointer(desc)
+    sc = ctx.scene(dp)
+    W, H, spp = 64, 48, 64
+    cam = ra.camera_frame((0, 1.6, 4.0), (0, 0.4, 0), (0, 1, 0), 45.0, W, H)
+    P = ra.job_params(W, H, cam, spp=spp, max_depth=8, seed=21, flags=ra.FLAG.NEE, background_intensity=0.5)
+    sums, _ = OracleScene(dp).render(P, nsamples=spp, camera=cam)
+    ref = sums / spp
+    for path_kind in ("plain", "batched"):
+        job = ctx.job(sc, P)
+        if path_kind == "plain":
+            job.render_plain(spp)
+        else:
+            ctx.render_batched([job], spp)
+        err = rel_rmse(job.resolve_linear().reshape(-1, 3).astype(np.float64), ref)
+        log(f"  {path_kind}: rel_rmse={err:.5f}")
+        assert err <= 0.01
+        job.close()
+    sc.close()
+    del keep
