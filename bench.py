@@ -114,6 +114,8 @@ def main(argv=None) -> int:
     ap.add_argument("--duration", type=float, default=20.0, help="serve: seconds of arrivals per rate")
     ap.add_argument("--grace", type=float, default=20.0, help="serve: seconds to finish after arrivals stop")
     ap.add_argument("--pool", type=int, default=4, help="serve: resident scenes per generator")
+    ap.add_argument("--target-pass-ms", type=float, default=0.0,
+                    help="serve: refinement pass duration the server aims for (0: library default)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device-to-device 4 times (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     a = ap.parse_args(argv)
@@ -598,7 +600,7 @@ def bench_serve(a, ra) -> int:
                 P, _ = job_params_for(ra, g, sp, extra_flags=ra.FLAG.ADAPTIVE_STDERR, min_adaptive_samples=16,
                                       adaptive_threshold=float(rng.uniform(0.005, 0.02)))
             plist.append((sc, P))
-        srv = ctx.server()
+        srv = ctx.server(**({"target_pass_ms": a.target_pass_ms} if a.target_pass_ms > 0 else {}))
         ids = []
         t0 = time.perf_counter()
         next_log = 5.0

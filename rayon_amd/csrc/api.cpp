@@ -535,14 +535,17 @@ rayon_status rayon_scene_set_batch_path(rayon_scene* scene, int32_t path) {
 int32_t rayon_scene_batch_path(const rayon_scene* scene) { return scene ? ry::scene_batch_path(scene) : -1; }
 
 rayon_status rayon_job_create(rayon_ctx* ctx, rayon_scene* sc, const rayon_job_params* p, rayon_job** out) {
-   return ry::job_create(ctx, sc, p, out, true);
+   return ry::job_create(ctx, sc, p, out, true, true);
 }
 
 }  // extern "C"
 
 // sync = false leaves the buffers' allocation and clears queued on the context's stream:
-// only for callers that order every later use after that stream.
-rayon_status ry::job_create(rayon_ctx* ctx, rayon_scene* sc, const rayon_job_params* p, rayon_job** out, bool sync) {
+// only for callers that order every later use after that stream. clear = false leaves the
+// accumulation buffers and counters uncleared: the caller clears them (job_clear_ranges)
+// before the job's first pass or statistics read.
+rayon_status ry::job_create(rayon_ctx* ctx, rayon_scene* sc, const rayon_job_params* p, rayon_job** out, bool sync,
+                            bool clear) {
    if (!ctx || !sc || !p || !out) {
       set_error("rayon_job_create: NULL argument");
       return RAYON_ERR_INVALID;
@@ -581,7 +584,7 @@ rayon_status ry::job_create(rayon_ctx* ctx, rayon_scene* sc, const rayon_job_par
    j->view.rng = (uint32_t*)rng;
    j->view.counters = (unsigned long long*)ctr;
    fill_jobview(j);
-   st = rayon_job_reset(j);
+   st = clear ? rayon_job_reset(j) : RAYON_OK;
    // the buffers are valid for every stream (and host copies) once the clears have run
    if (st == RAYON_OK && sync && hipStreamSynchronize(ctx->stream) != hipSuccess)
       st = hip_fail(hipGetLastError(), "job set-up");
@@ -592,6 +595,21 @@ rayon_status ry::job_create(rayon_ctx* ctx, rayon_scene* sc, const rayon_job_par
    }
    *out = j;
    return RAYON_OK;
+}
+
+// The buffers rayon_job_reset clears, as (pointer, 32-bit words) ranges; returns how many
+// (4, or 5 with a reference RNG stream, which is set by its own kernel instead).
+int ry::job_clear_ranges(const rayon_job* j, uint32_t** p, long long* n) {
+   const long long npix = (long long)j->params.width * j->params.height;
+   p[0] = (uint32_t*)j->view.acc;
+   n[0] = npix * 4;
+   p[1] = (uint32_t*)j->view.comp;
+   n[1] = npix * 4;
+   p[2] = (uint32_t*)j->view.counts;
+   n[2] = npix;
+   p[3] = (uint32_t*)j->view.counters;
+   n[3] = 2 * 8 * kCounterSlots;
+   return 4;
 }
 
 extern "C" {

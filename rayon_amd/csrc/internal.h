@@ -121,5 +121,7 @@ struct rayon_job {
 namespace ry {
 void fill_jobview(rayon_job* j);
 int scene_batch_path(const rayon_scene* s);
-rayon_status job_create(rayon_ctx* ctx, rayon_scene* sc, const rayon_job_params* p, rayon_job** out, bool sync);
+rayon_status job_create(rayon_ctx* ctx, rayon_scene* sc, const rayon_job_params* p, rayon_job** out, bool sync,
+                        bool clear);
+int job_clear_ranges(const rayon_job* j, uint32_t** p, long long* n);
 }

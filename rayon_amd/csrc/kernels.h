@@ -8,11 +8,32 @@
 
 namespace ry {
 
+// First-frame launches: up to kPreviewJobs jobs per launch, their views in the kernel
+// arguments (8 x 312 bytes, within the 4 KiB argument limit).
+constexpr int kPreviewJobs = 8;
+struct PreviewJob {
+   JobView view;
+   uint8_t* frame;  // RGBA8, width x height
+   int first_tile;  // first 16x16 tile of this job in the launch
+   int pad_;
+};
+struct PreviewBatch {
+   PreviewJob job[kPreviewJobs];
+   int njobs, ntiles, per_xcd, samples;
+   float inv_gamma;
+};
+constexpr int kFillRanges = 32;
+struct FillRanges {
+   uint32_t* p[kFillRanges];
+   long long n[kFillRanges];
+};
+
 hipError_t launch_plain(const JobView& J, int samples, hipStream_t st);
-// All tiles of `njobs` jobs in one launch (device tables: job views, tile prefix with
-// njobs + 1 entries, samples per job); layout: 0 binary, 1 wide, 2 mixed.
-hipError_t launch_multi(const JobView* jobs, const int* tile_base, const int* todo, int njobs, int total_tiles,
-                        bool count, int layout, hipStream_t st);
+// All tiles of `njobs` jobs (device tables: job views, tile prefix with njobs + 1 entries,
+// samples per job); the first nbin jobs (bin_tiles tiles) have binary trees, the rest
+// quantized ones, and each group runs in a launch of its own layout.
+hipError_t launch_multi(const JobView* jobs, const int* tile_base, const int* todo, int nbin, int bin_tiles,
+                        int njobs, int total_tiles, bool count, hipStream_t st);
 int plain_tiles(int width, int height);
 hipError_t launch_resolve_u8(const JobView& J, uint8_t* out, int channels, float gamma, hipStream_t st);
 hipError_t launch_resolve_linear(const JobView& J, float* out, hipStream_t st);
@@ -24,5 +45,11 @@ hipError_t launch_resolve_u8_raw(const float4* acc, const float4* comp, const in
                                  uint8_t* out, int channels, float gamma, hipStream_t st);
 hipError_t launch_heatmap(const int* counts, int npix, uint8_t* out, int channels, int max_samples, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t st);
+// One launch per kPreviewJobs jobs: `samples` paths per pixel from sample 0, resolved with
+// `gamma` into each job's RGBA8 frame (no accumulation buffers are touched).
+hipError_t launch_preview(const JobView* views, uint8_t* const* frames, int n, int samples, float gamma,
+                          hipStream_t st);
+// Sets `count` buffers of n[k] 32-bit words to v, kFillRanges buffers per launch.
+hipError_t launch_fill_ranges(uint32_t* const* p, const long long* n, int count, uint32_t v, hipStream_t st);
 
 }  // namespace ry

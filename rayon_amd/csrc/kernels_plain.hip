@@ -8,6 +8,9 @@
 // L2 sees spatially coherent BVH traffic.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "kernels.h"
 #include "pathtrace.h"
 
@@ -92,11 +95,13 @@ __global__ __launch_bounds__(kBlock) void k_plain(JobView J, int samples, int ti
 // holds each job's first tile, njobs + 1 entries), and each workgroup finds its job by a
 // binary search over that prefix. The per-pixel arithmetic is the plain kernel's, so a job
 // gets the same bits from this launch as from its own.
+// tile0 is the first tile of this launch: jobs whose trees have another layout run in a
+// launch of their own over the rest of the same tables (tile_base holds absolute tiles).
 template <bool CNT, int LAYOUT>
 __global__ __launch_bounds__(kBlock) void k_multi(const JobView* __restrict__ jobs, const int* __restrict__ tile_base,
-                                                  const int* __restrict__ todo, int njobs, int per_xcd) {
+                                                  const int* __restrict__ todo, int njobs, int per_xcd, int tile0) {
    __shared__ int stack[kStack * kBlock];
-   const int tile = xcd_tile(blockIdx.x, per_xcd);
+   const int tile = tile0 + xcd_tile(blockIdx.x, per_xcd);
    if (tile >= tile_base[njobs]) return;
    int lo = 0, hi = njobs - 1;
    while (lo < hi) {
@@ -109,22 +114,69 @@ __global__ __launch_bounds__(kBlock) void k_multi(const JobView* __restrict__ jo
    plain_tile<CNT, LAYOUT>(J, todo[lo], tiles_x, tile - tile_base[lo], blockIdx.x, stack);
 }
 
-// gammaCorrectKernel semantics (render_acc_kernel.cu:11-44) on the compensated sums.
-__global__ void k_resolve_u8(JobView J, uint8_t* out, int channels, float inv_gamma, int n_total) {
-   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
-   if (pix >= J.width * J.height) return;
-   const float4 a = J.acc[pix], c = J.comp[pix];
-   int n = J.counts ? abs(J.counts[pix]) : n_total;
+// gammaCorrectKernel semantics (render_acc_kernel.cu:11-44) on the compensated sums of
+// n samples.
+__device__ __forceinline__ void resolve_px(float4 a, float4 c, int n, float inv_gamma, uint8_t* o, int channels) {
    if (n <= 0) n = 1;
    const float inv = 1.0f / (float)n;
    const float r = fminf(powf(fmaxf((a.x + c.x) * inv, 0.0f), inv_gamma), 0.999f);
    const float g = fminf(powf(fmaxf((a.y + c.y) * inv, 0.0f), inv_gamma), 0.999f);
    const float b = fminf(powf(fmaxf((a.z + c.z) * inv, 0.0f), inv_gamma), 0.999f);
-   uint8_t* o = out + (size_t)pix * channels;
    o[0] = (uint8_t)(256.0f * r);
    o[1] = (uint8_t)(256.0f * g);
    o[2] = (uint8_t)(256.0f * b);
    if (channels == 4) o[3] = 255;
+}
+
+__global__ void k_resolve_u8(JobView J, uint8_t* out, int channels, float inv_gamma, int n_total) {
+   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+   if (pix >= J.width * J.height) return;
+   resolve_px(J.acc[pix], J.comp[pix], J.counts ? abs(J.counts[pix]) : n_total, inv_gamma,
+              out + (size_t)pix * channels, channels);
+}
+
+// First frames of up to kPreviewJobs jobs in one launch. The job table travels in the
+// kernel arguments and the resolve is fused, so a first frame is this one kernel: no table
+// upload, no cleared accumulation buffers, no separate resolve, nothing that would wait
+// for a free compute unit between steps while another stream's kernels hold the chip.
+// Each lane runs `samples` paths of its pixel from sample 0 in sample order, folds them
+// into compensated sums that start at zero, and writes the RGBA8 pixel: the bits of the
+// plain kernel's `samples`-sample render of the job resolved by k_resolve_u8.
+template <int LAYOUT>
+__global__ __launch_bounds__(kBlock) void k_preview(const PreviewBatch B) {
+   __shared__ int stack[kStack * kBlock];
+   const int tile = xcd_tile(blockIdx.x, B.per_xcd);
+   if (tile >= B.ntiles) return;
+   int k = 0;
+   while (k + 1 < B.njobs && B.job[k + 1].first_tile <= tile) ++k;
+   const JobView& J = B.job[k].view;
+   const int tiles_x = (J.width + kTile - 1) / kTile;
+   const int t = tile - B.job[k].first_tile;
+   const int tid = threadIdx.x;
+   const int x = (t % tiles_x) * kTile + (tid & (kTile - 1));
+   const int y = (t / tiles_x) * kTile + (tid / kTile);
+   if (x >= J.width || y >= J.height) return;
+   const int pix = y * J.width + x;
+   PathCounts pc;
+   float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c = a;
+   Rng R{0u, false};
+   for (int s = 0; s < B.samples; ++s) {
+      R.s = counter_init(J.seed_lo, J.seed_hi, (uint32_t)pix, (uint32_t)s);
+      V3 o, d;
+      camera_ray(J, x, y, R, o, d);
+      const V3 L = trace_path<kBlock, false, LAYOUT>(J, o, d, R, stack + tid, pc);
+      accumulate(a, c, L);
+   }
+   resolve_px(a, c, B.samples, B.inv_gamma, B.job[k].frame + (size_t)pix * 4, 4);
+}
+
+// One launch clears up to kFillRanges buffers (words of 32 bits): blockIdx.y picks the
+// buffer, the x blocks stride over it.
+__global__ void k_fill_ranges(const FillRanges R, uint32_t v) {
+   uint32_t* p = R.p[blockIdx.y];
+   const long long n = R.n[blockIdx.y];
+   const long long stride = (long long)gridDim.x * blockDim.x;
+   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
 __global__ void k_resolve_linear(JobView J, float* out) {
@@ -247,20 +299,29 @@ hipError_t launch_plain(const JobView& J, int samples, hipStream_t st) {
    return hipGetLastError();
 }
 
-hipError_t launch_multi(const JobView* jobs, const int* tile_base, const int* todo, int njobs, int total_tiles,
-                        bool count, int layout, hipStream_t st) {
+hipError_t launch_multi(const JobView* jobs, const int* tile_base, const int* todo, int nbin, int bin_tiles,
+                        int njobs, int total_tiles, bool count, hipStream_t st) {
    if (njobs <= 0 || total_tiles <= 0) return hipSuccess;
-   const int per = (total_tiles + 7) / 8;
-   const dim3 g(8 * per), b(kBlock);
-   if (count)
-      hipLaunchKernelGGL((k_multi<true, L_ANY>), g, b, 0, st, jobs, tile_base, todo, njobs, per);
-   else if (layout == L_BIN)
-      hipLaunchKernelGGL((k_multi<false, L_BIN>), g, b, 0, st, jobs, tile_base, todo, njobs, per);
-   else if (layout == L_WIDE)
-      hipLaunchKernelGGL((k_multi<false, L_WIDE>), g, b, 0, st, jobs, tile_base, todo, njobs, per);
-   else
-      hipLaunchKernelGGL((k_multi<false, L_ANY>), g, b, 0, st, jobs, tile_base, todo, njobs, per);
-   return hipGetLastError();
+   const dim3 b(kBlock);
+   if (count) {
+      const int per = (total_tiles + 7) / 8;
+      hipLaunchKernelGGL((k_multi<true, L_ANY>), dim3(8 * per), b, 0, st, jobs, tile_base, todo, njobs, per, 0);
+      return hipGetLastError();
+   }
+   // one launch per tree layout, so a job runs the same kernel whatever it is batched with
+   if (nbin > 0) {
+      const int per = (bin_tiles + 7) / 8;
+      hipLaunchKernelGGL((k_multi<false, L_BIN>), dim3(8 * per), b, 0, st, jobs, tile_base, todo, nbin, per, 0);
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+   }
+   if (njobs > nbin) {
+      const int per = (total_tiles - bin_tiles + 7) / 8;
+      hipLaunchKernelGGL((k_multi<false, L_WIDE>), dim3(8 * per), b, 0, st, jobs + nbin, tile_base + nbin, todo + nbin,
+                         njobs - nbin, per, bin_tiles);
+      return hipGetLastError();
+   }
+   return hipSuccess;
 }
 
 int plain_tiles(int width, int height) { return ((width + kTile - 1) / kTile) * ((height + kTile - 1) / kTile); }
@@ -269,6 +330,59 @@ hipError_t launch_resolve_u8(const JobView& J, uint8_t* out, int channels, float
    const int n = J.width * J.height;
    hipLaunchKernelGGL(k_resolve_u8, dim3((n + 255) / 256), dim3(256), 0, st, J, out, channels, 1.0f / gamma, 1);
    return hipGetLastError();
+}
+
+hipError_t launch_preview(const JobView* views, uint8_t* const* frames, int n, int samples, float gamma,
+                          hipStream_t st) {
+   // binary-tree jobs first, then quantized-tree jobs, kPreviewJobs of one layout per launch
+   std::vector<int> order;
+   for (int wide = 0; wide < 2; ++wide)
+      for (int k = 0; k < n; ++k)
+         if ((views[k].scene.wnodes != nullptr) == (wide != 0)) order.push_back(k);
+   for (size_t base = 0; base < order.size();) {
+      const bool wide = views[order[base]].scene.wnodes != nullptr;
+      PreviewBatch B{};
+      B.samples = samples;
+      B.inv_gamma = 1.0f / gamma;
+      while (base < order.size() && B.njobs < kPreviewJobs && (views[order[base]].scene.wnodes != nullptr) == wide) {
+         const int k = order[base++];
+         const JobView& J = views[k];
+         if ((J.flags & RAYON_FLAG_REF_RNG) || J.width <= 0 || J.height <= 0 || !frames[k]) return hipErrorInvalidValue;
+         B.job[B.njobs].view = J;
+         B.job[B.njobs].frame = frames[k];
+         B.job[B.njobs].first_tile = B.ntiles;
+         B.ntiles += plain_tiles(J.width, J.height);
+         ++B.njobs;
+      }
+      B.per_xcd = (B.ntiles + 7) / 8;
+      const dim3 g(8 * B.per_xcd), b(kBlock);
+      if (wide)
+         hipLaunchKernelGGL((k_preview<L_WIDE>), g, b, 0, st, B);
+      else
+         hipLaunchKernelGGL((k_preview<L_BIN>), g, b, 0, st, B);
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+   }
+   return hipSuccess;
+}
+
+hipError_t launch_fill_ranges(uint32_t* const* p, const long long* n, int count, uint32_t v, hipStream_t st) {
+   for (int base = 0; base < count; base += kFillRanges) {
+      FillRanges R{};
+      long long most = 0;
+      const int m = std::min(kFillRanges, count - base);
+      for (int k = 0; k < m; ++k) {
+         R.p[k] = p[base + k];
+         R.n[k] = n[base + k];
+         most = std::max(most, n[base + k]);
+      }
+      if (most <= 0) continue;
+      const dim3 g((unsigned)std::min<long long>((most + 255) / 256, 1024), m), b(256);
+      hipLaunchKernelGGL(k_fill_ranges, g, b, 0, st, R, v);
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+   }
+   return hipSuccess;
 }
 
 hipError_t launch_resolve_linear(const JobView& J, float* out, hipStream_t st) {

@@ -54,7 +54,11 @@ struct SJob {
    bool cancel = false;
    bool released = false;
    bool accounted = false;  // bytes (and a scene reference) are held in the budget
+   bool needs_clear = false;  // accumulation buffers not cleared yet (before the first pass)
+   bool previewing = false;   // its first frame is being rendered
 };
+
+constexpr int kPreviewStreams = 2;
 
 int64_t job_bytes(const rayon_job_params& p) {
    const int64_t npix = (int64_t)p.width * p.height;
@@ -67,7 +71,9 @@ int64_t job_bytes(const rayon_job_params& p) {
 
 struct rayon_server {
    rayon_ctx* ctx = nullptr;
-   rayon_ctx* first_ctx = nullptr;  // first passes, on a stream and wavefront state of their own
+   rayon_ctx* first_ctx = nullptr;  // job and first-frame allocations, on a stream of their own
+   hipStream_t pv_stream[kPreviewStreams] = {};  // first-frame kernels
+   int next_pv = 0;
    rayon_server_config cfg{};
    std::thread th;
    std::mutex mu;
@@ -162,15 +168,37 @@ struct rayon_server {
       return finished;
    }
 
-   // The server thread. First frames run on a context of their own (first_ctx), so they
-   // overlap the refinement pass in flight on the main context instead of waiting for it:
-   //   - admit queued jobs, create them, run their first pass and first-frame resolve on
-   //     first_ctx, wait for that alone, stamp time-to-first-frame;
-   //   - when no refinement pass is in flight, start one over the oldest ready jobs;
+   // First frames in flight: one launch on a preview stream, and the event after it.
+   struct FirstBatch {
+      std::vector<SJob*> jobs;
+      hipEvent_t ready = nullptr, done = nullptr;
+      double t0 = 0;
+   };
+   std::vector<hipEvent_t> event_pool;
+   hipEvent_t take_event() {
+      hipEvent_t e = nullptr;
+      if (!event_pool.empty()) {
+         e = event_pool.back();
+         event_pool.pop_back();
+      } else if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+         e = nullptr;
+      }
+      return e;
+   }
+
+   // The server thread. First frames run on streams of their own, so they overlap the
+   // refinement pass in flight on the main context instead of waiting for it, and the
+   // thread does not wait for them either:
+   //   - admit queued jobs, create them, launch their first frames on the next preview
+   //     stream (no wait); a job's time-to-first-frame is stamped when the thread sees
+   //     that launch complete;
+   //   - when no refinement pass is in flight, start one over the oldest jobs that have
+   //     a first frame;
    //   - when the pass in flight has completed, check its jobs and steer the next size.
    void loop() {
       (void)hipSetDevice(ctx->device);
       std::vector<SJob*> in_flight;
+      std::vector<FirstBatch> firsts;
       bool flying = false;
       double t_pass = 0;
       hipEvent_t pass_done = nullptr;
@@ -179,7 +207,7 @@ struct rayon_server {
          std::vector<SJob*> fresh;
          {
             std::unique_lock<std::mutex> lk(mu);
-            if (flying)  // wake up for arrivals, or soon to look at the pass in flight
+            if (flying || !firsts.empty())  // wake up for arrivals, or soon to look at the GPU work
                cv.wait_for(lk, std::chrono::microseconds(200), [&] { return stop || !queue.empty(); });
             else
                cv.wait(lk, [&] { return stop || !queue.empty() || !active.empty(); });
@@ -187,9 +215,9 @@ struct rayon_server {
             // retire cancelled jobs that no pass is using
             for (auto it = active.begin(); it != active.end();) {
                SJob* j = *it;
-               if (j->cancel && std::find(in_flight.begin(), in_flight.end(), j) == in_flight.end()) {
+               if (j->cancel && !j->previewing && std::find(in_flight.begin(), in_flight.end(), j) == in_flight.end()) {
                   rayon_job_stats js;
-                  if (j->job && rayon_job_stats_get(ctx, j->job, &js) == RAYON_OK) {
+                  if (j->job && !j->needs_clear && rayon_job_stats_get(ctx, j->job, &js) == RAYON_OK) {
                      j->samples = js.samples_done;
                      j->segments = js.segments;
                   }
@@ -224,15 +252,18 @@ struct rayon_server {
                active.push_back(j);  // counted as active from here, so drain() waits for it
             }
          }
-         // (1) new jobs: buffers, first pass and first frame on first_ctx
+         // (1) new jobs: buffers on first_ctx, first frames on a preview stream
          if (!fresh.empty()) {
             const double t0 = now_s();
             std::vector<SJob*> ok_fresh;
             for (SJob* j : fresh) {
-               // created on first_ctx: its allocations and clears must not queue behind
-               // the refinement pass in flight on the main stream
-               // (no wait here: the previews' synchronisation below orders every later use)
-               rayon_status st = ry::job_create(first_ctx, j->scene, &j->params, &j->job, false);
+               // created on first_ctx: its allocations must not queue behind the refinement
+               // pass in flight on the main stream. The preview stream waits for them, and
+               // every other use follows the host's sight of the preview's completion. The
+               // buffers are cleared on the main stream just before the job's first
+               // refinement pass, so the first frame waits for nothing but its own kernel.
+               rayon_status st = ry::job_create(first_ctx, j->scene, &j->params, &j->job, false, false);
+               j->needs_clear = st == RAYON_OK;
                if (st == RAYON_OK && hipMallocAsync((void**)&j->frame, (size_t)j->params.width * j->params.height * 4,
                                                     first_ctx->stream) != hipSuccess) {
                   ry::set_error("allocation of the first-frame buffer failed");
@@ -254,47 +285,65 @@ struct rayon_server {
                ok_fresh.push_back(j);
             }
             if (!ok_fresh.empty()) {
-               // The first frame is a preview: the job's first samples rendered by the
-               // one-launch plain kernel into a scratch job and resolved to RGBA8. It does
-               // not enter the job's sums (its samples arrive with the refinement passes),
-               // so a served job's result equals the job rendered alone.
-               // All new jobs' previews share one launch (the multi-job form of the plain
-               // kernel, which gives the plain kernel's bits).
-               bool ok = true;
-               std::vector<rayon_job*> pvs;
+               // The first frame is a preview: the job's first samples rendered and resolved
+               // to RGBA8 by one launch of the preview kernel for all new jobs (the plain
+               // kernel's per-pixel arithmetic, tables in the kernel arguments). It does not
+               // enter the job's sums (its samples arrive with the refinement passes), so a
+               // served job's result equals the job rendered alone. The preview stream
+               // waits for the allocations made on first_ctx's stream.
+               FirstBatch fb;
+               fb.jobs = ok_fresh;
+               fb.t0 = t0;
+               fb.ready = take_event();
+               fb.done = take_event();
+               hipStream_t ps = pv_stream[next_pv++ % kPreviewStreams];
+               std::vector<ry::JobView> views;
+               std::vector<uint8_t*> frames;
                for (SJob* j : ok_fresh) {
-                  rayon_job_params pp = j->params;
-                  pp.spp = cfg.first_pass_samples;
-                  pp.flags &= ~(uint32_t)(RAYON_FLAG_ADAPTIVE_REF | RAYON_FLAG_ADAPTIVE_STDERR);
-                  rayon_job* pv = nullptr;
-                  ok = ok && ry::job_create(first_ctx, j->scene, &pp, &pv, false) == RAYON_OK;
-                  if (!pv) break;
-                  pv->path = RAYON_PATH_MEGAKERNEL;
-                  pvs.push_back(pv);
+                  views.push_back(j->job->view);
+                  frames.push_back(j->frame);
+                  j->previewing = true;
                }
-               ok = ok && rayon_render_batched(first_ctx, pvs.data(), (int)pvs.size(), cfg.first_pass_samples) == RAYON_OK;
-               for (size_t k = 0; ok && k < pvs.size(); ++k)
-                  ok = ry::launch_resolve_u8(pvs[k]->view, ok_fresh[k]->frame, 4, 2.0f, first_ctx->stream) == hipSuccess;
-               ok = ok && hipStreamSynchronize(first_ctx->stream) == hipSuccess;
-               for (rayon_job* pv : pvs) rayon_job_destroy(pv);
-               const double t = now_s();
-               std::lock_guard<std::mutex> lk(mu);
-               ++first_passes;
-               busy_s += t - t0;
-               first_ms_sum += (t - t0) * 1e3;
-               for (SJob* j : ok_fresh) {
-                  if (!ok) {
-                     j->state = RAYON_JOB_FAILED;
-                     ++failed;
-                  } else {
-                     j->t_first = t;
-                  }
+               bool ok = fb.ready && fb.done && hipEventRecord(fb.ready, first_ctx->stream) == hipSuccess &&
+                         hipStreamWaitEvent(ps, fb.ready, 0) == hipSuccess &&
+                         ry::launch_preview(views.data(), frames.data(), (int)views.size(), cfg.first_pass_samples,
+                                            2.0f, ps) == hipSuccess &&
+                         hipEventRecord(fb.done, ps) == hipSuccess;
+               if (!ok) {
+                  (void)hipStreamSynchronize(ps);
+                  fb.done = nullptr;  // completes below as failed
                }
-               std::vector<SJob*> failed_now;
-               for (SJob* j : ok_fresh)
-                  if (j->state == RAYON_JOB_FAILED) failed_now.push_back(j);
-               finish_locked(failed_now);
+               firsts.push_back(std::move(fb));
             }
+         }
+         // (1b) first frames that have completed
+         for (size_t k = 0; k < firsts.size();) {
+            FirstBatch& fb = firsts[k];
+            const hipError_t q = fb.done ? hipEventQuery(fb.done) : hipErrorInvalidValue;
+            if (q == hipErrorNotReady) {
+               ++k;
+               continue;
+            }
+            const double t = now_s();
+            std::lock_guard<std::mutex> lk(mu);
+            if (q != hipSuccess) last_error = std::string("first frame: ") + hipGetErrorString(q);
+            ++first_passes;
+            first_ms_sum += (t - fb.t0) * 1e3;
+            std::vector<SJob*> failed_now;
+            for (SJob* j : fb.jobs) {
+               j->previewing = false;
+               if (q == hipSuccess) {
+                  j->t_first = t;
+               } else {
+                  j->state = RAYON_JOB_FAILED;
+                  ++failed;
+                  failed_now.push_back(j);
+               }
+            }
+            finish_locked(failed_now);
+            for (hipEvent_t e : {fb.ready, fb.done})
+               if (e) event_pool.push_back(e);
+            firsts.erase(firsts.begin() + (long)k);
          }
          // (2) the refinement pass in flight, once complete
          if (flying && hipEventQuery(pass_done) == hipSuccess) {
@@ -336,9 +385,24 @@ struct rayon_server {
             }
             if (!run.empty()) {
                std::vector<rayon_job*> h;
-               for (SJob* j : run) h.push_back(j->job);
+               std::vector<uint32_t*> cp;
+               std::vector<long long> cn;
+               for (SJob* j : run) {
+                  h.push_back(j->job);
+                  if (j->needs_clear) {
+                     uint32_t* p[8];
+                     long long n[8];
+                     const int m = ry::job_clear_ranges(j->job, p, n);
+                     cp.insert(cp.end(), p, p + m);
+                     cn.insert(cn.end(), n, n + m);
+                  }
+               }
                t_pass = now_s();
-               if (rayon_render_batched(ctx, h.data(), (int)h.size(), pass_samples) != RAYON_OK ||
+               const bool cleared =
+                   cp.empty() || ry::launch_fill_ranges(cp.data(), cn.data(), (int)cp.size(), 0u, ctx->stream) == hipSuccess;
+               if (cleared)
+                  for (SJob* j : run) j->needs_clear = false;
+               if (!cleared || rayon_render_batched(ctx, h.data(), (int)h.size(), pass_samples) != RAYON_OK ||
                    hipEventRecord(pass_done, ctx->stream) != hipSuccess) {
                   fail_all(run, "batched pass");
                   (void)rayon_ctx_sync(ctx);
@@ -352,6 +416,14 @@ struct rayon_server {
          }
       }
       (void)rayon_ctx_sync(ctx);
+      for (hipStream_t ps : pv_stream) (void)hipStreamSynchronize(ps);
+      for (FirstBatch& fb : firsts) {
+         for (SJob* j : fb.jobs) j->previewing = false;
+         for (hipEvent_t e : {fb.ready, fb.done})
+            if (e) event_pool.push_back(e);
+      }
+      for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
+      event_pool.clear();
       (void)hipEventDestroy(pass_done);
    }
 };
@@ -389,6 +461,15 @@ rayon_status rayon_server_create(rayon_ctx* ctx, const rayon_server_config* cfg,
       delete s;
       return RAYON_ERR_HIP;
    }
+   for (hipStream_t& ps : s->pv_stream)
+      if (hipStreamCreateWithFlags(&ps, hipStreamNonBlocking) != hipSuccess) {
+         ry::set_error("rayon_server_create: hipStreamCreateWithFlags failed");
+         for (hipStream_t q : s->pv_stream)
+            if (q) (void)hipStreamDestroy(q);
+         rayon_ctx_destroy(s->first_ctx);
+         delete s;
+         return RAYON_ERR_HIP;
+      }
    // passes change their job set every step: replaying captured graphs would not pay
    rayon_ctx_set_option(ctx, RAYON_OPT_GRAPH, 0);
    rayon_ctx_set_option(s->first_ctx, RAYON_OPT_GRAPH, 0);
@@ -559,6 +640,7 @@ rayon_status rayon_server_destroy(rayon_server* s) {
       delete kv.second;
    }
    (void)rayon_ctx_sync(s->first_ctx);
+   for (hipStream_t ps : s->pv_stream) (void)hipStreamDestroy(ps);
    rayon_ctx_destroy(s->first_ctx);
    delete s;
    return RAYON_OK;

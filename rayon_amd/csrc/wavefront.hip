@@ -1064,15 +1064,15 @@ void launch_stage(Wave* W, int stage, K kernel, dim3 grid, dim3 block, hipStream
 
 // Enqueues every launch of one pass (the sequence a graph captures).
 static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jobs, const std::vector<int>& todo,
-                               long long total, int levels, int nmega, int mega_tiles, int mega_layout) {
+                               long long total, int levels, int nmega, int mega_tiles, int mbin, int mbin_tiles) {
    Wave* W = ctx->wave;
    hipStream_t st = ctx->stream;
    if (nmega > 0) {
       hipEvent_t e0 = nullptr, e1 = nullptr;
       const bool timed = W->opt_timing && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
       if (timed) (void)hipEventRecord(e0, st);
-      hipError_t e = launch_multi(W->mjobs_dev, W->mtile_dev, W->mtodo_dev, nmega, mega_tiles, W->opt_count != 0,
-                                  mega_layout, st);
+      hipError_t e = launch_multi(W->mjobs_dev, W->mtile_dev, W->mtodo_dev, mbin, mbin_tiles, nmega, mega_tiles,
+                                  W->opt_count != 0, st);
       if (e != hipSuccess) return e;
       if (timed) {
          (void)hipEventRecord(e1, st);
@@ -1298,21 +1298,28 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
    W->host_mjobs.clear();
    W->host_mtile.assign(1, 0);
    W->host_mtodo.clear();
-   int mwide = 0;
-   for (size_t k = 0; k < jobs.size(); ++k) {
-      if (todo[k] <= 0) continue;
-      rayon_job* j = jobs[k];
-      fill_jobview(j);
-      j->view.sample_base = (int)j->samples_done;
-      if (j->path == RAYON_PATH_MEGAKERNEL) {
+   // megakernel jobs: binary-tree jobs first, then quantized-tree ones (a launch per layout)
+   int mbin = 0;
+   for (int wide = 0; wide < 2; ++wide)
+      for (size_t k = 0; k < jobs.size(); ++k) {
+         rayon_job* j = jobs[k];
+         if (todo[k] <= 0 || j->path != RAYON_PATH_MEGAKERNEL || (j->scene->view.wnodes != nullptr) != (wide != 0))
+            continue;
+         fill_jobview(j);
+         j->view.sample_base = (int)j->samples_done;
          mactive.push_back(j);
          mtodo.push_back(todo[k]);
-         mwide += j->scene->view.wnodes != nullptr;
+         mbin += wide == 0;
          W->host_mjobs.push_back(j->view);
          W->host_mtile.push_back(W->host_mtile.back() + plain_tiles(j->params.width, j->params.height));
          W->host_mtodo.push_back(todo[k]);
-         continue;
       }
+   for (size_t k = 0; k < jobs.size(); ++k) {
+      if (todo[k] <= 0) continue;
+      rayon_job* j = jobs[k];
+      if (j->path == RAYON_PATH_MEGAKERNEL) continue;
+      fill_jobview(j);
+      j->view.sample_base = (int)j->samples_done;
       if (std::find(listed.begin(), listed.end(), k) != listed.end()) {
          j->view.live = j->live;
          j->view.nlive = (int)units[k];
@@ -1322,7 +1329,6 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
       W->host_jobs.push_back(j->view);
       W->host_base.push_back(W->host_base.back() + (long long)todo[k] * units[k]);
    }
-   const int mlayout = mwide == 0 ? L_BIN : (mwide == nmega ? L_WIDE : L_ANY);
    if (!active.empty()) {
       RY_HIP(hipMemcpyAsync(W->jobs_dev, W->host_jobs.data(), W->host_jobs.size() * sizeof(JobView),
                             hipMemcpyHostToDevice, ctx->stream));
@@ -1339,7 +1345,7 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
    }
    // the table copies read pageable host memory: finish them before the vectors change again
    RY_HIP(hipStreamSynchronize(ctx->stream));
-   const int mtiles = W->host_mtile.back();
+   const int mtiles = W->host_mtile.back(), mbin_tiles = W->host_mtile[mbin];
    // a graph keyed by adaptive jobs must be forgotten with them, which only their own
    // context's job destruction does: passes over other contexts' jobs run eagerly
    bool foreign = false;
@@ -1347,7 +1353,7 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
    // passes over pixel lists change size every time: not worth a graph each
    const bool use_graph = W->opt_graph && !W->opt_timing && !foreign && listed.empty();
    if (!use_graph) {
-      RY_HIP(enqueue_pass(ctx, active, atodo, total, levels, nmega, mtiles, mlayout));
+      RY_HIP(enqueue_pass(ctx, active, atodo, total, levels, nmega, mtiles, mbin, mbin_tiles));
    } else {
       // The captured launches read every job through the device tables, so a graph fits
       // any job set with the same work per job; only the adaptive pass-begin/end kernels
@@ -1376,7 +1382,8 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
       key.push_back(-1);
       key.push_back(nmega);
       key.push_back(mtiles);
-      key.push_back(mlayout);
+      key.push_back(mbin);
+      key.push_back(mbin_tiles);
       auto it = W->graphs.find(key);
       if (it == W->graphs.end() && W->graphs.size() >= 64) {  // bounded cache: start over
          for (auto& kv : W->graphs) {
@@ -1389,7 +1396,7 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
       if (it == W->graphs.end()) {
          Wave::GraphEntry ge;
          RY_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-         hipError_t e = enqueue_pass(ctx, active, atodo, total, levels, nmega, mtiles, mlayout);
+         hipError_t e = enqueue_pass(ctx, active, atodo, total, levels, nmega, mtiles, mbin, mbin_tiles);
          hipGraph_t g = nullptr;
          hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
          if (e != hipSuccess) return hip_fail(e, "wavefront pass capture");

@@ -130,3 +130,43 @@ def test_cancel_and_adaptive(world):
         srv.release(big)
         srv.release(ad)
         assert srv.stats().used_bytes == 0
+
+
+def test_first_frames_of_many_jobs(world):
+    """More new jobs than one first-frame launch holds (8), over binary and quantized trees:
+    every first frame equals the job's one-sample plain render resolved to RGBA8, and every
+    result equals the job rendered alone."""
+    ctx, gens, scenes = world
+    quant = ctx.scene(gens["corn"].desc_ptr(), ra.BVH.QUANTIZED)
+    try:
+        sc = dict(scenes, cornq=quant)
+        gens_q = dict(gens, cornq=gens["corn"])
+        jobs = []
+        for i in range(20):
+            key = ["corn", "book", "sdf", "cornq"][i % 4]
+            kw = dict(spp=3 + i % 5, seed=100 + i, max_depth=8 if key != "book" else 20)
+            if key.startswith("corn"):
+                kw["flags"] = ra.FLAG.NEE
+            jobs.append((key, 24 + 8 * (i % 3), 16 + 4 * (i % 4), kw))
+        refs, firsts = [], []
+        for key, W, H, kw in jobs:
+            j = alone(ctx, sc, gens_q, key, W, H, kw)
+            refs.append(j.resolve_linear())
+            j.close()
+            f = ctx.job(sc[key], params(gens_q, key, W, H, **dict(kw, spp=1)))
+            f.render_plain(1)
+            firsts.append(f.resolve_u8(4, 2.0))
+            f.close()
+        log("alone renders done")
+        with ctx.server(target_pass_ms=5.0, max_pass_samples=2) as srv:
+            ids = [srv.submit(sc[key], params(gens_q, key, W, H, **kw)) for key, W, H, kw in jobs]
+            srv.drain(120.0)
+            st = srv.stats()
+            log(f"first passes {st.first_passes} for {len(ids)} jobs, passes {st.passes}")
+            assert st.completed == len(jobs) and st.failed == 0
+            for i, jid in enumerate(ids):
+                np.testing.assert_array_equal(srv.first_frame(jid), firsts[i], err_msg=f"first frame {i}")
+                np.testing.assert_array_equal(srv.result(jid), refs[i], err_msg=f"job {i}")
+                srv.release(jid)
+    finally:
+        quant.close()
