@@ -110,9 +110,9 @@ def main(argv=None) -> int:
     ap.add_argument("--pass-spp", type=int, default=32, help="batch256: samples per batched pass")
     ap.add_argument("--batch-path", choices=["auto", "wavefront", "megakernel"], default="auto",
                     help="batch256: batched path of every scene (auto: measured per scene by the autotuner)")
-    ap.add_argument("--lam", type=str, default="5,20,50", help="serve: arrival rates (jobs/s), comma separated")
-    ap.add_argument("--duration", type=float, default=20.0, help="serve: seconds of arrivals per rate")
-    ap.add_argument("--grace", type=float, default=20.0, help="serve: seconds to finish after arrivals stop")
+    ap.add_argument("--lam", type=str, default="5,50,200", help="serve: arrival rates (jobs/s), comma separated")
+    ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals per rate")
+    ap.add_argument("--grace", type=float, default=5.0, help="serve: seconds to finish after arrivals stop")
     ap.add_argument("--pool", type=int, default=4, help="serve: resident scenes per generator")
     ap.add_argument("--target-pass-ms", type=float, default=0.0,
                     help="serve: refinement pass duration the server aims for (0: library default)")

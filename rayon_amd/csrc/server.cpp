@@ -56,7 +56,10 @@ struct SJob {
    bool accounted = false;  // bytes (and a scene reference) are held in the budget
    bool needs_clear = false;  // accumulation buffers not cleared yet (before the first pass)
    bool previewing = false;   // its first frame is being rendered
+   bool preview_held = false;  // its frame is held in the first-frame reserve (not admitted)
 };
+
+int64_t frame_bytes(const SJob* j) { return (int64_t)j->params.width * j->params.height * 4; }
 
 constexpr int kPreviewStreams = 2;
 
@@ -84,7 +87,9 @@ struct rayon_server {
    std::vector<SJob*> active;
    std::unordered_map<uint64_t, SJob*> jobs;
    std::unordered_map<rayon_scene*, int> scene_refs;
-   int64_t used_bytes = 0;
+   int64_t used_bytes = 0;     // admitted jobs and their scenes
+   int64_t preview_bytes = 0;  // first frames of jobs not admitted yet
+   uint64_t wake_seq = 0;      // arrivals and cancellations, for the server thread
    int pass_samples = 1;
    double pass_work = 4.0 * (1 << 20);  // pixel-samples of the next refinement pass
    // statistics
@@ -120,10 +125,41 @@ struct rayon_server {
             scene_refs.erase(j->scene);
          }
       }
-      if (j->frame) {
+      // a frame still being rendered is freed when its first frame completes
+      if (j->frame && !j->previewing) {
          (void)hipFreeAsync(j->frame, first_ctx->stream);
          j->frame = nullptr;
+         if (j->preview_held) {
+            preview_bytes -= frame_bytes(j);
+            j->preview_held = false;
+         }
       }
+   }
+
+   // A job whose set-up failed, queued or admitted.
+   void fail_locked(SJob* j) {
+      last_error = rayon_last_error();
+      j->state = RAYON_JOB_FAILED;
+      j->t_done = now_s();
+      ++failed;
+      auto q = std::find(queue.begin(), queue.end(), j);
+      if (q != queue.end()) queue.erase(q);
+      auto a = std::find(active.begin(), active.end(), j);
+      if (a != active.end()) active.erase(a);
+      free_job_locked(j);
+      done_cv.notify_all();
+   }
+
+   // The view a first frame renders: the job's own, or one built from its parameters
+   // (a job not admitted yet has no buffers; the preview kernel touches none).
+   ry::JobView preview_view(const SJob* j) {
+      if (j->job) return j->job->view;
+      rayon_job tmp;
+      tmp.ctx = first_ctx;
+      tmp.scene = j->scene;
+      tmp.params = j->params;
+      ry::fill_jobview(&tmp);
+      return tmp.view;
    }
 
    // Marks finished jobs DONE/FAILED and takes them out of the active set.
@@ -187,11 +223,13 @@ struct rayon_server {
    }
 
    // The server thread. First frames run on streams of their own, so they overlap the
-   // refinement pass in flight on the main context instead of waiting for it, and the
-   // thread does not wait for them either:
-   //   - admit queued jobs, create them, launch their first frames on the next preview
-   //     stream (no wait); a job's time-to-first-frame is stamped when the thread sees
-   //     that launch complete;
+   //   refinement pass in flight on the main context instead of waiting for it, and the
+   //   thread does not wait for them either:
+   //   - admit queued jobs by the HBM budget less a first-frame reserve (1/8), create them;
+   //   - launch first frames on the next preview stream (no wait) for admitted jobs and
+   //     for queued jobs whose frames fit the reserve, so a first frame does not wait for
+   //     admission; a job's time-to-first-frame is stamped when the thread sees that
+   //     launch complete;
    //   - when no refinement pass is in flight, start one over the oldest jobs that have
    //     a first frame;
    //   - when the pass in flight has completed, check its jobs and steer the next size.
@@ -203,14 +241,18 @@ struct rayon_server {
       double t_pass = 0;
       hipEvent_t pass_done = nullptr;
       (void)hipEventCreateWithFlags(&pass_done, hipEventDisableTiming);
+      uint64_t seen_seq = 0;
       for (;;) {
-         std::vector<SJob*> fresh;
+         std::vector<SJob*> fresh, previews;
          {
             std::unique_lock<std::mutex> lk(mu);
             if (flying || !firsts.empty())  // wake up for arrivals, or soon to look at the GPU work
-               cv.wait_for(lk, std::chrono::microseconds(200), [&] { return stop || !queue.empty(); });
+               cv.wait_for(lk, std::chrono::microseconds(200), [&] { return stop || wake_seq != seen_seq; });
             else
-               cv.wait(lk, [&] { return stop || !queue.empty() || !active.empty(); });
+               cv.wait(lk, [&] {  // a queued job is admitted whatever its size once nothing is active
+                  return stop || wake_seq != seen_seq || !active.empty() || !queue.empty();
+               });
+            seen_seq = wake_seq;
             if (stop) break;
             // retire cancelled jobs that no pass is using
             for (auto it = active.begin(); it != active.end();) {
@@ -232,85 +274,95 @@ struct rayon_server {
             }
             // admission by HBM budget and active-job cap (FIFO: a job that does not fit
             // blocks the ones behind it, so large jobs are not starved)
+            const int64_t reserve = cfg.hbm_budget_bytes / 8;
             while (!queue.empty() && (int)active.size() < cfg.max_active_jobs) {
                SJob* j = queue.front();
                if (j->cancel) {
                   queue.pop_front();
                   j->state = RAYON_JOB_CANCELLED;
                   j->t_done = now_s();
+                  free_job_locked(j);
                   done_cv.notify_all();
                   continue;
                }
                const int64_t need = j->bytes + (scene_refs.count(j->scene) ? 0 : scene_bytes(j->scene));
-               if (used_bytes + need > cfg.hbm_budget_bytes && (!active.empty() || !fresh.empty())) break;
+               if (used_bytes + need > cfg.hbm_budget_bytes - reserve && (!active.empty() || !fresh.empty())) break;
                queue.pop_front();
                used_bytes += need;
                scene_refs[j->scene]++;
                j->accounted = true;
+               if (j->preview_held) {  // the frame is part of the job's bytes from here
+                  preview_bytes -= frame_bytes(j);
+                  j->preview_held = false;
+               }
                j->t_admit = now_s();
                fresh.push_back(j);
                active.push_back(j);  // counted as active from here, so drain() waits for it
             }
-         }
-         // (1) new jobs: buffers on first_ctx, first frames on a preview stream
-         if (!fresh.empty()) {
-            const double t0 = now_s();
-            std::vector<SJob*> ok_fresh;
-            for (SJob* j : fresh) {
-               // created on first_ctx: its allocations must not queue behind the refinement
-               // pass in flight on the main stream. The preview stream waits for them, and
-               // every other use follows the host's sight of the preview's completion. The
-               // buffers are cleared on the main stream just before the job's first
-               // refinement pass, so the first frame waits for nothing but its own kernel.
-               rayon_status st = ry::job_create(first_ctx, j->scene, &j->params, &j->job, false, false);
-               j->needs_clear = st == RAYON_OK;
-               if (st == RAYON_OK && hipMallocAsync((void**)&j->frame, (size_t)j->params.width * j->params.height * 4,
-                                                    first_ctx->stream) != hipSuccess) {
-                  ry::set_error("allocation of the first-frame buffer failed");
-                  st = RAYON_ERR_OOM;
-               }
-               std::lock_guard<std::mutex> lk(mu);
-               if (st != RAYON_OK) {
-                  last_error = rayon_last_error();
-                  j->state = RAYON_JOB_FAILED;
-                  j->t_done = now_s();
-                  ++failed;
-                  free_job_locked(j);
-                  active.erase(std::find(active.begin(), active.end(), j));
-                  done_cv.notify_all();
-                  continue;
-               }
-               j->state = RAYON_JOB_RUNNING;
-               j->live_px = (int64_t)j->params.width * j->params.height;
-               ok_fresh.push_back(j);
+            for (SJob* j : fresh)
+               if (!j->frame) previews.push_back(j);
+            for (SJob* j : queue) {
+               if (j->cancel || j->frame) continue;
+               if (preview_bytes + frame_bytes(j) > reserve) break;
+               preview_bytes += frame_bytes(j);
+               j->preview_held = true;
+               previews.push_back(j);
             }
-            if (!ok_fresh.empty()) {
-               // The first frame is a preview: the job's first samples rendered and resolved
-               // to RGBA8 by one launch of the preview kernel for all new jobs (the plain
-               // kernel's per-pixel arithmetic, tables in the kernel arguments). It does not
-               // enter the job's sums (its samples arrive with the refinement passes), so a
-               // served job's result equals the job rendered alone. The preview stream
-               // waits for the allocations made on first_ctx's stream.
-               FirstBatch fb;
-               fb.jobs = ok_fresh;
-               fb.t0 = t0;
+         }
+         // (1) admitted jobs: buffers on first_ctx. Its allocations must not queue behind
+         // the refinement pass in flight on the main stream; a preview stream waits for them
+         // and every other use follows the host's sight of the first frame's completion. The
+         // buffers are cleared on the main stream just before the job's first refinement
+         // pass, so a first frame waits for nothing but its own kernel.
+         for (SJob* j : fresh) {
+            rayon_status st = ry::job_create(first_ctx, j->scene, &j->params, &j->job, false, false);
+            std::lock_guard<std::mutex> lk(mu);
+            if (st != RAYON_OK) {
+               fail_locked(j);
+               continue;
+            }
+            j->needs_clear = true;
+            j->state = RAYON_JOB_RUNNING;
+            j->live_px = (int64_t)j->params.width * j->params.height;
+         }
+         // (1a) first frames: the job's first samples rendered and resolved to RGBA8 by one
+         // launch of the preview kernel for all of them (the plain kernel's per-pixel
+         // arithmetic, tables in the kernel arguments). A first frame does not enter the
+         // job's sums (its samples arrive with the refinement passes), so a served job's
+         // result equals the job rendered alone.
+         if (!previews.empty()) {
+            FirstBatch fb;
+            fb.t0 = now_s();
+            std::vector<ry::JobView> views;
+            std::vector<uint8_t*> frames;
+            {
+               std::lock_guard<std::mutex> lk(mu);
+               for (SJob* j : previews) {
+                  if (j->state == RAYON_JOB_FAILED || j->state == RAYON_JOB_CANCELLED) continue;
+                  if (hipMallocAsync((void**)&j->frame, (size_t)frame_bytes(j), first_ctx->stream) != hipSuccess) {
+                     j->frame = nullptr;
+                     ry::set_error("allocation of the first-frame buffer failed");
+                     fail_locked(j);
+                     continue;
+                  }
+                  views.push_back(preview_view(j));
+                  frames.push_back(j->frame);
+                  j->previewing = true;
+                  fb.jobs.push_back(j);
+               }
+            }
+            if (!fb.jobs.empty()) {
                fb.ready = take_event();
                fb.done = take_event();
                hipStream_t ps = pv_stream[next_pv++ % kPreviewStreams];
-               std::vector<ry::JobView> views;
-               std::vector<uint8_t*> frames;
-               for (SJob* j : ok_fresh) {
-                  views.push_back(j->job->view);
-                  frames.push_back(j->frame);
-                  j->previewing = true;
-               }
-               bool ok = fb.ready && fb.done && hipEventRecord(fb.ready, first_ctx->stream) == hipSuccess &&
-                         hipStreamWaitEvent(ps, fb.ready, 0) == hipSuccess &&
-                         ry::launch_preview(views.data(), frames.data(), (int)views.size(), cfg.first_pass_samples,
-                                            2.0f, ps) == hipSuccess &&
-                         hipEventRecord(fb.done, ps) == hipSuccess;
+               const bool ok = fb.ready && fb.done && hipEventRecord(fb.ready, first_ctx->stream) == hipSuccess &&
+                               hipStreamWaitEvent(ps, fb.ready, 0) == hipSuccess &&
+                               ry::launch_preview(views.data(), frames.data(), (int)views.size(),
+                                                  cfg.first_pass_samples, 2.0f, ps) == hipSuccess &&
+                               hipEventRecord(fb.done, ps) == hipSuccess;
                if (!ok) {
                   (void)hipStreamSynchronize(ps);
+                  if (fb.done) event_pool.push_back(fb.done);
                   fb.done = nullptr;  // completes below as failed
                }
                firsts.push_back(std::move(fb));
@@ -329,18 +381,16 @@ struct rayon_server {
             if (q != hipSuccess) last_error = std::string("first frame: ") + hipGetErrorString(q);
             ++first_passes;
             first_ms_sum += (t - fb.t0) * 1e3;
-            std::vector<SJob*> failed_now;
             for (SJob* j : fb.jobs) {
                j->previewing = false;
-               if (q == hipSuccess) {
+               if (j->state == RAYON_JOB_CANCELLED || j->state == RAYON_JOB_FAILED) {
+                  if (j->released || j->state == RAYON_JOB_CANCELLED) free_job_locked(j);  // the frame now
+               } else if (q == hipSuccess) {
                   j->t_first = t;
                } else {
-                  j->state = RAYON_JOB_FAILED;
-                  ++failed;
-                  failed_now.push_back(j);
+                  fail_locked(j);
                }
             }
-            finish_locked(failed_now);
             for (hipEvent_t e : {fb.ready, fb.done})
                if (e) event_pool.push_back(e);
             firsts.erase(firsts.begin() + (long)k);
@@ -503,6 +553,7 @@ rayon_status rayon_server_submit(rayon_server* s, rayon_scene* scene, const rayo
    s->jobs[j->id] = j;
    s->queue.push_back(j);
    if (id) *id = j->id;
+   ++s->wake_seq;
    s->cv.notify_one();
    return RAYON_OK;
 }
@@ -571,6 +622,7 @@ rayon_status rayon_server_cancel(rayon_server* s, uint64_t id) {
       return RAYON_ERR_INVALID;
    }
    it->second->cancel = true;
+   ++s->wake_seq;
    s->cv.notify_one();
    return RAYON_OK;
 }
@@ -587,6 +639,8 @@ rayon_status rayon_server_release(rayon_server* s, uint64_t id) {
    if (j->state == RAYON_JOB_DONE || j->state == RAYON_JOB_FAILED || j->state == RAYON_JOB_CANCELLED) {
       (void)hipSetDevice(s->ctx->device);
       s->free_job_locked(j);
+      ++s->wake_seq;  // budget freed: queued jobs may fit now
+      s->cv.notify_one();
    }
    return RAYON_OK;
 }

@@ -91,12 +91,17 @@ def test_served_jobs_equal_alone_renders(world):
 
 def test_admission_by_budget(world):
     """Jobs that do not all fit the budget run in admission waves and still finish with the
-    same results; a job larger than the whole budget is refused at submission."""
+    same results; a job larger than the whole budget is refused at submission. Queued jobs
+    get their first frames before admission (an eighth of the budget is held for them)."""
     ctx, gens, scenes = world
     key, W, H, kw = JOBS[0]
     ref = alone(ctx, scenes, gens, key, W, H, kw)
     img = ref.resolve_linear()
     ref.close()
+    f = ctx.job(scenes[key], params(gens, key, W, H, **dict(kw, spp=1)))
+    f.render_plain(1)
+    first = f.resolve_u8(4, 2.0)
+    f.close()
     scene_bytes = scenes[key].info().device_bytes
     with ctx.server(hbm_budget_bytes=scene_bytes + 2 * W * H * 48, max_pass_pixels=W * H) as srv:
         ids = [srv.submit(scenes[key], params(gens, key, W, H, **kw)) for _ in range(5)]
@@ -106,10 +111,14 @@ def test_admission_by_budget(world):
         assert srv.stats().completed == 5
         infos = [srv.info(i) for i in ids]
         log(f"queue ms {[round(x.queue_ms, 2) for x in infos]}, ttc ms {[round(x.ttc_ms, 2) for x in infos]}")
-        # two jobs fit at once: the third waits until one of the first two has finished
-        assert infos[2].queue_ms >= min(infos[0].ttc_ms, infos[1].ttc_ms) - 5.0
+        log(f"ttff ms {[round(x.ttff_ms, 2) for x in infos]}")
+        # the jobs do not all fit at once: the last waits until an earlier one has finished,
+        # and its first frame came before its admission
+        assert infos[4].queue_ms >= min(infos[0].ttc_ms, infos[1].ttc_ms) - 5.0
+        assert infos[4].ttff_ms < infos[4].queue_ms
         for i in ids:
             np.testing.assert_array_equal(srv.result(i), img)
+            np.testing.assert_array_equal(srv.first_frame(i), first)
             srv.release(i)
 
 
