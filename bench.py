@@ -365,6 +365,22 @@ def main(argv=None) -> int:
         result["rel_rmse_vs_oracle"] = round(err, 5)
         result["rmse_note"] = f"same random streams, {spp} spp, {len(pix)}-pixel subset, linear RGB"
         log(f"cpu baseline {osegs / t_cpu / 1e6:.2f} Mrays/s on {cores} cores, rel RMSE {err:.5f}")
+        # converged image: the same scene and camera at 64x36, 4096 spp, batched path vs oracle
+        cw, chh, cspp = 64, 36, 4096
+        ccam = ra.camera_frame(lf, la, up, fov, cw, chh)
+        cP = ra.job_params(cw, chh, ccam, spp=cspp, max_depth=job0.params.max_depth, flags=job0.params.flags, seed=31)
+        cj = ctx.job(sc, cP)
+        ctx.render_batched([cj], cspp)
+        cimg = cj.resolve_linear().reshape(-1, 3).astype(np.float64)
+        cj.close()
+        t_c = time.perf_counter()
+        csums, _ = osc.render(cP, nsamples=cspp, threads=cores, camera=ccam)
+        cref = csums / cspp
+        cerr = float(np.sqrt(((cimg - cref) ** 2).sum() / (cref ** 2).sum()))
+        result["rel_rmse_4096spp"] = {"value": round(cerr, 6), "pixels": cw * chh, "spp": cspp,
+                                      "note": "same scene and camera at 64x36, batched path vs FP64 oracle, "
+                                              "same random streams, linear RGB"}
+        log(f"converged {cspp} spp: rel RMSE {cerr:.6f} (oracle {time.perf_counter() - t_c:.1f} s)")
 
     b_job.close()
     sc.close()
