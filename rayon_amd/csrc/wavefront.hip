@@ -642,6 +642,182 @@ __device__ void shadow_rays_bin(const WfView& V, int level, int* lds, TravCount&
    }
 }
 
+// The 8-wide quantized tree with lane refill: one traversal step visits internal nodes
+// until one with hit leaf children has had its primitives tested (or the traversal ends),
+// then the wave refills idle lanes. The visiting order, the stack (two LDS words per
+// partly visited node: its first internal child, the remaining hit mask with the node's
+// internal-child mask) and the primitive tests are those of closest_hit_wide and
+// any_hit_wide, so a ray gets the same hit as there.
+constexpr uint32_t kWideDone = 0xFFFFFFFFu;
+
+struct WideRay {
+   V3 o, d, inv;
+   uint32_t perm;
+};
+
+__device__ __forceinline__ WideRay make_wideray(V3 o, V3 d) {
+   WideRay r;
+   r.o = o;
+   r.d = d;
+   r.inv = mk(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
+   r.perm = 7u ^ ((d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u));
+   return r;
+}
+
+// The next node after a node whose internal children `hit_int` (visiting order) were hit:
+// the first of them (the rest pushed), else the nearest pending child of an ancestor, else
+// kWideDone.
+__device__ __forceinline__ uint32_t wide_next(uint32_t hit_int, uint32_t base, uint32_t imask, uint32_t perm,
+                                              int* stack, int& sp) {
+   if (!hit_int) {
+      if (sp == 0) return kWideDone;
+      base = (uint32_t)stack[(2 * (sp - 1)) * kWfBlock];
+      const uint32_t bits = (uint32_t)stack[(2 * (sp - 1) + 1) * kWfBlock];
+      hit_int = bits & 0xFFu;
+      const uint32_t im = bits >> 8;
+      const int i = __ffs(hit_int) - 1;
+      hit_int &= hit_int - 1u;
+      const uint32_t sl = (uint32_t)i ^ perm;
+      if (hit_int)
+         stack[(2 * (sp - 1) + 1) * kWfBlock] = (int)(hit_int | (im << 8));
+      else
+         --sp;
+      return base + (uint32_t)__popc(im & ((1u << sl) - 1u));
+   }
+   const int i = __ffs(hit_int) - 1;
+   hit_int &= hit_int - 1u;
+   const uint32_t sl = (uint32_t)i ^ perm;
+   if (hit_int && sp < kStack / 2) {
+      stack[(2 * sp) * kWfBlock] = (int)base;
+      stack[(2 * sp + 1) * kWfBlock] = (int)(hit_int | (imask << 8));
+      ++sp;
+   }
+   return base + (uint32_t)__popc(imask & ((1u << sl) - 1u));
+}
+
+template <bool CNT>
+__device__ void ext_rays_wide(const WfView& V, int level, int* lds, TravCount& tc) {
+   unsigned* ctl = V.ctl + kCtl * level;
+   const unsigned count = ctl[0];
+   const int* q = (level & 1) ? V.q1 : V.q0;
+   unsigned* mctl = V.sctl + 2 * (kSdfSlices + 1) * level;
+   const unsigned step = 64u * (unsigned)V.chunk;
+   WavePool pool = start_pool(&ctl[1], count);
+   int slot = -1, pend = -1, sp = 0;
+   uint32_t node = kWideDone;
+   QueueRes res;
+   auto queue_marches = [&]() {  // as in ext_rays_bin
+      res.push(&mctl[0], V.qs0, pend >= 0, pend);
+      pend = -1;
+   };
+   const SceneView* S = nullptr;
+   WideRay r;
+   HitMin best;
+   for (;;) {
+      if (!pool.drained || pool.next != pool.end) {
+         if (__popcll(__ballot(slot < 0)) >= V.refill) queue_marches();
+         const long long idx = pool.refill(&ctl[1], count, step, slot < 0, V.refill);
+         if (idx >= 0) {
+            slot = q[idx];
+            S = &V.jobs[V.ids[slot].x].scene;
+            r = make_wideray(mk(V.o[slot]), mk(V.d[slot]));
+            best.t = 3.402823466e38f;
+            best.slot = -1;
+            best.orig = -1;
+            best.u = best.v = 0.0f;
+            best.sdf = -1;
+            sp = 0;
+            node = 0;
+         }
+      }
+      if (__ballot(slot >= 0) == 0) break;
+      if (slot >= 0) {
+         const WNode* wn = S->wnodes;
+         for (;;) {
+            const WNode n = load_wnode(wn, node);
+            if constexpr (CNT) ++tc.nodes;
+            uint32_t hit_int, hit_leaf;
+            wide_children(n, r.o, r.inv, r.perm, 0.001f, best.t, hit_int, hit_leaf);
+            const bool leaves = hit_leaf != 0;
+            while (hit_leaf) {
+               const int sl = __ffs(hit_leaf) - 1;
+               hit_leaf &= hit_leaf - 1u;
+               const uint32_t m = ((sl < 4 ? n.meta[0] : n.meta[1]) >> (8 * (sl & 3))) & 0xFFu;
+               const int first = (int)n.prim_base + (int)(m & 31u);
+               const int cnt = (int)(m >> 5) + 1;
+               for (int i = first; i < first + cnt; ++i) test_closest<CNT, true>(*S, i, r.o, r.d, 0.001f, best, &tc);
+            }
+            node = wide_next(hit_int, n.child_base, n.e_imask >> 24, r.perm, lds, sp);
+            if (leaves || node == kWideDone) break;
+         }
+         if (node == kWideDone) {
+            V.hit[slot] = make_float4(best.t, __int_as_float(best.slot), best.u, best.v);
+            if (best.sdf >= 0) pend = slot;  // an SDF field lies on the way
+            slot = -1;
+         }
+      }
+   }
+   queue_marches();
+   res.close(V.qs0);
+}
+
+template <bool CNT>
+__device__ void shadow_rays_wide(const WfView& V, int level, int* lds, TravCount& tc) {
+   unsigned* sctl = V.ctl + kCtl * level + 3;
+   const unsigned count = sctl[0];
+   const unsigned step = 64u * (unsigned)V.chunk;
+   WavePool pool = start_pool(&sctl[1], count);
+   int slot = -1, sp = 0;
+   uint32_t node = kWideDone;
+   const SceneView* S = nullptr;
+   WideRay r;
+   constexpr float tmin = 1e-4f, tmax = 1.0f - 1e-4f;
+   for (;;) {
+      if (!pool.drained || pool.next != pool.end) {
+         const long long idx = pool.refill(&sctl[1], count, step, slot < 0, V.refill);
+         if (idx >= 0) {
+            slot = V.sq[idx];
+            S = &V.jobs[V.ids[slot].x].scene;
+            r = make_wideray(mk(V.so[slot]), mk(V.sd[slot]));
+            sp = 0;
+            node = 0;
+         }
+      }
+      if (__ballot(slot >= 0) == 0) break;
+      if (slot >= 0) {
+         const WNode* wn = S->wnodes;
+         bool occluded = false;
+         for (;;) {
+            const WNode n = load_wnode(wn, node);
+            if constexpr (CNT) ++tc.nodes;
+            uint32_t hit_int, hit_leaf;
+            wide_children(n, r.o, r.inv, r.perm, tmin, tmax, hit_int, hit_leaf);
+            const bool leaves = hit_leaf != 0;
+            while (hit_leaf && !occluded) {
+               const int sl = __ffs(hit_leaf) - 1;
+               hit_leaf &= hit_leaf - 1u;
+               const uint32_t m = ((sl < 4 ? n.meta[0] : n.meta[1]) >> (8 * (sl & 3))) & 0xFFu;
+               const int first = (int)n.prim_base + (int)(m & 31u);
+               const int cnt = (int)(m >> 5) + 1;
+               for (int i = first; i < first + cnt && !occluded; ++i)
+                  occluded = test_any<CNT>(*S, i, r.o, r.d, tmin, tmax, &tc);
+            }
+            node = occluded ? kWideDone : wide_next(hit_int, n.child_base, n.e_imask >> 24, r.perm, lds, sp);
+            if (leaves || node == kWideDone) break;
+         }
+         if (node == kWideDone) {
+            if (!occluded) {
+               const float4 c = V.sc[slot];
+               const float4 rd = V.rad[slot];
+               const V3 L = mk(rd) + mk(c);
+               V.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+            }
+            slot = -1;
+         }
+      }
+   }
+}
+
 // Shadow rays of one level, each wave tracing batches of 64 rays to completion.
 template <bool CNT, int LAYOUT, int STACK>
 __device__ void shadow_rays_drain(const WfView& V, int level, int* lds, TravCount& ts) {
@@ -680,6 +856,14 @@ __global__ __launch_bounds__(kWfBlock, (LAYOUT == L_BIN && STACK == kShortStack)
          // times would march one by one, so passes with fields keep whole batches together
          if (V.shadow_refill)
             shadow_rays_bin<CNT, STACK>(V, shadow_level, stack + threadIdx.x, ts);
+         else
+            shadow_rays_drain<CNT, LAYOUT, STACK>(V, shadow_level, stack + threadIdx.x, ts);
+      }
+   } else if constexpr (LAYOUT == L_WIDE) {
+      if (ext_level >= 0) ext_rays_wide<CNT>(V, ext_level, stack + threadIdx.x, te);
+      if (shadow_level >= 0) {
+         if (V.shadow_refill)
+            shadow_rays_wide<CNT>(V, shadow_level, stack + threadIdx.x, ts);
          else
             shadow_rays_drain<CNT, LAYOUT, STACK>(V, shadow_level, stack + threadIdx.x, ts);
       }

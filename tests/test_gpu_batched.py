@@ -309,3 +309,46 @@ def test_autotune_keeps_results_and_jobs(ctx, scenes):
             ctx.set_option(o, v)
         for sc in (BOOK, SDF, CORN):
             scenes.get(*sc)[1].set_batch_path(ra.PATH.AUTO)
+
+
+@pytest.mark.parametrize("scene,kw", [(CORN, dict(max_depth=8, flags=ra.FLAG.NEE)), (SDF, dict(max_depth=8)),
+                                      (BOOK, dict(max_depth=50))], ids=["cornell_nee", "sdf", "book"])
+def test_quantized_wavefront_refill_equals_per_ray_traversal(ctx, scene, kw):
+    """Quantized trees on the wavefront: a pass over quantized-tree jobs alone runs the
+    lane-refill wide traversal; batched with a binary-tree job it runs the per-ray wide
+    traversal (closest_hit_wide / any_hit_wide). Both find the same hits, so the images are
+    identical bit for bit, and so are other refill thresholds."""
+    g = ra.Generated.generate(scene[0], scene[1], detail=scene[2])
+    q = ctx.scene(g.desc_ptr(), ra.BVH.QUANTIZED)
+    gb = ra.Generated.generate(*BOOK[:2], detail=BOOK[2])
+    b = ctx.scene(gb.desc_ptr())
+    try:
+        q.set_batch_path(ra.PATH.WAVEFRONT)
+        b.set_batch_path(ra.PATH.WAVEFRONT)
+        lf, la, up, fov = g.camera()
+        cam = ra.camera_frame(lf, la, up, fov, 72, 40)
+        P = ra.job_params(72, 40, cam, spp=12, seed=17, **kw)
+        lfb, lab, upb, fovb = gb.camera()
+        Pb = ra.job_params(40, 24, ra.camera_frame(lfb, lab, upb, fovb, 40, 24), spp=12, seed=3, max_depth=8)
+
+        def run(with_binary, refill):
+            ctx.set_option(ra.OPT.TRACE_REFILL, refill)
+            j = ctx.job(q, P)
+            other = ctx.job(b, Pb) if with_binary else None
+            ctx.render_batched([j] + ([other] if other else []), 12)
+            img, segs = j.resolve_linear(), j.stats().segments
+            j.close()
+            if other:
+                other.close()
+            return img, segs
+
+        ref, segs = run(True, 16)
+        log(f"quantized {scene}: per-ray traversal, {segs} segments")
+        for refill in (16, 1, 64):
+            img, s2 = run(False, refill)
+            np.testing.assert_array_equal(img, ref, err_msg=f"refill {refill}")
+            assert s2 == segs
+    finally:
+        ctx.set_option(ra.OPT.TRACE_REFILL, 16)
+        q.close()
+        b.close()
