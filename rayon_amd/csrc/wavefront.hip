@@ -860,13 +860,11 @@ __global__ __launch_bounds__(kWfBlock, (LAYOUT == L_BIN && STACK == kShortStack)
             shadow_rays_drain<CNT, LAYOUT, STACK>(V, shadow_level, stack + threadIdx.x, ts);
       }
    } else if constexpr (LAYOUT == L_WIDE) {
+      // shadow rays refill lanes also in passes with SDF fields (marched inline): the
+      // per-ray drain's traversal would raise the kernel to 136 VGPRs (3 waves per SIMD,
+      // against 5 with the refilling traversals alone)
       if (ext_level >= 0) ext_rays_wide<CNT>(V, ext_level, stack + threadIdx.x, te);
-      if (shadow_level >= 0) {
-         if (V.shadow_refill)
-            shadow_rays_wide<CNT>(V, shadow_level, stack + threadIdx.x, ts);
-         else
-            shadow_rays_drain<CNT, LAYOUT, STACK>(V, shadow_level, stack + threadIdx.x, ts);
-      }
+      if (shadow_level >= 0) shadow_rays_wide<CNT>(V, shadow_level, stack + threadIdx.x, ts);
    } else {
    if (ext_level >= 0) {
       unsigned* ctl = V.ctl + kCtl * ext_level;
