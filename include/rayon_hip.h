@@ -158,7 +158,7 @@ typedef struct rayon_scene_info {
    double build_ms;           /* BVH build time (host, or device including its sort)      */
    double upload_ms;          /* host-to-device copies of the scene                      */
    int32_t builder;           /* RAYON_BVH_BUILDER_HOST or RAYON_BVH_BUILDER_DEVICE        */
-   int32_t pad_;
+   int32_t wide_depth;        /* quantized wide tree depth (root = 0; 0 without one)     */
 } rayon_scene_info;
 
 /* ---------------------------------------------------------------------------------

@@ -66,7 +66,7 @@ class SceneDesc(C.Structure):
 class SceneInfo(C.Structure):
     _fields_ = [("num_prims", C.c_int64), ("num_nodes", C.c_int64), ("num_qnodes", C.c_int64),
                 ("max_depth", C.c_int32), ("num_lights", C.c_int32), ("device_bytes", C.c_int64),
-                ("build_ms", C.c_double), ("upload_ms", C.c_double), ("builder", C.c_int32), ("pad_", C.c_int32)]
+                ("build_ms", C.c_double), ("upload_ms", C.c_double), ("builder", C.c_int32), ("wide_depth", C.c_int32)]
 
 
 class Camera(C.Structure):

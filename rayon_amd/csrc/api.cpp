@@ -467,6 +467,8 @@ rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* d, uint3
    I.num_nodes = num_nodes;
    I.num_qnodes = (int64_t)br.wnodes.size();
    I.max_depth = br.max_depth;
+   sc->wide_depth = br.wnodes.empty() ? 0 : br.wide_depth;
+   I.wide_depth = sc->wide_depth;
    I.num_lights = V.num_lights;
    for (auto& b : sc->bufs) I.device_bytes += (int64_t)b.bytes;
    I.build_ms = build_ms;

@@ -99,6 +99,7 @@ struct rayon_scene {
    ry::SceneView view{};
    std::vector<ry::DevBuf> bufs;
    rayon_scene_info info{};
+   int wide_depth = 0;  // levels below the root of the 8-wide quantized tree (0 without one)
    std::vector<int64_t> orig_of_slot;
    std::vector<int32_t> order;  // slot -> primitive of the gathered description
    int32_t batch_path = RAYON_PATH_AUTO;  // path of this scene's jobs in batched passes

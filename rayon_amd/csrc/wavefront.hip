@@ -38,6 +38,10 @@ constexpr int kWfBlock = 256;
 // Binary trees up to 23 levels deep take a 23-entry stack (22 LDS entries per lane + the
 // register top), which fits 7 workgroups of the trace kernel per CU instead of 5.
 constexpr int kShortStack = 23;
+// LDS words per lane of the quantized-tree trace kernel for trees at most kShortWide / 2
+// levels deep (two words per partly visited node): 20 KiB per workgroup, so LDS allows 8
+// workgroups per CU and registers 6 waves per SIMD, against 5 with the full 32-word stack.
+constexpr int kShortWide = 20;
 constexpr int kSdfSlice = 8;   // steps of the first marching slice; slice k takes kSdfSlice << k
 constexpr int kSdfSlices = 8;  // slices per level at most (the last one runs to the end)
 constexpr long long kMegaSceneBytes = 8ll << 20;  // automatic path choice: megakernel up to this scene size
@@ -95,7 +99,7 @@ struct Wave {
    int opt_chunk = 4;
    int opt_shade_chunk = 16;
    int opt_refill = 16;
-   int occ_trace = 0, occ_trace_wide = 0, occ_trace_short = 0, occ_shade = 0, occ_sdf = 0, occ_sdf_rest = 0;
+   int occ_trace = 0, occ_trace_wide = 0, occ_trace_short = 0, occ_trace_wide_short = 0, occ_shade = 0, occ_sdf = 0, occ_sdf_rest = 0;
    struct Pending {
       int stage;
       hipEvent_t e0, e1;
@@ -666,7 +670,8 @@ __device__ __forceinline__ WideRay make_wideray(V3 o, V3 d) {
 
 // The next node after a node whose internal children `hit_int` (visiting order) were hit:
 // the first of them (the rest pushed), else the nearest pending child of an ancestor, else
-// kWideDone.
+// kWideDone. STACK: LDS words per lane (trees are chosen no deeper than STACK / 2 levels).
+template <int STACK>
 __device__ __forceinline__ uint32_t wide_next(uint32_t hit_int, uint32_t base, uint32_t imask, uint32_t perm,
                                               int* stack, int& sp) {
    if (!hit_int) {
@@ -687,7 +692,7 @@ __device__ __forceinline__ uint32_t wide_next(uint32_t hit_int, uint32_t base, u
    const int i = __ffs(hit_int) - 1;
    hit_int &= hit_int - 1u;
    const uint32_t sl = (uint32_t)i ^ perm;
-   if (hit_int && sp < kStack / 2) {
+   if (hit_int && sp < STACK / 2) {
       stack[(2 * sp) * kWfBlock] = (int)base;
       stack[(2 * sp + 1) * kWfBlock] = (int)(hit_int | (imask << 8));
       ++sp;
@@ -695,7 +700,7 @@ __device__ __forceinline__ uint32_t wide_next(uint32_t hit_int, uint32_t base, u
    return base + (uint32_t)__popc(imask & ((1u << sl) - 1u));
 }
 
-template <bool CNT>
+template <bool CNT, int STACK>
 __device__ void ext_rays_wide(const WfView& V, int level, int* lds, TravCount& tc) {
    unsigned* ctl = V.ctl + kCtl * level;
    const unsigned count = ctl[0];
@@ -747,7 +752,7 @@ __device__ void ext_rays_wide(const WfView& V, int level, int* lds, TravCount& t
                const int cnt = (int)(m >> 5) + 1;
                for (int i = first; i < first + cnt; ++i) test_closest<CNT, true>(*S, i, r.o, r.d, 0.001f, best, &tc);
             }
-            node = wide_next(hit_int, n.child_base, n.e_imask >> 24, r.perm, lds, sp);
+            node = wide_next<STACK>(hit_int, n.child_base, n.e_imask >> 24, r.perm, lds, sp);
             if (leaves || node == kWideDone) break;
          }
          if (node == kWideDone) {
@@ -761,7 +766,7 @@ __device__ void ext_rays_wide(const WfView& V, int level, int* lds, TravCount& t
    res.close(V.qs0);
 }
 
-template <bool CNT>
+template <bool CNT, int STACK>
 __device__ void shadow_rays_wide(const WfView& V, int level, int* lds, TravCount& tc) {
    unsigned* sctl = V.ctl + kCtl * level + 3;
    const unsigned count = sctl[0];
@@ -802,7 +807,7 @@ __device__ void shadow_rays_wide(const WfView& V, int level, int* lds, TravCount
                for (int i = first; i < first + cnt && !occluded; ++i)
                   occluded = test_any<CNT>(*S, i, r.o, r.d, tmin, tmax, &tc);
             }
-            node = occluded ? kWideDone : wide_next(hit_int, n.child_base, n.e_imask >> 24, r.perm, lds, sp);
+            node = occluded ? kWideDone : wide_next<STACK>(hit_int, n.child_base, n.e_imask >> 24, r.perm, lds, sp);
             if (leaves || node == kWideDone) break;
          }
          if (node == kWideDone) {
@@ -844,10 +849,13 @@ __device__ void shadow_rays_drain(const WfView& V, int level, int* lds, TravCoun
 // disjoint state, so one persistent launch drains both and waves that finish their
 // extension work fill the launch's tail with shadow work. Either level may be -1.
 template <bool CNT, int LAYOUT, int STACK = kStack>
-__global__ __launch_bounds__(kWfBlock, (LAYOUT == L_BIN && STACK == kShortStack) ? 7 : 1) void k_wf_trace(
+__global__ __launch_bounds__(kWfBlock, (LAYOUT == L_BIN && STACK == kShortStack)  ? 7
+                                       : (LAYOUT == L_WIDE && STACK == kShortWide) ? 6
+                                                                                   : 1) void k_wf_trace(
     WfView V, int ext_level, int shadow_level) {
-   // binary traversal: STACK - 1 LDS entries per lane; the wide one keeps kStack / 2 pairs
-   __shared__ int stack[(LAYOUT == L_BIN ? STACK - 1 : kStack) * kWfBlock];
+   // binary traversal: STACK - 1 LDS entries per lane (the top stays in a register); the
+   // wide one keeps STACK / 2 pairs (the per-ray traversal of mixed layouts kStack / 2)
+   __shared__ int stack[(LAYOUT == L_BIN ? STACK - 1 : (LAYOUT == L_WIDE ? STACK : kStack)) * kWfBlock];
    TravCount te, ts;
    if constexpr (LAYOUT == L_BIN) {
       if (ext_level >= 0) ext_rays_bin<CNT, STACK>(V, ext_level, stack + threadIdx.x, te);
@@ -863,8 +871,8 @@ __global__ __launch_bounds__(kWfBlock, (LAYOUT == L_BIN && STACK == kShortStack)
       // shadow rays refill lanes also in passes with SDF fields (marched inline): the
       // per-ray drain's traversal would raise the kernel to 136 VGPRs (3 waves per SIMD,
       // against 5 with the refilling traversals alone)
-      if (ext_level >= 0) ext_rays_wide<CNT>(V, ext_level, stack + threadIdx.x, te);
-      if (shadow_level >= 0) shadow_rays_wide<CNT>(V, shadow_level, stack + threadIdx.x, ts);
+      if (ext_level >= 0) ext_rays_wide<CNT, STACK>(V, ext_level, stack + threadIdx.x, te);
+      if (shadow_level >= 0) shadow_rays_wide<CNT, STACK>(V, shadow_level, stack + threadIdx.x, ts);
    } else {
    if (ext_level >= 0) {
       unsigned* ctl = V.ctl + kCtl * ext_level;
@@ -1120,6 +1128,8 @@ static rayon_status wave_reserve(rayon_ctx* ctx, int capacity, int levels, int n
       W->occ_trace_wide = n > 0 ? n : 1;
       RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_trace<false, L_BIN, kShortStack>, kWfBlock, 0));
       W->occ_trace_short = n > 0 ? n : 1;
+      RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_trace<false, L_WIDE, kShortWide>, kWfBlock, 0));
+      W->occ_trace_wide_short = n > 0 ? n : 1;
       RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_shade<false>, kWfBlock, 0));
       W->occ_shade = n > 0 ? n : 1;
       RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_sdf<false>, kWfBlock, 0));
@@ -1291,7 +1301,12 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
    int depth = 0;
    for (rayon_job* j : jobs) depth = std::max(depth, j->scene->info.max_depth);
    const bool short_stack = layout == L_BIN && depth <= kShortStack;
-   const dim3 g_trace(grid_of(short_stack        ? W->occ_trace_short
+   // quantized trees no deeper than kShortWide / 2 levels: the wide kernel with the short stack
+   int wdepth = 0;
+   for (rayon_job* j : jobs) wdepth = std::max(wdepth, j->scene->wide_depth);
+   const bool short_wide = layout == L_WIDE && wdepth <= kShortWide / 2;
+   const dim3 g_trace(grid_of(short_stack       ? W->occ_trace_short
+                              : short_wide      ? W->occ_trace_wide_short
                               : layout == L_BIN ? W->occ_trace
                                                 : std::min(W->occ_trace, W->occ_trace_wide)));
    auto trace = [&](int ext, int sh) {
@@ -1302,6 +1317,7 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
       } else {
          if (short_stack) launch_stage(W, 1, k_wf_trace<false, L_BIN, kShortStack>, g_trace, blk, st, V, ext, sh);
          else if (layout == L_BIN) launch_stage(W, 1, k_wf_trace<false, L_BIN>, g_trace, blk, st, V, ext, sh);
+         else if (short_wide) launch_stage(W, 1, k_wf_trace<false, L_WIDE, kShortWide>, g_trace, blk, st, V, ext, sh);
          else if (layout == L_WIDE) launch_stage(W, 1, k_wf_trace<false, L_WIDE>, g_trace, blk, st, V, ext, sh);
          else launch_stage(W, 1, k_wf_trace<false, L_ANY>, g_trace, blk, st, V, ext, sh);
       }
@@ -1561,6 +1577,9 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
       int depth = 0;  // picks the trace kernel's stack size
       for (rayon_job* j : active) depth = std::max(depth, j->scene->info.max_depth);
       key.push_back(depth <= kShortStack);
+      int wdepth = 0;  // picks the quantized-tree kernel's stack size
+      for (rayon_job* j : active) wdepth = std::max(wdepth, j->scene->wide_depth);
+      key.push_back(wdepth <= kShortWide / 2);
       key.push_back(-1);
       key.push_back(nmega);
       key.push_back(mtiles);
