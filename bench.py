@@ -38,6 +38,7 @@ HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per fetch of each device record (layout.h)
 NODE_B, PRIM_B, MAT_B, LIGHT_B = 64, 48, 96, 48
+WNODE_B = 80  # one 8-wide quantized node
 PIXEL_RMW_B = 2 * (16 + 16 + 4)  # acc + comp + count, read and written once per launch
 
 
@@ -106,7 +107,7 @@ def main(argv=None) -> int:
                          "jobs, plain one-job-per-launch vs batched passes; serve: Poisson arrivals into the server")
     ap.add_argument("--jobs", type=int, default=256, help="batch256: number of jobs")
     ap.add_argument("--layout", choices=["binary", "quantized"], default="binary",
-                    help="batch256/serve: BVH node layout of the scenes")
+                    help="BVH node layout of the scenes")
     ap.add_argument("--pass-spp", type=int, default=32, help="batch256: samples per batched pass")
     ap.add_argument("--batch-path", choices=["auto", "wavefront", "megakernel"], default="auto",
                     help="batch256: batched path of every scene (auto: measured per scene by the autotuner)")
@@ -155,8 +156,9 @@ def main(argv=None) -> int:
     if a.refill:
         ctx.set_option(ra.OPT.TRACE_REFILL, a.refill)
     t0 = time.perf_counter()
-    sc = ctx.scene(g.desc_ptr())
+    sc = ctx.scene(g.desc_ptr(), ra.BVH.QUANTIZED if a.layout == "quantized" else ra.BVH.DEFAULT)
     create_ms = (time.perf_counter() - t0) * 1e3
+    node_b = WNODE_B if a.layout == "quantized" else NODE_B  # bytes per visited node
     info = sc.info()
     log(f"scene: {info.num_prims} prims, {info.num_nodes} pair nodes, depth {info.max_depth}, "
         f"build {info.build_ms:.0f} ms, upload {info.upload_ms:.0f} ms, {info.device_bytes / 2**20:.1f} MiB")
@@ -218,7 +220,7 @@ def main(argv=None) -> int:
     cj.render_plain(spp)
     cs = cj.stats()
     cj.close()
-    plain_bps = (NODE_B * cs.node_fetches + PRIM_B * cs.prim_fetches + MAT_B * cs.material_fetches
+    plain_bps = (node_b * cs.node_fetches + PRIM_B * cs.prim_fetches + MAT_B * cs.material_fetches
                  + LIGHT_B * cs.light_fetches) / cs.segments
     p_wall, p_ms, p_stats, p_job = timed_steps("plain")
     p_segs = sum(s.segments for s in p_stats)
@@ -252,7 +254,7 @@ def main(argv=None) -> int:
     sh_rays = cjs.shadow_segments
     # trace kernel: extension rays (nodes + primitives + ray in 48 B + hit out 16 B) and
     # shadow rays (nodes + primitives + ray/contribution in 48 B + radiance RMW 32 B)
-    ext_bytes = (NODE_B * (st_c.extend_node_fetches + st_c.shadow_node_fetches)
+    ext_bytes = (node_b * (st_c.extend_node_fetches + st_c.shadow_node_fetches)
                  + PRIM_B * (st_c.extend_prim_fetches + st_c.shadow_prim_fetches) + 64 * ext_rays + 80 * sh_rays)
     ext_ms = st_t.stage_ms[1]
     ext_launches = st_t.launches[1]
@@ -265,7 +267,7 @@ def main(argv=None) -> int:
     # kernel's counts) for this exact workload, kept beside the kernel-trace summary
     traffic = None
     tf = Path(__file__).resolve().parent / "profiles" / "k_wf_trace_cornell_icosphere_1080p_64spp.traffic.json"
-    if tf.exists() and (W, H, spp, a.depth, a.detail) == (1920, 1080, 64, 8, 8):
+    if tf.exists() and (W, H, spp, a.depth, a.detail, a.layout) == (1920, 1080, 64, 8, 8, "binary"):
         tr = json.loads(tf.read_text())
         traffic = round(tr["hbm_bytes_per_segment"] * (ext_rays + sh_rays) / max(ext_launches, 1))
 
@@ -323,6 +325,7 @@ def main(argv=None) -> int:
                         f"{W}x{H}, {spp} spp, max depth {a.depth}, NEE, GGX",
             "path": "batched wavefront (hipGraph pass)" if use_batched else "plain megakernel",
             "width": W, "height": H, "spp": spp, "max_depth": a.depth, "triangles": int(ntris),
+            "layout": a.layout,
         },
         "plain_mrays_per_s": round(plain_mrays, 2),
         "batched_mrays_per_s": round(batched_mrays, 2),
