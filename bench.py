@@ -39,6 +39,7 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per fetch of each device record (layout.h)
 NODE_B, PRIM_B, MAT_B, LIGHT_B = 64, 48, 96, 48
 WNODE_B = 80  # one 8-wide quantized node
+REF_NODE_B, REF_PRIM_B = 64, 112  # the reference's BVHNode and Geometry records
 PIXEL_RMW_B = 2 * (16 + 16 + 4)  # acc + comp + count, read and written once per launch
 
 
@@ -102,9 +103,12 @@ def main(argv=None) -> int:
     ap.add_argument("--chunk", type=int, default=0, help="queue entries (x64) a trace wave claims (0: default)")
     ap.add_argument("--shade-chunk", type=int, default=0, help="queue entries (x64) a shade wave claims (0: default)")
     ap.add_argument("--refill", type=int, default=0, help="idle lanes at which a trace wave takes new rays (0: default)")
-    ap.add_argument("--workload", choices=["cornell_1080p", "batch256", "serve"], default="cornell_1080p",
+    ap.add_argument("--workload", choices=["cornell_1080p", "batch256", "serve", "book_640", "sdf_720p"],
+                    default="cornell_1080p",
                     help="cornell_1080p: one 1080p 64-spp mesh frame per step (default); batch256: 256 mixed "
-                         "jobs, plain one-job-per-launch vs batched passes; serve: Poisson arrivals into the server")
+                         "jobs, plain one-job-per-launch vs batched passes; serve: Poisson arrivals into the "
+                         "server; book_640: the sphere field, 640x360, 32 spp, depth 50, CPU baseline at full "
+                         "size; sdf_720p: SDF + terrain, 1280x720, adaptive 16..1024 spp to 0.5% std error")
     ap.add_argument("--jobs", type=int, default=256, help="batch256: number of jobs")
     ap.add_argument("--layout", choices=["binary", "quantized"], default="binary",
                     help="BVH node layout of the scenes")
@@ -142,6 +146,8 @@ def main(argv=None) -> int:
         return bench_batch256(a, ra)
     if a.workload == "serve":
         return bench_serve(a, ra)
+    if a.workload in ("book_640", "sdf_720p"):
+        return bench_scene(a, ra)
 
     log("generating scene")
     g = ra.Generated.generate(ra.GEN.CORNELL_ICOSPHERE, 7, detail=a.detail)
@@ -259,6 +265,11 @@ def main(argv=None) -> int:
     ext_ms = st_t.stage_ms[1]
     ext_launches = st_t.launches[1]
     ext_achieved = ext_bytes / (ext_ms * 1e-3) / 1e9
+    # the same visits priced at the reference's record sizes (64 B per node, 112 B per
+    # primitive record): a fixed yardstick independent of this build's layouts
+    ref_bytes = (REF_NODE_B * (st_c.extend_node_fetches + st_c.shadow_node_fetches)
+                 + REF_PRIM_B * (st_c.extend_prim_fetches + st_c.shadow_prim_fetches) + 64 * ext_rays + 80 * sh_rays)
+    ref_achieved = ref_bytes / (ext_ms * 1e-3) / 1e9
     stage_names = ["generate", "trace", "shade", "megakernel", "accumulate", "bookkeeping", "sdf_march", "unused"]
     stage_ms = {n: round(st_t.stage_ms[k], 2) for k, n in enumerate(stage_names) if n != "unused"}
     log(f"stages (ms per frame): {stage_ms}; trace {ext_achieved:.0f} GB/s algorithmic")
@@ -301,6 +312,9 @@ def main(argv=None) -> int:
                 "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4), "launches_per_frame": int(ext_launches),
                 "alg_bytes_per_ray": round(ext_bytes / max(ext_rays + sh_rays, 1), 1),
                 "alg_bytes_per_launch": round(ext_bytes / max(ext_launches, 1)),
+                "achieved_ref_layout": round(ref_achieved, 1),
+                "ref_layout_note": "the same node/primitive visits at the reference's record sizes "
+                                   "(64 B per node, 112 B per primitive)",
                 "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)"}
     else:
         roof = {"bound": "hbm", "achieved": round(plain_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -579,6 +593,118 @@ def bench_batch256(a, ra) -> int:
         sc.close()
     ctx.close()
     print(json.dumps(result), flush=True)
+    return 0
+
+
+def bench_scene(a, ra) -> int:
+    """The sphere field (640x360, 32 spp, depth 50) or the SDF + terrain scene (1280x720,
+    adaptive: 16 spp, then passes of 64 until every pixel's relative std error is below 0.5 %
+    or it has 1024 samples). Plain kernel and batched path timed on the same job; the CPU baseline is
+    the FP64 oracle on the host cores, over every pixel of the sphere field and over a pixel
+    subset (every 4th row and column, 16 spp) of the SDF scene, where the relative RMSE is
+    taken against a 16-spp GPU render of the same pixels and streams."""
+    book = a.workload == "book_640"
+    gen = 0 if book else 2
+    kind = ra.GEN.BOOK_SPHERES if book else ra.GEN.SDF_TERRAIN
+    g = ra.Generated.generate(kind, 1984 if book else 11)
+    ctx = ra.Context(0)
+    sc = ctx.scene(g.desc_ptr(), ra.BVH.QUANTIZED if a.layout == "quantized" else ra.BVH.DEFAULT)
+    W, H = (640, 360) if book else (1280, 720)
+    spec = dict(gen=gen, W=W, H=H, spp=32 if book else 1024, seed=1234)
+    extra = {} if book else dict(extra_flags=ra.FLAG.ADAPTIVE_STDERR, min_adaptive_samples=16, adaptive_threshold=0.005)
+    depth = 50 if book else 8
+    P, cam = job_params_for(ra, g, spec, depth=depth, **dict(extra))
+
+    def render(batched):
+        j = ctx.job(sc, P)
+        ctx.sync()
+        t0 = time.perf_counter()
+        if book:
+            if batched:
+                ctx.render_batched([j], 32)
+            else:
+                j.render_plain(32)
+        else:  # the first 16 samples, then passes of 64 until the job is done
+            n = 16
+            while not j.stats().done:
+                if batched:
+                    ctx.render_batched([j], n)
+                else:
+                    j.render_plain(n)
+                n = 64
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        st = j.stats()
+        return j, wall, st
+
+    res = {}
+    keep = None  # the last batched frame, for the image comparison
+    for name, batched in (("plain", False), ("batched", True)):
+        j, _, _ = render(batched)  # warm-up
+        j.close()
+        walls, segs = [], 0
+        for _ in range(max(1, a.steps)):
+            j, wall, st = render(batched)
+            walls.append(wall)
+            segs += st.segments
+            last = st
+            if keep is not None:
+                keep.close()
+                keep = None
+            if batched:
+                keep = j
+            else:
+                j.close()
+        res[name] = {"mrays_per_s": round(segs / sum(walls) / 1e6, 2), "ms_per_frame": round(1e3 * np.mean(walls), 2),
+                     "segments_per_frame": int(last.segments), "samples": int(last.samples_done),
+                     "converged_pixels": int(last.converged_pixels)}
+        log(f"{name}: {res[name]}")
+    out = {"metric": METRIC, "value": res["batched"]["mrays_per_s"], "unit": "Mrays/s", "n_gpus": 1,
+           "steps": max(1, a.steps), "warmup": 1, "ms_per_step": res["batched"]["ms_per_frame"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (procedural scene generated in-process)",
+           "config": {"workload": ("book spheres (~480 spheres), 640x360, 32 spp, depth 50" if book else
+                                   "SDF (64 primitives, smooth union k=0.1) + 200k-triangle terrain, 1280x720, "
+                                   "adaptive 16..1024 spp to 0.5% relative std error (passes of 16, then 64), depth 8"),
+                      "layout": a.layout, "scene_path": int(sc.batch_path)},
+           "plain_mrays_per_s": res["plain"]["mrays_per_s"], "batched_mrays_per_s": res["batched"]["mrays_per_s"],
+           "batched_over_plain": round(res["batched"]["mrays_per_s"] / max(res["plain"]["mrays_per_s"], 1e-9), 3),
+           "frames": res}
+    if not a.no_cpu_baseline:
+        from oracle import OracleScene
+        cores = min(16, os.cpu_count() or 1)
+        osc = OracleScene(g.desc_ptr())
+        if book:
+            pix, nspp, Pc = None, 32, P
+            img = keep.resolve_linear().reshape(-1, 3).astype(np.float64)
+        else:
+            ys, xs = np.meshgrid(np.arange(2, H, 4), np.arange(2, W, 4), indexing="ij")
+            pix = (ys * W + xs).reshape(-1).astype(np.int64)
+            nspp = 16
+            Pc, _ = job_params_for(ra, g, dict(spec, spp=16), depth=depth)
+            fj = ctx.job(sc, Pc)
+            ctx.render_batched([fj], 16)
+            img = fj.resolve_linear().reshape(-1, 3).astype(np.float64)[pix]
+            fj.close()
+        t0 = time.perf_counter()
+        sums, osegs = osc.render(Pc, pixels=pix, nsamples=nspp, threads=cores, camera=cam)
+        t_cpu = time.perf_counter() - t0
+        ref = sums / nspp
+        err = float(np.sqrt(((img - ref) ** 2).sum() / (ref ** 2).sum()))
+        npx = W * H if pix is None else len(pix)
+        out["cpu_baseline"] = {"value": round(osegs / t_cpu / 1e6, 3), "unit": "Mrays/s", "cores": cores,
+                               "kind": "port",
+                               "sample": (f"FP64 oracle, {'every pixel' if book else 'every 4th row and column'} "
+                                          f"({npx} pixels x {nspp} spp, {osegs} segments, {t_cpu:.1f} s)"),
+                               "cpu": cpu_model()}
+        out["rel_rmse_vs_oracle"] = round(err, 6)
+        out["rmse_note"] = f"same random streams, {nspp} spp, {npx} pixels, linear RGB"
+        log(f"cpu baseline {osegs / t_cpu / 1e6:.2f} Mrays/s on {cores} cores, rel RMSE {err:.6f}")
+        osc.close()
+    keep.close()
+    sc.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
     return 0
 
 
