@@ -184,9 +184,19 @@ struct QueueRes {
 // when it can tell the queue is already drained: an empty queue, a head already past the
 // end (a plain read, which can only lag behind the true head), or a claim of its own that
 // reached the end.
+// Batches of 64 queue entries a wave claims at once: `chunk`, or fewer when the queue is too
+// short to give every wave of the launch that many, so a short queue (a pass's deep levels,
+// a small pass) spreads over all waves instead of lining up behind a few of them.
+__device__ __forceinline__ int fit_chunk(unsigned count, int chunk) {
+   const unsigned waves = gridDim.x * (blockDim.x >> 6);
+   const unsigned per = (count + 64u * waves - 1u) / (64u * waves);
+   return (int)min((unsigned)chunk, max(per, 1u));
+}
+
 template <class F>
 __device__ __forceinline__ void drain(unsigned* head, unsigned count, int chunk, F&& body) {
    if (count == 0) return;
+   chunk = fit_chunk(count, chunk);
    unsigned seen = 0;
    if (lane_id() == 0) seen = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
    if (__shfl(seen, 0) >= count) return;
@@ -402,7 +412,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_sdf_rest(WfView V, int level) {
    unsigned long long n_steps = 0, n_evals = 0, n_slots = 0;
    unsigned* cin = V.sctl + 2 * ((kSdfSlices + 1) * level + 1);
    const unsigned count = cin[0];
-   const unsigned step = 64u * (unsigned)V.chunk;
+   const unsigned step = 64u * (unsigned)fit_chunk(count, V.chunk);
    WavePool pool = start_pool(&cin[1], count);
    int slot = -1;
    const SceneView* S = nullptr;
@@ -461,7 +471,7 @@ __device__ void ext_rays_bin(const WfView& V, int level, int* lds, TravCount& tc
    const unsigned count = ctl[0];
    const int* q = (level & 1) ? V.q1 : V.q0;
    unsigned* mctl = V.sctl + 2 * (kSdfSlices + 1) * level;
-   const unsigned step = 64u * (unsigned)V.chunk;
+   const unsigned step = 64u * (unsigned)fit_chunk(count, V.chunk);
    WavePool pool = start_pool(&ctl[1], count);
    int slot = -1, cur = kSentinel;
    // slot of a finished ray whose SDF march is still to be queued. The lanes' marches are
@@ -571,7 +581,7 @@ template <bool CNT, int STACK>
 __device__ void shadow_rays_bin(const WfView& V, int level, int* lds, TravCount& tc) {
    unsigned* sctl = V.ctl + kCtl * level + 3;
    const unsigned count = sctl[0];
-   const unsigned step = 64u * (unsigned)V.chunk;
+   const unsigned step = 64u * (unsigned)fit_chunk(count, V.chunk);
    WavePool pool = start_pool(&sctl[1], count);
    int slot = -1, cur = kSentinel;
    const SceneView* S = nullptr;
@@ -706,7 +716,7 @@ __device__ void ext_rays_wide(const WfView& V, int level, int* lds, TravCount& t
    const unsigned count = ctl[0];
    const int* q = (level & 1) ? V.q1 : V.q0;
    unsigned* mctl = V.sctl + 2 * (kSdfSlices + 1) * level;
-   const unsigned step = 64u * (unsigned)V.chunk;
+   const unsigned step = 64u * (unsigned)fit_chunk(count, V.chunk);
    WavePool pool = start_pool(&ctl[1], count);
    int slot = -1, pend = -1, sp = 0;
    uint32_t node = kWideDone;
@@ -770,7 +780,7 @@ template <bool CNT, int STACK>
 __device__ void shadow_rays_wide(const WfView& V, int level, int* lds, TravCount& tc) {
    unsigned* sctl = V.ctl + kCtl * level + 3;
    const unsigned count = sctl[0];
-   const unsigned step = 64u * (unsigned)V.chunk;
+   const unsigned step = 64u * (unsigned)fit_chunk(count, V.chunk);
    WavePool pool = start_pool(&sctl[1], count);
    int slot = -1, sp = 0;
    uint32_t node = kWideDone;
@@ -934,7 +944,7 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_wf_shade(WfView V, int level) {
    int* qn = (level & 1) ? V.q0 : V.q1;
    unsigned* ctl_next = V.ctl + kCtl * (level + 1);
    unsigned* sctl = ctl + 3;
-   const int chunk = min(V.shade_chunk, kMaxChunk);
+   const int chunk = fit_chunk(count, min(V.shade_chunk, kMaxChunk));
    const unsigned long long lt = (1ull << lane) - 1ull;
    PathCounts pc;
    unsigned seen = count;  // skip the claims of an empty or drained queue, as drain() does
