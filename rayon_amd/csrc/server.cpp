@@ -129,10 +129,10 @@ struct rayon_server {
       if (j->frame && !j->previewing) {
          (void)hipFreeAsync(j->frame, first_ctx->stream);
          j->frame = nullptr;
-         if (j->preview_held) {
-            preview_bytes -= frame_bytes(j);
-            j->preview_held = false;
-         }
+      }
+      if (!j->frame && j->preview_held) {  // (also a reserve whose allocation failed)
+         preview_bytes -= frame_bytes(j);
+         j->preview_held = false;
       }
    }
 
