@@ -136,8 +136,9 @@ def main(argv=None) -> int:
         return 0
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
-              "(the sharding design is described in DESIGN.md).", file=sys.stderr)
+        print("bench.py: runs on one GPU only; jobs shard across GPUs with no exchange, and one large "
+              "job by sample range with one sum of the per-pixel accumulators, but neither is built "
+              "here.", file=sys.stderr)
         return 2
 
     import rayon_amd as ra
@@ -266,7 +267,7 @@ def main(argv=None) -> int:
     ext_launches = st_t.launches[1]
     ext_achieved = ext_bytes / (ext_ms * 1e-3) / 1e9
     # the same visits priced at the reference's record sizes (64 B per node, 112 B per
-    # primitive record): a fixed yardstick independent of this build's layouts
+    # primitive record): a fixed yardstick independent of the layouts used here
     ref_bytes = (REF_NODE_B * (st_c.extend_node_fetches + st_c.shadow_node_fetches)
                  + REF_PRIM_B * (st_c.extend_prim_fetches + st_c.shadow_prim_fetches) + 64 * ext_rays + 80 * sh_rays)
     ref_achieved = ref_bytes / (ext_ms * 1e-3) / 1e9
