@@ -513,28 +513,40 @@ __device__ void ext_rays_bin(const WfView& V, int level, int* lds, TravCount& tc
       if (__ballot(slot >= 0) == 0) break;
       if (slot >= 0) {
          bool finished;
-         while (cur >= 0) {
-            const Node2 n = gload(nodes + cur);
-            if constexpr (CNT) ++tc.nodes;
-            float tn0, tn1;
-            bool h0, h1;
-            box2(r, n, 0.001f, best.t, tn0, tn1, h0, h1);
-            const int r0 = n.meta.z ? pack_leaf(n.meta.x, n.meta.z) : n.meta.x;
-            const int r1 = n.meta.w ? pack_leaf(n.meta.y, n.meta.w) : n.meta.y;
-            if (h0 && h1) {
-               const bool swap = tn1 < tn0;
-               st.push(swap ? r0 : r1);
-               cur = swap ? r1 : r0;
-            } else if (h0) {
-               cur = r0;
-            } else if (h1) {
-               cur = r1;
-            } else {
+         // Internal nodes until every lane of the wave holds a leaf or has finished: the
+         // first leaf a lane meets is set aside and the lane goes on with its next node
+         // (postponed leaves, Aila and Laine 2009), so lanes that reach a leaf early keep
+         // working while the others descend. The closest hit does not depend on the order
+         // of the primitive tests (equal distances go to the larger original index).
+         int leaf = kSentinel;
+         for (;;) {
+            if (cur >= 0) {
+               const Node2 n = gload(nodes + cur);
+               if constexpr (CNT) ++tc.nodes;
+               float tn0, tn1;
+               bool h0, h1;
+               box2(r, n, 0.001f, best.t, tn0, tn1, h0, h1);
+               const int r0 = n.meta.z ? pack_leaf(n.meta.x, n.meta.z) : n.meta.x;
+               const int r1 = n.meta.w ? pack_leaf(n.meta.y, n.meta.w) : n.meta.y;
+               if (h0 && h1) {
+                  const bool swap = tn1 < tn0;
+                  st.push(swap ? r0 : r1);
+                  cur = swap ? r1 : r0;
+               } else if (h0) {
+                  cur = r0;
+               } else if (h1) {
+                  cur = r1;
+               } else {
+                  cur = st.pop();
+               }
+            } else if (cur != kSentinel && leaf == kSentinel) {
+               leaf = cur;
                cur = st.pop();
             }
+            if (__all(leaf != kSentinel || cur == kSentinel)) break;
          }
-         if (cur != kSentinel) {  // one leaf
-            const int packed = ~cur;
+         if (leaf != kSentinel) {  // one leaf
+            const int packed = ~leaf;
             const int first = packed >> 3, cnt = (packed & 7) + 1;
             for (int i = first; i < first + cnt; ++i) {
                const PrimRec p = gload(prims + i);
@@ -561,7 +573,6 @@ __device__ void ext_rays_bin(const WfView& V, int level, int* lds, TravCount& tc
                   }
                }
             }
-            cur = st.pop();
          }
          finished = cur == kSentinel;
          if (finished) {
@@ -603,27 +614,34 @@ __device__ void shadow_rays_bin(const WfView& V, int level, int* lds, TravCount&
       if (__ballot(slot >= 0) == 0) break;
       if (slot >= 0) {
          bool occluded = false;
-         while (cur >= 0) {
-            const Node2 n = gload(S->nodes + cur);
-            if constexpr (CNT) ++tc.nodes;
-            float tn0, tn1;
-            bool h0, h1;
-            box2(r, n, tmin, tmax, tn0, tn1, h0, h1);
-            const int r0 = n.meta.z ? pack_leaf(n.meta.x, n.meta.z) : n.meta.x;
-            const int r1 = n.meta.w ? pack_leaf(n.meta.y, n.meta.w) : n.meta.y;
-            if (h0 && h1) {
-               st.push(r1);
-               cur = r0;
-            } else if (h0) {
-               cur = r0;
-            } else if (h1) {
-               cur = r1;
-            } else {
+         int leaf = kSentinel;  // postponed leaves, as for extension rays
+         for (;;) {
+            if (cur >= 0) {
+               const Node2 n = gload(S->nodes + cur);
+               if constexpr (CNT) ++tc.nodes;
+               float tn0, tn1;
+               bool h0, h1;
+               box2(r, n, tmin, tmax, tn0, tn1, h0, h1);
+               const int r0 = n.meta.z ? pack_leaf(n.meta.x, n.meta.z) : n.meta.x;
+               const int r1 = n.meta.w ? pack_leaf(n.meta.y, n.meta.w) : n.meta.y;
+               if (h0 && h1) {
+                  st.push(r1);
+                  cur = r0;
+               } else if (h0) {
+                  cur = r0;
+               } else if (h1) {
+                  cur = r1;
+               } else {
+                  cur = st.pop();
+               }
+            } else if (cur != kSentinel && leaf == kSentinel) {
+               leaf = cur;
                cur = st.pop();
             }
+            if (__all(leaf != kSentinel || cur == kSentinel)) break;
          }
-         if (cur != kSentinel) {  // one leaf
-            const int packed = ~cur;
+         if (leaf != kSentinel) {  // one leaf
+            const int packed = ~leaf;
             const int first = packed >> 3, cnt = (packed & 7) + 1;
             for (int i = first; i < first + cnt && !occluded; ++i) {
                const PrimRec p = gload(S->prims + i);
@@ -641,7 +659,7 @@ __device__ void shadow_rays_bin(const WfView& V, int level, int* lds, TravCount&
                else
                   occluded = isect_sdf(*S, p, r.o, r.d, tmin, tmax, t);
             }
-            cur = occluded ? kSentinel : st.pop();
+            if (occluded) cur = kSentinel;
          }
          if (cur == kSentinel) {
             if (!occluded) {
