@@ -766,22 +766,40 @@ __device__ void ext_rays_wide(const WfView& V, int level, int* lds, TravCount& t
       if (__ballot(slot >= 0) == 0) break;
       if (slot >= 0) {
          const WNode* wn = S->wnodes;
+         // Postponed leaves, as on binary trees: the hit leaf children of the first node that
+         // has any are set aside (mask, meta words, primitive base) and the lane goes on with
+         // its next node, until every lane of the wave holds leaves or has finished. A lane
+         // that meets another node with hit leaves meanwhile waits on it and visits it again
+         // after its leaves.
+         uint32_t pmask = 0, pm0 = 0, pm1 = 0, pbase = 0;
+         bool held = false;
          for (;;) {
-            const WNode n = load_wnode(wn, node);
-            if constexpr (CNT) ++tc.nodes;
-            uint32_t hit_int, hit_leaf;
-            wide_children(n, r.o, r.inv, r.perm, 0.001f, best.t, hit_int, hit_leaf);
-            const bool leaves = hit_leaf != 0;
-            while (hit_leaf) {
-               const int sl = __ffs(hit_leaf) - 1;
-               hit_leaf &= hit_leaf - 1u;
-               const uint32_t m = ((sl < 4 ? n.meta[0] : n.meta[1]) >> (8 * (sl & 3))) & 0xFFu;
-               const int first = (int)n.prim_base + (int)(m & 31u);
-               const int cnt = (int)(m >> 5) + 1;
-               for (int i = first; i < first + cnt; ++i) test_closest<CNT, true>(*S, i, r.o, r.d, 0.001f, best, &tc);
+            if (node != kWideDone && !held) {
+               const WNode n = load_wnode(wn, node);
+               uint32_t hit_int, hit_leaf;
+               wide_children(n, r.o, r.inv, r.perm, 0.001f, best.t, hit_int, hit_leaf);
+               if (hit_leaf && pmask) {
+                  held = true;
+               } else {
+                  if constexpr (CNT) ++tc.nodes;
+                  if (hit_leaf) {
+                     pmask = hit_leaf;
+                     pm0 = n.meta[0];
+                     pm1 = n.meta[1];
+                     pbase = n.prim_base;
+                  }
+                  node = wide_next<STACK>(hit_int, n.child_base, n.e_imask >> 24, r.perm, lds, sp);
+               }
             }
-            node = wide_next<STACK>(hit_int, n.child_base, n.e_imask >> 24, r.perm, lds, sp);
-            if (leaves || node == kWideDone) break;
+            if (__all(pmask != 0 || node == kWideDone)) break;
+         }
+         while (pmask) {
+            const int sl = __ffs(pmask) - 1;
+            pmask &= pmask - 1u;
+            const uint32_t m = ((sl < 4 ? pm0 : pm1) >> (8 * (sl & 3))) & 0xFFu;
+            const int first = (int)pbase + (int)(m & 31u);
+            const int cnt = (int)(m >> 5) + 1;
+            for (int i = first; i < first + cnt; ++i) test_closest<CNT, true>(*S, i, r.o, r.d, 0.001f, best, &tc);
          }
          if (node == kWideDone) {
             V.hit[slot] = make_float4(best.t, __int_as_float(best.slot), best.u, best.v);
@@ -820,24 +838,38 @@ __device__ void shadow_rays_wide(const WfView& V, int level, int* lds, TravCount
       if (slot >= 0) {
          const WNode* wn = S->wnodes;
          bool occluded = false;
+         uint32_t pmask = 0, pm0 = 0, pm1 = 0, pbase = 0;  // postponed leaves, as for extension rays
+         bool held = false;
          for (;;) {
-            const WNode n = load_wnode(wn, node);
-            if constexpr (CNT) ++tc.nodes;
-            uint32_t hit_int, hit_leaf;
-            wide_children(n, r.o, r.inv, r.perm, tmin, tmax, hit_int, hit_leaf);
-            const bool leaves = hit_leaf != 0;
-            while (hit_leaf && !occluded) {
-               const int sl = __ffs(hit_leaf) - 1;
-               hit_leaf &= hit_leaf - 1u;
-               const uint32_t m = ((sl < 4 ? n.meta[0] : n.meta[1]) >> (8 * (sl & 3))) & 0xFFu;
-               const int first = (int)n.prim_base + (int)(m & 31u);
-               const int cnt = (int)(m >> 5) + 1;
-               for (int i = first; i < first + cnt && !occluded; ++i)
-                  occluded = test_any<CNT>(*S, i, r.o, r.d, tmin, tmax, &tc);
+            if (node != kWideDone && !held) {
+               const WNode n = load_wnode(wn, node);
+               uint32_t hit_int, hit_leaf;
+               wide_children(n, r.o, r.inv, r.perm, tmin, tmax, hit_int, hit_leaf);
+               if (hit_leaf && pmask) {
+                  held = true;
+               } else {
+                  if constexpr (CNT) ++tc.nodes;
+                  if (hit_leaf) {
+                     pmask = hit_leaf;
+                     pm0 = n.meta[0];
+                     pm1 = n.meta[1];
+                     pbase = n.prim_base;
+                  }
+                  node = wide_next<STACK>(hit_int, n.child_base, n.e_imask >> 24, r.perm, lds, sp);
+               }
             }
-            node = occluded ? kWideDone : wide_next<STACK>(hit_int, n.child_base, n.e_imask >> 24, r.perm, lds, sp);
-            if (leaves || node == kWideDone) break;
+            if (__all(pmask != 0 || node == kWideDone)) break;
          }
+         while (pmask && !occluded) {
+            const int sl = __ffs(pmask) - 1;
+            pmask &= pmask - 1u;
+            const uint32_t m = ((sl < 4 ? pm0 : pm1) >> (8 * (sl & 3))) & 0xFFu;
+            const int first = (int)pbase + (int)(m & 31u);
+            const int cnt = (int)(m >> 5) + 1;
+            for (int i = first; i < first + cnt && !occluded; ++i)
+               occluded = test_any<CNT>(*S, i, r.o, r.d, tmin, tmax, &tc);
+         }
+         if (occluded) node = kWideDone;
          if (node == kWideDone) {
             if (!occluded) {
                const float4 c = V.sc[slot];
