@@ -230,6 +230,15 @@ rayon_status rayon_ctx_create(int32_t device, rayon_ctx** out) {
 rayon_status rayon_ctx_destroy(rayon_ctx* c) {
    if (!c) return RAYON_OK;
    (void)hipSetDevice(c->device);
+   if (c->serve_pv) {
+      (void)hipStreamSynchronize(c->serve_pv);
+      (void)hipStreamDestroy(c->serve_pv);
+      c->serve_pv = nullptr;
+   }
+   if (c->serve_first) {
+      (void)rayon_ctx_destroy(c->serve_first);
+      c->serve_first = nullptr;
+   }
    (void)hipStreamSynchronize(c->stream);
    wave_destroy(c);
    (void)hipStreamDestroy(c->stream);

