@@ -121,6 +121,8 @@ def main(argv=None) -> int:
     ap.add_argument("--pool", type=int, default=4, help="serve: resident scenes per generator")
     ap.add_argument("--target-pass-ms", type=float, default=0.0,
                     help="serve: refinement pass duration the server aims for (0: library default)")
+    ap.add_argument("--fixed-spp", type=int, default=0,
+                    help="sdf_720p: render this many samples per pixel without the adaptive rule (0: adaptive)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device-to-device 4 times (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     a = ap.parse_args(argv)
@@ -611,8 +613,10 @@ def bench_scene(a, ra) -> int:
     ctx = ra.Context(0)
     sc = ctx.scene(g.desc_ptr(), ra.BVH.QUANTIZED if a.layout == "quantized" else ra.BVH.DEFAULT)
     W, H = (640, 360) if book else (1280, 720)
-    spec = dict(gen=gen, W=W, H=H, spp=32 if book else 1024, seed=1234)
-    extra = {} if book else dict(extra_flags=ra.FLAG.ADAPTIVE_STDERR, min_adaptive_samples=16, adaptive_threshold=0.005)
+    fixed = 0 if book else a.fixed_spp
+    spec = dict(gen=gen, W=W, H=H, spp=32 if book else (fixed or 1024), seed=1234)
+    extra = {} if (book or fixed) else dict(extra_flags=ra.FLAG.ADAPTIVE_STDERR, min_adaptive_samples=16,
+                                            adaptive_threshold=0.005)
     depth = 50 if book else 8
     P, cam = job_params_for(ra, g, spec, depth=depth, **dict(extra))
 
@@ -620,11 +624,11 @@ def bench_scene(a, ra) -> int:
         j = ctx.job(sc, P)
         ctx.sync()
         t0 = time.perf_counter()
-        if book:
+        if book or fixed:
             if batched:
-                ctx.render_batched([j], 32)
+                ctx.render_batched([j], spec["spp"])
             else:
-                j.render_plain(32)
+                j.render_plain(spec["spp"])
         else:  # the first 16 samples, then passes of 64 until the job is done
             n = 16
             while not j.stats().done:
@@ -666,7 +670,9 @@ def bench_scene(a, ra) -> int:
            "data": "synthetic (procedural scene generated in-process)",
            "config": {"workload": ("book spheres (~480 spheres), 640x360, 32 spp, depth 50" if book else
                                    "SDF (64 primitives, smooth union k=0.1) + 200k-triangle terrain, 1280x720, "
-                                   "adaptive 16..1024 spp to 0.5% relative std error (passes of 16, then 64), depth 8"),
+                                   + (f"{fixed} spp" if fixed else
+                                      "adaptive 16..1024 spp to 0.5% relative std error (passes of 16, then 64)")
+                                   + ", depth 8"),
                       "layout": a.layout, "scene_path": int(sc.batch_path)},
            "plain_mrays_per_s": res["plain"]["mrays_per_s"], "batched_mrays_per_s": res["batched"]["mrays_per_s"],
            "batched_over_plain": round(res["batched"]["mrays_per_s"] / max(res["plain"]["mrays_per_s"], 1e-9), 3),
