@@ -318,7 +318,11 @@ def main(argv=None) -> int:
                 "achieved_ref_layout": round(ref_achieved, 1),
                 "ref_layout_note": "the same node/primitive visits at the reference's record sizes "
                                    "(64 B per node, 112 B per primitive)",
-                "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)"}
+                "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
+                "traffic_frac": (None if traffic is None else
+                                 round(traffic / (ext_ms / max(ext_launches, 1) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)),
+                "frac_note": "achieved counts the bytes the traversal asks for; above 1 when L2 and the MALL "
+                             "serve them; traffic_frac is the measured HBM traffic over the same launch time"}
     else:
         roof = {"bound": "hbm", "achieved": round(plain_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(plain_achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_plain",
