@@ -424,12 +424,14 @@ rayon_status rayon_progressive_destroy(rayon_progressive* p);
 
 /* ---------------------------------------------------------------------------------
  * Render server: continuous batching of many jobs on one context (no counterpart in the
- * reference). A server thread admits queued jobs by HBM budget, renders every newly
- * admitted job's first frame (on a stream of its own, overlapping the pass in flight)
- * and resolves it on the device, advances the
- * oldest active jobs together in batched passes whose work (pixels x samples) a
- * controller sizes toward target_pass_ms, and retires jobs whose sample budget is spent
- * or whose pixels all converged.
+ * reference). A server thread admits queued jobs by HBM budget (seven eighths of it; the
+ * rest holds the first frames of jobs still queued), renders each job's first frame as
+ * soon as it is admitted or its frame fits that reserve (one kernel on a first-frame
+ * stream, overlapping the pass in flight, resolved on the device), advances the oldest
+ * active jobs together in batched passes whose work (pixels x samples) a controller sizes
+ * toward target_pass_ms, and retires jobs whose sample budget is spent or whose pixels all
+ * converged. The first-frame streams belong to the context (created by its first server,
+ * destroyed with the context).
  * ------------------------------------------------------------------------------- */
 typedef struct rayon_server rayon_server;
 
@@ -444,8 +446,8 @@ enum {
 typedef struct rayon_server_config {
    int64_t hbm_budget_bytes;    /* accumulation + frame buffers + referenced scenes      */
    int32_t max_active_jobs;
-   int32_t first_pass_samples;  /* samples of a job's first frame (a plain-kernel preview
-                                   of its first samples, outside its sums)              */
+   int32_t first_pass_samples;  /* samples of a job's first frame (the plain kernel's
+                                   arithmetic for its first samples, outside its sums)  */
    int32_t max_pass_samples;    /* cap of the controlled pass size                       */
    int32_t pad_;
    double target_pass_ms;       /* controller target for one batched pass               */
