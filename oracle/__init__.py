@@ -60,8 +60,70 @@ def lib() -> C.CDLL:
         L.oracle_scatter_probe.argtypes = [P(Material), C.c_uint32, C.c_float, C.c_float, C.c_float,
                                            P(C.c_double), P(C.c_double), i32, P(C.c_double), P(C.c_float), i32,
                                            P(C.c_double), P(C.c_double), P(C.c_double), P(i32)]
+        d3 = P(C.c_double)
+        L.oracle_set_fault.argtypes = [i32]
+        L.oracle_sample_vndf.argtypes = [d3, C.c_double, C.c_double, d3, i64, d3]
+        L.oracle_ggx_lambda.restype = C.c_double
+        L.oracle_ggx_lambda.argtypes = [d3, C.c_double, C.c_double]
+        L.oracle_fresnel_dielectric.restype = C.c_double
+        L.oracle_fresnel_dielectric.argtypes = [C.c_double, C.c_double]
+        L.oracle_fresnel_conductor.restype = C.c_double
+        L.oracle_fresnel_conductor.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.oracle_unit_disk.restype = i64
+        L.oracle_unit_disk.argtypes = [P(C.c_float), i64, i64, d3]
+        L.oracle_scatter_batch.argtypes = [P(Material), C.c_uint32, C.c_float, C.c_float, C.c_float, d3, d3, i32,
+                                           d3, P(C.c_float), i32, i64, d3, d3, P(i32)]
         _lib = L
     return _lib
+
+
+class fault:
+    """Context manager planting one of the oracle's known mistakes (tests of the tests)."""
+
+    VNDF_NO_WARP, VNDF_SWAP_ALPHA, CONDUCTOR_RP_SIGN, DIELECTRIC_RP, DISK_NO_REJECT, LAMBDA_HALF = 1, 2, 3, 4, 5, 6
+
+    def __init__(self, which: int):
+        self.which = which
+
+    def __enter__(self):
+        lib().oracle_set_fault(self.which)
+        return self
+
+    def __exit__(self, *exc):
+        lib().oracle_set_fault(0)
+        return False
+
+
+def sample_vndf(wo, ax, ay, u):
+    """VNDF normals for the (n, 2) uniforms u."""
+    w = np.ascontiguousarray(wo, np.float64)
+    uu = np.ascontiguousarray(u, np.float64)
+    out = np.empty((uu.shape[0], 3), np.float64)
+    lib().oracle_sample_vndf(_dp(w), ax, ay, _dp(uu), uu.shape[0], _dp(out))
+    return out
+
+
+def unit_disk(rnd, n):
+    r = np.ascontiguousarray(rnd, np.float32)
+    out = np.empty((n, 2), np.float64)
+    used = lib().oracle_unit_disk(_fp(r), r.size, n, _dp(out))
+    return out, used
+
+
+def scatter_batch(m: Material, dir_, normal, rnd, front=True, tangent=(1.0, 0.0, 0.0), flags=0, fuzz=0.8,
+                  glass_ior=1.5, light=1.0):
+    """Scatters `len(rnd)` samples of one incident direction; rnd is (count, k) uniforms."""
+    r = np.ascontiguousarray(rnd, np.float32)
+    count, k = r.shape
+    d = np.ascontiguousarray(dir_, np.float64)
+    nn = np.ascontiguousarray(normal, np.float64)
+    t = np.ascontiguousarray(tangent, np.float64)
+    od = np.empty((count, 3), np.float64)
+    at = np.empty((count, 3), np.float64)
+    ok = np.empty(count, np.int32)
+    lib().oracle_scatter_batch(C.byref(m), flags, fuzz, glass_ior, light, _dp(d), _dp(nn), int(front), _dp(t),
+                               _fp(r), k, count, _dp(od), _dp(at), ok.ctypes.data_as(C.POINTER(C.c_int32)))
+    return ok.astype(bool), od, at
 
 
 def _fp(a):

@@ -21,6 +21,18 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 
+// Planted mistakes for the pinning tests (oracle_set_fault): each one is a plausible
+// error in a formula the tests must catch. 0 = none, the only value outside those tests.
+int g_fault = 0;
+enum Fault {
+   FAULT_VNDF_NO_WARP = 1,        // VNDF without the hemisphere warp of the disk sample
+   FAULT_VNDF_SWAP_ALPHA = 2,     // VNDF stretch with alpha_x and alpha_y swapped
+   FAULT_CONDUCTOR_RP_SIGN = 3,   // conductor Fresnel: wrong sign of 2 a cos sin^2 in Rp's numerator
+   FAULT_DIELECTRIC_RP = 4,       // dielectric Fresnel: Rs used in place of Rp
+   FAULT_DISK_NO_REJECT = 5,      // unit-disk sampler that keeps points of the square
+   FAULT_LAMBDA_HALF = 6,         // Smith Lambda without its factor 1/2
+};
+
 struct V3 {
    double x, y, z;
 };
@@ -491,11 +503,21 @@ bool closest(const oracle_scene& s, const Ray& r, double tmin, double tmax, bool
    return any;
 }
 
+// spec: the emitters next-event estimation samples — LIGHT-material rectangles, triangles
+// and spheres. Shadow rays pass through them, and a BSDF-sampled path that reaches one
+// after a Lambertian bounce does not add its emission (NEE already counted it). Every
+// other emitter (golf-ball or SDF lights, CONSTANT and SHOW_NORMALS surfaces) is reached
+// by BSDF sampling only, so its emission always counts and it occludes shadow rays.
+bool nee_light(const oracle_scene& s, const Prim& p) {
+   return p.mat >= 0 && p.mat < (int)s.mats.size() && s.mats[p.mat].type == RAYON_MAT_LIGHT &&
+          (p.type == P_RECT || p.type == P_TRI || p.type == P_SPHERE);
+}
+
 bool any_hit(const oracle_scene& s, const Ray& r, double tmin, double tmax) {
    Hit h;
    auto test = [&](int64_t pi) {
       const Prim& p = s.prims[pi];
-      if (s.mats[p.mat].type == RAYON_MAT_LIGHT) return false;
+      if (nee_light(s, p)) return false;
       return isect_prim(s, p, r, tmin, tmax, h);
    };
    if (s.nodes.empty()) {
@@ -579,7 +601,7 @@ double lambda_ggx(V3 w, double ax, double ay) {
    double wz2 = w.z * w.z;
    if (wz2 < 1e-16) return 0;
    double a2 = (ax * w.x) * (ax * w.x) + (ay * w.y) * (ay * w.y);
-   return (std::sqrt(1 + a2 / wz2) - 1) * 0.5;
+   return (std::sqrt(1 + a2 / wz2) - 1) * (g_fault == FAULT_LAMBDA_HALF ? 1.0 : 0.5);
 }
 double g1_ggx(V3 w, double ax, double ay) { return 1 / (1 + lambda_ggx(w, ax, ay)); }
 double g2_ggx(V3 wo, V3 wi, double ax, double ay) {
@@ -588,6 +610,7 @@ double g2_ggx(V3 wo, V3 wi, double ax, double ay) {
 
 // VNDF sampling, microfacet_ggx.cuh:45-76
 V3 sample_vndf(V3 wo, double ax, double ay, double u1, double u2) {
+   if (g_fault == FAULT_VNDF_SWAP_ALPHA) std::swap(ax, ay);
    V3 wh = unit(v3(ax * wo.x, ay * wo.y, wo.z));
    if (wh.z < 0) wh = -wh;
    V3 t1 = wh.z < 0.99999 ? unit(cross(v3(0, 0, 1), wh)) : v3(1, 0, 0);
@@ -596,7 +619,7 @@ V3 sample_vndf(V3 wo, double ax, double ay, double u1, double u2) {
    double px = r * std::cos(phi), py = r * std::sin(phi);
    double hh = std::sqrt(std::max(0.0, 1 - px * px));
    double sm = (1 + wh.z) * 0.5;
-   py = (1 - sm) * hh + sm * py;
+   if (g_fault != FAULT_VNDF_NO_WARP) py = (1 - sm) * hh + sm * py;
    double pz = std::sqrt(std::max(0.0, 1 - px * px - py * py));
    V3 nh = px * t1 + py * t2 + pz * wh;
    return unit(v3(ax * nh.x, ay * nh.y, std::max(1e-6, nh.z)));
@@ -609,7 +632,9 @@ double fr_complex1(double cos2, double sin2, double ci, double eta, double k) {
    double ab = std::sqrt(std::max(t0 * t0 + 4 * eta2 * k2, 0.0));
    double a = std::sqrt(std::max((ab + t0) * 0.5, 0.0));
    double rs = (ab + cos2 - 2 * a * ci) / std::max(ab + cos2 + 2 * a * ci, 1e-10);
-   double rp = rs * (ab * cos2 + sin2 * sin2 - 2 * a * ci * sin2) / std::max(ab * cos2 + sin2 * sin2 + 2 * a * ci * sin2, 1e-10);
+   const double sg = g_fault == FAULT_CONDUCTOR_RP_SIGN ? -1.0 : 1.0;
+   double rp = rs * (ab * cos2 + sin2 * sin2 - sg * 2 * a * ci * sin2) /
+               std::max(ab * cos2 + sin2 * sin2 + 2 * a * ci * sin2, 1e-10);
    return (rs + rp) * 0.5;
 }
 V3 fr_complex(double ci, V3 eta, V3 k) {
@@ -626,6 +651,7 @@ double fresnel_dielectric(double ci, double eta) {
    double ct = std::sqrt(1 - s2t);
    double rs = (eta * ci - ct) / (eta * ci + ct);
    double rp = (ci - eta * ct) / (ci + eta * ct);
+   if (g_fault == FAULT_DIELECTRIC_RP) rp = rs;
    return 0.5 * (rs * rs + rp * rp);
 }
 
@@ -906,6 +932,15 @@ struct Cam {
    V3 C, p00, du, dv, u, v;
 };
 
+// random_in_unit_disk (cuda_raytracer.cuh:114-122): rejection from the square [-1,1)^2,
+// two draws per attempt.
+void unit_disk(Draws& R, double& dx, double& dy) {
+   do {
+      dx = 2 * R.next() - 1;
+      dy = 2 * R.next() - 1;
+   } while (dx * dx + dy * dy >= 1 && g_fault != FAULT_DISK_NO_REJECT);
+}
+
 // One camera sample: render_acc_kernel.cu:198-229 + ray_color, cuda_raytracer.cuh:858-933
 // (+ spec NEE on Lambertian surfaces).
 V3 trace_sample(const oracle_scene& s, const rayon_job_params& P, const Cam& cam, int64_t x, int64_t y, Rng& rng,
@@ -921,10 +956,7 @@ V3 trace_sample(const oracle_scene& s, const rayon_job_params& P, const Cam& cam
       V3 nd = unit(ray.d);
       V3 focus = cam.C + (double)P.dof_focus_distance * nd;
       double dx, dy;
-      do {  // random_in_unit_disk, cuda_raytracer.cuh:114-122
-         dx = 2 * R.next() - 1;
-         dy = 2 * R.next() - 1;
-      } while (dx * dx + dy * dy >= 1);
+      unit_disk(R, dx, dy);
       V3 off = (double)P.dof_aperture * (dx * cam.u + dy * cam.v);
       ray.o = cam.C + off;
       ray.d = focus - ray.o;
@@ -952,7 +984,7 @@ V3 trace_sample(const oracle_scene& s, const rayon_job_params& P, const Cam& cam
       ScatterIn in{h.p, h.normal, h.tangent, ray.d, h.front};
       ScatterOut o;
       scatter(m, color, K, in, R, o);
-      if (len2(o.emitted) > 0 && !(nee && prev_lambert)) L = L + T * o.emitted;
+      if (len2(o.emitted) > 0 && !(nee && prev_lambert && nee_light(s, pr))) L = L + T * o.emitted;
       if (!o.scattered) break;
       if (nee && o.lambert) {
          double ul = R.next(), ua = R.next(), ub = R.next();
@@ -1118,8 +1150,7 @@ oracle_scene* oracle_scene_create(const rayon_scene_desc* d) {
    if (!s->prims.empty()) build_node(*s, 0, (int64_t)s->prims.size());
    for (size_t i = 0; i < s->prims.size(); ++i) {
       const Prim& p = s->prims[i];
-      if (p.mat >= 0 && p.mat < (int)s->mats.size() && s->mats[p.mat].type == RAYON_MAT_LIGHT &&
-          (p.type == P_RECT || p.type == P_TRI || p.type == P_SPHERE)) {
+      if (nee_light(*s, p)) {
          s->lights.push_back({(int64_t)i, 0});
       }
    }
@@ -1317,6 +1348,56 @@ int32_t oracle_scatter_probe(const rayon_material* m, uint32_t flags, float fuzz
    em[2] = o.emitted.z;
    *used = R.used;
    return o.scattered ? 1 : 0;
+}
+
+void oracle_set_fault(int32_t fault) { g_fault = fault; }
+
+void oracle_sample_vndf(const double wo[3], double ax, double ay, const double* u, int64_t n, double* out) {
+   const V3 w = dv(wo);
+   for (int64_t i = 0; i < n; ++i) {
+      V3 m = sample_vndf(w, ax, ay, u[2 * i], u[2 * i + 1]);
+      out[3 * i] = m.x;
+      out[3 * i + 1] = m.y;
+      out[3 * i + 2] = m.z;
+   }
+}
+
+double oracle_ggx_lambda(const double w[3], double ax, double ay) { return lambda_ggx(dv(w), ax, ay); }
+
+double oracle_fresnel_dielectric(double ci, double eta) { return fresnel_dielectric(ci, eta); }
+
+double oracle_fresnel_conductor(double ci, double eta, double k) {
+   ci = std::clamp(ci, 0.0, 1.0);
+   return fr_complex1(ci * ci, 1 - ci * ci, ci, eta, k);
+}
+
+int64_t oracle_unit_disk(const float* rnd, int64_t nrnd, int64_t n, double* out_xy) {
+   Draws R{nullptr, rnd, (int)std::min<int64_t>(nrnd, 0x7fffffff)};
+   for (int64_t i = 0; i < n; ++i) {
+      unit_disk(R, out_xy[2 * i], out_xy[2 * i + 1]);
+      if (R.used > R.nfixed) return -1;
+   }
+   return R.used;
+}
+
+void oracle_scatter_batch(const rayon_material* m, uint32_t flags, float fuzz, float glass_ior, float light,
+                          const double dir[3], const double n[3], int32_t front, const double tangent[3],
+                          const float* rnd, int32_t nrnd, int64_t count, double* out_dir, double* atten,
+                          int32_t* scattered) {
+   const Knobs K{flags, fuzz, glass_ior, light};
+   const ScatterIn in{v3(0, 0, 0), dv(n), dv(tangent), dv(dir), front != 0};
+   for (int64_t i = 0; i < count; ++i) {
+      Draws R{nullptr, rnd + i * nrnd, nrnd};
+      ScatterOut o;
+      scatter(*m, fv(m->albedo), K, in, R, o);
+      out_dir[3 * i] = o.ray.d.x;
+      out_dir[3 * i + 1] = o.ray.d.y;
+      out_dir[3 * i + 2] = o.ray.d.z;
+      atten[3 * i] = o.atten.x;
+      atten[3 * i + 1] = o.atten.y;
+      atten[3 * i + 2] = o.atten.z;
+      scattered[i] = o.scattered ? 1 : 0;
+   }
 }
 
 }  // extern "C"

@@ -35,7 +35,8 @@ void oracle_trace_closest(const oracle_scene* s, const float* origins, const flo
 double oracle_intersect_prim(const oracle_scene* s, int64_t prim, const float* origin, const float* dir,
                              double t_min, double t_max);
 
-/* 1 if any primitive other than an emitter is hit in [t_min, t_max]. */
+/* 1 if any primitive other than an emitter that NEE samples (LIGHT-material rectangle,
+ * triangle or sphere) is hit in [t_min, t_max]. */
 void oracle_trace_any(const oracle_scene* s, const float* origins, const float* dirs, int64_t n, double t_min,
                       double t_max, int32_t* out_hit);
 
@@ -82,6 +83,23 @@ int32_t oracle_scatter_probe(const rayon_material* m, uint32_t flags, float meta
                              float light_intensity, const double dir[3], const double n[3], int32_t front_face,
                              const double tangent[3], const float* rnd, int32_t nrnd, double out_dir[3],
                              double atten[3], double emitted[3], int32_t* used);
+
+/* Pinning hooks: the pieces of the material models on their own, so the tests can hold
+ * them against closed forms, analytic densities and numerical integrals. */
+void oracle_set_fault(int32_t fault); /* plant a known mistake (tests of the tests); 0 = none */
+/* VNDF sample (microfacet_ggx.cuh:45-76) for each (u1, u2) pair of u; out: n unit normals. */
+void oracle_sample_vndf(const double wo[3], double ax, double ay, const double* u, int64_t n, double* out);
+double oracle_ggx_lambda(const double w[3], double ax, double ay); /* microfacet_ggx.cuh:21-28 */
+double oracle_fresnel_dielectric(double cos_i, double eta);        /* eta = n_incident / n_transmitted */
+double oracle_fresnel_conductor(double cos_i, double eta, double k); /* FrComplex1, :83-101 */
+/* random_in_unit_disk (cuda_raytracer.cuh:114-122) n times from the given uniforms; returns
+ * the number consumed, or -1 if they ran out. */
+int64_t oracle_unit_disk(const float* rnd, int64_t nrnd, int64_t n, double* out_xy);
+/* oracle_scatter_probe for `count` independent samples, nrnd uniforms each. */
+void oracle_scatter_batch(const rayon_material* m, uint32_t flags, float metal_fuzziness, float glass_ior,
+                          float light_intensity, const double dir[3], const double n[3], int32_t front_face,
+                          const double tangent[3], const float* rnd, int32_t nrnd, int64_t count,
+                          double* out_dir, double* atten, int32_t* scattered);
 
 #ifdef __cplusplus
 }

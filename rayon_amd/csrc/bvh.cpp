@@ -461,11 +461,16 @@ rayon_status gather_prims(const rayon_scene_desc* d, std::vector<HostPrim>& prim
       return RAYON_ERR_INVALID;
    }
    auto mat_ok = [&](int m) { return m >= 0 && m < d->num_materials; };
-   auto flags_for = [&](int mat, bool visible, bool normals) {
+   // F_EMITTER marks the emitters next-event estimation samples: LIGHT-material triangles,
+   // spheres and rectangles. Other emitters (golf-ball and SDF lights, CONSTANT surfaces)
+   // are only reached by BSDF sampling, so they occlude shadow rays and always add
+   // their emission.
+   auto flags_for = [&](int mat, bool visible, bool normals, uint32_t kind) {
       uint32_t f = 0;
       if (visible) f |= F_VISIBLE;
       if (normals) f |= F_NORMALS;
-      if (d->materials[mat].type == RAYON_MAT_LIGHT) f |= F_EMITTER;
+      if (d->materials[mat].type == RAYON_MAT_LIGHT && (kind == K_TRI || kind == K_SPHERE || kind == K_RECT))
+         f |= F_EMITTER;
       return f;
    };
    prims.clear();
@@ -500,7 +505,7 @@ rayon_status gather_prims(const rayon_scene_desc* d, std::vector<HostPrim>& prim
          return RAYON_ERR_INVALID;
       }
       p.mat = (uint32_t)g.material_id;
-      p.flags = flags_for(g.material_id, g.visible != 0, p.kind == K_TRI && g.has_normals);
+      p.flags = flags_for(g.material_id, g.visible != 0, p.kind == K_TRI && g.has_normals, p.kind);
       p.orig = orig++;
       std::memcpy(p.a, g.p, 12);
       std::memcpy(p.b, g.p + 3, 12);
@@ -515,7 +520,7 @@ rayon_status gather_prims(const rayon_scene_desc* d, std::vector<HostPrim>& prim
          set_error("mesh %d: bad material id or NULL arrays", mi);
          return RAYON_ERR_INVALID;
       }
-      const uint32_t fl = flags_for(m.material_id, m.visible != 0, m.normals != nullptr);
+      const uint32_t fl = flags_for(m.material_id, m.visible != 0, m.normals != nullptr, K_TRI);
       for (int64_t t = 0; t < m.num_triangles; ++t) {
          HostPrim p{};
          p.kind = K_TRI;
@@ -548,7 +553,7 @@ rayon_status gather_prims(const rayon_scene_desc* d, std::vector<HostPrim>& prim
       HostPrim p{};
       p.kind = K_SDF;
       p.mat = (uint32_t)d->sdf_material_id;
-      p.flags = flags_for(d->sdf_material_id, true, false);
+      p.flags = flags_for(d->sdf_material_id, true, false, K_SDF);
       p.orig = orig++;
       double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
       for (int i = 0; i < d->num_sdf_prims; ++i) {

@@ -1253,7 +1253,8 @@ __device__ __forceinline__ V3 sky(const JobView& J, V3 d) {  // cuda_raytracer.c
    return J.background_intensity * mk(1.0f - 0.5f * t, 1.0f - 0.3f * t, 1.0f);
 }
 
-// Neumaier-compensated add; plain adds only, so no contraction can change it.
+// Neumaier-compensated add of an already rounded x; the body has adds only, so no
+// contraction can change it (callers round products with __fmul_rn before passing them).
 __device__ __forceinline__ void neumaier(float& sum, float& c, float x) {
    float t = sum + x;
    if (fabsf(sum) >= fabsf(x))

@@ -89,7 +89,9 @@ __device__ inline bool shade_step(const JobView& J, const HitMin& h, int bounce,
    const V3 color = surface_color(M, s.p, s.center);
    Scatter sc;
    scatter(M, color, J, s, ps.d, R, sc);
-   if (len2(sc.emitted) > 0.0f && !(nee && ps.prev_lambert)) ps.L = ps.L + ps.T * sc.emitted;
+   // An emitter that NEE samples (F_EMITTER) was already counted by the shadow ray of the
+   // Lambertian bounce that led here; every other emitter is reached only this way.
+   if (len2(sc.emitted) > 0.0f && !(nee && ps.prev_lambert && (s.info & F_EMITTER))) ps.L = ps.L + ps.T * sc.emitted;
    if (!sc.scattered) return false;
    if (nee && sc.lambert) nee_request<CNT>(J, s, color, ps.T, R, sh, pc);
    ps.T = ps.T * sc.atten;
@@ -126,7 +128,9 @@ __device__ __forceinline__ void accumulate(float4& a, float4& c, V3 L) {
    neumaier(a.y, c.y, L.y);
    neumaier(a.z, c.z, L.z);
    float l = luminance(L);
-   neumaier(a.w, c.w, l * l);
+   // the square is rounded on its own (no fused multiply-add into the sum), so the
+   // compensation term sees the same addend as the sum
+   neumaier(a.w, c.w, __fmul_rn(l, l));
 }
 
 // New signed per-pixel count after a pass: render_acc_kernel.cu:240-271 (REF) or the

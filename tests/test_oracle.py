@@ -373,3 +373,38 @@ def test_gamma_and_adaptive_closed_form():
     # std error of n identical samples is 0 -> converged; of 0/1 samples ~ 0.5/sqrt(n)
     assert L.oracle_adaptive_stderr(16.0, 16.0, 16, 16, 0.005) == 1
     assert L.oracle_adaptive_stderr(8.0, 8.0, 16, 16, 0.005) == 0
+
+
+def _mixed_emitter_scene(emissive=True):
+    """Lambertian floor lit by a quad light (sampled by NEE), an emissive golf-ball sphere
+    and a CONSTANT sphere (both reached by BSDF sampling only)."""
+    mats = [mat(0, (0.7, 0.7, 0.7)), mat(ra.MAT.LIGHT, emission=(4, 4, 4)),
+            mat(ra.MAT.LIGHT if emissive else 0, (0.5, 0.5, 0.5), emission=(3, 2, 1)),
+            mat(ra.MAT.CONSTANT if emissive else 0, (0.2, 0.9, 0.3))]
+    golf = sphere((1.5, 0.6, 0.0), 0.5, 2)
+    golf.type = ra.GEOM.DISPLACED_SPHERE
+    geoms = [rect((-5, 0, -5), (10, 0, 0), (0, 0, 10), 0), rect((-1, 4, -1), (2, 0, 0), (0, 0, 2), 1), golf,
+             sphere((-1.5, 0.5, 0.5), 0.4, 3)]
+    return make_desc(mats, geoms)
+
+
+def test_nee_counts_emitters_it_does_not_sample():
+    # With NEE on, only the emitters NEE samples may drop their BSDF-hit emission after a
+    # Lambertian bounce; the golf-ball light and the CONSTANT sphere must still light the
+    # floor, so NEE and pure BSDF sampling agree (an estimator that dropped them would be
+    # 10% or more darker, checked below).
+    W, H, n = 8, 8, 4000
+    cam_args = ((0, 3, 6), (0, 0.3, 0), (0, 1, 0), 50.0)
+    res = {}
+    for emissive in (True, False):
+        desc, keep = _mixed_emitter_scene(emissive)
+        s = OracleScene(C.pointer(desc))
+        assert s.num_lights == 1
+        cam = ra.camera_frame(*cam_args, W, H)
+        for flags in (0, ra.FLAG.NEE):
+            P = ra.job_params(W, H, cam, spp=1, max_depth=4, seed=5, flags=flags, background_intensity=0.0)
+            sums, _ = s.render(P, nsamples=n, camera=cam)
+            res[emissive, flags] = sums.sum(0) / (n * W * H)
+    np.testing.assert_allclose(res[True, 0], res[True, ra.FLAG.NEE], rtol=0.04)
+    np.testing.assert_allclose(res[False, 0], res[False, ra.FLAG.NEE], rtol=0.04)
+    assert (res[True, ra.FLAG.NEE] / res[False, ra.FLAG.NEE]).max() > 1.1
