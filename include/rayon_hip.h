@@ -337,6 +337,26 @@ rayon_status rayon_job_stats_get(rayon_ctx* ctx, rayon_job* job, rayon_job_stats
 rayon_status rayon_trace_closest(rayon_ctx* ctx, rayon_scene* scene, const float* origins, const float* dirs,
                                  int64_t n, float t_min, float t_max, int64_t* out_prim, float* out_t);
 
+/* Material probe: scatters n rays with the renderers' own device code for material m
+ * (scatter_material, cuda_raytracer.cuh:613-850, after apply_material's mapping :418-485),
+ * each at a surface point with normal normals[i] (facing the ray), tangent tangents[i] and
+ * front-face flag front[i], drawing its random numbers in order from rnd[i*nrnd ...]
+ * instead of the path's stream. flags / fuzziness / glass IOR / light intensity act as
+ * the job's. All arrays are host memory, 3 floats per vector. For parity tests against
+ * a CPU restatement; out_used receives how many numbers each sample drew. */
+rayon_status rayon_scatter_probe(rayon_ctx* ctx, const rayon_material* m, uint32_t flags, float metal_fuzziness,
+                                 float glass_ior, float light_intensity, const float* dirs, const float* normals,
+                                 const float* tangents, const int32_t* front, const float* rnd, int32_t nrnd,
+                                 int64_t n, float* out_dir, float* out_atten, float* out_emitted,
+                                 int32_t* out_scattered, int32_t* out_used);
+
+/* Accumulation probe: folds n sample values (RGB, host memory) in order into one pixel's
+ * compensated FP32 sums with the renderers' own device code (the acc += ray_color of
+ * render_acc_kernel.cu:224, compensated), returning the sums (rgb, luminance^2) and
+ * their compensation terms. For tests of the summation against exact sums. */
+rayon_status rayon_accumulate_probe(rayon_ctx* ctx, const float* rgb, int64_t n, float out_sum[4],
+                                    float out_comp[4]);
+
 /* Primary camera rays of sample `sample` of a job, exactly as the renderers generate
  * them (origins/dirs 3 floats per pixel). For parity tests. */
 rayon_status rayon_job_primary_rays(rayon_ctx* ctx, rayon_job* job, int32_t sample, float* origins,

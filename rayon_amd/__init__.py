@@ -192,6 +192,37 @@ class Context:
         _check(lib().rayon_ctx_get_option(self._h, opt, C.byref(v)), "rayon_ctx_get_option")
         return v.value
 
+    def scatter_probe(self, m: Material, dirs, normals, tangents, front, rnd, flags: int = 0, fuzz: float = 0.8,
+                      glass_ior: float = 1.5, light: float = 1.0):
+        """The device's scatter_material for n given hits and random-number rows (rnd: n x k).
+        Returns (scattered, dir, atten, emitted, used)."""
+        f32p = C.POINTER(C.c_float)
+        i32p = C.POINTER(C.c_int32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        nn = np.ascontiguousarray(normals, np.float32)
+        t = np.ascontiguousarray(tangents, np.float32)
+        fr = np.ascontiguousarray(front, np.int32)
+        r = np.ascontiguousarray(rnd, np.float32)
+        n, k = r.shape
+        od, at, em = (np.empty((n, 3), np.float32) for _ in range(3))
+        ok, used = np.empty(n, np.int32), np.empty(n, np.int32)
+        _check(lib().rayon_scatter_probe(self._h, C.byref(m), flags, fuzz, glass_ior, light, d.ctypes.data_as(f32p),
+                                         nn.ctypes.data_as(f32p), t.ctypes.data_as(f32p), fr.ctypes.data_as(i32p),
+                                         r.ctypes.data_as(f32p), k, n, od.ctypes.data_as(f32p),
+                                         at.ctypes.data_as(f32p), em.ctypes.data_as(f32p), ok.ctypes.data_as(i32p),
+                                         used.ctypes.data_as(i32p)), "rayon_scatter_probe")
+        return ok.astype(bool), od, at, em, used
+
+    def accumulate_probe(self, rgb):
+        """The device's compensated per-pixel accumulation over the series rgb (n x 3):
+        (sum rgb + luminance^2, compensation), each 4 floats."""
+        v = np.ascontiguousarray(rgb, np.float32)
+        s, c = np.empty(4, np.float32), np.empty(4, np.float32)
+        f32p = C.POINTER(C.c_float)
+        _check(lib().rayon_accumulate_probe(self._h, v.ctypes.data_as(f32p), v.shape[0], s.ctypes.data_as(f32p),
+                                            c.ctypes.data_as(f32p)), "rayon_accumulate_probe")
+        return s, c
+
     def autotune(self, jobs, samples: int) -> TuneResult:
         """Times wavefront launch shapes on clones of `jobs` and keeps the fastest on this context."""
         arr = (C.c_void_p * len(jobs))(*[j._h for j in jobs])

@@ -180,6 +180,9 @@ def declare(lib: C.CDLL) -> C.CDLL:
         "rayon_job_stats_get": (st, [vp, vp, P(JobStats)]),
         "rayon_trace_closest": (st, [vp, vp, P(f32), P(f32), i64, f32, f32, P(i64), P(f32)]),
         "rayon_job_primary_rays": (st, [vp, vp, i32, P(f32), P(f32)]),
+        "rayon_scatter_probe": (st, [vp, P(Material), C.c_uint32, f32, f32, f32, P(f32), P(f32), P(f32), P(i32),
+                                     P(f32), i32, i64, P(f32), P(f32), P(f32), P(i32), P(i32)]),
+        "rayon_accumulate_probe": (st, [vp, P(f32), i64, P(f32), P(f32)]),
         "rayon_scene_set_batch_path": (st, [vp, i32]),
         "rayon_scene_batch_path": (i32, [vp]),
         "rayon_progressive_config_default": (None, [P(ProgressiveConfig)]),

@@ -26,7 +26,7 @@ ORACLE_LIB = ORACLE_DIR / "liboracle.so"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-# Packed FP32 ops are disabled for every device compile; the Neumaier sums in the
+# Packed FP32 ops are disabled for every device compile; the compensated sums in the
 # accumulation rely on plain IEEE adds, so no fast-math either.
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

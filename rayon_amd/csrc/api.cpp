@@ -755,6 +755,83 @@ rayon_status rayon_trace_closest(rayon_ctx* ctx, rayon_scene* sc, const float* o
    return RAYON_OK;
 }
 
+rayon_status rayon_scatter_probe(rayon_ctx* ctx, const rayon_material* m, uint32_t flags, float fuzz,
+                                 float glass_ior, float light, const float* dirs, const float* normals,
+                                 const float* tangents, const int32_t* front, const float* rnd, int32_t nrnd,
+                                 int64_t n, float* out_dir, float* out_atten, float* out_emitted,
+                                 int32_t* out_scattered, int32_t* out_used) {
+   if (!ctx || !m || !dirs || !normals || !tangents || !front || (!rnd && nrnd > 0) || nrnd < 0 || n < 0 ||
+       !out_dir || !out_atten || !out_emitted || !out_scattered || !out_used) {
+      set_error("rayon_scatter_probe: bad arguments");
+      return RAYON_ERR_INVALID;
+   }
+   if (n == 0) return RAYON_OK;
+   RY_HIP(hipSetDevice(ctx->device));
+   JobView J{};
+   J.flags = flags;
+   J.metal_fuzziness = fuzz;
+   J.glass_ior = glass_ior;
+   J.light_intensity = light;
+   const MatRec M = pack_material(*m);
+   const size_t b3 = (size_t)n * 3 * sizeof(float), bi = (size_t)n * sizeof(int32_t);
+   const size_t br = (size_t)n * (size_t)nrnd * sizeof(float);
+   // one allocation: 3 inputs, 3 outputs of 3 floats, front, scattered, used, random numbers
+   char* base = nullptr;
+   hipError_t e = hipMalloc(&base, 6 * b3 + 3 * bi + br + 64);
+   if (e != hipSuccess) return hip_fail(e, "scatter_probe");
+   float* dD = (float*)base;
+   float* dN = (float*)(base + b3);
+   float* dT = (float*)(base + 2 * b3);
+   float* dO = (float*)(base + 3 * b3);
+   float* dA = (float*)(base + 4 * b3);
+   float* dE = (float*)(base + 5 * b3);
+   int* dF = (int*)(base + 6 * b3);
+   int* dS = (int*)(base + 6 * b3 + bi);
+   int* dU = (int*)(base + 6 * b3 + 2 * bi);
+   float* dR = (float*)(base + 6 * b3 + 3 * bi);
+   e = hipMemcpyAsync(dD, dirs, b3, hipMemcpyHostToDevice, ctx->stream);
+   if (e == hipSuccess) e = hipMemcpyAsync(dN, normals, b3, hipMemcpyHostToDevice, ctx->stream);
+   if (e == hipSuccess) e = hipMemcpyAsync(dT, tangents, b3, hipMemcpyHostToDevice, ctx->stream);
+   if (e == hipSuccess) e = hipMemcpyAsync(dF, front, bi, hipMemcpyHostToDevice, ctx->stream);
+   if (e == hipSuccess && br) e = hipMemcpyAsync(dR, rnd, br, hipMemcpyHostToDevice, ctx->stream);
+   if (e == hipSuccess)
+      e = launch_scatter_probe(M, J, dD, dN, dT, dF, dR, nrnd, n, dO, dA, dE, dS, dU, ctx->stream);
+   if (e == hipSuccess) e = hipMemcpyAsync(out_dir, dO, b3, hipMemcpyDeviceToHost, ctx->stream);
+   if (e == hipSuccess) e = hipMemcpyAsync(out_atten, dA, b3, hipMemcpyDeviceToHost, ctx->stream);
+   if (e == hipSuccess) e = hipMemcpyAsync(out_emitted, dE, b3, hipMemcpyDeviceToHost, ctx->stream);
+   if (e == hipSuccess) e = hipMemcpyAsync(out_scattered, dS, bi, hipMemcpyDeviceToHost, ctx->stream);
+   if (e == hipSuccess) e = hipMemcpyAsync(out_used, dU, bi, hipMemcpyDeviceToHost, ctx->stream);
+   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+   (void)hipFree(base);
+   if (e != hipSuccess) return hip_fail(e, "scatter_probe");
+   return RAYON_OK;
+}
+
+rayon_status rayon_accumulate_probe(rayon_ctx* ctx, const float* rgb, int64_t n, float out_sum[4],
+                                    float out_comp[4]) {
+   if (!ctx || (!rgb && n > 0) || n < 0 || !out_sum || !out_comp) {
+      set_error("rayon_accumulate_probe: bad arguments");
+      return RAYON_ERR_INVALID;
+   }
+   RY_HIP(hipSetDevice(ctx->device));
+   const size_t b = (size_t)n * 3 * sizeof(float);
+   char* base = nullptr;
+   hipError_t e = hipMalloc(&base, b + 2 * sizeof(float4) + 64);
+   if (e != hipSuccess) return hip_fail(e, "accumulate_probe");
+   float4* dOut = (float4*)base;
+   float* dV = (float*)(base + 2 * sizeof(float4));
+   if (n) e = hipMemcpyAsync(dV, rgb, b, hipMemcpyHostToDevice, ctx->stream);
+   if (e == hipSuccess) e = launch_accumulate_probe(dV, n, dOut, ctx->stream);
+   float4 r[2];
+   if (e == hipSuccess) e = hipMemcpyAsync(r, dOut, sizeof(r), hipMemcpyDeviceToHost, ctx->stream);
+   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+   (void)hipFree(base);
+   if (e != hipSuccess) return hip_fail(e, "accumulate_probe");
+   out_sum[0] = r[0].x, out_sum[1] = r[0].y, out_sum[2] = r[0].z, out_sum[3] = r[0].w;
+   out_comp[0] = r[1].x, out_comp[1] = r[1].y, out_comp[2] = r[1].z, out_comp[3] = r[1].w;
+   return RAYON_OK;
+}
+
 rayon_status rayon_job_primary_rays(rayon_ctx* ctx, rayon_job* j, int32_t sample, float* o, float* d) {
    if (!ctx || !j || !o || !d || sample < 0) {
       set_error("rayon_job_primary_rays: bad arguments");

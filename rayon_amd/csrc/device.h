@@ -935,14 +935,16 @@ __device__ __forceinline__ void onb(V3 n, V3& u, V3& v) {  // cuda_utils.cuh:99-
    v = cross(n, u);
 }
 
-__device__ __forceinline__ V3 on_sphere(Rng& R) {  // cuda_utils.cuh:66-77
+template <class RNG>
+__device__ __forceinline__ V3 on_sphere(RNG& R) {  // cuda_utils.cuh:66-77
    float a = R.next();
    float z = 1.0f - 2.0f * R.next();
    float s = sqrtf(fmaxf(0.0f, 1.0f - z * z));
    return mk(s * cos2pi(a), s * sin2pi(a), z);
 }
 
-__device__ __forceinline__ V3 cosine_dir(V3 n, Rng& R) {  // cuda_raytracer.cuh:621-638
+template <class RNG>
+__device__ __forceinline__ V3 cosine_dir(V3 n, RNG& R) {  // cuda_raytracer.cuh:621-638
    V3 w = unit(n), u, v;
    onb(w, u, v);
    float u1 = R.next(), u2 = R.next();
@@ -1006,8 +1008,10 @@ struct Scatter {
 };
 
 // apply_material's type mapping (cuda_raytracer.cuh:418-485) and scatter_material
-// (:613-850); RAYON_MAT_GGX_* are additions of this library.
-__device__ inline void scatter(const MatRec& M, V3 color, const JobView& J, const Surf& s, V3 dir, Rng& R,
+// (:613-850); RAYON_MAT_GGX_* are additions of this library. RNG is the path's stream
+// (Rng), or a table of given numbers for the material probe.
+template <class RNG>
+__device__ inline void scatter(const MatRec& M, V3 color, const JobView& J, const Surf& s, V3 dir, RNG& R,
                                Scatter& o) {
    const uint32_t mt = __float_as_uint(M.m0.w) & 0xFFu;
    const bool quirks = (J.flags & RAYON_FLAG_REF_QUIRKS) != 0;
@@ -1253,15 +1257,20 @@ __device__ __forceinline__ V3 sky(const JobView& J, V3 d) {  // cuda_raytracer.c
    return J.background_intensity * mk(1.0f - 0.5f * t, 1.0f - 0.3f * t, 1.0f);
 }
 
-// Neumaier-compensated add of an already rounded x; the body has adds only, so no
-// contraction can change it (callers round products with __fmul_rn before passing them).
-__device__ __forceinline__ void neumaier(float& sum, float& c, float x) {
-   float t = sum + x;
-   if (fabsf(sum) >= fabsf(x))
-      c += (sum - t) + x;
-   else
-      c += (x - t) + sum;
-   sum = t;
+// Compensated add of an already rounded x into the float-float pair (sum, c): the exact
+// error of sum + x (Knuth's TwoSum) joins the low part, and the pair is renormalised
+// (Fast2Sum) so |c| stays within half an ulp of sum. The pair carries about 48 bits, so
+// a pixel's sum over millions of samples stays within an ulp of the exact FP32-rounded
+// sum; a Neumaier sum, whose FP32 compensation term grows with the number of samples,
+// drifts by hundreds of ulp over 2^20 equal samples. Adds only, so no contraction can
+// change it (callers round products with __fmul_rn before passing them).
+__device__ __forceinline__ void ff_add(float& sum, float& c, float x) {
+   const float t = sum + x;
+   const float z = t - sum;
+   float e = (sum - (t - z)) + (x - z);
+   e += c;
+   sum = t + e;
+   c = e - (sum - t);
 }
 
 __device__ __forceinline__ float luminance(V3 c) { return 0.299f * c.x + 0.587f * c.y + 0.114f * c.z; }

@@ -49,6 +49,11 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t st)
 // `gamma` into each job's RGBA8 frame (no accumulation buffers are touched).
 hipError_t launch_preview(const JobView* views, uint8_t* const* frames, int n, int samples, float gamma,
                           hipStream_t st);
+// Device material and accumulation code on caller-given inputs (parity probes).
+hipError_t launch_scatter_probe(const MatRec& M, const JobView& J, const float* dirs, const float* nrm,
+                                const float* tan, const int* front, const float* rnd, int nrnd, long long n,
+                                float* od, float* at, float* em, int* ok, int* used, hipStream_t st);
+hipError_t launch_accumulate_probe(const float* rgb, long long n, float4* out, hipStream_t st);
 // Sets `count` buffers of n[k] 32-bit words to v, kFillRanges buffers per launch.
 hipError_t launch_fill_ranges(uint32_t* const* p, const long long* n, int count, uint32_t v, hipStream_t st);
 

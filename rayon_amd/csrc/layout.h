@@ -126,7 +126,7 @@ struct SceneView {
 struct JobView {
    SceneView scene;
    float4* acc;        // per pixel: sum.rgb, sum of luminance^2
-   float4* comp;       // per pixel: Neumaier compensation rgb, luminance compensation
+   float4* comp;       // per pixel: low parts of the float-float sums in acc (rgb, luminance^2)
    int* counts;        // per pixel sample count, negative = converged
    uint32_t* rng;      // per pixel sequential stream (RAYON_FLAG_REF_RNG only)
    unsigned long long* counters;  // kCounterSlots x 8: segments, shadow segments, primary

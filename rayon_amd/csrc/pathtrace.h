@@ -124,13 +124,13 @@ __device__ inline V3 trace_path(const JobView& J, V3 o, V3 d, Rng& R, int* stack
 
 // Adds one sample's radiance to a pixel's compensated sums.
 __device__ __forceinline__ void accumulate(float4& a, float4& c, V3 L) {
-   neumaier(a.x, c.x, L.x);
-   neumaier(a.y, c.y, L.y);
-   neumaier(a.z, c.z, L.z);
+   ff_add(a.x, c.x, L.x);
+   ff_add(a.y, c.y, L.y);
+   ff_add(a.z, c.z, L.z);
    float l = luminance(L);
    // the square is rounded on its own (no fused multiply-add into the sum), so the
    // compensation term sees the same addend as the sum
-   neumaier(a.w, c.w, __fmul_rn(l, l));
+   ff_add(a.w, c.w, __fmul_rn(l, l));
 }
 
 // New signed per-pixel count after a pass: render_acc_kernel.cu:240-271 (REF) or the

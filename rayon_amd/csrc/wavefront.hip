@@ -1073,7 +1073,7 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_wf_shade(WfView V, int level) {
 }
 
 // One thread per pixel present in the round: adds that pixel's samples of the round in
-// sample order (Neumaier), updates its count and the job's counters.
+// sample order (float-float compensated sums), updates its count and the job's counters.
 __global__ __launch_bounds__(kWfBlock) void k_wf_accumulate(WfView V, long long w0, int n) {
    const int i = blockIdx.x * kWfBlock + threadIdx.x;
    bool active = false;
