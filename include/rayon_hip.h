@@ -327,6 +327,15 @@ rayon_status rayon_ctx_wave_stats(rayon_ctx* ctx, rayon_wave_stats* out, int32_t
 rayon_status rayon_job_resolve_linear(rayon_ctx* ctx, rayon_job* job, float* rgb_out);
 /* Gamma-corrected 8-bit image (gammaCorrectKernel, render_acc_kernel.cu:11-44). */
 rayon_status rayon_job_resolve_u8(rayon_ctx* ctx, rayon_job* job, uint8_t* out, int32_t channels, float gamma);
+/* Allocation-free variant of rayon_job_resolve_u8, with the same result. It has no
+ * counterpart in the reference: it is for callers that cannot allocate device memory
+ * (code under graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * image is staged in a fixed 64 MiB device array of the library (one per GPU, held until
+ * the process ends) instead of a buffer allocated per call, so width * height * channels
+ * must not exceed 64 MiB (RAYON_ERR_INVALID otherwise). Every call on one GPU uses the same
+ * array: calls that overlap on one GPU race, so make them from one thread per device. */
+rayon_status rayon_job_resolve_u8_noalloc(rayon_ctx* ctx, rayon_job* job, uint8_t* out, int32_t channels,
+                                          float gamma);
 /* Per-pixel sample counts (negative = converged, as the reference's count buffer). */
 rayon_status rayon_job_sample_counts(rayon_ctx* ctx, rayon_job* job, int32_t* out);
 rayon_status rayon_job_stats_get(rayon_ctx* ctx, rayon_job* job, rayon_job_stats* out);

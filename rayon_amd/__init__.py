@@ -329,6 +329,14 @@ class Job:
                                           channels, gamma), "rayon_job_resolve_u8")
         return out
 
+    def resolve_u8_noalloc(self, channels: int = 3, gamma: float = 2.0) -> np.ndarray:
+        """As resolve_u8, staged in the library's fixed device array (no device allocation)."""
+        h, w = self.shape
+        out = np.empty((h, w, channels), np.uint8)
+        _check(lib().rayon_job_resolve_u8_noalloc(self.ctx._h, self._h, out.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                                  channels, gamma), "rayon_job_resolve_u8_noalloc")
+        return out
+
     def write_png(self, path: str, gamma: float = 2.0):
         """Resolves the image (gamma-corrected RGB) and writes it as a PNG file."""
         _check(lib().rayon_job_write_png(self.ctx._h, self._h, str(path).encode(), gamma),

@@ -176,6 +176,7 @@ def declare(lib: C.CDLL) -> C.CDLL:
         "rayon_ctx_wave_stats": (st, [vp, P(WaveStats), i32]),
         "rayon_job_resolve_linear": (st, [vp, vp, P(f32)]),
         "rayon_job_resolve_u8": (st, [vp, vp, P(C.c_uint8), i32, f32]),
+        "rayon_job_resolve_u8_noalloc": (st, [vp, vp, P(C.c_uint8), i32, f32]),
         "rayon_job_sample_counts": (st, [vp, vp, P(i32)]),
         "rayon_job_stats_get": (st, [vp, vp, P(JobStats)]),
         "rayon_trace_closest": (st, [vp, vp, P(f32), P(f32), i64, f32, f32, P(i64), P(f32)]),

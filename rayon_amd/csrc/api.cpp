@@ -210,8 +210,24 @@ rayon_status rayon_ctx_create(int32_t device, rayon_ctx** out) {
       return RAYON_ERR_INVALID;
    }
    RY_HIP(hipSetDevice(device));
+   // the device address of the fixed staging array, once per device, with it current
+   void* scratch = nullptr;
+   {
+      static std::mutex mu;
+      static std::map<int, void*> addr;
+      std::lock_guard<std::mutex> lk(mu);
+      auto it = addr.find(device);
+      if (it == addr.end()) {
+         hipError_t e = noalloc_scratch_address(&scratch);
+         if (e != hipSuccess || !scratch) return hip_fail(e, "hipGetSymbolAddress (staging array)");
+         addr[device] = scratch;
+      } else {
+         scratch = it->second;
+      }
+   }
    auto* c = new rayon_ctx();
    c->device = device;
+   c->scratch = scratch;
    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
    if (e != hipSuccess) {
       delete c;
@@ -685,6 +701,36 @@ rayon_status rayon_job_resolve_u8(rayon_ctx* ctx, rayon_job* j, uint8_t* out, in
    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
    (void)hipFree(d);
    if (e != hipSuccess) return hip_fail(e, "resolve_u8");
+   return RAYON_OK;
+}
+
+rayon_status rayon_job_resolve_u8_noalloc(rayon_ctx* ctx, rayon_job* j, uint8_t* out, int32_t channels,
+                                          float gamma) {
+   if (!ctx || !j || !out || (channels != 3 && channels != 4) || !(gamma > 0.0f)) {
+      set_error("rayon_job_resolve_u8_noalloc: bad arguments");
+      return RAYON_ERR_INVALID;
+   }
+   if (!j->ctx || j->ctx->device != ctx->device) {
+      set_error("rayon_job_resolve_u8_noalloc: the job lives on another device than the context");
+      return RAYON_ERR_INVALID;
+   }
+   const size_t bytes = (size_t)j->params.width * j->params.height * channels;
+   if (bytes > kNoAllocBytes) {
+      set_error("rayon_job_resolve_u8_noalloc: %dx%d x %d channels is %zu bytes, beyond the %zu-byte staging "
+                "array; use rayon_job_resolve_u8",
+                j->params.width, j->params.height, channels, bytes, kNoAllocBytes);
+      return RAYON_ERR_INVALID;
+   }
+   if (!ctx->scratch) {
+      set_error("rayon_job_resolve_u8_noalloc: no staging array on device %d", ctx->device);
+      return RAYON_ERR_HIP;
+   }
+   RY_HIP(hipSetDevice(ctx->device));
+   uint8_t* d = (uint8_t*)ctx->scratch;
+   hipError_t e = launch_resolve_u8(j->view, d, channels, gamma, ctx->stream);
+   if (e == hipSuccess) e = hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, ctx->stream);
+   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+   if (e != hipSuccess) return hip_fail(e, "resolve_u8_noalloc");
    return RAYON_OK;
 }
 

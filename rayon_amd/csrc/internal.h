@@ -92,6 +92,9 @@ struct rayon_ctx {
    // allocations and every other first frame, and a second first-frame stream
    rayon_ctx* serve_first = nullptr;
    hipStream_t serve_pv = nullptr;
+   // device address of the fixed staging array on this context's device (looked up once
+   // per device at context creation)
+   void* scratch = nullptr;
    std::mutex mu;
    // wavefront state (shared by all batched launches of this context)
    ry::Wave* wave = nullptr;

@@ -49,6 +49,10 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t st)
 // `gamma` into each job's RGBA8 frame (no accumulation buffers are touched).
 hipError_t launch_preview(const JobView* views, uint8_t* const* frames, int n, int samples, float gamma,
                           hipStream_t st);
+// Fixed staging array of the allocation-free resolve, one per device (64 MiB: a 4096 x 4096
+// RGBA8 image); noalloc_scratch_address returns its address on the current device.
+constexpr size_t kNoAllocBytes = (size_t)64 << 20;
+hipError_t noalloc_scratch_address(void** ptr);
 // Device material and accumulation code on caller-given inputs (parity probes).
 hipError_t launch_scatter_probe(const MatRec& M, const JobView& J, const float* dirs, const float* nrm,
                                 const float* tan, const int* front, const float* rnd, int nrnd, long long n,

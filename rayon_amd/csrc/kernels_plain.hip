@@ -391,6 +391,14 @@ hipError_t launch_resolve_linear(const JobView& J, float* out, hipStream_t st) {
    return hipGetLastError();
 }
 
+// Staging area of the allocation-free resolve (rayon_job_resolve_u8_noalloc): a fixed
+// array of kNoAllocBytes per device, held until the process ends. Host code never uses
+// this name as an address: it asks the runtime for the device address once per device
+// (noalloc_scratch_address, with that device current) and passes that pointer on.
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+hipError_t noalloc_scratch_address(void** ptr) { return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch)); }
+
 hipError_t launch_trace_closest(const SceneView& S, const float* o, const float* d, long long n, float tmin,
                                 float tmax, long long* prim, float* t, hipStream_t st) {
    if (n <= 0) return hipSuccess;
