@@ -127,3 +127,52 @@ def test_null_scene_returns_zero():
     cam = ra.camera_frame((0, 0, 1), (0, 0, 0), (0, 1, 0), 40, 8, 8)
     assert call_render(L, None, cam, 8, 8, 4, 4, states, accp) == 0
     assert not states.value
+
+
+@pytest.mark.parametrize("dof", [False, True])
+def test_mode2_loop_matches_oracle_reference_stream(book, dof):
+    """The offline mode-2 loop of renderer_cuda_host.hpp:35-60 (five batches into a host
+    float3 buffer) against the FP64 oracle run with the reference's sequential stream
+    (cuda_utils.cu:26, cuda_utils.cuh:24-36) and material quirks (REF_RNG | REF_QUIRKS), with
+    the lens model of render_acc_kernel.cu:209-221 off and on. Tolerance: relative RMSE of
+    the summed image <= 1% (measured ~1e-3: FP32 against FP64 on identical random numbers)."""
+    from oracle import OracleScene
+    L = legacy()
+    W, H, spp, depth = 96, 54, 10, 16
+    scene = L.rayon_legacy_scene_create(book.desc_ptr())
+    lf, la, up, fov = book.camera()
+    cam = ra.camera_frame(lf, la, up, fov, W, H)
+    aperture, focus = 0.25, 10.0
+    L.setDOFEnabled(dof)
+    L.setDOFAperture(aperture)
+    L.setDOFFocusDistance(focus)
+    try:
+        states, accp = C.c_void_p(), C.c_void_p()
+        host = np.zeros(W * H * 3, np.float32)
+        per, done = max(1, spp // 5), 0
+        while done < spp:
+            k = min(per, spp - done)
+            call_render(L, scene, cam, W, H, k, depth, states, accp, host=host)
+            done += k
+        L.freeDeviceRandomStates(states)
+        L.freeDeviceAccumBuffer(accp)
+    finally:
+        L.setDOFEnabled(False)
+        L.rayon_legacy_scene_destroy(scene)
+    flags = ra.FLAG.REF_RNG | ra.FLAG.REF_QUIRKS | (ra.FLAG.DOF if dof else 0)
+    P = ra.job_params(W, H, cam, spp=spp, max_depth=depth, flags=flags, light_intensity=1.0,
+                      background_intensity=1.0, metal_fuzziness=0.8, glass_ior=1.5, dof_aperture=aperture,
+                      dof_focus_distance=focus)
+    osc = OracleScene(book.desc_ptr())
+    rng = np.array([olib().oracle_ref_rng_init(i) for i in range(W * H)], np.uint32)
+    sums, _ = osc.render(P, nsamples=spp, rng_state=rng)  # the FP32 camera of P, as the device casts it
+    ref = sums.reshape(-1)
+    err = float(np.sqrt(((host - ref) ** 2).sum() / (ref ** 2).sum()))
+    print(f"mode 2 dof={dof}: rel RMSE vs oracle {err:.5f}", flush=True)
+    assert err <= 0.01
+    if dof:  # the lens really moved the rays: the pinhole oracle image differs
+        P0 = ra.job_params(W, H, cam, spp=spp, max_depth=depth, flags=ra.FLAG.REF_RNG | ra.FLAG.REF_QUIRKS,
+                           metal_fuzziness=0.8, glass_ior=1.5)
+        rng0 = np.array([olib().oracle_ref_rng_init(i) for i in range(W * H)], np.uint32)
+        pin, _ = osc.render(P0, nsamples=spp, rng_state=rng0)
+        assert float(np.sqrt(((pin.reshape(-1) - ref) ** 2).sum() / (ref ** 2).sum())) > 0.05

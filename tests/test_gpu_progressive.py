@@ -82,3 +82,50 @@ def test_progressive_depth_schedule_and_reset(ctx, cornell):
     f = pr.step()
     assert f.samples == 4 and f.depth == 4
     pr.close()
+
+
+def reference_depth(samples, max_depth):
+    """calculateProgressiveMaxDepth (renderer_cuda_progressive_host.hpp:853-876), not moving."""
+    for lim, d in ((4, 4), (16, 5), (32, 6), (64, 7), (128, 8), (512, 16), (1024, 24)):
+        if samples <= lim:
+            return d
+    return min(512, max_depth)
+
+
+def test_progressive_depth_schedule_matches_oracle(ctx, cornell):
+    """The progressive renderer with the depth schedule on, against the FP64 oracle rendering
+    the same batches at the same depths (sample ranges and counter streams identical). Each
+    batch's depth follows calculateProgressiveMaxDepth of the samples after the batch, as
+    renderBatch computes it (:884-910). Tolerance: relative RMSE <= 1%."""
+    from oracle import OracleScene
+    g, sc = cornell
+    W, H, spp = 48, 32, 160
+    P = params(g, W, H, spp=spp, seed=6, max_depth=12, flags=ra.FLAG.NEE,
+               background_intensity=g.background_intensity())
+    pr = ctx.progressive(sc, P, target_fps=2000.0, max_batch=32, first_batch=4, adaptive_depth=1)
+    steps, before = [], 0
+    while True:
+        f = pr.step()
+        if f.batch:
+            steps.append((before, f.batch, f.depth))
+            before = f.samples
+        if f.done:
+            break
+    log(f"(sample0, batch, depth): {steps}")
+    assert before == spp
+    for s0, b, d in steps:
+        assert d == reference_depth(s0 + b, 12), (s0, b, d)
+    assert len({d for _, _, d in steps}) >= 3  # the schedule really changed depth
+    img = pr.job.resolve_linear().reshape(-1, 3).astype(np.float64)
+    osc = OracleScene(g.desc_ptr())
+    sums = np.zeros((W * H, 3))
+    for s0, b, d in steps:
+        Pd = params(g, W, H, spp=spp, seed=6, max_depth=d, flags=ra.FLAG.NEE,
+                    background_intensity=g.background_intensity())
+        part, _ = osc.render(Pd, sample0=s0, nsamples=b)
+        sums += part
+    ref = sums / spp
+    err = float(np.sqrt(((img - ref) ** 2).sum() / (ref ** 2).sum()))
+    log(f"progressive depth schedule vs oracle: rel RMSE {err:.5f}")
+    assert err <= 0.01
+    pr.close()
