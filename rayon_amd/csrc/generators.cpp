@@ -187,9 +187,11 @@ void icosphere(int levels, std::vector<double>& V, std::vector<int32_t>& F) {
 }
 
 // Cornell box [0,555]^3 (5 walls as 10 triangles), a 130x105 quad light of radiance 15
-// at y=554, and an icosphere (center (278,200,278), r=150, `levels` subdivisions)
-// displaced radially by r*0.05*vnoise(4*dir; seed 7). The sphere's material is a GGX
-// conductor or GGX dielectric with roughness U(0.05,0.8), drawn from seed+1.
+// at y=554, an icosphere (center (278,200,278), r=150, `levels` subdivisions) displaced
+// radially by r*0.05*vnoise(4*dir; seed 7), and a sphere (center (440,70,110), r=70) on the
+// floor in front of it. One of the two objects is a GGX conductor (gold), the other a GGX
+// dielectric (IOR 1.5), each with its own roughness U(0.05,0.8); which one is the
+// icosphere and both roughnesses are drawn from seed+1.
 void gen_cornell(rayon_generated* g, uint64_t seed, int levels) {
    g->background_intensity = 0.0f;  // the quad light is the only source
    int white = add_mat(g, mat_base(RAYON_MAT_LAMBERTIAN, 0.73f, 0.73f, 0.73f));
@@ -222,20 +224,31 @@ void gen_cornell(rayon_generated* g, uint64_t seed, int levels) {
    g->geoms.push_back(q);
 
    SplitMix R{seed + 1};
-   rayon_material sm;
-   const bool conductor = R.uni() < 0.5;
+   const bool conductor = R.uni() < 0.5;  // the icosphere's type; the sphere takes the other
    const float rough = (float)R.uni(0.05, 0.8);
-   if (conductor) {
-      sm = mat_base(RAYON_MAT_GGX_CONDUCTOR, 1, 1, 1);
-      const float eta[3] = {0.143f, 0.374f, 1.442f}, kk[3] = {3.983f, 2.385f, 1.603f};  // gold
-      std::memcpy(sm.eta, eta, sizeof eta);
-      std::memcpy(sm.k, kk, sizeof kk);
-   } else {
-      sm = mat_base(RAYON_MAT_GGX_DIELECTRIC, 1, 1, 1);
-      sm.refractive_index = 1.5f;
-   }
-   sm.roughness = rough;
-   int smat = add_mat(g, sm);
+   const float rough2 = (float)R.uni(0.05, 0.8);
+   auto ggx = [&](bool cond, float r) {
+      rayon_material m;
+      if (cond) {
+         m = mat_base(RAYON_MAT_GGX_CONDUCTOR, 1, 1, 1);
+         const float eta[3] = {0.143f, 0.374f, 1.442f}, kk[3] = {3.983f, 2.385f, 1.603f};  // gold
+         std::memcpy(m.eta, eta, sizeof eta);
+         std::memcpy(m.k, kk, sizeof kk);
+      } else {
+         m = mat_base(RAYON_MAT_GGX_DIELECTRIC, 1, 1, 1);
+         m.refractive_index = 1.5f;
+      }
+      m.roughness = r;
+      return m;
+   };
+   int smat = add_mat(g, ggx(conductor, rough));
+   rayon_geometry ball{};
+   ball.type = RAYON_GEOM_SPHERE;
+   ball.material_id = add_mat(g, ggx(!conductor, rough2));
+   ball.visible = 1;
+   const float bp[4] = {440, 70, 110, 70};
+   std::memcpy(ball.p, bp, sizeof bp);
+   g->geoms.push_back(ball);
    std::vector<double> V;
    std::vector<int32_t> F;
    icosphere(levels, V, F);

@@ -56,9 +56,17 @@ def test_book_scene_generator_is_deterministic_and_shaped():
 def test_cornell_generator_counts():
     g = ra.Generated.generate(ra.GEN.CORNELL_ICOSPHERE, 7, detail=3)
     d = g.desc
-    assert d.num_geometries == 11  # 10 wall triangles + the quad light
+    assert d.num_geometries == 12  # 10 wall triangles + the quad light + the sphere
     assert d.num_meshes == 1 and d.meshes[0].num_triangles == 20 * 4 ** 3
     assert d.materials[d.geometries[10].material_id].type == ra.MAT.LIGHT
+    # one GGX conductor and one GGX dielectric object, roughness each in [0.05, 0.8]
+    m_ico = d.materials[d.meshes[0].material_id]
+    m_ball = d.materials[d.geometries[11].material_id]
+    assert d.geometries[11].type == ra.GEOM.SPHERE
+    assert {m_ico.type, m_ball.type} == {ra.MAT.GGX_CONDUCTOR, ra.MAT.GGX_DIELECTRIC}
+    assert 0.05 <= m_ico.roughness <= 0.8 and 0.05 <= m_ball.roughness <= 0.8
+    ball = np.array(d.geometries[11].p[:4])
+    assert np.linalg.norm(ball[:3] - np.array([278, 200, 278])) > 150 * 1.05 + ball[3]  # apart from the icosphere
     pos = np.ctypeslib.as_array(d.meshes[0].positions, (d.meshes[0].num_vertices * 3,)).reshape(-1, 3)
     r = np.linalg.norm(pos - np.array([278, 200, 278]), axis=1)
     assert 150 * 0.95 - 1e-3 <= r.min() and r.max() <= 150 * 1.05 + 1e-3  # displacement 0.05 * [-1,1]
