@@ -18,3 +18,16 @@ echo "$HERE/_ref/ref_yaml_dump"
 g++ -std=c++17 -O1 -w -I"$SRC" -I"$SRC/camera" -I"$SRC/data_structures" -I"$SRC/cpu_renderers/utils" -I"$REF/src/external" \
     "$HERE/ref_output_dump.cpp" -o "$HERE/_ref/ref_output_dump"
 echo "$HERE/_ref/ref_output_dump"
+# the reference's offline GPU renderer (renderer_cuda_host.hpp, unchanged) linked against
+# librayon_hip.so through the scene shim a RayON maintainer would add
+# (rayon_amd/integration/scene_builder_hip.cc); needs the library built first
+LIB=$HERE/../rayon_amd/librayon_hip.so
+if [ -f "$LIB" ]; then
+  g++ -std=c++17 -O1 -w -I"$SRC" -I"$SRC/scenes" -I"$SRC/gpu_renderers" -I"$SRC/data_structures" -I"$SRC/camera" \
+      -I"$SRC/cpu_renderers" -I"$SRC/cpu_renderers/utils" -I"$SRC/cpu_renderers/cpu_shapes" -I"$HERE/../include" \
+      "$HERE/ref_cuda_host_drive.cpp" "$HERE/../rayon_amd/integration/scene_builder_hip.cc" \
+      "$SRC/scenes/yaml_scene_loader.cc" \
+      -L"$HERE/../rayon_amd" -lrayon_hip -Wl,-rpath,'$ORIGIN/../../rayon_amd' -pthread \
+      -o "$HERE/_ref/ref_cuda_host_drive"
+  echo "$HERE/_ref/ref_cuda_host_drive"
+fi
