@@ -12,11 +12,14 @@ Both paths are measured: the plain per-pixel megakernel (one launch per frame) a
 batched wavefront path (one hipGraph-replayed pass per frame); `value` is the batched
 one unless --path plain.
 
-Printed: one JSON line with the metric, the per-kernel roofline (algorithmic bytes from
-the traversal counters / kernel time from HIP events on the library's stream), and the
-CPU baseline (the FP64 oracle on a bounded pixel subset of the same job, all host cores),
-which also gives the relative RMSE of the GPU image on that subset with identical
-random-number streams.
+Printed: one JSON line with the metric, the roofline of the dominant kernel (compulsory
+bytes per launch / its launch time from HIP events on the library's stream; the bytes the
+traversal requests and the HBM traffic rocprofv3 measured for this workload beside them),
+the CPU baseline on a bounded pixel subset of the same job, which also gives the relative
+RMSE of the GPU image on that subset with identical random-number streams, and the metric's
+latency half: p50/p99 job completion, batched vs plain, over the 256-job batching workload,
+and p50/p99 time-to-first-frame and time-to-converge of the serving workload at 200 and 5
+jobs/s, both with every job's spp divided by 8 so the run stays within a few minutes.
 """
 from __future__ import annotations
 
@@ -121,6 +124,10 @@ def main(argv=None) -> int:
     ap.add_argument("--pool", type=int, default=4, help="serve: resident scenes per generator")
     ap.add_argument("--target-pass-ms", type=float, default=0.0,
                     help="serve: refinement pass duration the server aims for (0: library default)")
+    ap.add_argument("--spp-div", type=int, default=1,
+                    help="batch256 / serve: divide every job's spp (and the adaptive cap) by this")
+    ap.add_argument("--no-latency", action="store_true",
+                    help="cornell_1080p: skip the shortened batching and serving runs that give job latency")
     ap.add_argument("--fixed-spp", type=int, default=0,
                     help="sdf_720p: render this many samples per pixel without the adaptive rule (0: adaptive)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -154,7 +161,7 @@ def main(argv=None) -> int:
 
     log("generating scene")
     g = ra.Generated.generate(ra.GEN.CORNELL_ICOSPHERE, 7, detail=a.detail)
-    ntris = g.desc.meshes[0].num_triangles + 10
+    ntris = g.desc.meshes[0].num_triangles + 10  # the mesh and the walls (plus a sphere and the light)
     ctx = ra.Context(0)
     if a.capacity:
         ctx.set_option(ra.OPT.WAVE_CAPACITY, a.capacity)
@@ -278,12 +285,27 @@ def main(argv=None) -> int:
     log(f"stages (ms per frame): {stage_ms}; trace {ext_achieved:.0f} GB/s algorithmic")
 
     # measured HBM bytes per traced ray (rocprofv3 FETCH_SIZE/WRITE_SIZE, scaled by a copy
-    # kernel's counts) for this exact workload, kept beside the kernel-trace summary
-    traffic = None
-    tf = Path(__file__).resolve().parent / "profiles" / "k_wf_trace_cornell_icosphere_1080p_64spp.traffic.json"
-    if tf.exists() and (W, H, spp, a.depth, a.detail, a.layout) == (1920, 1080, 64, 8, 8, "binary"):
+    # kernel's counts) for this exact workload, from the committed profile of the same command
+    traffic, traffic_src = None, None
+    prof = Path(__file__).resolve().parent / "profiles"
+    tf = prof / "k_wf_trace_cornell_icosphere_1080p_64spp.traffic.json"
+    default_cfg = (W, H, spp, a.depth, a.detail, a.layout) == (1920, 1080, 64, 8, 8, "binary")
+    if tf.exists() and default_cfg:
         tr = json.loads(tf.read_text())
         traffic = round(tr["hbm_bytes_per_segment"] * (ext_rays + sh_rays) / max(ext_launches, 1))
+        traffic_src = f"profiles/{tf.name} (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, of this workload)"
+    valu = None
+    vf = prof / "valu_cornell_icosphere_1080p_64spp.json"
+    if vf.exists() and default_cfg:
+        valu = json.loads(vf.read_text())
+    # compulsory (algorithmic) bytes of one trace launch: every ray's state in and out once
+    # (extension: ray 48 B in, hit 16 B out; shadow: ray and contribution 48 B in, radiance
+    # read-modify-write 32 B) and the scene's node and primitive arrays read at most once
+    scene_b = int(info.device_bytes)
+    state_b = 64 * ext_rays + 80 * sh_rays
+    comp_per_launch = (state_b + scene_b * ext_launches) / max(ext_launches, 1)
+    avg_launch_s = ext_ms / max(ext_launches, 1) * 1e-3
+    comp_achieved = comp_per_launch / avg_launch_s / 1e9
 
     # one batched frame over the device-built tree (same job, identical image)
     dj = ctx.job(dsc, ra.job_params(W, H, cam, spp=spp, max_depth=a.depth, seed=1, flags=flags,
@@ -310,23 +332,30 @@ def main(argv=None) -> int:
     prim_samples = sum(s.primary_samples for s in (b_stats if use_batched else p_stats))
     wall = b_wall if use_batched else p_wall
     if use_batched:
-        roof = {"bound": "hbm", "achieved": round(ext_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ext_achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_wf_trace",
-                "avg_launch_ms": round(ext_ms / max(ext_launches, 1), 4), "launches_per_frame": int(ext_launches),
-                "alg_bytes_per_ray": round(ext_bytes / max(ext_rays + sh_rays, 1), 1),
-                "alg_bytes_per_launch": round(ext_bytes / max(ext_launches, 1)),
-                "achieved_ref_layout": round(ref_achieved, 1),
-                "ref_layout_note": "the same node/primitive visits at the reference's record sizes "
-                                   "(64 B per node, 112 B per primitive)",
-                "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
-                "traffic_frac": (None if traffic is None else
-                                 round(traffic / (ext_ms / max(ext_launches, 1) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)),
-                "frac_note": "achieved counts the bytes the traversal asks for; above 1 when L2 and the MALL "
-                             "serve them; traffic_frac is the measured HBM traffic over the same launch time"}
+        roof = {"bound": "hbm", "achieved": round(comp_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(comp_achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_wf_trace",
+                "avg_launch_ms": round(avg_launch_s * 1e3, 4), "launches_per_frame": int(ext_launches),
+                "alg_bytes_per_launch": round(comp_per_launch),
+                "alg_bytes_note": "compulsory bytes: ray state in/out (64 B per extension ray, 80 B per shadow "
+                                  "ray) + the scene's arrays once per launch",
+                "traffic_source": traffic_src,
+                "traffic_frac": (None if traffic is None else round(traffic / avg_launch_s / 1e9 / HBM_PEAK_GBS, 4)),
+                "traffic_over_alg": None if traffic is None else round(traffic / comp_per_launch, 2),
+                "requested_bytes_per_launch": round(ext_bytes / max(ext_launches, 1)),
+                "requested_GBs": round(ext_achieved, 1),
+                "requested_note": "every node (64 B) and primitive (48 B) record the traversal fetches, plus ray "
+                                  "state; most are served by L2 and the 256 MB MALL, so this rate exceeds HBM peak",
+                "requested_bytes_per_ray": round(ext_bytes / max(ext_rays + sh_rays, 1), 1),
+                "requested_GBs_ref_layout": round(ref_achieved, 1),
+                "valu": valu}
     else:
-        roof = {"bound": "hbm", "achieved": round(plain_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(plain_achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_plain",
-                "avg_launch_ms": round(p_avg_ms, 3), "alg_bytes_per_segment": round(plain_bps, 1)}
+        plain_comp = W * H * PIXEL_RMW_B + int(info.device_bytes)
+        roof = {"bound": "hbm", "achieved": round(plain_comp / (p_avg_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(plain_comp / (p_avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+                "traffic": None, "kernel": "k_plain", "avg_launch_ms": round(p_avg_ms, 3),
+                "alg_bytes_per_launch": plain_comp,
+                "alg_bytes_note": "compulsory bytes: accumulation read-modify-write per pixel + the scene once",
+                "requested_GBs": round(plain_achieved, 1), "requested_bytes_per_segment": round(plain_bps, 1)}
 
     result = {
         "metric": METRIC,
@@ -342,8 +371,8 @@ def main(argv=None) -> int:
         "dtype": "f32",
         "data": "synthetic (procedural scene generated in-process)",
         "config": {
-            "workload": f"cornell_icosphere: Cornell box + displaced icosphere ({ntris} triangles), "
-                        f"{W}x{H}, {spp} spp, max depth {a.depth}, NEE, GGX",
+            "workload": f"cornell_icosphere: Cornell box + displaced icosphere ({ntris} triangles) + sphere, "
+                        f"{W}x{H}, {spp} spp, max depth {a.depth}, NEE, GGX conductor and dielectric",
             "path": "batched wavefront (hipGraph pass)" if use_batched else "plain megakernel",
             "width": W, "height": H, "spp": spp, "max_depth": a.depth, "triangles": int(ntris),
             "layout": a.layout,
@@ -358,8 +387,8 @@ def main(argv=None) -> int:
                 "scene_create_ms": round(create_ms, 1),
                 "device_build": dev_tree},
         "roofline": roof,
-        "roofline_plain": {"kernel": "k_plain", "achieved": round(plain_achieved, 1),
-                           "frac": round(plain_achieved / HBM_PEAK_GBS, 4), "avg_launch_ms": round(p_avg_ms, 3)},
+        "roofline_plain": {"kernel": "k_plain", "requested_GBs": round(plain_achieved, 1),
+                           "avg_launch_ms": round(p_avg_ms, 3)},
     }
     timed = [b_job if use_batched else None]
 
@@ -409,6 +438,37 @@ def main(argv=None) -> int:
     b_job.close()
     sc.close()
     ctx.close()
+
+    if not a.no_latency:
+        # the metric's latency half, batched vs plain and served, on shortened runs of the
+        # batching and serving workloads: the same job mixes, scenes and rates with every
+        # job's spp (and the adaptive cap) divided by 8, so the default run stays short
+        div = 8
+        lctx = ra.Context(0)
+        log("job latency: batching workload, 256 jobs, spp / 8, quantized nodes")
+        rb = run_batch(a, ra, lctx, draw_jobs(256, 3, div), "quantized", 32 // div, "wavefront", oracle=False)
+        lctx.close()
+        lctx = ra.Context(0)
+        pool = create_scenes(ra, lctx, [(gen, 100 + s) for gen in range(3) for s in range(4)], "binary")
+        pool = {(gen, s - 100): v for (gen, s), v in pool.items()}
+        serve = []
+        for lam in (200.0, 5.0):
+            log(f"job latency: serving, {lam:g} jobs/s for 10 s, spp / 8")
+            serve.append(run_serve(a, ra, lctx, pool, lam, 10.0, 5.0, div))
+        for g2, sc2 in pool.values():
+            sc2.close()
+        lctx.close()
+        keep_b = ("jobs", "megapixels", "samples", "pass_spp", "layout", "batched_mrays_per_s", "plain_mrays_per_s",
+                  "batched_over_plain", "latency_ms", "latency_note", "autotune", "batched_vs_plain_max_rel_diff")
+        keep_s = ("lambda", "duration_s", "jobs", "completed_fraction", "first_frame_fraction", "ttff_p50_ms",
+                  "ttff_p99_ms", "ttc_p50_ms", "ttc_p99_ms", "ttc_note", "ttc_p50_ms_completed",
+                  "ttc_p99_ms_completed", "mrays_per_s", "offered_msamples_per_s")
+        result["job_latency"] = {
+            "note": "shortened runs: the batching (256 jobs) and serving (Poisson, 10 s per rate) workloads with "
+                    "every job's spp divided by 8; the full-length runs are bench.py --workload batch256 / serve",
+            "batched_vs_plain": {k: rb[k] for k in keep_b},
+            "served": [{k: r[k] for k in keep_s} for r in serve],
+        }
     print(json.dumps(result), flush=True)
     return 0
 
@@ -421,15 +481,17 @@ RESOLUTIONS = [(256, 256), (512, 512), (1024, 1024), (1920, 1080)]
 GEN_NAMES = ["book_spheres", "cornell_icosphere", "sdf_terrain"]
 
 
-def draw_jobs(n, seed):
+def draw_jobs(n, seed, spp_div=1):
     """Job mix of the batching workload: resolution uniform over RESOLUTIONS, generator
-    uniform over the three scene generators, per-job scene seed, spp uniform in 16..256."""
+    uniform over the three scene generators, per-job scene seed, spp uniform in 16..256
+    (divided by spp_div, at least 2, in the shortened runs)."""
     rng = np.random.default_rng(seed)
     out = []
     for j in range(n):
         W, H = RESOLUTIONS[int(rng.integers(len(RESOLUTIONS)))]
+        spp = int(rng.integers(16, 257))
         out.append(dict(index=j, gen=int(rng.integers(3)), scene_seed=j + 1, W=W, H=H,
-                        spp=int(rng.integers(16, 257)), seed=1000 + j))
+                        spp=max(2, spp // spp_div), seed=1000 + j))
     return out
 
 
@@ -456,42 +518,45 @@ def job_params_for(ra, g, spec, depth=8, **kw):
                          flags=flags, background_intensity=g.background_intensity(), **kw), cam
 
 
-def bench_batch256(a, ra) -> int:
-    """256 jobs of uneven size: the plain kernel one job per launch, then the batched
-    wavefront (all jobs in each hipGraph-replayed pass of --pass-spp samples, launch shape
-    from the autotuner). Same jobs, same seeds, same results; reports Mrays/s of both and
-    p50/p99 job completion latency (all jobs submitted at t = 0)."""
-    specs = draw_jobs(a.jobs, 3)
-    ctx = ra.Context(0)
+def create_scenes(ra, ctx, keys, layout):
+    """Scenes for (generator, seed) keys, generated and built on host threads in parallel
+    (every call releases the interpreter lock); returns {key: (generated, scene)}."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        made = list(ex.map(lambda k: make_scene(ra, ctx, k[0], k[1], layout), keys))
+    return dict(zip(keys, made))
+
+
+def run_batch(a, ra, ctx, specs, layout, pass_spp, batch_path="auto", oracle=True):
+    """The batching workload over `specs`: batched (autotune on clones, then passes of pass_spp
+    samples over all unfinished jobs) and plain (one launch per job, submission order). All
+    jobs are submitted at t = 0; a job's latency is its completion time. Returns a dict."""
     ev = HipEvents()
     t0 = time.perf_counter()
-    gens, scenes, params = [], [], []
-    dev_bytes = 0
-    for k, sp in enumerate(specs):
-        g, sc = make_scene(ra, ctx, sp["gen"], sp["scene_seed"], a.layout)
-        P, cam = job_params_for(ra, g, sp)
-        gens.append(g)
-        scenes.append(sc)
-        params.append((P, cam))
-        dev_bytes += sc.info().device_bytes
-        if (k + 1) % 32 == 0:
-            log(f"scenes {k + 1}/{len(specs)} ({time.perf_counter() - t0:.0f} s, {dev_bytes / 2**30:.1f} GiB)")
+    made = create_scenes(ra, ctx, [(sp["gen"], sp["scene_seed"]) for sp in specs], layout)
+    gens = [made[(sp["gen"], sp["scene_seed"])][0] for sp in specs]
+    scenes = [made[(sp["gen"], sp["scene_seed"])][1] for sp in specs]
+    params = [job_params_for(ra, g, sp) for g, sp in zip(gens, specs)]
+    dev_bytes = sum(sc.info().device_bytes for sc in scenes)
     px = sum(sp["W"] * sp["H"] for sp in specs)
     samples = sum(sp["W"] * sp["H"] * sp["spp"] for sp in specs)
-    log(f"{len(specs)} jobs, {px / 1e6:.1f} Mpixels, {samples / 1e9:.2f} G samples; scenes {dev_bytes / 2**30:.1f} GiB")
+    log(f"{len(specs)} jobs, {px / 1e6:.1f} Mpixels, {samples / 1e9:.2f} G samples; scenes {dev_bytes / 2**30:.1f} GiB "
+        f"created in {time.perf_counter() - t0:.0f} s")
     stream = ctx.stream
 
     def completion(evs, e_start, done_at):
         return [ev.elapsed_ms(e_start, evs[i]) for i in done_at]
 
     # ---- batched: autotune on clones, then progressive passes over all unfinished jobs
-    if a.batch_path != "auto":
+    if batch_path != "auto":
         for sc in scenes:
-            sc.set_batch_path(ra.PATH.WAVEFRONT if a.batch_path == "wavefront" else ra.PATH.MEGAKERNEL)
+            sc.set_batch_path(ra.PATH.WAVEFRONT if batch_path == "wavefront" else ra.PATH.MEGAKERNEL)
     jobs = [ctx.job(scenes[k], params[k][0]) for k in range(len(specs))]
+    t_tune = time.perf_counter()
     tune = ctx.autotune(jobs, 2)
     log(f"autotune: capacity {tune.capacity} chunk {tune.chunk} wg/CU {tune.wg_per_cu} "
-        f"({tune.best_ms:.1f} ms vs {tune.start_ms:.1f} ms, {tune.candidates} shapes)")
+        f"({tune.best_ms:.1f} ms vs {tune.start_ms:.1f} ms, {tune.candidates} shapes, "
+        f"{time.perf_counter() - t_tune:.0f} s)")
     ctx.sync()
     e_start = ev.create()
     pass_events, done_at = [], [None] * len(jobs)
@@ -500,13 +565,13 @@ def bench_batch256(a, ra) -> int:
     left = list(range(len(jobs)))
     done_spp = [0] * len(jobs)
     while left:
-        ctx.render_batched([jobs[i] for i in left], a.pass_spp)
+        ctx.render_batched([jobs[i] for i in left], pass_spp)
         e = ev.create()
         ev.record(e, stream)
         pass_events.append(e)
         nxt = []
         for i in left:
-            done_spp[i] = min(specs[i]["spp"], done_spp[i] + a.pass_spp)
+            done_spp[i] = min(specs[i]["spp"], done_spp[i] + pass_spp)
             if done_spp[i] >= specs[i]["spp"]:
                 done_at[i] = len(pass_events) - 1
             else:
@@ -552,7 +617,7 @@ def bench_batch256(a, ra) -> int:
 
     # ---- FP64 oracle on a pixel subset of one job per generator (same streams)
     rmse = {}
-    if not a.no_cpu_baseline:
+    if oracle:
         from oracle import OracleScene
         for gen in range(3):
             cand = [i for i in range(len(specs)) if specs[i]["gen"] == gen]
@@ -572,32 +637,46 @@ def bench_batch256(a, ra) -> int:
             log(f"oracle {GEN_NAMES[gen]} job {i}: rel RMSE {rmse[GEN_NAMES[gen]]}")
     for j in jobs:
         j.close()
-
+    for sc in scenes:
+        sc.close()
     b_mrays, p_mrays = b_segs / b_wall / 1e6, p_segs / p_wall / 1e6
-    result = {
-        "metric": METRIC, "value": round(b_mrays, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": 1, "warmup": 0,
-        "ms_per_step": round(b_wall * 1e3, 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic (procedural scenes generated in-process)",
-        "config": {"workload": f"batch256: {len(specs)} jobs, resolutions uniform over 256^2/512^2/1024^2/1920x1080, "
-                               f"generators uniform over book spheres / Cornell icosphere (1.31 M tris, NEE) / "
-                               f"SDF+terrain, per-job scene seeds, spp U{{16..256}}, depth 8, {a.layout} BVH",
-                   "jobs": len(specs), "megapixels": round(px / 1e6, 2), "samples": samples,
-                   "pass_spp": a.pass_spp, "layout": a.layout},
-        "batched_mrays_per_s": round(b_mrays, 2), "plain_mrays_per_s": round(p_mrays, 2),
-        "batched_over_plain": round(b_mrays / p_mrays, 3),
+    return {
+        "jobs": len(specs), "megapixels": round(px / 1e6, 2), "samples": samples, "pass_spp": pass_spp,
+        "layout": layout, "batched_mrays_per_s": round(b_mrays, 2), "plain_mrays_per_s": round(p_mrays, 2),
+        "batched_over_plain": round(b_mrays / p_mrays, 3), "batched_s": round(b_wall, 2), "plain_s": round(p_wall, 2),
         "latency_ms": {"batched_p50": round(percentile(b_lat, 50), 1), "batched_p99": round(percentile(b_lat, 99), 1),
                        "plain_p50": round(percentile(p_lat, 50), 1), "plain_p99": round(percentile(p_lat, 99), 1)},
+        "latency_note": "all jobs submitted at t = 0; latency = completion time of each job",
         "segments": int(b_segs), "plain_segments": int(p_segs),
         "autotune": {"capacity": tune.capacity, "chunk": tune.chunk, "shade_chunk": tune.shade_chunk,
-                     "refill": tune.refill,
-                     "wg_per_cu": tune.wg_per_cu, "mega_scenes": tune.mega_scenes, "wave_scenes": tune.wave_scenes,
-                     "best_ms": round(tune.best_ms, 2), "start_ms": round(tune.start_ms, 2),
-                     "shapes": tune.candidates},
+                     "refill": tune.refill, "wg_per_cu": tune.wg_per_cu, "mega_scenes": tune.mega_scenes,
+                     "wave_scenes": tune.wave_scenes, "best_ms": round(tune.best_ms, 2),
+                     "start_ms": round(tune.start_ms, 2), "shapes": tune.candidates},
         "batched_vs_plain_max_rel_diff": worst,
         "rel_rmse_vs_oracle": rmse,
     }
-    for sc in scenes:
-        sc.close()
+
+
+def bench_batch256(a, ra) -> int:
+    """256 jobs of uneven size: the plain kernel one job per launch, then the batched
+    wavefront (all jobs in each hipGraph-replayed pass of --pass-spp samples, launch shape
+    from the autotuner). Same jobs, same seeds, same results; reports Mrays/s of both and
+    p50/p99 job completion latency (all jobs submitted at t = 0)."""
+    specs = draw_jobs(a.jobs, 3, a.spp_div)
+    ctx = ra.Context(0)
+    r = run_batch(a, ra, ctx, specs, a.layout, a.pass_spp, a.batch_path, oracle=not a.no_cpu_baseline)
+    result = {
+        "metric": METRIC, "value": r["batched_mrays_per_s"], "unit": "Mrays/s", "n_gpus": 1, "steps": 1,
+        "warmup": 0, "ms_per_step": round(r["batched_s"] * 1e3, 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (procedural scenes generated in-process)",
+        "config": {"workload": f"batch256: {len(specs)} jobs, resolutions uniform over 256^2/512^2/1024^2/1920x1080, "
+                               f"generators uniform over book spheres / Cornell icosphere (1.31 M tris, NEE) / "
+                               f"SDF+terrain, per-job scene seeds, spp U{{16..256}}"
+                               + (f" / {a.spp_div}" if a.spp_div > 1 else "") + f", depth 8, {a.layout} BVH",
+                   "jobs": len(specs), "megapixels": r["megapixels"], "samples": r["samples"],
+                   "pass_spp": a.pass_spp, "layout": a.layout},
+    }
+    result.update({k: v for k, v in r.items() if k not in ("jobs", "megapixels", "samples", "pass_spp", "layout")})
     ctx.close()
     print(json.dumps(result), flush=True)
     return 0
@@ -719,95 +798,108 @@ def bench_scene(a, ra) -> int:
     return 0
 
 
+def run_serve(a, ra, ctx, pool, lam, duration, grace, spp_div=1, target_pass_ms=0.0):
+    """Poisson arrivals (rate lam, `duration` s, seed 42 + lam) into a render server. Jobs are
+    drawn like the batching workload from the scene pool; half fixed spp U{16..256}, half a
+    noise target U(0.5%, 2%) (relative std error, 16..1024 spp); spp divided by spp_div in the
+    shortened runs. After the arrivals, `grace` s to finish, then the rest is cancelled.
+    Time-to-first-frame and time-to-converge are reported over all arrivals; a job that did
+    not finish counts with its time up to the end of the run (censored), and the fraction of
+    completed jobs is given beside it."""
+    npool = max(s for _, s in pool) + 1
+    rng = np.random.default_rng(42 + int(lam))
+    arrivals, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / lam)
+        if t >= duration:
+            break
+        arrivals.append(t)
+    specs = draw_jobs(len(arrivals), 7 + int(lam), spp_div)
+    plist = []
+    for sp in specs:
+        g, sc = pool[(sp["gen"], int(rng.integers(npool)))]
+        if rng.random() < 0.5:
+            P, _ = job_params_for(ra, g, sp)
+        else:
+            sp = dict(sp, spp=max(2, 1024 // spp_div))
+            P, _ = job_params_for(ra, g, sp, extra_flags=ra.FLAG.ADAPTIVE_STDERR,
+                                  min_adaptive_samples=max(2, 16 // spp_div),
+                                  adaptive_threshold=float(rng.uniform(0.005, 0.02)))
+        plist.append((sc, P))
+    srv = ctx.server(**({"target_pass_ms": target_pass_ms} if target_pass_ms > 0 else {}))
+    ids = []
+    t0 = time.perf_counter()
+    next_log = 5.0
+    for (sc, P), ta in zip(plist, arrivals):
+        dt = t0 + ta - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        ids.append(srv.submit(sc, P))
+        if ta >= next_log:
+            st = srv.stats()
+            log(f"lam {lam:g}: t={ta:.0f}s submitted {len(ids)} active {st.active} queued {st.queued} "
+                f"completed {st.completed}")
+            next_log += 5.0
+    try:
+        srv.drain(grace)
+    except ra.RayonError:
+        pass
+    wall = time.perf_counter() - t0
+    infos = [srv.info(i) for i in ids]
+    for i, inf in zip(ids, infos):
+        if inf.state in (ra.JOB.QUEUED, ra.JOB.RUNNING):
+            srv.cancel(i)
+    srv.drain(60.0)
+    infos = [srv.info(i) for i in ids]
+    segs = sum(inf.segments for inf in infos)
+    st = srv.stats()
+    srv.close()
+    done = [inf.state == ra.JOB.DONE for inf in infos]
+    censor = [(wall - ta) * 1e3 for ta in arrivals]  # arrival to the end of the run
+    ttff = [inf.ttff_ms if inf.ttff_ms >= 0 else c for inf, c in zip(infos, censor)]
+    ttc = [inf.ttc_ms if (d and inf.ttc_ms >= 0) else c for inf, d, c in zip(infos, done, censor)]
+    ttc_done = [inf.ttc_ms for inf, d in zip(infos, done) if d and inf.ttc_ms >= 0]
+    mrays = segs / wall / 1e6
+    r = {"lambda": lam, "duration_s": duration, "jobs": len(ids), "completed": int(sum(done)),
+         "completed_fraction": round(sum(done) / max(len(ids), 1), 4),
+         "first_frame_fraction": round(sum(inf.ttff_ms >= 0 for inf in infos) / max(len(ids), 1), 4),
+         "ttff_p50_ms": percentile(ttff, 50), "ttff_p99_ms": percentile(ttff, 99),
+         "ttc_p50_ms": percentile(ttc, 50), "ttc_p99_ms": percentile(ttc, 99),
+         "ttc_note": "over all arrivals; unfinished jobs count with arrival-to-end-of-run time (censored)",
+         "ttc_p50_ms_completed": percentile(ttc_done, 50), "ttc_p99_ms_completed": percentile(ttc_done, 99),
+         "mrays_per_s": round(mrays, 2), "wall_s": round(wall, 1), "passes": int(st.passes),
+         "first_passes": int(st.first_passes),
+         "refine_s": round(st.pass_ms / 1e3, 2), "first_pass_s": round(st.first_ms / 1e3, 2),
+         "stats_read_s": round(st.check_ms / 1e3, 2),
+         "refine_msamples_per_s": round(st.pass_paths / max(st.pass_ms, 1e-9) / 1e3, 1),
+         "segments_by_generator": {GEN_NAMES[gi]: int(sum(inf.segments for inf, sp in zip(infos, specs)
+                                                         if sp["gen"] == gi)) for gi in range(3)}}
+    # offered load: the sample budget (pixels x spp cap) of the arrivals per second of the run
+    r["offered_msamples_per_s"] = round(sum(P.width * P.height * P.spp for _, P in plist) / max(wall, 1e-9) / 1e6, 1)
+    for k in ("ttff_p50_ms", "ttff_p99_ms", "ttc_p50_ms", "ttc_p99_ms", "ttc_p50_ms_completed",
+              "ttc_p99_ms_completed"):
+        r[k] = None if r[k] is None or not np.isfinite(r[k]) else round(r[k], 1)
+    log(json.dumps(r))
+    return r
+
+
 def bench_serve(a, ra) -> int:
-    """Poisson arrivals into the render server; per rate: p50/p99 time-to-first-frame and
-    time-to-converge, completed jobs and aggregate Mrays/s. Jobs are drawn like the batching
-    workload (scenes from a resident pool of --pool seeds per generator); half have a fixed
-    spp U{16..256}, half a noise target U(0.5%, 2%) (relative std-error, 16..1024 spp)."""
+    """Poisson arrivals into the render server at each rate of --lam (see run_serve)."""
     ctx = ra.Context(0)
-    pool = {}
-    for gen in range(3):
-        for s in range(a.pool):
-            pool[(gen, s)] = make_scene(ra, ctx, gen, 100 + s, a.layout)
+    pool = create_scenes(ra, ctx, [(gen, 100 + s) for gen in range(3) for s in range(a.pool)], a.layout)
+    pool = {(gen, s - 100): v for (gen, s), v in pool.items()}
     log(f"scene pool: {len(pool)} scenes")
     out = {"metric": METRIC, "unit": "Mrays/s", "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic (procedural scenes generated in-process)",
            "config": {"workload": f"serve: Poisson arrivals, jobs as batch256 (pool of {a.pool} scenes per "
-                                  f"generator), 50% fixed spp U{{16..256}}, 50% noise target U(0.5%,2%), "
-                                  f"{a.duration:.0f} s per rate, HBM budget 200 GiB", "layout": a.layout},
+                                  f"generator), 50% fixed spp U{{16..256}}, 50% noise target U(0.5%,2%)"
+                                  + (f", spp / {a.spp_div}" if a.spp_div > 1 else "")
+                                  + f", {a.duration:.0f} s per rate, HBM budget 200 GiB", "layout": a.layout},
            "rates": []}
     best = 0.0
     for lam in [float(x) for x in a.lam.split(",") if x]:
-        rng = np.random.default_rng(42 + int(lam))
-        arrivals, t = [], 0.0
-        while True:
-            t += rng.exponential(1.0 / lam)
-            if t >= a.duration:
-                break
-            arrivals.append(t)
-        specs = draw_jobs(len(arrivals), 7 + int(lam))
-        plist = []
-        for sp in specs:
-            g, sc = pool[(sp["gen"], int(rng.integers(a.pool)))]
-            if rng.random() < 0.5:
-                P, _ = job_params_for(ra, g, sp)
-            else:
-                sp = dict(sp, spp=1024)
-                P, _ = job_params_for(ra, g, sp, extra_flags=ra.FLAG.ADAPTIVE_STDERR, min_adaptive_samples=16,
-                                      adaptive_threshold=float(rng.uniform(0.005, 0.02)))
-            plist.append((sc, P))
-        srv = ctx.server(**({"target_pass_ms": a.target_pass_ms} if a.target_pass_ms > 0 else {}))
-        ids = []
-        t0 = time.perf_counter()
-        next_log = 5.0
-        for (sc, P), ta in zip(plist, arrivals):
-            dt = t0 + ta - time.perf_counter()
-            if dt > 0:
-                time.sleep(dt)
-            ids.append(srv.submit(sc, P))
-            if ta >= next_log:
-                st = srv.stats()
-                log(f"lam {lam:g}: t={ta:.0f}s submitted {len(ids)} active {st.active} queued {st.queued} "
-                    f"completed {st.completed}")
-                next_log += 5.0
-        try:
-            srv.drain(a.grace)
-        except ra.RayonError:
-            pass
-        wall = time.perf_counter() - t0
-        infos = [srv.info(i) for i in ids]
-        for i, inf in zip(ids, infos):
-            if inf.state in (ra.JOB.QUEUED, ra.JOB.RUNNING):
-                srv.cancel(i)
-        srv.drain(60.0)
-        infos = [srv.info(i) for i in ids]
-        segs = sum(inf.segments for inf in infos)
-        ttff = [inf.ttff_ms if inf.ttff_ms >= 0 else float("inf") for inf in infos]
-        ttc = [inf.ttc_ms if inf.ttc_ms >= 0 else float("inf") for inf in infos]
-        st = srv.stats()
-        srv.close()
-        mrays = segs / wall / 1e6
-        best = max(best, mrays)
-        ttc_done = [x for x in ttc if np.isfinite(x)]
-        r = {"lambda": lam, "jobs": len(ids), "completed": int(sum(inf.state == ra.JOB.DONE for inf in infos)),
-             "offered_msamples_per_s": None,
-             "ttff_p50_ms": percentile(ttff, 50), "ttff_p99_ms": percentile(ttff, 99),
-             "ttc_p50_ms": percentile(ttc, 50), "ttc_p99_ms": percentile(ttc, 99),
-             "ttc_p50_ms_completed": percentile(ttc_done, 50), "ttc_p99_ms_completed": percentile(ttc_done, 99),
-             "mrays_per_s": round(mrays, 2), "wall_s": round(wall, 1), "passes": int(st.passes),
-             "first_passes": int(st.first_passes),
-             "refine_s": round(st.pass_ms / 1e3, 2), "first_pass_s": round(st.first_ms / 1e3, 2),
-             "stats_read_s": round(st.check_ms / 1e3, 2),
-             "refine_msamples_per_s": round(st.pass_paths / max(st.pass_ms, 1e-9) / 1e3, 1),
-             "segments_by_generator": {GEN_NAMES[gi]: int(sum(inf.segments for inf, (sc, P), sp in
-                                                             zip(infos, plist, specs) if sp["gen"] == gi))
-                                       for gi in range(3)}}
-        # offered load: the sample budget (pixels x spp cap) of the arrivals per second of the run
-        r["offered_msamples_per_s"] = round(sum(P.width * P.height * P.spp for _, P in plist) / max(wall, 1e-9) / 1e6, 1)
-        for k in ("ttff_p50_ms", "ttff_p99_ms", "ttc_p50_ms", "ttc_p99_ms", "ttc_p50_ms_completed",
-                  "ttc_p99_ms_completed"):
-            r[k] = None if r[k] is None or not np.isfinite(r[k]) else round(r[k], 1)
-        log(json.dumps(r))
+        r = run_serve(a, ra, ctx, pool, lam, a.duration, a.grace, a.spp_div, a.target_pass_ms)
+        best = max(best, r["mrays_per_s"])
         out["rates"].append(r)
     out["value"] = round(best, 2)
     out["steps"], out["warmup"], out["ms_per_step"] = len(out["rates"]), 0, None
