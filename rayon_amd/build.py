@@ -27,9 +27,13 @@ ORACLE_LIB = ORACLE_DIR / "liboracle.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 # Packed FP32 ops are disabled for every device compile; the compensated sums in the
-# accumulation rely on plain IEEE adds, so no fast-math either.
+# accumulation rely on plain IEEE adds, so no fast-math either. Multiply-adds are fused
+# only within one source expression (-ffp-contract=on), never across statements after
+# inlining: the same device function then rounds the same way in every kernel that
+# inlines it, so a job's bits do not depend on which kernel variant (stack size, tree
+# layout, counting) a batched pass picks for the company it keeps.
 HIP_FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=on",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result", "-Wno-unused-command-line-argument",
 ]

@@ -108,7 +108,7 @@ struct Rng {
 // traversal
 // ------------------------------------------------------------------------------------
 struct RayInv {
-   V3 o, d, inv, ood;
+   V3 o, d, inv;
 };
 
 __device__ __forceinline__ float safe_inv(float d) {
@@ -121,27 +121,35 @@ __device__ __forceinline__ RayInv make_rayinv(V3 o, V3 d) {
    r.o = o;
    r.d = d;
    r.inv = mk(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
-   r.ood = r.o * r.inv;
    return r;
 }
 
-// Slab test of both children of a pair node; returns entry distances, hit flags.
+// Slab test of both children of a pair node; returns entry distances, hit flags. Each slab
+// distance is (b - o) * inv: two roundings and the reciprocal's, so it is within 3 * 2^-24
+// relative of the exact distance whatever the magnitudes of b and o (the form b * inv -
+// o * inv is off by an ulp of o * inv, which dwarfs a short distance from a far origin).
 __device__ __forceinline__ void box2(const RayInv& r, const Node2& n, float tmin, float tmax, float& tn0,
                                      float& tn1, bool& h0, bool& h1) {
-   float ax0 = fmaf(n.bx.x, r.inv.x, -r.ood.x), bx0 = fmaf(n.bx.y, r.inv.x, -r.ood.x);
-   float ay0 = fmaf(n.by.x, r.inv.y, -r.ood.y), by0 = fmaf(n.by.y, r.inv.y, -r.ood.y);
-   float az0 = fmaf(n.bz.x, r.inv.z, -r.ood.z), bz0 = fmaf(n.bz.y, r.inv.z, -r.ood.z);
-   float ax1 = fmaf(n.bx.z, r.inv.x, -r.ood.x), bx1 = fmaf(n.bx.w, r.inv.x, -r.ood.x);
-   float ay1 = fmaf(n.by.z, r.inv.y, -r.ood.y), by1 = fmaf(n.by.w, r.inv.y, -r.ood.y);
-   float az1 = fmaf(n.bz.z, r.inv.z, -r.ood.z), bz1 = fmaf(n.bz.w, r.inv.z, -r.ood.z);
+   float ax0 = (n.bx.x - r.o.x) * r.inv.x, bx0 = (n.bx.y - r.o.x) * r.inv.x;
+   float ay0 = (n.by.x - r.o.y) * r.inv.y, by0 = (n.by.y - r.o.y) * r.inv.y;
+   float az0 = (n.bz.x - r.o.z) * r.inv.z, bz0 = (n.bz.y - r.o.z) * r.inv.z;
+   float ax1 = (n.bx.z - r.o.x) * r.inv.x, bx1 = (n.bx.w - r.o.x) * r.inv.x;
+   float ay1 = (n.by.z - r.o.y) * r.inv.y, by1 = (n.by.w - r.o.y) * r.inv.y;
+   float az1 = (n.bz.z - r.o.z) * r.inv.z, bz1 = (n.bz.w - r.o.z) * r.inv.z;
    tn0 = fmaxf(fmaxf(fminf(ax0, bx0), fminf(ay0, by0)), fmaxf(fminf(az0, bz0), tmin));
    float tf0 = fminf(fminf(fmaxf(ax0, bx0), fmaxf(ay0, by0)), fminf(fmaxf(az0, bz0), tmax));
    tn1 = fmaxf(fmaxf(fminf(ax1, bx1), fminf(ay1, by1)), fmaxf(fminf(az1, bz1), tmin));
    float tf1 = fminf(fminf(fmaxf(ax1, bx1), fmaxf(ay1, by1)), fminf(fmaxf(az1, bz1), tmax));
-   // widen the exit by 2^-22 relative so FP32 rounding of the slab distances can never
-   // reject a box the exact test accepts (flat boxes included)
-   h0 = tn0 <= tf0 * 1.00000024f;
-   h1 = tn1 <= tf1 * 1.00000024f;
+   // The exit (or the current closest hit) is widened by 2^-16 relative, so neither the
+   // rounding of the slab distances (3 * 2^-24) nor that of a primitive's own hit distance
+   // (Moller-Trumbore's relative error grows as 1 / sin of the angle between ray and
+   // triangle: 2^-16 covers angles down to about half a degree) rejects a box holding a
+   // primitive whose computed hit is no farther than the closest so far. The set of
+   // primitives a closest-hit traversal tests then does not depend on the order it meets
+   // them in (postponed leaves make that order depend on the other rays of the wave, whose
+   // composition depends on the timing of the queue claims), so neither does its result.
+   h0 = tn0 <= tf0 * 1.0000153f;
+   h1 = tn1 <= tf1 * 1.0000153f;
 }
 
 struct HitMin {
@@ -616,8 +624,9 @@ __device__ __forceinline__ uint32_t byte_of(const uint32_t* w, int i) { return (
 // Tests the eight quantized child boxes of a wide node. Returns the hit internal
 // children as a mask in visiting order (bit i = slot i ^ perm) and the hit leaf children
 // as a slot mask. Each child plane is q * (2^e / d) + (p - o) / d; the exit distance is
-// widened by 2^-22 relative plus 2^-21 of the largest plane offset so FP32 rounding never
-// rejects a box the exact test accepts.
+// widened by 2^-16 relative plus 2^-21 of the largest plane offset (see box2) so FP32
+// rounding never rejects a box the exact test accepts, nor one holding a primitive whose
+// computed hit is no farther than the closest so far.
 __device__ __forceinline__ void wide_children(const WNode& n, V3 o, V3 inv, uint32_t perm, float tmin, float tmax,
                                               uint32_t& hit_int, uint32_t& hit_leaf) {
    const uint32_t em = n.e_imask;
@@ -639,7 +648,7 @@ __device__ __forceinline__ void wide_children(const WNode& n, V3 o, V3 inv, uint
       const float tf = fminf(fminf(fmaxf(lx, hx), fmaxf(ly, hy)), fminf(fmaxf(lz, hz), tmax));
       // an empty slot stores lo = 255 > hi = 0 on every axis: its box is inverted
       const bool valid = byte_of(n.q, sl) <= byte_of(n.q, 24 + sl);
-      if (valid && tn <= tf * 1.00000024f + slack) {
+      if (valid && tn <= fmaf(tf, 1.0000153f, slack)) {
          if ((imask >> sl) & 1u)
             hit_int |= 1u << (sl ^ perm);
          else

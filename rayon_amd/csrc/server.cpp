@@ -239,8 +239,9 @@ struct rayon_server {
       std::vector<FirstBatch> firsts;
       bool flying = false;
       double t_pass = 0;
-      hipEvent_t pass_done = nullptr;
+      hipEvent_t pass_done = nullptr, alloc_done = nullptr;
       (void)hipEventCreateWithFlags(&pass_done, hipEventDisableTiming);
+      (void)hipEventCreateWithFlags(&alloc_done, hipEventDisableTiming);
       uint64_t seen_seq = 0;
       for (;;) {
          std::vector<SJob*> fresh, previews;
@@ -310,10 +311,11 @@ struct rayon_server {
             }
          }
          // (1) admitted jobs: buffers on first_ctx. Its allocations must not queue behind
-         // the refinement pass in flight on the main stream; a preview stream waits for them
-         // and every other use follows the host's sight of the first frame's completion. The
-         // buffers are cleared on the main stream just before the job's first refinement
-         // pass, so a first frame waits for nothing but its own kernel.
+         // the refinement pass in flight on the main stream; the main stream waits for them
+         // by event (the stream-ordered pool may hand out memory whose earlier use on
+         // first_ctx's stream is still in flight), and so does a preview stream. The buffers
+         // are cleared on the main stream just before the job's first refinement pass, so a
+         // first frame waits for nothing but its own kernel.
          for (SJob* j : fresh) {
             rayon_status st = ry::job_create(first_ctx, j->scene, &j->params, &j->job, false, false);
             std::lock_guard<std::mutex> lk(mu);
@@ -324,6 +326,10 @@ struct rayon_server {
             j->needs_clear = true;
             j->state = RAYON_JOB_RUNNING;
             j->live_px = (int64_t)j->params.width * j->params.height;
+         }
+         if (!fresh.empty() && (hipEventRecord(alloc_done, first_ctx->stream) != hipSuccess ||
+                                hipStreamWaitEvent(ctx->stream, alloc_done, 0) != hipSuccess)) {
+            (void)hipStreamSynchronize(first_ctx->stream);  // the same order, by the host
          }
          // (1a) first frames: the job's first samples rendered and resolved to RGBA8 by one
          // launch of the preview kernel for all of them (the plain kernel's per-pixel
@@ -475,6 +481,7 @@ struct rayon_server {
       for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
       event_pool.clear();
       (void)hipEventDestroy(pass_done);
+      (void)hipEventDestroy(alloc_done);
    }
 };
 
