@@ -393,31 +393,43 @@ def main(argv=None) -> int:
     timed = [b_job if use_batched else None]
 
     if not a.no_cpu_baseline:
-        from oracle import OracleScene
-        log("cpu baseline: oracle on a pixel subset")
+        from oracle import OracleScene, cpu_baseline_lib
         job0 = b_job
         img = job0.resolve_linear().reshape(-1, 3).astype(np.float64)
         ys, xs = np.meshgrid(np.arange(a.cpu_stride // 2, H, a.cpu_stride),
                              np.arange(a.cpu_stride // 2, W, a.cpu_stride), indexing="ij")
         pix = (ys * W + xs).reshape(-1).astype(np.int64)
-        t_b = time.perf_counter()
-        osc = OracleScene(g.desc_ptr())
-        t_build = time.perf_counter() - t_b
         cores = min(16, os.cpu_count() or 1)  # the host CPU share of one GPU
+        # CPU baseline: the FP32 build of the path's restatement with the GPU's tree rule
+        # (binned SAH) and the reference CPU renderer's row chunks, on the same pixels and streams
+        log("cpu baseline: FP32 CPU renderer on a pixel subset")
+        t_b = time.perf_counter()
+        bsc = OracleScene(g.desc_ptr(), cpu_baseline_lib())
+        t_bbuild = time.perf_counter() - t_b
         t_r = time.perf_counter()
-        sums, osegs = osc.render(job0.params, pixels=pix, nsamples=spp, threads=cores, camera=cam)
+        bsums, bsegs = bsc.render(job0.params, pixels=pix, nsamples=spp, threads=cores, camera=cam)
         t_cpu = time.perf_counter() - t_r
+        bsc.close()
+        # checker: the FP64 oracle on the same pixels and streams
+        log("oracle on the same pixel subset")
+        osc = OracleScene(g.desc_ptr())
+        sums, osegs = osc.render(job0.params, pixels=pix, nsamples=spp, threads=cores, camera=cam)
         ref = sums / spp
         err = float(np.sqrt(((img[pix] - ref) ** 2).sum() / (ref ** 2).sum()))
+        berr = float(np.sqrt(((bsums / spp - ref) ** 2).sum() / (ref ** 2).sum()))
         result["cpu_baseline"] = {
-            "value": round(osegs / t_cpu / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
-            "sample": f"FP64 oracle, every {a.cpu_stride}th row and column of the same job ({len(pix)} pixels x "
-                      f"{spp} spp, {osegs} segments, {t_cpu:.1f} s; oracle BVH build {t_build:.1f} s excluded)",
-            "cpu": cpu_model(),
+            "value": round(bsegs / t_cpu / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
+            "sample": f"FP32 multi-threaded CPU renderer (the path's restatement built in FP32, binned-SAH tree, "
+                      f"one contiguous chunk of rows per thread as renderer_cpu_parallel.hpp:56-63), every "
+                      f"{a.cpu_stride}th row and column of the same job ({len(pix)} pixels x {spp} spp, {bsegs} "
+                      f"segments, {t_cpu:.1f} s; tree build {t_bbuild:.1f} s excluded)",
+            "nproc": os.cpu_count(), "cpu": cpu_model(),
+            "primary_samples_per_s": round(len(pix) * spp / t_cpu, 1),
+            "rel_rmse_vs_oracle": round(berr, 5),
         }
         result["rel_rmse_vs_oracle"] = round(err, 5)
         result["rmse_note"] = f"same random streams, {spp} spp, {len(pix)}-pixel subset, linear RGB"
-        log(f"cpu baseline {osegs / t_cpu / 1e6:.2f} Mrays/s on {cores} cores, rel RMSE {err:.5f}")
+        log(f"cpu baseline {bsegs / t_cpu / 1e6:.2f} Mrays/s on {cores} threads; GPU rel RMSE vs oracle {err:.5f}")
         # converged image: the same scene and camera at 64x36, 4096 spp, batched path vs oracle
         cw, chh, cspp = 64, 36, 4096
         ccam = ra.camera_frame(lf, la, up, fov, cw, chh)
@@ -776,20 +788,26 @@ def bench_scene(a, ra) -> int:
             ctx.render_batched([fj], 16)
             img = fj.resolve_linear().reshape(-1, 3).astype(np.float64)[pix]
             fj.close()
+        from oracle import cpu_baseline_lib
+        bsc = OracleScene(g.desc_ptr(), cpu_baseline_lib())
         t0 = time.perf_counter()
-        sums, osegs = osc.render(Pc, pixels=pix, nsamples=nspp, threads=cores, camera=cam)
+        bsums, bsegs = bsc.render(Pc, pixels=pix, nsamples=nspp, threads=cores, camera=cam)
         t_cpu = time.perf_counter() - t0
+        bsc.close()
+        sums, osegs = osc.render(Pc, pixels=pix, nsamples=nspp, threads=cores, camera=cam)
         ref = sums / nspp
         err = float(np.sqrt(((img - ref) ** 2).sum() / (ref ** 2).sum()))
+        berr = float(np.sqrt(((bsums / nspp - ref) ** 2).sum() / (ref ** 2).sum()))
         npx = W * H if pix is None else len(pix)
-        out["cpu_baseline"] = {"value": round(osegs / t_cpu / 1e6, 3), "unit": "Mrays/s", "cores": cores,
+        out["cpu_baseline"] = {"value": round(bsegs / t_cpu / 1e6, 3), "unit": "Mrays/s", "cores": cores,
                                "kind": "port",
-                               "sample": (f"FP64 oracle, {'every pixel' if book else 'every 4th row and column'} "
-                                          f"({npx} pixels x {nspp} spp, {osegs} segments, {t_cpu:.1f} s)"),
-                               "cpu": cpu_model()}
+                               "sample": (f"FP32 multi-threaded CPU renderer (binned-SAH tree, row chunks per "
+                                          f"thread), {'every pixel' if book else 'every 4th row and column'} "
+                                          f"({npx} pixels x {nspp} spp, {bsegs} segments, {t_cpu:.1f} s)"),
+                               "nproc": os.cpu_count(), "cpu": cpu_model(), "rel_rmse_vs_oracle": round(berr, 6)}
         out["rel_rmse_vs_oracle"] = round(err, 6)
         out["rmse_note"] = f"same random streams, {nspp} spp, {npx} pixels, linear RGB"
-        log(f"cpu baseline {osegs / t_cpu / 1e6:.2f} Mrays/s on {cores} cores, rel RMSE {err:.6f}")
+        log(f"cpu baseline {bsegs / t_cpu / 1e6:.2f} Mrays/s on {cores} threads; GPU rel RMSE {err:.6f}")
         osc.close()
     keep.close()
     sc.close()

@@ -13,15 +13,22 @@ import numpy as np
 from rayon_amd.abi import Camera, JobParams, Material, SceneDesc
 
 LIB_PATH = Path(__file__).resolve().parent / "liboracle.so"
-_lib = None
+# The benchmark's CPU baseline: the same restatement built in FP32 with the GPU path's tree
+# rule and the reference CPU renderer's row-chunk schedule (rayon_amd/build.py). Timed by
+# bench.py; never the checker.
+CPU_BASE_PATH = Path(__file__).resolve().parent / "libcpubase.so"
+_libs = {}
 
 
-def lib() -> C.CDLL:
-    global _lib
-    if _lib is None:
-        if not LIB_PATH.exists():
-            raise RuntimeError(f"{LIB_PATH} is not built; run `python -m rayon_amd.build --oracle`")
-        L = C.CDLL(str(LIB_PATH))
+def cpu_baseline_lib() -> C.CDLL:
+    return lib(CPU_BASE_PATH)
+
+
+def lib(path: Path = LIB_PATH) -> C.CDLL:
+    if path not in _libs:
+        if not path.exists():
+            raise RuntimeError(f"{path} is not built; run `python -m rayon_amd.build --oracle`")
+        L = C.CDLL(str(path))
         vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
         P = C.POINTER
         L.oracle_scene_create.restype = vp
@@ -73,8 +80,8 @@ def lib() -> C.CDLL:
         L.oracle_unit_disk.argtypes = [P(C.c_float), i64, i64, d3]
         L.oracle_scatter_batch.argtypes = [P(Material), C.c_uint32, C.c_float, C.c_float, C.c_float, d3, d3, i32,
                                            d3, P(C.c_float), i32, i64, d3, d3, P(i32)]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 class fault:
@@ -135,16 +142,17 @@ def _dp(a):
 
 
 class OracleScene:
-    def __init__(self, desc_ptr):
-        self._h = lib().oracle_scene_create(desc_ptr)
+    def __init__(self, desc_ptr, library: C.CDLL = None):
+        self._L = library or lib()
+        self._h = self._L.oracle_scene_create(desc_ptr)
 
     @property
     def num_prims(self) -> int:
-        return lib().oracle_scene_num_prims(self._h)
+        return self._L.oracle_scene_num_prims(self._h)
 
     @property
     def num_lights(self) -> int:
-        return lib().oracle_scene_num_lights(self._h)
+        return self._L.oracle_scene_num_lights(self._h)
 
     def trace_closest(self, origins, dirs, t_min=0.001, t_max=3.4e38, brute=False):
         o = np.ascontiguousarray(origins, np.float32)
@@ -152,20 +160,20 @@ class OracleScene:
         n = o.shape[0]
         prim = np.empty(n, np.int64)
         t = np.empty(n, np.float64)
-        lib().oracle_trace_closest(self._h, _fp(o), _fp(d), n, t_min, t_max, int(brute),
+        self._L.oracle_trace_closest(self._h, _fp(o), _fp(d), n, t_min, t_max, int(brute),
                                    prim.ctypes.data_as(C.POINTER(C.c_int64)), _dp(t))
         return prim, t
 
     def intersect_prim(self, prim, origin, d, t_min=0.001, t_max=3.4e38) -> float:
         o = np.ascontiguousarray(origin, np.float32)
         dd = np.ascontiguousarray(d, np.float32)
-        return lib().oracle_intersect_prim(self._h, int(prim), _fp(o), _fp(dd), t_min, t_max)
+        return self._L.oracle_intersect_prim(self._h, int(prim), _fp(o), _fp(dd), t_min, t_max)
 
     def trace_any(self, origins, dirs, t_min, t_max):
         o = np.ascontiguousarray(origins, np.float32)
         d = np.ascontiguousarray(dirs, np.float32)
         out = np.empty(o.shape[0], np.int32)
-        lib().oracle_trace_any(self._h, _fp(o), _fp(d), o.shape[0], t_min, t_max,
+        self._L.oracle_trace_any(self._h, _fp(o), _fp(d), o.shape[0], t_min, t_max,
                                out.ctypes.data_as(C.POINTER(C.c_int32)))
         return out
 
@@ -183,7 +191,7 @@ class OracleScene:
         rs = None
         if rng_state is not None:
             rs = rng_state.ctypes.data_as(C.POINTER(C.c_uint32))
-        lib().oracle_render(self._h, C.byref(params), C.byref(camera) if camera is not None else None,
+        self._L.oracle_render(self._h, C.byref(params), C.byref(camera) if camera is not None else None,
                             pix.ctypes.data_as(C.POINTER(C.c_int64)) if pix is not None else None, npix, sample0,
                             nsamples, threads, _dp(sums), rs, C.byref(segs))
         return sums, segs.value
@@ -191,13 +199,13 @@ class OracleScene:
     def render_samples(self, params: JobParams, pixels, sample0, nsamples, threads=0):
         pix = np.ascontiguousarray(pixels, np.int64)
         out = np.zeros((pix.shape[0], nsamples, 3), np.float64)
-        lib().oracle_render_samples(self._h, C.byref(params), None, pix.ctypes.data_as(C.POINTER(C.c_int64)),
+        self._L.oracle_render_samples(self._h, C.byref(params), None, pix.ctypes.data_as(C.POINTER(C.c_int64)),
                                     pix.shape[0], sample0, nsamples, threads, _dp(out))
         return out
 
     def close(self):
         if self._h:
-            lib().oracle_scene_free(self._h)
+            self._L.oracle_scene_free(self._h)
             self._h = None
 
     def __del__(self):

@@ -15,11 +15,42 @@
 #include <cmath>
 #include <cstring>
 #include <thread>
+#include <type_traits>
 #include <vector>
+
+// The checker computes in FP64 (real = double). The benchmark's CPU baseline is this same
+// restatement built with ORACLE_REAL=float, the binned-SAH tree (ORACLE_BVH_BINNED) and
+// static row chunks over the host threads (ORACLE_STATIC_ROWS); see oracle/build_cpu.sh.
+#ifndef ORACLE_REAL
+#define ORACLE_REAL double
+#endif
 
 namespace {
 
-constexpr double kPi = 3.14159265358979323846;
+using real = ORACLE_REAL;
+
+// min / max / clamp that compute in `real` whenever a floating-point operand is involved
+// (literals are double), and in the common type of integer operands otherwise
+template <class A, class B>
+inline auto mx(A a, B b) {
+   if constexpr (std::is_floating_point_v<A> || std::is_floating_point_v<B>)
+      return std::max<real>((real)a, (real)b);
+   else
+      return std::max<std::common_type_t<A, B>>(a, b);
+}
+template <class A, class B>
+inline auto mn(A a, B b) {
+   if constexpr (std::is_floating_point_v<A> || std::is_floating_point_v<B>)
+      return std::min<real>((real)a, (real)b);
+   else
+      return std::min<std::common_type_t<A, B>>(a, b);
+}
+template <class A, class B, class C>
+inline real cl(A v, B lo, C hi) {
+   return std::clamp<real>((real)v, (real)lo, (real)hi);
+}
+
+constexpr real kPi = (real)3.14159265358979323846;
 
 // Planted mistakes for the pinning tests (oracle_set_fault): each one is a plausible
 // error in a formula the tests must catch. 0 = none, the only value outside those tests.
@@ -34,23 +65,23 @@ enum Fault {
 };
 
 struct V3 {
-   double x, y, z;
+   real x, y, z;
 };
-inline V3 v3(double x, double y, double z) { return {x, y, z}; }
+inline V3 v3(real x, real y, real z) { return {x, y, z}; }
 inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 inline V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
-inline V3 operator*(double s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline V3 operator*(real s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
 inline V3 operator*(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
-inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline real dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 inline V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
-inline double len2(V3 a) { return dot(a, a); }
-inline double len(V3 a) { return std::sqrt(dot(a, a)); }
+inline real len2(V3 a) { return dot(a, a); }
+inline real len(V3 a) { return std::sqrt(dot(a, a)); }
 inline V3 unit(V3 a) { return (1.0 / len(a)) * a; }
-inline double maxc(V3 a) { return std::max(a.x, std::max(a.y, a.z)); }
+inline real maxc(V3 a) { return mx(a.x, mx(a.y, a.z)); }
 inline V3 fv(const float* p) { return {p[0], p[1], p[2]}; }
-inline V3 dv(const double* p) { return {p[0], p[1], p[2]}; }
-inline double axis(V3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+inline V3 dv(const double* p) { return {(real)p[0], (real)p[1], (real)p[2]}; }
+inline real axis(V3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
 
 // ------------------------------------------------------------------------------------
 // RNG
@@ -101,7 +132,7 @@ struct Prim {
 struct SdfPrim {
    int type;
    V3 c;
-   double p[4];
+   real p[4];
 };
 
 struct Node {
@@ -112,7 +143,7 @@ struct Node {
 
 struct Light {
    int64_t prim;
-   double area;
+   real area;
 };
 
 }  // namespace
@@ -121,9 +152,9 @@ struct oracle_scene {
    std::vector<Prim> prims;
    std::vector<rayon_material> mats;
    std::vector<SdfPrim> sdf;
-   double sdf_k = 0.1;
+   real sdf_k = 0.1;
    int sdf_steps = 256;
-   double sdf_eps = 1e-4;
+   real sdf_eps = 1e-4;
    std::vector<Node> nodes;
    std::vector<int64_t> order;  // leaf order -> prim index
    std::vector<Light> lights;
@@ -135,7 +166,7 @@ void prim_bounds(Prim& p) {
    switch (p.type) {
    case P_SPHERE:
    case P_DISPLACED: {
-      double r = p.b.x * (p.type == P_DISPLACED ? 1.4 : 1.0);
+      real r = p.b.x * (p.type == P_DISPLACED ? 1.4 : 1.0);
       p.lo = p.a - v3(r, r, r);
       p.hi = p.a + v3(r, r, r);
       break;
@@ -144,20 +175,21 @@ void prim_bounds(Prim& p) {
       V3 c[4] = {p.a, p.a + p.b, p.a + p.c, p.a + p.b + p.c};
       p.lo = p.hi = c[0];
       for (int i = 1; i < 4; ++i) {
-         p.lo = v3(std::min(p.lo.x, c[i].x), std::min(p.lo.y, c[i].y), std::min(p.lo.z, c[i].z));
-         p.hi = v3(std::max(p.hi.x, c[i].x), std::max(p.hi.y, c[i].y), std::max(p.hi.z, c[i].z));
+         p.lo = v3(mn(p.lo.x, c[i].x), mn(p.lo.y, c[i].y), mn(p.lo.z, c[i].z));
+         p.hi = v3(mx(p.hi.x, c[i].x), mx(p.hi.y, c[i].y), mx(p.hi.z, c[i].z));
       }
       break;
    }
    case P_TRI: {
-      p.lo = v3(std::min({p.a.x, p.b.x, p.c.x}), std::min({p.a.y, p.b.y, p.c.y}), std::min({p.a.z, p.b.z, p.c.z}));
-      p.hi = v3(std::max({p.a.x, p.b.x, p.c.x}), std::max({p.a.y, p.b.y, p.c.y}), std::max({p.a.z, p.b.z, p.c.z}));
+      p.lo = v3(mn(mn(p.a.x, p.b.x), p.c.x), mn(mn(p.a.y, p.b.y), p.c.y), mn(mn(p.a.z, p.b.z), p.c.z));
+      p.hi = v3(mx(mx(p.a.x, p.b.x), p.c.x), mx(mx(p.a.y, p.b.y), p.c.y), mx(mx(p.a.z, p.b.z), p.c.z));
       break;
    }
    default:
       break;
    }
-   const double e = 1e-9 * (1.0 + std::max(len(p.lo), len(p.hi)));
+   // padded outward by far more than the rounding of the bounds (FP32 builds round more)
+   const real e = (sizeof(real) < 8 ? 1e-6 : 1e-9) * (1.0 + mx(len(p.lo), len(p.hi)));
    p.lo = p.lo - v3(e, e, e);
    p.hi = p.hi + v3(e, e, e);
 }
@@ -171,11 +203,11 @@ int64_t build_node(oracle_scene& s, int64_t first, int64_t count) {
    V3 clo = n.lo, chi = n.hi;
    for (int64_t i = first; i < first + count; ++i) {
       const Prim& p = s.prims[s.order[i]];
-      n.lo = v3(std::min(n.lo.x, p.lo.x), std::min(n.lo.y, p.lo.y), std::min(n.lo.z, p.lo.z));
-      n.hi = v3(std::max(n.hi.x, p.hi.x), std::max(n.hi.y, p.hi.y), std::max(n.hi.z, p.hi.z));
+      n.lo = v3(mn(n.lo.x, p.lo.x), mn(n.lo.y, p.lo.y), mn(n.lo.z, p.lo.z));
+      n.hi = v3(mx(n.hi.x, p.hi.x), mx(n.hi.y, p.hi.y), mx(n.hi.z, p.hi.z));
       V3 c = 0.5 * (p.lo + p.hi);
-      clo = v3(std::min(clo.x, c.x), std::min(clo.y, c.y), std::min(clo.z, c.z));
-      chi = v3(std::max(chi.x, c.x), std::max(chi.y, c.y), std::max(chi.z, c.z));
+      clo = v3(mn(clo.x, c.x), mn(clo.y, c.y), mn(clo.z, c.z));
+      chi = v3(mx(chi.x, c.x), mx(chi.y, c.y), mx(chi.z, c.z));
    }
    int64_t idx = (int64_t)s.nodes.size();
    s.nodes.push_back(n);
@@ -187,10 +219,80 @@ int64_t build_node(oracle_scene& s, int64_t first, int64_t count) {
    V3 ext = chi - clo;
    int ax = ext.x > ext.y ? (ext.x > ext.z ? 0 : 2) : (ext.y > ext.z ? 1 : 2);
    int64_t mid = first + count / 2;
+#ifdef ORACLE_BVH_BINNED
+   // the GPU path's build rule instead: binned SAH over 16 centroid bins per axis, cost
+   // A_L n_L + A_R n_R, the cheapest split over the three axes; the median if none splits
+   {
+      constexpr int B = 16;
+      auto area = [](V3 lo, V3 hi) {
+         V3 e = v3(mx(0.0, hi.x - lo.x), mx(0.0, hi.y - lo.y), mx(0.0, hi.z - lo.z));
+         return 2 * (e.x * e.y + e.y * e.z + e.z * e.x);
+      };
+      real best = 1e30;
+      int best_ax = -1, best_b = -1;
+      for (int a = 0; a < 3; ++a) {
+         const real lo_a = axis(clo, a), ext_a = axis(chi, a) - lo_a;
+         if (!(ext_a > 0)) continue;
+         V3 blo[B], bhi[B];
+         int64_t cnt[B] = {0};
+         for (int b = 0; b < B; ++b) blo[b] = v3(1e30, 1e30, 1e30), bhi[b] = v3(-1e30, -1e30, -1e30);
+         auto bin = [&](const Prim& p) {
+            int b = (int)((axis(0.5 * (p.lo + p.hi), a) - lo_a) * (B / ext_a));
+            return b < 0 ? 0 : (b >= B ? B - 1 : b);
+         };
+         for (int64_t i = first; i < first + count; ++i) {
+            const Prim& p = s.prims[s.order[i]];
+            const int b = bin(p);
+            ++cnt[b];
+            blo[b] = v3(mn(blo[b].x, p.lo.x), mn(blo[b].y, p.lo.y), mn(blo[b].z, p.lo.z));
+            bhi[b] = v3(mx(bhi[b].x, p.hi.x), mx(bhi[b].y, p.hi.y), mx(bhi[b].z, p.hi.z));
+         }
+         real rarea[B];
+         int64_t rcnt[B];
+         V3 rl = v3(1e30, 1e30, 1e30), rh = v3(-1e30, -1e30, -1e30);
+         int64_t rc = 0;
+         for (int b = B - 1; b > 0; --b) {
+            rl = v3(mn(rl.x, blo[b].x), mn(rl.y, blo[b].y), mn(rl.z, blo[b].z));
+            rh = v3(mx(rh.x, bhi[b].x), mx(rh.y, bhi[b].y), mx(rh.z, bhi[b].z));
+            rc += cnt[b];
+            rarea[b] = area(rl, rh);
+            rcnt[b] = rc;
+         }
+         V3 ll = v3(1e30, 1e30, 1e30), lh = v3(-1e30, -1e30, -1e30);
+         int64_t lc = 0;
+         for (int b = 0; b < B - 1; ++b) {
+            ll = v3(mn(ll.x, blo[b].x), mn(ll.y, blo[b].y), mn(ll.z, blo[b].z));
+            lh = v3(mx(lh.x, bhi[b].x), mx(lh.y, bhi[b].y), mx(lh.z, bhi[b].z));
+            lc += cnt[b];
+            if (lc == 0 || rcnt[b + 1] == 0) continue;
+            const real cost = area(ll, lh) * (real)lc + rarea[b + 1] * (real)rcnt[b + 1];
+            if (cost < best) best = cost, best_ax = a, best_b = b + 1;
+         }
+      }
+      if (best_ax >= 0) {
+         const real lo_a = axis(clo, best_ax), ext_a = axis(chi, best_ax) - lo_a;
+         auto it = std::partition(s.order.begin() + first, s.order.begin() + first + count, [&](int64_t pi) {
+            const Prim& p = s.prims[pi];
+            int b = (int)((axis(0.5 * (p.lo + p.hi), best_ax) - lo_a) * (B / ext_a));
+            b = b < 0 ? 0 : (b >= B ? B - 1 : b);
+            return b < best_b;
+         });
+         const int64_t m = it - s.order.begin();
+         if (m > first && m < first + count) {
+            const int64_t l = build_node(s, first, m - first);
+            const int64_t r = build_node(s, m, first + count - m);
+            s.nodes[idx].left = l;
+            s.nodes[idx].right = r;
+            s.nodes[idx].count = 0;
+            return idx;
+         }
+      }
+   }
+#endif
    std::nth_element(s.order.begin() + first, s.order.begin() + mid, s.order.begin() + first + count,
                     [&](int64_t i, int64_t j) {
-                       double ci = axis(s.prims[i].lo + s.prims[i].hi, ax);
-                       double cj = axis(s.prims[j].lo + s.prims[j].hi, ax);
+                       real ci = axis(s.prims[i].lo + s.prims[i].hi, ax);
+                       real cj = axis(s.prims[j].lo + s.prims[j].hi, ax);
                        return ci < cj || (ci == cj && i < j);
                     });
    int64_t l = build_node(s, first, mid - first);
@@ -206,40 +308,40 @@ struct Ray {
 };
 
 struct Hit {
-   double t;
+   real t;
    V3 p, normal, outward, tangent;
    bool front;
    int64_t prim;
 };
 
 // slab test, cuda_raytracer.cuh:154-197 (inclusive bounds)
-bool hit_box(const Ray& r, V3 lo, V3 hi, double tmin, double tmax) {
-   double t0 = -1e300, t1 = 1e300;
+bool hit_box(const Ray& r, V3 lo, V3 hi, real tmin, real tmax) {
+   real t0 = -1e300, t1 = 1e300;
    for (int k = 0; k < 3; ++k) {
-      double o = axis(r.o, k), d = axis(r.d, k);
-      double inv = 1.0 / d;
-      double a = (axis(lo, k) - o) * inv, b = (axis(hi, k) - o) * inv;
+      real o = axis(r.o, k), d = axis(r.d, k);
+      real inv = 1.0 / d;
+      real a = (axis(lo, k) - o) * inv, b = (axis(hi, k) - o) * inv;
       if (std::isnan(a) || std::isnan(b)) {  // d == 0 and origin on a slab plane
          if (o < axis(lo, k) || o > axis(hi, k)) return false;
          continue;
       }
       if (a > b) std::swap(a, b);
-      t0 = std::max(t0, a);
-      t1 = std::min(t1, b);
+      t0 = mx(t0, a);
+      t1 = mn(t1, b);
    }
    return t0 <= t1 && t1 >= tmin && t0 <= tmax;
 }
 
 // hit_sphere, cuda_raytracer.cuh:203-233
-bool isect_sphere(V3 c, double rad, const Ray& r, double tmin, double tmax, Hit& h) {
+bool isect_sphere(V3 c, real rad, const Ray& r, real tmin, real tmax, Hit& h) {
    V3 oc = r.o - c;
-   double a = dot(r.d, r.d);
-   double hb = dot(oc, r.d);
-   double cc = dot(oc, oc) - rad * rad;
-   double disc = hb * hb - a * cc;
+   real a = dot(r.d, r.d);
+   real hb = dot(oc, r.d);
+   real cc = dot(oc, oc) - rad * rad;
+   real disc = hb * hb - a * cc;
    if (disc < 0) return false;
-   double sq = std::sqrt(disc);
-   double root = (-hb - sq) / a;
+   real sq = std::sqrt(disc);
+   real root = (-hb - sq) / a;
    if (root < tmin || tmax < root) {
       root = (-hb + sq) / a;
       if (root < tmin || tmax < root) return false;
@@ -256,15 +358,15 @@ bool isect_sphere(V3 c, double rad, const Ray& r, double tmin, double tmax, Hit&
 }
 
 // hit_rectangle, cuda_raytracer.cuh:235-271
-bool isect_rect(V3 q, V3 u, V3 v, const Ray& r, double tmin, double tmax, Hit& h) {
+bool isect_rect(V3 q, V3 u, V3 v, const Ray& r, real tmin, real tmax, Hit& h) {
    V3 n = unit(cross(u, v));
-   double denom = dot(n, r.d);
+   real denom = dot(n, r.d);
    if (std::fabs(denom) < 1e-8) return false;
-   double t = dot(n, q - r.o) / denom;
+   real t = dot(n, q - r.o) / denom;
    if (t < tmin || t > tmax) return false;
    V3 p = r.o + t * r.d;
    V3 rel = p - q;
-   double al = dot(rel, u) / dot(u, u), be = dot(rel, v) / dot(v, v);
+   real al = dot(rel, u) / dot(u, u), be = dot(rel, v) / dot(v, v);
    if (al < 0 || al > 1 || be < 0 || be > 1) return false;
    h.t = t;
    h.p = p;
@@ -276,19 +378,19 @@ bool isect_rect(V3 q, V3 u, V3 v, const Ray& r, double tmin, double tmax, Hit& h
 }
 
 // hit_triangle (Moller-Trumbore), cuda_raytracer.cuh:277-347
-bool isect_tri(const Prim& pr, const Ray& r, double tmin, double tmax, Hit& h) {
+bool isect_tri(const Prim& pr, const Ray& r, real tmin, real tmax, Hit& h) {
    V3 e1 = pr.b - pr.a, e2 = pr.c - pr.a;
    V3 hh = cross(r.d, e2);
-   double a = dot(e1, hh);
+   real a = dot(e1, hh);
    if (std::fabs(a) < 1e-8) return false;
-   double f = 1.0 / a;
+   real f = 1.0 / a;
    V3 s = r.o - pr.a;
-   double u = f * dot(s, hh);
+   real u = f * dot(s, hh);
    if (u < 0 || u > 1) return false;
    V3 q = cross(s, e1);
-   double v = f * dot(r.d, q);
+   real v = f * dot(r.d, q);
    if (v < 0 || u + v > 1) return false;
-   double t = f * dot(e2, q);
+   real t = f * dot(e2, q);
    if (t < tmin || t > tmax) return false;
    h.t = t;
    h.p = r.o + t * r.d;
@@ -297,7 +399,7 @@ bool isect_tri(const Prim& pr, const Ray& r, double tmin, double tmax, Hit& h) {
    h.outward = geo;
    V3 sn = geo;
    if (pr.has_n) {
-      double w = 1 - u - v;
+      real w = 1 - u - v;
       sn = unit(w * pr.n0 + u * pr.n1 + v * pr.n2);
       if (dot(sn, geo) < 0) sn = -sn;
    }
@@ -309,54 +411,54 @@ bool isect_tri(const Prim& pr, const Ray& r, double tmin, double tmax, Hit& h) {
 
 // fibonacci_point, shaders/shader_golf.cu:4-13
 V3 fib_point(int i, int n) {
-   const double ga = 2.39996323;
-   double k = i + 0.5;
-   double v = k / n;
-   double phi = std::acos(1.0 - 2.0 * v);
-   double theta = ga * k;
-   double s = std::sin(phi);
+   const real ga = 2.39996323;
+   real k = i + 0.5;
+   real v = k / n;
+   real phi = std::acos(1.0 - 2.0 * v);
+   real theta = ga * k;
+   real s = std::sin(phi);
    return v3(std::cos(theta) * s, std::sin(theta) * s, std::cos(phi));
 }
 
 // hexagonalDimplePattern / distanceToNearestDimple, shader_golf.cu:15-43
-double dimple(V3 p) {
+real dimple(V3 p) {
    V3 q = unit(p);
-   double best = -1;
-   for (int i = 0; i < 150; ++i) best = std::max(best, dot(q, fib_point(i, 150)));
-   best = std::clamp(best, -1.0, 1.0);
-   double ang = std::acos(best);
+   real best = -1;
+   for (int i = 0; i < 150; ++i) best = mx(best, dot(q, fib_point(i, 150)));
+   best = cl(best, -1.0, 1.0);
+   real ang = std::acos(best);
    if (ang < 0.24) return -0.35 * std::cos(ang / 0.24 * kPi * 0.5);
    return 0.0;
 }
 
-double smoothstep(double e0, double e1, double x) {
-   double t = std::clamp((x - e0) / (e1 - e0), 0.0, 1.0);
+real smoothstep(real e0, real e1, real x) {
+   real t = cl((x - e0) / (e1 - e0), 0.0, 1.0);
    return t * t * (3 - 2 * t);
 }
 
 // hit_golf_ball_sphere, shader_golf.cu:53-140
-bool isect_golf(V3 c, double rad, const Ray& r, double tmin, double tmax, Hit& h) {
+bool isect_golf(V3 c, real rad, const Ray& r, real tmin, real tmax, Hit& h) {
    if (!isect_sphere(c, rad, r, tmin, tmax, h)) return false;
    V3 sp = h.p;
-   double disp = dimple(unit(sp - c));
+   real disp = dimple(unit(sp - c));
    V3 bo = unit(sp - c);
-   double dn = std::min(1.0, std::max(0.0, -disp / 0.35));
-   double push = rad * 0.35 * (1.0 - dn);
+   real dn = mn(1.0, mx(0.0, -disp / 0.35));
+   real push = rad * 0.35 * (1.0 - dn);
    h.p = sp + push * bo;
    V3 bn = unit(h.p - c);
    if (disp < -0.001) {
       V3 helper = std::fabs(bn.x) > 0.8 ? v3(0, 1, 0) : v3(1, 0, 0);
       V3 t1 = unit(cross(helper, bn));
       V3 t2 = cross(bn, t1);
-      const double hs = 0.015;
-      double d0 = dimple(bn);
-      double d1 = dimple(unit(bn + hs * t1));
-      double d2 = dimple(unit(bn + hs * t2));
+      const real hs = 0.015;
+      real d0 = dimple(bn);
+      real d1 = dimple(unit(bn + hs * t1));
+      real d2 = dimple(unit(bn + hs * t2));
       V3 g = ((d1 - d0) / hs) * t1 + ((d2 - d0) / hs) * t2;
       V3 dnv = -0.2 * g;
-      double ndv = std::max(0.0, dot(bn, unit(-r.d)));
+      real ndv = mx(0.0, dot(bn, unit(-r.d)));
       dnv = smoothstep(0.1, 0.4, ndv) * dnv;
-      double l = len(dnv);
+      real l = len(dnv);
       if (l > 0.4 && l > 1e-6) dnv = (0.4 / l) * dnv;
       V3 pert = unit(bn + dnv);
       if (dot(pert, bn) < 0) pert = -pert;
@@ -365,7 +467,7 @@ bool isect_golf(V3 c, double rad, const Ray& r, double tmin, double tmax, Hit& h
    } else {
       h.normal = bn;
    }
-   double ndotv = dot(r.d, h.normal);
+   real ndotv = dot(r.d, h.normal);
    if (std::isnan(ndotv)) {
       h.normal = bn;
       ndotv = dot(r.d, h.normal);
@@ -377,57 +479,57 @@ bool isect_golf(V3 c, double rad, const Ray& r, double tmin, double tmax, Hit& h
 }
 
 // ---- SDF field (spec; primitive formulas of cpu_shapes/sdf.hpp:105-143, smooth union :410-414)
-double sdf_prim(const SdfPrim& s, V3 p) {
+real sdf_prim(const SdfPrim& s, V3 p) {
    V3 o = p - s.c;
    switch (s.type) {
    case RAYON_SDF_SPHERE:
       return len(o) - s.p[0];
    case RAYON_SDF_ROUNDED_BOX: {
-      double rr = s.p[3];
+      real rr = s.p[3];
       V3 q = v3(std::fabs(o.x) - (s.p[0] - rr), std::fabs(o.y) - (s.p[1] - rr), std::fabs(o.z) - (s.p[2] - rr));
-      V3 m = v3(std::max(q.x, 0.0), std::max(q.y, 0.0), std::max(q.z, 0.0));
-      return len(m) + std::min(std::max(q.x, std::max(q.y, q.z)), 0.0) - rr;
+      V3 m = v3(mx(q.x, 0.0), mx(q.y, 0.0), mx(q.z, 0.0));
+      return len(m) + mn(mx(q.x, mx(q.y, q.z)), 0.0) - rr;
    }
    default: {
-      double qx = std::sqrt(o.x * o.x + o.z * o.z) - s.p[0];
+      real qx = std::sqrt(o.x * o.x + o.z * o.z) - s.p[0];
       return std::sqrt(qx * qx + o.y * o.y) - s.p[1];
    }
    }
 }
 
-double smooth_union(double d1, double d2, double k) {
-   double h = std::clamp(0.5 + 0.5 * (d2 - d1) / k, 0.0, 1.0);
+real smooth_union(real d1, real d2, real k) {
+   real h = cl(0.5 + 0.5 * (d2 - d1) / k, 0.0, 1.0);
    return d2 * (1.0 - h) + d1 * h - k * h * (1.0 - h);
 }
 
-double sdf_field(const oracle_scene& s, V3 p) {
-   double d = sdf_prim(s.sdf[0], p);
+real sdf_field(const oracle_scene& s, V3 p) {
+   real d = sdf_prim(s.sdf[0], p);
    for (size_t i = 1; i < s.sdf.size(); ++i) d = smooth_union(d, sdf_prim(s.sdf[i], p), s.sdf_k);
    return d;
 }
 
 // Sphere tracing along the unit direction (rayMarch, sdf.hpp:518-563) from the entry of
 // the field's bounding box; normal by central differences (estimateNormal, :499-507).
-bool isect_sdf(const oracle_scene& s, const Prim& pr, const Ray& r, double tmin, double tmax, Hit& h) {
-   double dl = len(r.d);
+bool isect_sdf(const oracle_scene& s, const Prim& pr, const Ray& r, real tmin, real tmax, Hit& h) {
+   real dl = len(r.d);
    V3 dn = (1.0 / dl) * r.d;
-   double t0 = tmin * dl, t1 = tmax * dl;
+   real t0 = tmin * dl, t1 = tmax * dl;
    for (int k = 0; k < 3; ++k) {
-      double o = axis(r.o, k), d = axis(dn, k);
-      double inv = 1.0 / d;
-      double a = (axis(pr.lo, k) - o) * inv, b = (axis(pr.hi, k) - o) * inv;
+      real o = axis(r.o, k), d = axis(dn, k);
+      real inv = 1.0 / d;
+      real a = (axis(pr.lo, k) - o) * inv, b = (axis(pr.hi, k) - o) * inv;
       if (a > b) std::swap(a, b);
       if (std::isnan(a)) continue;
-      t0 = std::max(t0, a);
-      t1 = std::min(t1, b);
+      t0 = mx(t0, a);
+      t1 = mn(t1, b);
    }
    if (t0 > t1) return false;
-   double t = t0;
+   real t = t0;
    for (int step = 0; step < s.sdf_steps; ++step) {
       V3 p = r.o + t * dn;
-      double dist = sdf_field(s, p);
+      real dist = sdf_field(s, p);
       if (dist < s.sdf_eps) {
-         const double e = s.sdf_eps;
+         const real e = s.sdf_eps;
          V3 g = v3(sdf_field(s, p + v3(e, 0, 0)) - sdf_field(s, p - v3(e, 0, 0)),
                    sdf_field(s, p + v3(0, e, 0)) - sdf_field(s, p - v3(0, e, 0)),
                    sdf_field(s, p + v3(0, 0, e)) - sdf_field(s, p - v3(0, 0, e)));
@@ -448,7 +550,7 @@ bool isect_sdf(const oracle_scene& s, const Prim& pr, const Ray& r, double tmin,
    return false;
 }
 
-bool isect_prim(const oracle_scene& s, const Prim& p, const Ray& r, double tmin, double tmax, Hit& h) {
+bool isect_prim(const oracle_scene& s, const Prim& p, const Ray& r, real tmin, real tmax, Hit& h) {
    switch (p.type) {
    case P_SPHERE:
       return isect_sphere(p.a, p.b.x, r, tmin, tmax, h);
@@ -467,7 +569,7 @@ bool isect_prim(const oracle_scene& s, const Prim& p, const Ray& r, double tmin,
 
 // closest hit; equal t goes to the larger primitive index (linear-scan order of
 // cuda_raytracer.cuh:568-585, inclusive bound :215-220).
-bool closest(const oracle_scene& s, const Ray& r, double tmin, double tmax, bool brute, Hit& best) {
+bool closest(const oracle_scene& s, const Ray& r, real tmin, real tmax, bool brute, Hit& best) {
    bool any = false;
    best.t = tmax;
    best.prim = -1;
@@ -513,7 +615,7 @@ bool nee_light(const oracle_scene& s, const Prim& p) {
           (p.type == P_RECT || p.type == P_TRI || p.type == P_SPHERE);
 }
 
-bool any_hit(const oracle_scene& s, const Ray& r, double tmin, double tmax) {
+bool any_hit(const oracle_scene& s, const Ray& r, real tmin, real tmax) {
    Hit h;
    auto test = [&](int64_t pi) {
       const Prim& p = s.prims[pi];
@@ -549,15 +651,15 @@ bool any_hit(const oracle_scene& s, const Ray& r, double tmin, double tmax) {
 // ------------------------------------------------------------------------------------
 inline V3 reflect(V3 v, V3 n) { return v - (2 * dot(v, n)) * n; }  // cuda_raytracer.cuh:84
 
-V3 refract(V3 uv, V3 n, double eta) {  // cuda_raytracer.cuh:86-92
-   double c = std::min(dot(-uv, n), 1.0);
+V3 refract(V3 uv, V3 n, real eta) {  // cuda_raytracer.cuh:86-92
+   real c = mn(dot(-uv, n), 1.0);
    V3 perp = eta * (uv + c * n);
    V3 par = (-std::sqrt(std::fabs(1.0 - len2(perp)))) * n;
    return perp + par;
 }
 
-double schlick(double c, double ri) {  // cuda_raytracer.cuh:94-99
-   double r0 = (1 - ri) / (1 + ri);
+real schlick(real c, real ri) {  // cuda_raytracer.cuh:94-99
+   real r0 = (1 - ri) / (1 + ri);
    r0 *= r0;
    return r0 + (1 - r0) * std::pow(1 - c, 5);
 }
@@ -575,7 +677,7 @@ struct Draws {  // random numbers in consumption order
    const float* fixed;
    int nfixed;
    int used = 0;
-   double next() {
+   real next() {
       ++used;
       if (rng) return rng->next();
       return used - 1 < nfixed ? fixed[used - 1] : 0.5;
@@ -583,74 +685,74 @@ struct Draws {  // random numbers in consumption order
 };
 
 V3 on_sphere(Draws& R) {  // randOnUnitSphere, cuda_utils.cuh:66-77
-   double th = 2 * kPi * R.next();
-   double ph = std::acos(1 - 2 * R.next());
+   real th = 2 * kPi * R.next();
+   real ph = std::acos(1 - 2 * R.next());
    return v3(std::sin(ph) * std::cos(th), std::sin(ph) * std::sin(th), std::cos(ph));
 }
 
 V3 cosine_dir(V3 n, Draws& R) {  // Lambert lobe, cuda_raytracer.cuh:621-638
    V3 w = unit(n), u, v;
    onb(w, u, v);
-   double u1 = R.next(), u2 = R.next();
-   double r = std::sqrt(u1), th = 2 * kPi * u2;
-   return (r * std::cos(th)) * u + (r * std::sin(th)) * v + std::sqrt(std::max(0.0, 1 - u1)) * w;
+   real u1 = R.next(), u2 = R.next();
+   real r = std::sqrt(u1), th = 2 * kPi * u2;
+   return (r * std::cos(th)) * u + (r * std::sin(th)) * v + std::sqrt(mx(0.0, 1 - u1)) * w;
 }
 
 // microfacet_ggx.cuh:21-39
-double lambda_ggx(V3 w, double ax, double ay) {
-   double wz2 = w.z * w.z;
+real lambda_ggx(V3 w, real ax, real ay) {
+   real wz2 = w.z * w.z;
    if (wz2 < 1e-16) return 0;
-   double a2 = (ax * w.x) * (ax * w.x) + (ay * w.y) * (ay * w.y);
+   real a2 = (ax * w.x) * (ax * w.x) + (ay * w.y) * (ay * w.y);
    return (std::sqrt(1 + a2 / wz2) - 1) * (g_fault == FAULT_LAMBDA_HALF ? 1.0 : 0.5);
 }
-double g1_ggx(V3 w, double ax, double ay) { return 1 / (1 + lambda_ggx(w, ax, ay)); }
-double g2_ggx(V3 wo, V3 wi, double ax, double ay) {
+real g1_ggx(V3 w, real ax, real ay) { return 1 / (1 + lambda_ggx(w, ax, ay)); }
+real g2_ggx(V3 wo, V3 wi, real ax, real ay) {
    return 1 / (1 + lambda_ggx(wo, ax, ay) + lambda_ggx(wi, ax, ay));
 }
 
 // VNDF sampling, microfacet_ggx.cuh:45-76
-V3 sample_vndf(V3 wo, double ax, double ay, double u1, double u2) {
+V3 sample_vndf(V3 wo, real ax, real ay, real u1, real u2) {
    if (g_fault == FAULT_VNDF_SWAP_ALPHA) std::swap(ax, ay);
    V3 wh = unit(v3(ax * wo.x, ay * wo.y, wo.z));
    if (wh.z < 0) wh = -wh;
    V3 t1 = wh.z < 0.99999 ? unit(cross(v3(0, 0, 1), wh)) : v3(1, 0, 0);
    V3 t2 = cross(wh, t1);
-   double r = std::sqrt(u1), phi = 2 * kPi * u2;
-   double px = r * std::cos(phi), py = r * std::sin(phi);
-   double hh = std::sqrt(std::max(0.0, 1 - px * px));
-   double sm = (1 + wh.z) * 0.5;
+   real r = std::sqrt(u1), phi = 2 * kPi * u2;
+   real px = r * std::cos(phi), py = r * std::sin(phi);
+   real hh = std::sqrt(mx(0.0, 1 - px * px));
+   real sm = (1 + wh.z) * 0.5;
    if (g_fault != FAULT_VNDF_NO_WARP) py = (1 - sm) * hh + sm * py;
-   double pz = std::sqrt(std::max(0.0, 1 - px * px - py * py));
+   real pz = std::sqrt(mx(0.0, 1 - px * px - py * py));
    V3 nh = px * t1 + py * t2 + pz * wh;
-   return unit(v3(ax * nh.x, ay * nh.y, std::max(1e-6, nh.z)));
+   return unit(v3(ax * nh.x, ay * nh.y, mx(1e-6, nh.z)));
 }
 
 // FrComplex, microfacet_ggx.cuh:83-111
-double fr_complex1(double cos2, double sin2, double ci, double eta, double k) {
-   double eta2 = eta * eta, k2 = k * k;
-   double t0 = eta2 - k2 - sin2;
-   double ab = std::sqrt(std::max(t0 * t0 + 4 * eta2 * k2, 0.0));
-   double a = std::sqrt(std::max((ab + t0) * 0.5, 0.0));
-   double rs = (ab + cos2 - 2 * a * ci) / std::max(ab + cos2 + 2 * a * ci, 1e-10);
-   const double sg = g_fault == FAULT_CONDUCTOR_RP_SIGN ? -1.0 : 1.0;
-   double rp = rs * (ab * cos2 + sin2 * sin2 - sg * 2 * a * ci * sin2) /
-               std::max(ab * cos2 + sin2 * sin2 + 2 * a * ci * sin2, 1e-10);
+real fr_complex1(real cos2, real sin2, real ci, real eta, real k) {
+   real eta2 = eta * eta, k2 = k * k;
+   real t0 = eta2 - k2 - sin2;
+   real ab = std::sqrt(mx(t0 * t0 + 4 * eta2 * k2, 0.0));
+   real a = std::sqrt(mx((ab + t0) * 0.5, 0.0));
+   real rs = (ab + cos2 - 2 * a * ci) / mx(ab + cos2 + 2 * a * ci, 1e-10);
+   const real sg = g_fault == FAULT_CONDUCTOR_RP_SIGN ? -1.0 : 1.0;
+   real rp = rs * (ab * cos2 + sin2 * sin2 - sg * 2 * a * ci * sin2) /
+               mx(ab * cos2 + sin2 * sin2 + 2 * a * ci * sin2, 1e-10);
    return (rs + rp) * 0.5;
 }
-V3 fr_complex(double ci, V3 eta, V3 k) {
-   ci = std::clamp(ci, 0.0, 1.0);
-   double c2 = ci * ci, s2 = 1 - c2;
+V3 fr_complex(real ci, V3 eta, V3 k) {
+   ci = cl(ci, 0.0, 1.0);
+   real c2 = ci * ci, s2 = 1 - c2;
    return v3(fr_complex1(c2, s2, ci, eta.x, k.x), fr_complex1(c2, s2, ci, eta.y, k.y),
              fr_complex1(c2, s2, ci, eta.z, k.z));
 }
 
 // spec: exact unpolarised dielectric Fresnel; eta = n_incident / n_transmitted.
-double fresnel_dielectric(double ci, double eta) {
-   double s2t = eta * eta * std::max(0.0, 1 - ci * ci);
+real fresnel_dielectric(real ci, real eta) {
+   real s2t = eta * eta * mx(0.0, 1 - ci * ci);
    if (s2t >= 1) return 1;
-   double ct = std::sqrt(1 - s2t);
-   double rs = (eta * ci - ct) / (eta * ci + ct);
-   double rp = (ci - eta * ct) / (ci + eta * ct);
+   real ct = std::sqrt(1 - s2t);
+   real rs = (eta * ci - ct) / (eta * ci + ct);
+   real rp = (ci - eta * ct) / (ci + eta * ct);
    if (g_fault == FAULT_DIELECTRIC_RP) rp = rs;
    return 0.5 * (rs * rs + rp * rp);
 }
@@ -669,7 +771,7 @@ struct ScatterOut {
 
 struct Knobs {
    uint32_t flags;
-   double fuzz, glass_ior, light;
+   real fuzz, glass_ior, light;
 };
 
 // scatter_material, cuda_raytracer.cuh:613-850, after apply_material's type mapping
@@ -698,8 +800,8 @@ void scatter(const rayon_material& m, V3 color, const Knobs& K, const ScatterIn&
    }
    case RAYON_MAT_METAL:
    case RAYON_MAT_ROUGH_MIRROR: {
-      double rough = m.roughness;
-      double fz = 1.0;
+      real rough = m.roughness;
+      real fz = 1.0;
       if (quirks) {
          rough = rough > 0 ? rough : 0.3;  // cuda_raytracer.cuh:432-437
          fz = K.fuzz;                      // g_metal_fuzziness
@@ -714,10 +816,10 @@ void scatter(const rayon_material& m, V3 color, const Knobs& K, const ScatterIn&
    case RAYON_MAT_GLASS:
    case RAYON_MAT_DIELECTRIC: {
       V3 ud = unit(in.dir);
-      double ri = quirks ? K.glass_ior : m.refractive_index;
-      double ratio = in.front ? 1.0 / ri : ri;
-      double ct = std::min(dot(-ud, n), 1.0);
-      double st = std::sqrt(1 - ct * ct);
+      real ri = quirks ? K.glass_ior : m.refractive_index;
+      real ratio = in.front ? 1.0 / ri : ri;
+      real ct = mn(dot(-ud, n), 1.0);
+      real st = std::sqrt(1 - ct * ct);
       bool tir = ratio * st > 1;
       V3 d;
       if (tir || schlick(ct, ratio) > R.next())
@@ -744,20 +846,20 @@ void scatter(const rayon_material& m, V3 color, const Knobs& K, const ScatterIn&
    case RAYON_MAT_ANISOTROPIC_METAL:
    case RAYON_MAT_GGX_CONDUCTOR: {
       const bool aniso = m.type == RAYON_MAT_ANISOTROPIC_METAL;
-      double ax, ay;
+      real ax, ay;
       if (aniso) {
-         double aspect = std::sqrt(std::max(1e-4, 1 - 0.9 * (double)m.anisotropy));
-         double r2 = (double)m.roughness * m.roughness;
-         ax = std::max(1e-4, r2 / aspect);
-         ay = std::max(1e-4, r2 * aspect);
+         real aspect = std::sqrt(mx(1e-4, 1 - 0.9 * (real)m.anisotropy));
+         real r2 = (real)m.roughness * m.roughness;
+         ax = mx(1e-4, r2 / aspect);
+         ay = mx(1e-4, r2 * aspect);
       } else {
-         ax = ay = std::max(1e-4, (double)m.roughness * m.roughness);
+         ax = ay = mx(1e-4, (real)m.roughness * m.roughness);
       }
       V3 eta = fv(m.eta), kk = fv(m.k);
-      if (std::max(ax, ay) < 1e-3) {
+      if (mx(ax, ay) < 1e-3) {
          V3 rf = reflect(unit(in.dir), n);
          o.ray = {in.p + 1e-4 * n, rf};
-         double ci = std::max(dot(-unit(in.dir), n), 0.0);
+         real ci = mx(dot(-unit(in.dir), n), 0.0);
          o.atten = color * fr_complex(ci, eta, kk);
          o.scattered = dot(rf, n) > 0;
          return;
@@ -775,7 +877,7 @@ void scatter(const rayon_material& m, V3 color, const Knobs& K, const ScatterIn&
          o.scattered = false;
          return;
       }
-      double u1 = R.next(), u2 = R.next();
+      real u1 = R.next(), u2 = R.next();
       V3 wm = sample_vndf(wol, ax, ay, u1, u2);
       V3 wil = reflect(-wol, wm);
       if (wil.z <= 0) {
@@ -783,17 +885,17 @@ void scatter(const rayon_material& m, V3 color, const Knobs& K, const ScatterIn&
          return;
       }
       V3 wi = wil.x * T + wil.y * B + wil.z * N;
-      V3 F = fr_complex(std::max(dot(wol, wm), 0.0), eta, kk);
-      double G = g2_ggx(wol, wil, ax, ay), G1 = g1_ggx(wol, ax, ay);
+      V3 F = fr_complex(mx(dot(wol, wm), 0.0), eta, kk);
+      real G = g2_ggx(wol, wil, ax, ay), G1 = g1_ggx(wol, ax, ay);
       o.ray = {in.p + 1e-4 * n, wi};
-      o.atten = color * ((G / std::max(G1, 1e-6)) * F);
+      o.atten = color * ((G / mx(G1, 1e-6)) * F);
       o.scattered = true;
       return;
    }
    case RAYON_MAT_GGX_DIELECTRIC: {
-      double a = std::max(1e-4, (double)m.roughness * m.roughness);
-      double ri = m.refractive_index;
-      double eta = in.front ? 1.0 / ri : ri;
+      real a = mx(1e-4, (real)m.roughness * m.roughness);
+      real ri = m.refractive_index;
+      real eta = in.front ? 1.0 / ri : ri;
       V3 N = unit(n), T, B;
       onb(N, T, B);
       V3 wo = unit(-in.dir);
@@ -802,10 +904,10 @@ void scatter(const rayon_material& m, V3 color, const Knobs& K, const ScatterIn&
          o.scattered = false;
          return;
       }
-      double u1 = R.next(), u2 = R.next();
+      real u1 = R.next(), u2 = R.next();
       V3 wm = sample_vndf(wol, a, a, u1, u2);
-      double com = dot(wol, wm);
-      double F = fresnel_dielectric(com, eta);
+      real com = dot(wol, wm);
+      real F = fresnel_dielectric(com, eta);
       V3 wil;
       bool refl = R.next() < F;
       if (refl) {
@@ -822,54 +924,54 @@ void scatter(const rayon_material& m, V3 color, const Knobs& K, const ScatterIn&
          }
       }
       V3 wi = wil.x * T + wil.y * B + wil.z * N;
-      double G = g2_ggx(wol, wil, a, a), G1 = g1_ggx(wol, a, a);
+      real G = g2_ggx(wol, wil, a, a), G1 = g1_ggx(wol, a, a);
       o.ray = {refl ? in.p + 1e-4 * n : in.p - 1e-4 * n, wi};
-      o.atten = (G / std::max(G1, 1e-6)) * color;
+      o.atten = (G / mx(G1, 1e-6)) * color;
       o.scattered = true;
       return;
    }
    case RAYON_MAT_THIN_FILM: {  // cuda_raytracer.cuh:753-809
       V3 ud = unit(in.dir);
-      double ci = std::max(std::fabs(dot(-ud, n)), 0.001);
-      double n0 = m.refractive_index, n1 = m.film_ior, d = m.film_thickness;
-      double si = std::sqrt(std::max(0.0, 1 - ci * ci));
-      double st = (n0 / n1) * si;
-      double ct = st >= 1 ? 0 : std::sqrt(1 - st * st);
-      double r01 = (n0 - n1) / (n0 + n1);
+      real ci = mx(std::fabs(dot(-ud, n)), 0.001);
+      real n0 = m.refractive_index, n1 = m.film_ior, d = m.film_thickness;
+      real si = std::sqrt(mx(0.0, 1 - ci * ci));
+      real st = (n0 / n1) * si;
+      real ct = st >= 1 ? 0 : std::sqrt(1 - st * st);
+      real r01 = (n0 - n1) / (n0 + n1);
       r01 *= r01;
-      double x01 = 1 - ci;
-      double R01 = r01 + (1 - r01) * x01 * x01 * x01 * x01 * x01;
-      double r12 = (n1 - n0) / (n1 + n0);
+      real x01 = 1 - ci;
+      real R01 = r01 + (1 - r01) * x01 * x01 * x01 * x01 * x01;
+      real r12 = (n1 - n0) / (n1 + n0);
       r12 *= r12;
-      double x12 = 1 - ct;
-      double R12 = r12 + (1 - r12) * x12 * x12 * x12 * x12 * x12;
-      double sq = std::sqrt(R01 * R12);
-      auto airy = [&](double lam) {
-         double delta = 4 * kPi * n1 * d * ct / lam;
-         double c = std::cos(delta);
-         return (R01 + R12 + 2 * sq * c) / std::max(1 + R01 * R12 + 2 * sq * c, 1e-8);
+      real x12 = 1 - ct;
+      real R12 = r12 + (1 - r12) * x12 * x12 * x12 * x12 * x12;
+      real sq = std::sqrt(R01 * R12);
+      auto airy = [&](real lam) {
+         real delta = 4 * kPi * n1 * d * ct / lam;
+         real c = std::cos(delta);
+         return (R01 + R12 + 2 * sq * c) / mx(1 + R01 * R12 + 2 * sq * c, 1e-8);
       };
-      double Rr = airy(650), Rg = airy(550), Rb = airy(450);
-      double avg = (Rr + Rg + Rb) / 3;
+      real Rr = airy(650), Rg = airy(550), Rb = airy(450);
+      real avg = (Rr + Rg + Rb) / 3;
       if (R.next() < avg) {
          V3 rf = reflect(ud, n);
          o.ray = {in.p + 1e-4 * n, rf};
-         o.atten = (1 / std::max(avg, 0.001)) * v3(Rr, Rg, Rb);
+         o.atten = (1 / mx(avg, 0.001)) * v3(Rr, Rg, Rb);
          o.scattered = dot(rf, n) > 0;
       } else {
          o.ray = {in.p - 1e-4 * n, ud};
-         o.atten = (1 / std::max(1 - avg, 0.001)) * v3(1 - Rr, 1 - Rg, 1 - Rb);
+         o.atten = (1 / mx(1 - avg, 0.001)) * v3(1 - Rr, 1 - Rg, 1 - Rb);
          o.scattered = true;
       }
       return;
    }
    case RAYON_MAT_CLEAR_COAT: {  // cuda_raytracer.cuh:810-846 (+ ior default of :478)
       V3 ud = unit(in.dir);
-      double ct = std::min(dot(-ud, n), 1.0);
-      double ior = m.refractive_index > 1 ? m.refractive_index : 1.5;
-      double F = schlick(ct, ior);
+      real ct = mn(dot(-ud, n), 1.0);
+      real ior = m.refractive_index > 1 ? m.refractive_index : 1.5;
+      real F = schlick(ct, ior);
       if (R.next() < F) {
-         V3 pn = m.roughness > 1e-3 ? unit(n + (double)m.roughness * on_sphere(R)) : n;
+         V3 pn = m.roughness > 1e-3 ? unit(n + (real)m.roughness * on_sphere(R)) : n;
          V3 rf = reflect(ud, pn);
          o.ray = {in.p + 1e-4 * n, rf};
          o.atten = v3(1, 1, 1);
@@ -894,15 +996,15 @@ V3 surface_color(const rayon_material& m, V3 p, V3 center) {
    if (m.pattern != RAYON_PATTERN_FIBONACCI_DOTS) return base;
    V3 dir = unit(p - center);
    int N = (int)m.pattern_param1;
-   double best = -1;
-   for (int i = 0; i < N; ++i) best = std::max(best, dot(dir, fib_point(i, N)));
-   best = std::clamp(best, -1.0, 1.0);
-   double mask = std::acos(best) < m.pattern_param2 ? 0.0 : 1.0;
+   real best = -1;
+   for (int i = 0; i < N; ++i) best = mx(best, dot(dir, fib_point(i, N)));
+   best = cl(best, -1.0, 1.0);
+   real mask = std::acos(best) < m.pattern_param2 ? 0.0 : 1.0;
    return mask * base + (1 - mask) * fv(m.pattern_color);
 }
 
 // spec: uniform point on an emitter; returns point, geometric normal, area.
-void sample_light(const Prim& p, double a, double b, V3& q, V3& nl, double& area) {
+void sample_light(const Prim& p, real a, real b, V3& q, V3& nl, real& area) {
    switch (p.type) {
    case P_RECT:
       q = p.a + a * p.b + b * p.c;
@@ -911,7 +1013,7 @@ void sample_light(const Prim& p, double a, double b, V3& q, V3& nl, double& area
       nl = unit(nl);
       return;
    case P_TRI: {
-      double su = std::sqrt(a);
+      real su = std::sqrt(a);
       q = (1 - su) * p.a + (su * (1 - b)) * p.b + (su * b) * p.c;
       nl = cross(p.b - p.a, p.c - p.a);
       area = 0.5 * len(nl);
@@ -919,7 +1021,7 @@ void sample_light(const Prim& p, double a, double b, V3& q, V3& nl, double& area
       return;
    }
    default: {  // sphere: uniform on the surface
-      double z = 1 - 2 * a, r = std::sqrt(std::max(0.0, 1 - z * z)), ph = 2 * kPi * b;
+      real z = 1 - 2 * a, r = std::sqrt(mx(0.0, 1 - z * z)), ph = 2 * kPi * b;
       nl = v3(r * std::cos(ph), r * std::sin(ph), z);
       q = p.a + p.b.x * nl;
       area = 4 * kPi * p.b.x * p.b.x;
@@ -934,7 +1036,7 @@ struct Cam {
 
 // random_in_unit_disk (cuda_raytracer.cuh:114-122): rejection from the square [-1,1)^2,
 // two draws per attempt.
-void unit_disk(Draws& R, double& dx, double& dy) {
+void unit_disk(Draws& R, real& dx, real& dy) {
    do {
       dx = 2 * R.next() - 1;
       dy = 2 * R.next() - 1;
@@ -948,16 +1050,16 @@ V3 trace_sample(const oracle_scene& s, const rayon_job_params& P, const Cam& cam
    Draws R{&rng, nullptr, 0};
    const Knobs K{P.flags, P.metal_fuzziness, P.glass_ior, P.light_intensity};
    const bool nee = (P.flags & RAYON_FLAG_NEE) && !s.lights.empty();
-   double ou = R.next() - 0.5;
-   double ov = R.next() - 0.5;
-   V3 pc = cam.p00 + ((double)x + ou) * cam.du + ((double)y + ov) * cam.dv;
+   real ou = R.next() - 0.5;
+   real ov = R.next() - 0.5;
+   V3 pc = cam.p00 + ((real)x + ou) * cam.du + ((real)y + ov) * cam.dv;
    Ray ray{cam.C, pc - cam.C};
    if ((P.flags & RAYON_FLAG_DOF) && P.dof_aperture > 0) {  // render_acc_kernel.cu:209-221
       V3 nd = unit(ray.d);
-      V3 focus = cam.C + (double)P.dof_focus_distance * nd;
-      double dx, dy;
+      V3 focus = cam.C + (real)P.dof_focus_distance * nd;
+      real dx, dy;
       unit_disk(R, dx, dy);
-      V3 off = (double)P.dof_aperture * (dx * cam.u + dy * cam.v);
+      V3 off = (real)P.dof_aperture * (dx * cam.u + dy * cam.v);
       ray.o = cam.C + off;
       ray.d = focus - ray.o;
    }
@@ -968,9 +1070,9 @@ V3 trace_sample(const oracle_scene& s, const rayon_job_params& P, const Cam& cam
       Hit h;
       if (!closest(s, ray, 0.001, 3.402823466e38, false, h)) {
          V3 ud = unit(ray.d);
-         double t = 0.5 * (ud.y + 1);
+         real t = 0.5 * (ud.y + 1);
          V3 sky = (1 - t) * v3(1, 1, 1) + t * v3(0.5, 0.7, 1.0);
-         L = L + (double)P.background_intensity * (T * sky);
+         L = L + (real)P.background_intensity * (T * sky);
          break;
       }
       const Prim& pr = s.prims[h.prim];
@@ -987,23 +1089,23 @@ V3 trace_sample(const oracle_scene& s, const rayon_job_params& P, const Cam& cam
       if (len2(o.emitted) > 0 && !(nee && prev_lambert && nee_light(s, pr))) L = L + T * o.emitted;
       if (!o.scattered) break;
       if (nee && o.lambert) {
-         double ul = R.next(), ua = R.next(), ub = R.next();
-         int64_t li = std::min((int64_t)(ul * (double)s.lights.size()), (int64_t)s.lights.size() - 1);
+         real ul = R.next(), ua = R.next(), ub = R.next();
+         int64_t li = mn((int64_t)(ul * (real)s.lights.size()), (int64_t)s.lights.size() - 1);
          const Prim& lp = s.prims[s.lights[li].prim];
          V3 q, nl;
-         double area;
+         real area;
          sample_light(lp, ua, ub, q, nl, area);
          V3 org = h.p + 1e-4 * h.normal;
          V3 wd = q - org;
-         double d2 = len2(wd);
+         real d2 = len2(wd);
          V3 wi = (1.0 / std::sqrt(d2)) * wd;
-         double cs = dot(h.normal, wi);
-         double cl = lp.type == P_SPHERE ? dot(nl, -wi) : std::fabs(dot(nl, -wi));
+         real cs = dot(h.normal, wi);
+         real cl = lp.type == P_SPHERE ? dot(nl, -wi) : std::fabs(dot(nl, -wi));
          if (cs > 0 && cl > 0) {
             ++segments;
             if (!any_hit(s, {org, wd}, 1e-4, 1 - 1e-4)) {
                V3 Le = K.light * fv(s.mats[lp.mat].emission);
-               double g = cs * cl / d2 * area * (double)s.lights.size() / kPi;
+               real g = cs * cl / d2 * area * (real)s.lights.size() / kPi;
                L = L + g * (T * color * Le);
             }
          }
@@ -1012,7 +1114,7 @@ V3 trace_sample(const oracle_scene& s, const rayon_job_params& P, const Cam& cam
       prev_lambert = o.lambert;
       ray = o.ray;
       if (bounce > 0) {  // Russian roulette, cuda_raytracer.cuh:906-920
-         double q = std::min(maxc(T), 0.95);
+         real q = mn(maxc(T), 0.95);
          if (R.next() > q) break;
          T = (1.0 / q) * T;
       }
@@ -1042,14 +1144,30 @@ Cam make_cam(const rayon_job_params& P, const rayon_camera* c) {
 
 template <class F>
 void parallel_for(int64_t n, int threads, F&& f) {
-   if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+   if (threads <= 0) threads = (int)mx(1u, std::thread::hardware_concurrency());
    threads = (int)std::min<int64_t>(threads, std::max<int64_t>(1, n));
+#ifdef ORACLE_STATIC_ROWS
+   // the reference CPU renderer's schedule (renderer_cpu_parallel.hpp:56-63): one contiguous
+   // chunk of the (row-major) pixel list per thread, the last one taking the remainder
+   {
+      const int64_t per = std::max<int64_t>(1, n / threads);
+      std::vector<std::thread> pool;
+      for (int t = 0; t < threads; ++t) {
+         const int64_t a = t * per, b = t == threads - 1 ? n : std::min<int64_t>(n, a + per);
+         pool.emplace_back([&f, a, b] {
+            for (int64_t i = a; i < b; ++i) f(i);
+         });
+      }
+      for (auto& t : pool) t.join();
+      return;
+   }
+#endif
    std::atomic<int64_t> next{0};
    auto work = [&]() {
       for (;;) {
          int64_t i0 = next.fetch_add(16);
          if (i0 >= n) break;
-         for (int64_t i = i0; i < std::min(n, i0 + 16); ++i) f(i);
+         for (int64_t i = i0; i < mn(n, i0 + 16); ++i) f(i);
       }
    };
    if (threads == 1) {
@@ -1127,7 +1245,7 @@ oracle_scene* oracle_scene_create(const rayon_scene_desc* d) {
          const rayon_sdf_prim& q = d->sdf_prims[i];
          SdfPrim sp{q.type, fv(q.center), {q.params[0], q.params[1], q.params[2], q.params[3]}};
          s->sdf.push_back(sp);
-         double ex, ey, ez;
+         real ex, ey, ez;
          if (q.type == RAYON_SDF_SPHERE)
             ex = ey = ez = q.params[0];
          else if (q.type == RAYON_SDF_ROUNDED_BOX)
@@ -1374,7 +1492,10 @@ double oracle_fresnel_conductor(double ci, double eta, double k) {
 int64_t oracle_unit_disk(const float* rnd, int64_t nrnd, int64_t n, double* out_xy) {
    Draws R{nullptr, rnd, (int)std::min<int64_t>(nrnd, 0x7fffffff)};
    for (int64_t i = 0; i < n; ++i) {
-      unit_disk(R, out_xy[2 * i], out_xy[2 * i + 1]);
+      real x, y;
+      unit_disk(R, x, y);
+      out_xy[2 * i] = x;
+      out_xy[2 * i + 1] = y;
       if (R.used > R.nfixed) return -1;
    }
    return R.used;

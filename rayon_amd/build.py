@@ -75,6 +75,21 @@ def build_oracle() -> Path:
     return ORACLE_LIB
 
 
+CPU_BASE_LIB = ORACLE_DIR / "libcpubase.so"
+
+
+def build_cpu_baseline() -> Path:
+    """The benchmark's CPU baseline: the oracle's restatement of the path built in FP32 with
+    the GPU path's tree rule (binned SAH, 16 bins, leaves of at most 4) and the reference CPU
+    renderer's schedule (one contiguous chunk of rows per thread)."""
+    tmp = CPU_BASE_LIB.with_suffix(".so.tmp")
+    _run(["g++", "-std=c++17", "-O3", "-march=x86-64-v2", "-fPIC", "-shared", "-pthread", "-Wall",
+          "-DORACLE_REAL=float", "-DORACLE_BVH_BINNED", "-DORACLE_STATIC_ROWS",
+          "-o", str(tmp), str(ORACLE_DIR / "oracle.cpp")])
+    os.replace(tmp, CPU_BASE_LIB)
+    return CPU_BASE_LIB
+
+
 def build_reference() -> None:
     """The reference's own scene-file loader, compiled from its sources where they lie (only
     where they exist, i.e. not on a machine without the reference), into oracle/_ref/."""
@@ -92,6 +107,7 @@ def main(argv=None) -> int:
         print(build_product(a.j))
     if a.oracle or both:
         print(build_oracle())
+        print(build_cpu_baseline())
         build_reference()
     return 0
 
