@@ -619,8 +619,6 @@ __device__ __forceinline__ bool test_any(const SceneView& S, int i, V3 o, V3 d, 
    return isect_sdf(S, p, o, d, tmin, tmax, t);
 }
 
-__device__ __forceinline__ uint32_t byte_of(const uint32_t* w, int i) { return (w[i >> 2] >> (8 * (i & 3))) & 0xFFu; }
-
 // Tests the eight quantized child boxes of a wide node. Returns the hit internal
 // children as a mask in visiting order (bit i = slot i ^ perm) and the hit leaf children
 // as a slot mask. Each child plane is q * (2^e / d) + (p - o) / d; the exit distance is
@@ -637,17 +635,31 @@ __device__ __forceinline__ void wide_children(const WNode& n, V3 o, V3 inv, uint
    const float ax = sx * inv.x, ay = sy * inv.y, az = sz * inv.z;
    const float bx = (n.px - o.x) * inv.x, by = (n.py - o.y) * inv.y, bz = (n.pz - o.z) * inv.z;
    const float slack = 4.76837158e-7f * fmaxf(fmaxf(fabsf(bx), fabsf(by)), fabsf(bz));
+   // The entry plane of each axis is the lo plane for a positive direction and the hi plane
+   // for a negative one: pick the byte words once per node (the slots' planes then need no
+   // per-slot min/max to order them). Bytes: lo x/y/z in words 0-1/2-3/4-5, hi in 6-7/8-9/10-11.
+   const bool nx = inv.x < 0.0f, ny = inv.y < 0.0f, nz = inv.z < 0.0f;
+   const uint32_t nx0 = nx ? n.q[6] : n.q[0], nx1 = nx ? n.q[7] : n.q[1];
+   const uint32_t fx0 = nx ? n.q[0] : n.q[6], fx1 = nx ? n.q[1] : n.q[7];
+   const uint32_t ny0 = ny ? n.q[8] : n.q[2], ny1 = ny ? n.q[9] : n.q[3];
+   const uint32_t fy0 = ny ? n.q[2] : n.q[8], fy1 = ny ? n.q[3] : n.q[9];
+   const uint32_t nz0 = nz ? n.q[10] : n.q[4], nz1 = nz ? n.q[11] : n.q[5];
+   const uint32_t fz0 = nz ? n.q[4] : n.q[10], fz1 = nz ? n.q[5] : n.q[11];
    hit_int = 0;
    hit_leaf = 0;
 #pragma unroll
    for (int sl = 0; sl < 8; ++sl) {
-      const float lx = fmaf((float)byte_of(n.q, sl), ax, bx), hx = fmaf((float)byte_of(n.q, 24 + sl), ax, bx);
-      const float ly = fmaf((float)byte_of(n.q, 8 + sl), ay, by), hy = fmaf((float)byte_of(n.q, 32 + sl), ay, by);
-      const float lz = fmaf((float)byte_of(n.q, 16 + sl), az, bz), hz = fmaf((float)byte_of(n.q, 40 + sl), az, bz);
-      const float tn = fmaxf(fmaxf(fminf(lx, hx), fminf(ly, hy)), fmaxf(fminf(lz, hz), tmin));
-      const float tf = fminf(fminf(fmaxf(lx, hx), fmaxf(ly, hy)), fminf(fmaxf(lz, hz), tmax));
+      const int sh = 8 * (sl & 3);  // one byte-to-float conversion per plane (v_cvt_f32_ubyteN)
+      const float qnx = (float)(((sl < 4 ? nx0 : nx1) >> sh) & 0xFFu);
+      const float qfx = (float)(((sl < 4 ? fx0 : fx1) >> sh) & 0xFFu);
+      const float qny = (float)(((sl < 4 ? ny0 : ny1) >> sh) & 0xFFu);
+      const float qfy = (float)(((sl < 4 ? fy0 : fy1) >> sh) & 0xFFu);
+      const float qnz = (float)(((sl < 4 ? nz0 : nz1) >> sh) & 0xFFu);
+      const float qfz = (float)(((sl < 4 ? fz0 : fz1) >> sh) & 0xFFu);
+      const float tn = fmaxf(fmaxf(fmaf(qnx, ax, bx), fmaf(qny, ay, by)), fmaxf(fmaf(qnz, az, bz), tmin));
+      const float tf = fminf(fminf(fmaf(qfx, ax, bx), fmaf(qfy, ay, by)), fminf(fmaf(qfz, az, bz), tmax));
       // an empty slot stores lo = 255 > hi = 0 on every axis: its box is inverted
-      const bool valid = byte_of(n.q, sl) <= byte_of(n.q, 24 + sl);
+      const bool valid = nx ? qfx <= qnx : qnx <= qfx;
       if (valid && tn <= fmaf(tf, 1.0000153f, slack)) {
          if ((imask >> sl) & 1u)
             hit_int |= 1u << (sl ^ perm);
