@@ -887,6 +887,10 @@ def run_serve(a, ra, ctx, pool, lam, duration, grace, spp_div=1, target_pass_ms=
          "ttc_p50_ms_completed": percentile(ttc_done, 50), "ttc_p99_ms_completed": percentile(ttc_done, 99),
          "mrays_per_s": round(mrays, 2), "wall_s": round(wall, 1), "passes": int(st.passes),
          "first_passes": int(st.first_passes),
+         "first_frames_kept": int(st.first_kept),
+         "first_frames_kept_fraction": round(st.first_kept / max(len(ids), 1), 4),
+         "first_kept_note": "first frames of jobs admitted at arrival are their jobs' first samples; a queued "
+                            "job's first frame is rendered before it has buffers and its sample 0 again later",
          "refine_s": round(st.pass_ms / 1e3, 2), "first_pass_s": round(st.first_ms / 1e3, 2),
          "stats_read_s": round(st.check_ms / 1e3, 2),
          "refine_msamples_per_s": round(st.pass_paths / max(st.pass_ms, 1e-9) / 1e3, 1),

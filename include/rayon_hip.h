@@ -486,7 +486,7 @@ typedef struct rayon_server_config {
 
 typedef struct rayon_server_job_info {
    int32_t state;
-   int32_t pad_;
+   int32_t first_kept;          /* 1: its first frame's samples are its first samples      */
    int64_t samples_done;
    int64_t segments;
    double queue_ms;             /* arrival -> admission                                  */
@@ -503,6 +503,8 @@ typedef struct rayon_server_stats {
    double first_ms;             /* summed wall time of first passes                       */
    double check_ms;             /* summed time reading job statistics after passes        */
    double pass_paths;           /* summed pixel-samples of refinement passes              */
+   int64_t first_kept;          /* first frames kept as their jobs' first samples          */
+   int64_t first_kept_paths;    /* pixel-samples of those first frames                     */
 } rayon_server_stats;
 
 void rayon_server_config_default(rayon_server_config* cfg);

@@ -112,7 +112,7 @@ class ServerConfig(C.Structure):
 
 
 class ServerJobInfo(C.Structure):
-    _fields_ = [("state", C.c_int32), ("pad_", C.c_int32), ("samples_done", C.c_int64), ("segments", C.c_int64),
+    _fields_ = [("state", C.c_int32), ("first_kept", C.c_int32), ("samples_done", C.c_int64), ("segments", C.c_int64),
                 ("queue_ms", C.c_double), ("ttff_ms", C.c_double), ("ttc_ms", C.c_double)]
 
 
@@ -121,7 +121,8 @@ class ServerStats(C.Structure):
                 ("failed", C.c_int64), ("queued", C.c_int64), ("active", C.c_int64), ("segments", C.c_int64),
                 ("used_bytes", C.c_int64), ("busy_s", C.c_double), ("wall_s", C.c_double),
                 ("pass_samples", C.c_int32), ("pass_jobs", C.c_int32), ("pass_ms", C.c_double),
-                ("first_ms", C.c_double), ("check_ms", C.c_double), ("pass_paths", C.c_double)]
+                ("first_ms", C.c_double), ("check_ms", C.c_double), ("pass_paths", C.c_double),
+                ("first_kept", C.c_int64), ("first_kept_paths", C.c_int64)]
 
 
 class TuneResult(C.Structure):

@@ -758,10 +758,10 @@ rayon_status rayon_job_stats_get(rayon_ctx* ctx, rayon_job* j, rayon_job_stats* 
    for (int s = 0; s < kCounterSlots; ++s)
       for (int k = 0; k < 8; ++k) c[k] += slots[8 * s + k];
    out->samples_done = j->samples_done;
-   out->segments = (int64_t)c[0];
-   out->shadow_segments = (int64_t)c[1];
-   out->primary_samples = (int64_t)c[2];
-   out->converged_pixels = (int64_t)c[3];
+   out->segments = (int64_t)c[0] + j->pre_counts[0];
+   out->shadow_segments = (int64_t)c[1] + j->pre_counts[1];
+   out->primary_samples = (int64_t)c[2] + j->pre_counts[2];
+   out->converged_pixels = (int64_t)c[3] + j->pre_counts[3];
    out->pad_ = 0;
    out->node_fetches = (int64_t)c[4];
    out->prim_fetches = (int64_t)c[5];

@@ -122,6 +122,9 @@ struct rayon_job {
    std::vector<ry::DevBuf> bufs;       // hipMalloc
    std::vector<ry::DevBuf> pool_bufs;  // stream-ordered pool (accumulation, counts, counters)
    int64_t samples_done = 0;
+   // statistics of samples rendered outside the job's device counters (a server's first
+   // frame that the job keeps): segments, shadow segments, primary samples, converged pixels
+   int64_t pre_counts[4] = {0, 0, 0, 0};
    int32_t path = -1;       // batched path taken since the first pass (-1: not yet chosen)
    float4* prev = nullptr;  // sums at pass start (batched passes with the reference adaptive test)
    int* live = nullptr;      // pixels still sampling (batched passes of adaptive jobs)

@@ -9,12 +9,16 @@
 namespace ry {
 
 // First-frame launches: up to kPreviewJobs jobs per launch, their views in the kernel
-// arguments (8 x 312 bytes, within the 4 KiB argument limit).
+// arguments (8 x 336 bytes, within the 4 KiB argument limit).
 constexpr int kPreviewJobs = 8;
+// Per-tile statistics of an accumulating first frame: segments (extension + shadow),
+// shadow segments, primary samples, converged pixels of one 16x16 tile.
+constexpr int kPreviewTileCounts = 4;
 struct PreviewJob {
    JobView view;
-   uint8_t* frame;  // RGBA8, width x height
-   int first_tile;  // first 16x16 tile of this job in the launch
+   uint8_t* frame;            // RGBA8, width x height
+   unsigned long long* part;  // accumulating: kPreviewTileCounts per tile of the job; else NULL
+   int first_tile;            // first 16x16 tile of this job in the launch
    int pad_;
 };
 struct PreviewBatch {
@@ -46,9 +50,13 @@ hipError_t launch_resolve_u8_raw(const float4* acc, const float4* comp, const in
 hipError_t launch_heatmap(const int* counts, int npix, uint8_t* out, int channels, int max_samples, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t st);
 // One launch per kPreviewJobs jobs: `samples` paths per pixel from sample 0, resolved with
-// `gamma` into each job's RGBA8 frame (no accumulation buffers are touched).
-hipError_t launch_preview(const JobView* views, uint8_t* const* frames, int n, int samples, float gamma,
-                          hipStream_t st);
+// `gamma` into each job's RGBA8 frame. A job with a `parts` entry (NULL: none) also stores
+// those samples as its first `samples` accumulated samples: its sums, compensations and
+// per-pixel counts are written whole (no clear needed beforehand) with the plain kernel's
+// arithmetic, and each tile's statistics go to parts[k][tile * kPreviewTileCounts + i] (a
+// host-visible array, summed by the caller; the job's device counters are not touched).
+hipError_t launch_preview(const JobView* views, uint8_t* const* frames, unsigned long long* const* parts, int n,
+                          int samples, float gamma, hipStream_t st);
 // Fixed staging array of the allocation-free resolve, one per device (64 MiB: a 4096 x 4096
 // RGBA8 image); noalloc_scratch_address returns its address on the current device.
 constexpr size_t kNoAllocBytes = (size_t)64 << 20;

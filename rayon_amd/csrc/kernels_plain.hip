@@ -141,33 +141,66 @@ __global__ void k_resolve_u8(JobView J, uint8_t* out, int channels, float inv_ga
 // for a free compute unit between steps while another stream's kernels hold the chip.
 // Each lane runs `samples` paths of its pixel from sample 0 in sample order, folds them
 // into compensated sums that start at zero, and writes the RGBA8 pixel: the bits of the
-// plain kernel's `samples`-sample render of the job resolved by k_resolve_u8.
+// plain kernel's `samples`-sample render of the job resolved by k_resolve_u8. For a job
+// with a statistics array the lane also stores those sums, compensations and the pixel's
+// count as the job's first samples (what the plain kernel leaves after its first launch on
+// cleared buffers), and the workgroup writes its tile's statistics to that array.
 template <int LAYOUT>
-__global__ __launch_bounds__(kBlock) void k_preview(const PreviewBatch B) {
+__global__ __launch_bounds__(kBlock, 4) void k_preview(const PreviewBatch B) {
    __shared__ int stack[kStack * kBlock];
+   __shared__ unsigned long long tile_counts[kBlock / 64][kPreviewTileCounts];
    const int tile = xcd_tile(blockIdx.x, B.per_xcd);
    if (tile >= B.ntiles) return;
    int k = 0;
    while (k + 1 < B.njobs && B.job[k + 1].first_tile <= tile) ++k;
-   const JobView& J = B.job[k].view;
+   const PreviewJob& PJ = B.job[k];
+   const JobView& J = PJ.view;
    const int tiles_x = (J.width + kTile - 1) / kTile;
-   const int t = tile - B.job[k].first_tile;
+   const int t = tile - PJ.first_tile;
    const int tid = threadIdx.x;
    const int x = (t % tiles_x) * kTile + (tid & (kTile - 1));
    const int y = (t / tiles_x) * kTile + (tid / kTile);
-   if (x >= J.width || y >= J.height) return;
-   const int pix = y * J.width + x;
    PathCounts pc;
-   float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c = a;
-   Rng R{0u, false};
-   for (int s = 0; s < B.samples; ++s) {
-      R.s = counter_init(J.seed_lo, J.seed_hi, (uint32_t)pix, (uint32_t)s);
-      V3 o, d;
-      camera_ray(J, x, y, R, o, d);
-      const V3 L = trace_path<kBlock, false, LAYOUT>(J, o, d, R, stack + tid, pc);
-      accumulate(a, c, L);
+   int converged = 0;
+   if (x < J.width && y < J.height) {
+      const int pix = y * J.width + x;
+      float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c = a;
+      Rng R{0u, false};
+      for (int s = 0; s < B.samples; ++s) {
+         R.s = counter_init(J.seed_lo, J.seed_hi, (uint32_t)pix, (uint32_t)s);
+         V3 o, d;
+         camera_ray(J, x, y, R, o, d);
+         const V3 L = trace_path<kBlock, false, LAYOUT>(J, o, d, R, stack + tid, pc);
+         accumulate(a, c, L);
+      }
+      resolve_px(a, c, B.samples, B.inv_gamma, PJ.frame + (size_t)pix * 4, 4);
+      if (PJ.part) {
+         J.acc[pix] = a;
+         J.comp[pix] = c;
+         const int n = adaptive_count(J, make_float4(0.0f, 0.0f, 0.0f, 0.0f), 0, a, c, B.samples);
+         J.counts[pix] = n;
+         converged = n < 0 ? 1 : 0;
+      }
    }
-   resolve_px(a, c, B.samples, B.inv_gamma, B.job[k].frame + (size_t)pix * 4, 4);
+   if (PJ.part) {  // one job per workgroup: the branch is uniform
+      const int inside = x < J.width && y < J.height ? 1 : 0;
+      const unsigned long long v0 = wave_sum((unsigned long long)pc.segments + (unsigned long long)pc.shadow);
+      const unsigned long long v1 = wave_sum((unsigned long long)pc.shadow);
+      const unsigned long long v2 = wave_sum((unsigned long long)(inside * B.samples));
+      const unsigned long long v3 = wave_sum((unsigned long long)converged);
+      if ((tid & 63) == 0) {
+         tile_counts[tid >> 6][0] = v0;
+         tile_counts[tid >> 6][1] = v1;
+         tile_counts[tid >> 6][2] = v2;
+         tile_counts[tid >> 6][3] = v3;
+      }
+      __syncthreads();
+      if (tid < kPreviewTileCounts) {
+         unsigned long long v = 0;
+         for (int w = 0; w < kBlock / 64; ++w) v += tile_counts[w][tid];
+         PJ.part[(size_t)t * kPreviewTileCounts + tid] = v;
+      }
+   }
 }
 
 // One launch clears up to kFillRanges buffers (words of 32 bits): blockIdx.y picks the
@@ -332,8 +365,8 @@ hipError_t launch_resolve_u8(const JobView& J, uint8_t* out, int channels, float
    return hipGetLastError();
 }
 
-hipError_t launch_preview(const JobView* views, uint8_t* const* frames, int n, int samples, float gamma,
-                          hipStream_t st) {
+hipError_t launch_preview(const JobView* views, uint8_t* const* frames, unsigned long long* const* parts, int n,
+                          int samples, float gamma, hipStream_t st) {
    // binary-tree jobs first, then quantized-tree jobs, kPreviewJobs of one layout per launch
    std::vector<int> order;
    for (int wide = 0; wide < 2; ++wide)
@@ -347,9 +380,13 @@ hipError_t launch_preview(const JobView* views, uint8_t* const* frames, int n, i
       while (base < order.size() && B.njobs < kPreviewJobs && (views[order[base]].scene.wnodes != nullptr) == wide) {
          const int k = order[base++];
          const JobView& J = views[k];
+         unsigned long long* part = parts ? parts[k] : nullptr;
          if ((J.flags & RAYON_FLAG_REF_RNG) || J.width <= 0 || J.height <= 0 || !frames[k]) return hipErrorInvalidValue;
+         if (part && (!J.acc || !J.comp || !J.counts || (J.spp_target > 0 && J.spp_target < samples)))
+            return hipErrorInvalidValue;
          B.job[B.njobs].view = J;
          B.job[B.njobs].frame = frames[k];
+         B.job[B.njobs].part = part;
          B.job[B.njobs].first_tile = B.ntiles;
          B.ntiles += plain_tiles(J.width, J.height);
          ++B.njobs;

@@ -7,9 +7,11 @@
 //   1. admits queued jobs while their device memory (accumulation, counts and a u8 frame,
 //      plus each newly referenced scene) fits the HBM budget and the active-job cap;
 //   2. renders a first frame for each job admitted in this step: first_pass_samples
-//      samples of the job by the plain kernel's arithmetic (all new jobs in one launch)
-//      into scratch buffers, resolved on the device (a preview that does not enter the
-//      job's sums); time-to-first-frame is stamped when that work completes;
+//      samples of the job by the plain kernel's arithmetic (all new jobs in one launch),
+//      resolved on the device; those samples are the job's first accumulated samples
+//      (the same streams and arithmetic as the batched passes, so the result does not
+//      change); time-to-first-frame is stamped when that work completes. Queued jobs
+//      that have no buffers yet get a first frame too, which they cannot keep;
 //   3. advances the oldest other active jobs together by one batched pass whose work
 //      (pixels x samples) a controller steers toward target_pass_ms: the jobs are taken
 //      oldest first until their pixels reach the work budget (at least one job, at most
@@ -55,6 +57,9 @@ struct SJob {
    bool released = false;
    bool accounted = false;  // bytes (and a scene reference) are held in the budget
    bool needs_clear = false;  // accumulation buffers not cleared yet (before the first pass)
+   bool counters_only = false;  // its first frame holds its first samples: clear the counters only
+   bool pv_accum = false;       // its first frame in flight stores the job's first samples
+   bool first_kept = false;     // its first frame's samples are its first samples
    bool previewing = false;   // its first frame is being rendered
    bool preview_held = false;  // its frame is held in the first-frame reserve (not admitted)
 };
@@ -94,6 +99,7 @@ struct rayon_server {
    double pass_work = 4.0 * (1 << 20);  // pixel-samples of the next refinement pass
    // statistics
    int64_t passes = 0, first_passes = 0, total_segments = 0, completed = 0, failed = 0;
+   int64_t first_kept = 0, first_kept_paths = 0;
    double busy_s = 0, t_start = 0;
    double pass_ms_sum = 0, first_ms_sum = 0, check_ms_sum = 0, pass_paths_sum = 0;
    int pass_jobs = 0;
@@ -204,12 +210,36 @@ struct rayon_server {
       return finished;
    }
 
-   // First frames in flight: one launch on a preview stream, and the event after it.
+   // First frames in flight: one launch on a preview stream, and the event after it. Jobs
+   // whose first frame accumulates have a slice of `part` (pinned host memory) for the
+   // statistics of their tiles.
    struct FirstBatch {
       std::vector<SJob*> jobs;
+      std::vector<size_t> part_off;  // per job: first entry in `part`, or kNoPart
+      unsigned long long* part = nullptr;
+      size_t part_bytes = 0;
       hipEvent_t ready = nullptr, done = nullptr;
       double t0 = 0;
    };
+   static constexpr size_t kNoPart = ~(size_t)0;
+   // Pinned statistics buffers, kept for the server's lifetime and reused.
+   std::vector<std::pair<unsigned long long*, size_t>> pinned_free;
+   unsigned long long* take_pinned(size_t bytes, size_t& got) {
+      for (size_t i = 0; i < pinned_free.size(); ++i)
+         if (pinned_free[i].second >= bytes) {
+            auto r = pinned_free[i];
+            pinned_free.erase(pinned_free.begin() + (long)i);
+            got = r.second;
+            return r.first;
+         }
+      got = std::max<size_t>(bytes, (size_t)1 << 20);
+      void* p = nullptr;
+      if (hipHostMalloc(&p, got, hipHostMallocDefault) != hipSuccess) {
+         got = 0;
+         return nullptr;
+      }
+      return (unsigned long long*)p;
+   }
    std::vector<hipEvent_t> event_pool;
    hipEvent_t take_event() {
       hipEvent_t e = nullptr;
@@ -313,9 +343,11 @@ struct rayon_server {
          // (1) admitted jobs: buffers on first_ctx. Its allocations must not queue behind
          // the refinement pass in flight on the main stream; the main stream waits for them
          // by event (the stream-ordered pool may hand out memory whose earlier use on
-         // first_ctx's stream is still in flight), and so does a preview stream. The buffers
-         // are cleared on the main stream just before the job's first refinement pass, so a
-         // first frame waits for nothing but its own kernel.
+         // first_ctx's stream is still in flight), and so does a preview stream. The first
+         // frame writes the job's sums, compensations and counts whole; the counters (and,
+         // for a job whose first frame came before its buffers, everything) are cleared on
+         // the main stream just before the job's first refinement pass, so a first frame
+         // waits for nothing but its own kernel.
          for (SJob* j : fresh) {
             rayon_status st = ry::job_create(first_ctx, j->scene, &j->params, &j->job, false, false);
             std::lock_guard<std::mutex> lk(mu);
@@ -333,16 +365,20 @@ struct rayon_server {
          }
          // (1a) first frames: the job's first samples rendered and resolved to RGBA8 by one
          // launch of the preview kernel for all of them (the plain kernel's per-pixel
-         // arithmetic, tables in the kernel arguments). A first frame does not enter the
-         // job's sums (its samples arrive with the refinement passes), so a served job's
-         // result equals the job rendered alone.
+         // arithmetic, which the batched path matches bit for bit, tables in the kernel
+         // arguments). An admitted job keeps them as its first samples (the refinement
+         // passes go on from there), so a served job's result equals the job rendered
+         // alone; a queued job's first frame is only a frame: its samples are rendered
+         // again once it has buffers.
          if (!previews.empty()) {
             FirstBatch fb;
             fb.t0 = now_s();
             std::vector<ry::JobView> views;
             std::vector<uint8_t*> frames;
+            std::vector<unsigned long long*> parts;
             {
                std::lock_guard<std::mutex> lk(mu);
+               size_t entries = 0;
                for (SJob* j : previews) {
                   if (j->state == RAYON_JOB_FAILED || j->state == RAYON_JOB_CANCELLED) continue;
                   if (hipMallocAsync((void**)&j->frame, (size_t)frame_bytes(j), first_ctx->stream) != hipSuccess) {
@@ -354,7 +390,23 @@ struct rayon_server {
                   views.push_back(preview_view(j));
                   frames.push_back(j->frame);
                   j->previewing = true;
+                  // an admitted job whose buffers are still uncleared takes the first frame's
+                  // samples as its own first samples (same streams, same arithmetic as the
+                  // batched passes that follow)
+                  j->pv_accum = j->job && j->needs_clear && !j->counters_only &&
+                                !(j->params.flags & RAYON_FLAG_REF_RNG) && j->params.spp >= cfg.first_pass_samples;
+                  fb.part_off.push_back(j->pv_accum ? entries : kNoPart);
+                  if (j->pv_accum)
+                     entries += (size_t)ry::plain_tiles(j->params.width, j->params.height) * ry::kPreviewTileCounts;
                   fb.jobs.push_back(j);
+               }
+               if (entries) fb.part = take_pinned(entries * sizeof(unsigned long long), fb.part_bytes);
+               for (size_t k = 0; k < fb.jobs.size(); ++k) {
+                  if (!fb.part && fb.part_off[k] != kNoPart) {  // no pinned memory: a first frame only
+                     fb.part_off[k] = kNoPart;
+                     fb.jobs[k]->pv_accum = false;
+                  }
+                  parts.push_back(fb.part_off[k] == kNoPart ? nullptr : fb.part + fb.part_off[k]);
                }
             }
             if (!fb.jobs.empty()) {
@@ -363,7 +415,7 @@ struct rayon_server {
                hipStream_t ps = pv_stream[next_pv++ % kPreviewStreams];
                const bool ok = fb.ready && fb.done && hipEventRecord(fb.ready, first_ctx->stream) == hipSuccess &&
                                hipStreamWaitEvent(ps, fb.ready, 0) == hipSuccess &&
-                               ry::launch_preview(views.data(), frames.data(), (int)views.size(),
+                               ry::launch_preview(views.data(), frames.data(), parts.data(), (int)views.size(),
                                                   cfg.first_pass_samples, 2.0f, ps) == hipSuccess &&
                                hipEventRecord(fb.done, ps) == hipSuccess;
                if (!ok) {
@@ -387,16 +439,35 @@ struct rayon_server {
             if (q != hipSuccess) last_error = std::string("first frame: ") + hipGetErrorString(q);
             ++first_passes;
             first_ms_sum += (t - fb.t0) * 1e3;
-            for (SJob* j : fb.jobs) {
+            for (size_t m = 0; m < fb.jobs.size(); ++m) {
+               SJob* j = fb.jobs[m];
                j->previewing = false;
+               const bool accum = j->pv_accum;
+               j->pv_accum = false;
                if (j->state == RAYON_JOB_CANCELLED || j->state == RAYON_JOB_FAILED) {
                   if (j->released || j->state == RAYON_JOB_CANCELLED) free_job_locked(j);  // the frame now
                } else if (q == hipSuccess) {
                   j->t_first = t;
+                  if (accum) {  // the job's first samples are in its buffers; its counters are not
+                     const unsigned long long* pp = fb.part + fb.part_off[m];
+                     const size_t tiles = (size_t)ry::plain_tiles(j->params.width, j->params.height);
+                     int64_t c[ry::kPreviewTileCounts] = {0, 0, 0, 0};
+                     for (size_t i = 0; i < tiles; ++i)
+                        for (int k = 0; k < ry::kPreviewTileCounts; ++k) c[k] += (int64_t)pp[i * ry::kPreviewTileCounts + k];
+                     for (int k = 0; k < ry::kPreviewTileCounts; ++k) j->job->pre_counts[k] += c[k];
+                     j->job->samples_done = cfg.first_pass_samples;
+                     j->samples = j->job->samples_done;
+                     j->segments = c[0];
+                     j->counters_only = true;
+                     j->first_kept = true;
+                     ++first_kept;
+                     first_kept_paths += c[2];
+                  }
                } else {
                   fail_locked(j);
                }
             }
+            if (fb.part) pinned_free.push_back({fb.part, fb.part_bytes});
             for (hipEvent_t e : {fb.ready, fb.done})
                if (e) event_pool.push_back(e);
             firsts.erase(firsts.begin() + (long)k);
@@ -449,15 +520,16 @@ struct rayon_server {
                      uint32_t* p[8];
                      long long n[8];
                      const int m = ry::job_clear_ranges(j->job, p, n);
-                     cp.insert(cp.end(), p, p + m);
-                     cn.insert(cn.end(), n, n + m);
+                     const int first = j->counters_only ? m - 1 : 0;  // the counters are the last range
+                     cp.insert(cp.end(), p + first, p + m);
+                     cn.insert(cn.end(), n + first, n + m);
                   }
                }
                t_pass = now_s();
                const bool cleared =
                    cp.empty() || ry::launch_fill_ranges(cp.data(), cn.data(), (int)cp.size(), 0u, ctx->stream) == hipSuccess;
                if (cleared)
-                  for (SJob* j : run) j->needs_clear = false;
+                  for (SJob* j : run) j->needs_clear = j->counters_only = false;
                if (!cleared || rayon_render_batched(ctx, h.data(), (int)h.size(), pass_samples) != RAYON_OK ||
                    hipEventRecord(pass_done, ctx->stream) != hipSuccess) {
                   fail_all(run, "batched pass");
@@ -480,6 +552,10 @@ struct rayon_server {
       }
       for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
       event_pool.clear();
+      for (FirstBatch& fb : firsts)
+         if (fb.part) pinned_free.push_back({fb.part, fb.part_bytes});
+      for (auto& pb : pinned_free) (void)hipHostFree(pb.first);
+      pinned_free.clear();
       (void)hipEventDestroy(pass_done);
       (void)hipEventDestroy(alloc_done);
    }
@@ -585,6 +661,7 @@ rayon_status rayon_server_job_info_get(rayon_server* s, uint64_t id, rayon_serve
    }
    const SJob* j = it->second;
    out->state = j->state;
+   out->first_kept = j->first_kept ? 1 : 0;
    out->samples_done = j->samples;
    out->segments = j->segments;
    out->ttff_ms = j->t_first > 0 ? (j->t_first - j->t_arrive) * 1e3 : -1.0;
@@ -676,6 +753,8 @@ rayon_status rayon_server_stats_get(rayon_server* s, rayon_server_stats* out) {
    std::lock_guard<std::mutex> lk(s->mu);
    out->passes = s->passes;
    out->first_passes = s->first_passes;
+   out->first_kept = s->first_kept;
+   out->first_kept_paths = s->first_kept_paths;
    out->completed = s->completed;
    out->failed = s->failed;
    out->queued = (int64_t)s->queue.size();

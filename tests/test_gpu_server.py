@@ -4,7 +4,10 @@ Tolerances: a job served among others must equal the same job rendered alone wit
 rayon_render_batched bit for bit (the random streams depend only on seed, pixel and
 sample index, and each pixel accumulates its samples in sample order, whatever the
 pass sizes the server's controller picks); the first frame equals a one-sample render
-of the job by the plain kernel resolved to RGBA8 with gamma 2, byte for byte.
+of the job by the plain kernel resolved to RGBA8 with gamma 2, byte for byte. A job
+admitted at arrival keeps its first frame's sample as its sample 0 (the plain kernel and
+the batched path give the same bits per sample), so its result and its segment count
+still equal the alone render's exactly.
 """
 import sys
 
@@ -58,10 +61,11 @@ def alone(ctx, scenes, gens, key, W, H, kw, samples=None):
 
 def test_served_jobs_equal_alone_renders(world):
     ctx, gens, scenes = world
-    refs, firsts = [], []
+    refs, firsts, segs = [], [], []
     for key, W, H, kw in JOBS:
         j = alone(ctx, scenes, gens, key, W, H, kw)
         refs.append(j.resolve_linear())
+        segs.append(j.stats().segments)
         j.close()
         f = ctx.job(scenes[key], params(gens, key, W, H, **dict(kw, spp=1)))
         f.render_plain(1)
@@ -75,10 +79,15 @@ def test_served_jobs_equal_alone_renders(world):
         log(f"passes {st.passes} first passes {st.first_passes} completed {st.completed} pass_samples "
             f"{st.pass_samples}")
         assert st.completed == len(JOBS) and st.failed == 0 and st.queued == 0 and st.active == 0
-        for i, jid, (key, W, H, kw), ref, first in zip(range(len(ids)), ids, JOBS, refs, firsts):
+        # every job was admitted at arrival (the budget holds them all): each kept its first frame
+        assert st.first_kept == len(JOBS)
+        assert st.first_kept_paths == sum(W * H for _, W, H, _ in JOBS)
+        for i, jid, (key, W, H, kw), ref, first, sg in zip(range(len(ids)), ids, JOBS, refs, firsts, segs):
             inf = srv.info(jid)
             assert inf.state == ra.JOB.DONE
+            assert inf.first_kept == 1
             assert inf.samples_done == kw["spp"]
+            assert inf.segments == sg, f"job {i} {key}: {inf.segments} segments served, {sg} alone"
             assert 0 <= inf.queue_ms <= inf.ttff_ms <= inf.ttc_ms
             np.testing.assert_array_equal(srv.result(jid), ref, err_msg=f"job {i} {key}")
             np.testing.assert_array_equal(srv.first_frame(jid), first, err_msg=f"first frame {i} {key}")
@@ -116,6 +125,9 @@ def test_admission_by_budget(world):
         # and its first frame came before its admission
         assert infos[4].queue_ms >= min(infos[0].ttc_ms, infos[1].ttc_ms) - 5.0
         assert infos[4].ttff_ms < infos[4].queue_ms
+        # the first job kept its first frame's sample; the last one's came before it had
+        # buffers, so its sample 0 was rendered again by a refinement pass
+        assert infos[0].first_kept == 1 and infos[4].first_kept == 0
         for i in ids:
             np.testing.assert_array_equal(srv.result(i), img)
             np.testing.assert_array_equal(srv.first_frame(i), first)
