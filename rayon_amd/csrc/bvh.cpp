@@ -11,12 +11,18 @@
 // device's fixed LDS stack can never overflow (the reference drops pushes past 32,
 // cuda_raytracer.cuh:561-564).
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstring>
 #include <thread>
 
 #include "internal.h"
+
+// Cost of a primitive test relative to a wide-node visit in the surface-area collapse.
+#ifndef RY_WIDE_PRIM_COST
+#define RY_WIDE_PRIM_COST 0.6
+#endif
 
 namespace ry {
 
@@ -245,6 +251,72 @@ struct WideBuilder {
 
    WideBuilder(std::vector<BNode>& b, const std::vector<int32_t>& o) : bn(b), old_order(o) {}
 
+   // Which binary nodes become wide nodes: the surface-area cost of the wide tree is
+   // minimised by dynamic programming over the binary tree (the collapse of Ylitie, Karras
+   // and Laine, HPG 2017, with the binary leaves kept as they are). cost[b][k-1] is the
+   // least cost of covering b's subtree with at most k child slots of one wide node;
+   // cut[b][k-1] is 0 when b takes one slot itself (a leaf, or an internal child that
+   // becomes a wide node), else the number of slots its left child's subtree gets.
+   // A wide node costs kNodeCost per unit of surface area (testing its eight children), a
+   // primitive kPrimCost.
+   static constexpr double kNodeCost = 1.0;
+   static constexpr double kPrimCost = RY_WIDE_PRIM_COST;
+   std::vector<std::array<float, 8>> cost;
+   std::vector<std::array<int8_t, 8>> cut;
+
+   void plan(int b) {
+      BNode& n = bn[b];
+      auto& c = cost[b];
+      auto& k1 = cut[b];
+      if (n.count) {
+         c.fill((float)(n.box.area() * n.count * kPrimCost));
+         k1.fill(0);
+         return;
+      }
+      plan(n.left);
+      plan(n.right);
+      const auto& cl = cost[n.left];
+      const auto& cr = cost[n.right];
+      float split[9];  // split[k]: both children within k slots (k >= 2)
+      int8_t arg[9];
+      for (int k = 2; k <= 8; ++k) {
+         split[k] = INFINITY;
+         arg[k] = 1;
+         for (int a = 1; a < k; ++a) {
+            const float v = cl[a - 1] + cr[k - a - 1];
+            if (v < split[k]) {
+               split[k] = v;
+               arg[k] = (int8_t)a;
+            }
+         }
+      }
+      const float own = (float)(n.box.area() * kNodeCost) + split[8];  // b as a wide node
+      c[0] = own;
+      k1[0] = 0;
+      wide_cut[b] = arg[8];
+      for (int k = 2; k <= 8; ++k) {
+         if (own <= split[k]) {
+            c[k - 1] = own;
+            k1[k - 1] = 0;
+         } else {
+            c[k - 1] = split[k];
+            k1[k - 1] = arg[k];
+         }
+      }
+   }
+   std::vector<int8_t> wide_cut;  // slots of the left child's subtree when b is a wide node
+
+   // The binary nodes that fill at most k slots for b's subtree, as planned.
+   void cover(int b, int k, int* kids, int& nk) {
+      const int a = cut[b][k - 1];
+      if (a == 0) {
+         kids[nk++] = b;
+         return;
+      }
+      cover(bn[b].left, a, kids, nk);
+      cover(bn[b].right, k - a, kids, nk);
+   }
+
    static int exponent_for(double ext) {
       // smallest e with 255 * 2^e >= ext, within the normal float range
       int e = -126;
@@ -263,25 +335,9 @@ struct WideBuilder {
       if (bn[b].count) {
          kids[nk++] = b;
       } else {
-         kids[nk++] = bn[b].left;
-         kids[nk++] = bn[b].right;
-      }
-      while (nk < 8) {
-         int best = -1;
-         double best_area = -1;
-         for (int k = 0; k < nk; ++k) {
-            const BNode& c = bn[kids[k]];
-            if (c.count) continue;
-            const double a = c.box.area();
-            if (a > best_area) {
-               best_area = a;
-               best = k;
-            }
-         }
-         if (best < 0) break;
-         const int c = kids[best];
-         kids[best] = bn[c].left;
-         kids[nk++] = bn[c].right;
+         const int a = wide_cut[b];
+         cover(bn[b].left, a, kids, nk);
+         cover(bn[b].right, 8 - a, kids, nk);
       }
       Box nb;
       for (int k = 0; k < nk; ++k) nb.grow(bn[kids[k]].box);
@@ -329,11 +385,12 @@ struct WideBuilder {
       int n_int = 0;
       for (int sl = 0; sl < 8; ++sl) {
          const int c = slot_kid[sl];
-         if (c < 0) {  // empty slot: an inverted box no ray can hit
+         if (c < 0) {  // empty slot: an inverted box, and bit 7 of its meta byte set
             for (int a = 0; a < 3; ++a) {
                q[a][sl] = 255;
                q[3 + a][sl] = 0;
             }
+            meta[sl] = 0x80;
             continue;
          }
          const BNode& cn = bn[c];
@@ -408,6 +465,10 @@ bool build_bvh(const std::vector<HostPrim>& prims, BuildResult& out, bool wide) 
       // collapse into the 8-wide tree; this also reorders the primitives (each binary
       // leaf stays one contiguous block) and rewrites the binary leaves to match
       WideBuilder wb(b.nodes, b.idx);
+      wb.cost.resize(b.nodes.size());
+      wb.cut.resize(b.nodes.size());
+      wb.wide_cut.assign(b.nodes.size(), 1);
+      wb.plan(root);
       wb.out.resize(1);
       wb.order.reserve(n);
       wb.emit(0, root, 0);
@@ -660,7 +721,9 @@ bool check_wide(const BuildResult& br, const std::vector<HostPrim>& prims, uint3
    }
    int k = 0;
    for (int sl = 0; sl < 8; ++sl) {
-      if (q[sl] > q[24 + sl]) continue;  // empty slot
+      const bool empty = q[sl] > q[24 + sl];
+      if (empty != ((meta[sl] & 0x80u) != 0)) return false;  // the two marks of an empty slot agree
+      if (empty) continue;
       double clo[3], chi[3];
       for (int a = 0; a < 3; ++a) {
          clo[a] = p[a] + q[8 * a + sl] * sc[a];

@@ -619,19 +619,41 @@ __device__ __forceinline__ bool test_any(const SceneView& S, int i, V3 o, V3 d, 
    return isect_sdf(S, p, o, d, tmin, tmax, t);
 }
 
+// Empty slots of a wide node: bit 7 of a slot's meta byte (never set for a leaf, whose
+// offset and count take bits 0..6, nor for an internal child, whose byte is 0). The
+// multiply moves bits 7, 15, 23 and 31 of a word to bits 28..31 (the other partial
+// products land on distinct lower bits, so nothing carries into them).
+__device__ __forceinline__ uint32_t wide_empty_slots(uint32_t m0, uint32_t m1) {
+   const uint32_t lo = ((m0 & 0x80808080u) * 0x00204081u) >> 28;
+   const uint32_t hi = ((m1 & 0x80808080u) * 0x00204081u) >> 28;
+   return lo | (hi << 4);
+}
+
+// Moves bit i of an 8-bit slot mask to bit i ^ perm (visiting order of the ray's octant).
+__device__ __forceinline__ uint32_t wide_xor_slots(uint32_t x, uint32_t perm) {
+   uint32_t y = ((x & 0x55u) << 1) | ((x >> 1) & 0x55u);
+   x = (perm & 1u) ? y : x;
+   y = ((x & 0x33u) << 2) | ((x >> 2) & 0x33u);
+   x = (perm & 2u) ? y : x;
+   y = ((x & 0x0Fu) << 4) | (x >> 4);
+   return (perm & 4u) ? y : x;
+}
+
 // Tests the eight quantized child boxes of a wide node. Returns the hit internal
 // children as a mask in visiting order (bit i = slot i ^ perm) and the hit leaf children
 // as a slot mask. Each child plane is q * (2^e / d) + (p - o) / d; the exit distance is
 // widened by 2^-16 relative plus 2^-21 of the largest plane offset (see box2) so FP32
 // rounding never rejects a box the exact test accepts, nor one holding a primitive whose
-// computed hit is no farther than the closest so far.
+// computed hit is no farther than the closest so far. Per child the test is six byte
+// conversions, six multiply-adds, the entry/exit min-max and one compare into a slot mask;
+// which slots are internal, which are empty and the visiting order are applied to the
+// whole mask once per node.
 __device__ __forceinline__ void wide_children(const WNode& n, V3 o, V3 inv, uint32_t perm, float tmin, float tmax,
                                               uint32_t& hit_int, uint32_t& hit_leaf) {
    const uint32_t em = n.e_imask;
    const float sx = __uint_as_float((em & 0xFFu) << 23);
    const float sy = __uint_as_float(((em >> 8) & 0xFFu) << 23);
    const float sz = __uint_as_float(((em >> 16) & 0xFFu) << 23);
-   const uint32_t imask = em >> 24;
    const float ax = sx * inv.x, ay = sy * inv.y, az = sz * inv.z;
    const float bx = (n.px - o.x) * inv.x, by = (n.py - o.y) * inv.y, bz = (n.pz - o.z) * inv.z;
    const float slack = 4.76837158e-7f * fmaxf(fmaxf(fabsf(bx), fabsf(by)), fabsf(bz));
@@ -645,8 +667,7 @@ __device__ __forceinline__ void wide_children(const WNode& n, V3 o, V3 inv, uint
    const uint32_t fy0 = ny ? n.q[2] : n.q[8], fy1 = ny ? n.q[3] : n.q[9];
    const uint32_t nz0 = nz ? n.q[10] : n.q[4], nz1 = nz ? n.q[11] : n.q[5];
    const uint32_t fz0 = nz ? n.q[4] : n.q[10], fz1 = nz ? n.q[5] : n.q[11];
-   hit_int = 0;
-   hit_leaf = 0;
+   uint32_t raw = 0;
 #pragma unroll
    for (int sl = 0; sl < 8; ++sl) {
       const int sh = 8 * (sl & 3);  // one byte-to-float conversion per plane (v_cvt_f32_ubyteN)
@@ -658,15 +679,14 @@ __device__ __forceinline__ void wide_children(const WNode& n, V3 o, V3 inv, uint
       const float qfz = (float)(((sl < 4 ? fz0 : fz1) >> sh) & 0xFFu);
       const float tn = fmaxf(fmaxf(fmaf(qnx, ax, bx), fmaf(qny, ay, by)), fmaxf(fmaf(qnz, az, bz), tmin));
       const float tf = fminf(fminf(fmaf(qfx, ax, bx), fmaf(qfy, ay, by)), fminf(fmaf(qfz, az, bz), tmax));
-      // an empty slot stores lo = 255 > hi = 0 on every axis: its box is inverted
-      const bool valid = nx ? qfx <= qnx : qnx <= qfx;
-      if (valid && tn <= fmaf(tf, 1.0000153f, slack)) {
-         if ((imask >> sl) & 1u)
-            hit_int |= 1u << (sl ^ perm);
-         else
-            hit_leaf |= 1u << sl;
-      }
+      raw |= tn <= fmaf(tf, 1.0000153f, slack) ? (1u << sl) : 0u;
    }
+   // an empty slot stores lo = 255 > hi = 0 on every axis, an inverted box, and its meta
+   // byte has bit 7 set: it is dropped here even where the widening would let it pass
+   raw &= ~wide_empty_slots(n.meta[0], n.meta[1]);
+   const uint32_t imask = em >> 24;
+   hit_leaf = raw & ~imask;
+   hit_int = wide_xor_slots(raw & imask, perm);
 }
 
 __device__ __forceinline__ WNode load_wnode(const WNode* nodes, uint32_t i) {

@@ -77,7 +77,8 @@ struct alignas(16) WNode {
    uint32_t e_imask;     // ex | ey << 8 | ez << 16 (biased exponents) | imask << 24
    uint32_t child_base;  // first internal child
    uint32_t prim_base;   // first primitive slot of the leaf children
-   uint32_t meta[2];     // per slot: internal -> 0; leaf -> offset (bits 0..4) | (count-1) << 5
+   uint32_t meta[2];     // per slot: internal -> 0; leaf -> offset (bits 0..4) | (count-1) << 5;
+                         // empty -> 0x80 (its box is also inverted: lo 255, hi 0)
    uint32_t q[12];       // qlo_x[8], qlo_y[8], qlo_z[8], qhi_x[8], qhi_y[8], qhi_z[8] as bytes
 };
 

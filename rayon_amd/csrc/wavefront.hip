@@ -729,6 +729,8 @@ __device__ __forceinline__ uint32_t wide_next(uint32_t hit_int, uint32_t base, u
    return base + (uint32_t)__popc(imask & ((1u << sl) - 1u));
 }
 
+constexpr uint32_t kNoHeld = ~0u;  // no node set aside (masks take 16 bits)
+
 template <bool CNT, int STACK>
 __device__ void ext_rays_wide(const WfView& V, int level, int* lds, TravCount& tc) {
    unsigned* ctl = V.ctl + kCtl * level;
@@ -739,6 +741,7 @@ __device__ void ext_rays_wide(const WfView& V, int level, int* lds, TravCount& t
    WavePool pool = start_pool(&ctl[1], count);
    int slot = -1, pend = -1, sp = 0;
    uint32_t node = kWideDone;
+   uint32_t held_masks = kNoHeld;  // child test of the node set aside for the lane's leaves
    QueueRes res;
    auto queue_marches = [&]() {  // as in ext_rays_bin
       res.push(&mctl[0], V.qs0, pend >= 0, pend);
@@ -762,6 +765,7 @@ __device__ void ext_rays_wide(const WfView& V, int level, int* lds, TravCount& t
             best.sdf = -1;
             sp = 0;
             node = 0;
+            held_masks = kNoHeld;
          }
       }
       if (__ballot(slot >= 0) == 0) break;
@@ -770,17 +774,26 @@ __device__ void ext_rays_wide(const WfView& V, int level, int* lds, TravCount& t
          // Postponed leaves, as on binary trees: the hit leaf children of the first node that
          // has any are set aside (mask, meta words, primitive base) and the lane goes on with
          // its next node, until every lane of the wave holds leaves or has finished. A lane
-         // that meets another node with hit leaves meanwhile waits on it and visits it again
-         // after its leaves.
+         // that meets another node with hit leaves meanwhile waits on it, keeping that node's
+         // child masks, and takes the node up again after its leaves without testing its
+         // children again (masks found with a farther closest hit are a superset of the
+         // current ones, so the closest hit is the same).
          uint32_t pmask = 0, pm0 = 0, pm1 = 0, pbase = 0;
          bool held = false;
          for (;;) {
             if (node != kWideDone && !held) {
                const WNode n = load_wnode(wn, node);
                uint32_t hit_int, hit_leaf;
-               wide_children(n, r.o, r.inv, r.perm, 0.001f, best.t, hit_int, hit_leaf);
+               if (held_masks != kNoHeld) {
+                  hit_int = held_masks & 0xFFu;
+                  hit_leaf = held_masks >> 8;
+                  held_masks = kNoHeld;
+               } else {
+                  wide_children(n, r.o, r.inv, r.perm, 0.001f, best.t, hit_int, hit_leaf);
+               }
                if (hit_leaf && pmask) {
                   held = true;
+                  held_masks = hit_int | (hit_leaf << 8);
                } else {
                   if constexpr (CNT) ++tc.nodes;
                   if (hit_leaf) {
@@ -821,6 +834,7 @@ __device__ void shadow_rays_wide(const WfView& V, int level, int* lds, TravCount
    WavePool pool = start_pool(&sctl[1], count);
    int slot = -1, sp = 0;
    uint32_t node = kWideDone;
+   uint32_t held_masks = kNoHeld;  // as for extension rays
    const SceneView* S = nullptr;
    WideRay r;
    constexpr float tmin = 1e-4f, tmax = 1.0f - 1e-4f;
@@ -833,6 +847,7 @@ __device__ void shadow_rays_wide(const WfView& V, int level, int* lds, TravCount
             r = make_wideray(mk(V.so[slot]), mk(V.sd[slot]));
             sp = 0;
             node = 0;
+            held_masks = kNoHeld;
          }
       }
       if (__ballot(slot >= 0) == 0) break;
@@ -845,9 +860,16 @@ __device__ void shadow_rays_wide(const WfView& V, int level, int* lds, TravCount
             if (node != kWideDone && !held) {
                const WNode n = load_wnode(wn, node);
                uint32_t hit_int, hit_leaf;
-               wide_children(n, r.o, r.inv, r.perm, tmin, tmax, hit_int, hit_leaf);
+               if (held_masks != kNoHeld) {
+                  hit_int = held_masks & 0xFFu;
+                  hit_leaf = held_masks >> 8;
+                  held_masks = kNoHeld;
+               } else {
+                  wide_children(n, r.o, r.inv, r.perm, tmin, tmax, hit_int, hit_leaf);
+               }
                if (hit_leaf && pmask) {
                   held = true;
+                  held_masks = hit_int | (hit_leaf << 8);
                } else {
                   if constexpr (CNT) ++tc.nodes;
                   if (hit_leaf) {
@@ -870,7 +892,10 @@ __device__ void shadow_rays_wide(const WfView& V, int level, int* lds, TravCount
             for (int i = first; i < first + cnt && !occluded; ++i)
                occluded = test_any<CNT>(*S, i, r.o, r.d, tmin, tmax, &tc);
          }
-         if (occluded) node = kWideDone;
+         if (occluded) {
+            node = kWideDone;
+            held_masks = kNoHeld;
+         }
          if (node == kWideDone) {
             if (!occluded) {
                const float4 c = V.sc[slot];
