@@ -88,6 +88,12 @@ static float bits_as_float(uint32_t u) {
    return f;
 }
 
+static uint32_t ry_bits(float f) {
+   uint32_t u;
+   std::memcpy(&u, &f, 4);
+   return u;
+}
+
 }  // namespace ry
 
 using namespace ry;
@@ -409,8 +415,7 @@ rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* d, uint3
    }
    std::vector<MatRec> mats(d->num_materials);
    for (int i = 0; i < d->num_materials; ++i) mats[i] = pack_material(d->materials[i]);
-   std::vector<int> lslots;
-   std::vector<float> larea;
+   std::vector<LightRec> lights;
    for (size_t i = 0; i < n; ++i) {
       const HostPrim& p = prims[i];
       if (!(p.flags & F_EMITTER)) continue;
@@ -434,8 +439,8 @@ rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* d, uint3
       } else {
          continue;
       }
-      lslots.push_back(slot_of[i]);
-      larea.push_back((float)area);
+      const float4 em = mats[p.mat].m1;
+      lights.push_back({pack_rec(p, nullptr), {em.x, em.y, em.z, (float)area}});
    }
    std::vector<SdfRec> sdf;
    std::vector<float4> sdf_sph;
@@ -459,12 +464,11 @@ rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* d, uint3
    }
    SceneView& V = sc->view;
    MatRec* dm = nullptr;
-   int* dl = nullptr;
-   float* da = nullptr;
+   LightRec* dl = nullptr;
    SdfRec* ds = nullptr;
    float4* dsph = nullptr;
-   if ((st = upload(sc->bufs, mats, &dm)) != RAYON_OK || (st = upload(sc->bufs, lslots, &dl)) != RAYON_OK ||
-       (st = upload(sc->bufs, larea, &da)) != RAYON_OK || (st = upload(sc->bufs, sdf, &ds)) != RAYON_OK ||
+   if ((st = upload(sc->bufs, mats, &dm)) != RAYON_OK || (st = upload(sc->bufs, lights, &dl)) != RAYON_OK ||
+        (st = upload(sc->bufs, sdf, &ds)) != RAYON_OK ||
        (st = upload(sc->bufs, sdf_sph, &dsph)) != RAYON_OK)
       return fail(st);
    upload_ms += ms_between(c0, clock::now());
@@ -472,12 +476,11 @@ rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* d, uint3
    V.normals = dn;
    V.nodes = dnodes;
    V.mats = dm;
-   V.light_slots = dl;
-   V.light_area = da;
+   V.lights = dl;
    V.sdf = ds;
    V.sdf_sph = dsph;
    V.wnodes = br.wnodes.empty() ? nullptr : dw;
-   V.num_lights = (int)lslots.size();
+   V.num_lights = (int)lights.size();
    V.num_sdf = (int)sdf.size();
    V.root = br.root;
    V.num_prims = (int)n;
@@ -487,6 +490,20 @@ rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* d, uint3
    V.sdf_prim = -1;
    for (size_t i = 0; i < n; ++i)
       if (prims[i].kind == K_SDF) V.sdf_prim = slot_of[i];
+   // shading features beyond the lean set: kernels without them serve passes whose scenes
+   // use none of them
+   uint32_t feat = 0;
+   for (size_t i = 0; i < n; ++i) {
+      if (prims[i].kind == K_DISPLACED) feat |= SF_DISPLACED;
+      if (prims[i].kind == K_SDF) feat |= SF_SDF;
+   }
+   for (const MatRec& m : mats) {
+      const uint32_t w = ry_bits(m.m0.w), type = w & 0xFFu, pat = (w >> 8) & 0xFFu;
+      if (pat == RAYON_PATTERN_FIBONACCI_DOTS) feat |= SF_DOTS;
+      if (type == RAYON_MAT_THIN_FILM || type == RAYON_MAT_CLEAR_COAT) feat |= SF_LAYERED;
+      if (type == RAYON_MAT_ANISOTROPIC_METAL) feat |= SF_ANISO;
+   }
+   sc->features = feat;
    rayon_scene_info& I = sc->info;
    I.num_prims = (int64_t)n;
    I.num_nodes = num_nodes;

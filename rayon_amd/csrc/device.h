@@ -861,6 +861,8 @@ __device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
    return t * t * (3.0f - 2.0f * t);
 }
 
+// FT: the scene features (SceneFeature) whose code is compiled in.
+template <uint32_t FT = SF_ALL>
 __device__ inline Surf surface(const SceneView& S, const HitMin& h, V3 o, V3 d) {
    const PrimRec p = gload(S.prims + h.slot);
    Surf s;
@@ -888,7 +890,7 @@ __device__ inline Surf surface(const SceneView& S, const HitMin& h, V3 o, V3 d) 
       s.front = dot(d, n) < 0.0f;
       s.n = s.front ? n : -n;
       s.tangent = unit(mk(p.b));
-   } else if (kind == K_SDF) {  // central differences, sdf.hpp:499-507
+   } else if ((FT & SF_SDF) && kind == K_SDF) {  // central differences, sdf.hpp:499-507
       const float e = S.sdf_eps;
       const uint64_t m = sdf_point_mask(S, s.p);
       V3 g = mk(sdf_field_near(S, s.p + mk(e, 0, 0), m) - sdf_field_near(S, s.p - mk(e, 0, 0), m),
@@ -908,7 +910,7 @@ __device__ inline Surf surface(const SceneView& S, const HitMin& h, V3 o, V3 d) 
       s.n = s.front ? out : -out;
       V3 tg = cross(mk(0.0f, 1.0f, 0.0f), out);
       s.tangent = len2(tg) < 1e-6f ? mk(1.0f, 0.0f, 0.0f) : unit(tg);
-      if (kind == K_DISPLACED) {  // golf-ball displacement, shader_golf.cu:61-138
+      if ((FT & SF_DISPLACED) && kind == K_DISPLACED) {  // golf-ball displacement, shader_golf.cu:61-138
          V3 sp = s.p;
          V3 bo = unit(sp - c);
          float disp = dimple(bo);
@@ -1051,7 +1053,7 @@ struct Scatter {
 // apply_material's type mapping (cuda_raytracer.cuh:418-485) and scatter_material
 // (:613-850); RAYON_MAT_GGX_* are additions of this library. RNG is the path's stream
 // (Rng), or a table of given numbers for the material probe.
-template <class RNG>
+template <class RNG, uint32_t FT = SF_ALL>
 __device__ inline void scatter(const MatRec& M, V3 color, const JobView& J, const Surf& s, V3 dir, RNG& R,
                                Scatter& o) {
    const uint32_t mt = __float_as_uint(M.m0.w) & 0xFFu;
@@ -1122,7 +1124,7 @@ __device__ inline void scatter(const MatRec& M, V3 color, const JobView& J, cons
       return;
    case RAYON_MAT_ANISOTROPIC_METAL:
    case RAYON_MAT_GGX_CONDUCTOR: {
-      const bool aniso = mt == RAYON_MAT_ANISOTROPIC_METAL;
+      const bool aniso = (FT & SF_ANISO) && mt == RAYON_MAT_ANISOTROPIC_METAL;
       const float rough = M.m1.w;
       float ax, ay;
       if (aniso) {
@@ -1197,6 +1199,7 @@ __device__ inline void scatter(const MatRec& M, V3 color, const JobView& J, cons
       return;
    }
    case RAYON_MAT_THIN_FILM: {  // cuda_raytracer.cuh:753-809
+      if constexpr (!(FT & SF_LAYERED)) return;
       V3 ud = unit(dir);
       float ci = fmaxf(fabsf(dot(-ud, n)), 0.001f);
       float n0 = M.m2.w, n1 = M.m4.y, dd = M.m4.x;
@@ -1233,6 +1236,7 @@ __device__ inline void scatter(const MatRec& M, V3 color, const JobView& J, cons
       return;
    }
    case RAYON_MAT_CLEAR_COAT: {  // cuda_raytracer.cuh:810-846
+      if constexpr (!(FT & SF_LAYERED)) return;
       V3 ud = unit(dir);
       float ct = fminf(dot(-ud, n), 1.0f);
       float ior = M.m2.w > 1.0f ? M.m2.w : 1.5f;
@@ -1259,10 +1263,11 @@ __device__ inline void scatter(const MatRec& M, V3 color, const JobView& J, cons
 }
 
 // apply_procedural_pattern (cuda_raytracer.cuh:377-416)
+template <uint32_t FT = SF_ALL>
 __device__ inline V3 surface_color(const MatRec& M, V3 p, V3 center) {
    V3 base = mk(M.m0);
    const uint32_t pat = (__float_as_uint(M.m0.w) >> 8) & 0xFFu;
-   if (pat != RAYON_PATTERN_FIBONACCI_DOTS) return base;
+   if (!(FT & SF_DOTS) || pat != RAYON_PATTERN_FIBONACCI_DOTS) return base;
    V3 dir = unit(p - center);
    int N = (int)M.m4.z;
    float best = -1.0f;

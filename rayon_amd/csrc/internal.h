@@ -112,6 +112,7 @@ struct rayon_scene {
    std::vector<int64_t> orig_of_slot;
    std::vector<int32_t> order;  // slot -> primitive of the gathered description
    int32_t batch_path = RAYON_PATH_AUTO;  // path of this scene's jobs in batched passes
+   uint32_t features = ry::SF_ALL;        // SceneFeature bits its primitives and materials use
 };
 
 struct rayon_job {

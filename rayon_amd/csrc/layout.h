@@ -7,7 +7,7 @@
 //   nodes   : Node2[num_nodes], 64 B = one cache line per visited pair of children
 //   wnodes  : WNode[num_qnodes], 8-wide quantized nodes (80 B), built on request
 //   mats    : MatRec[num_mats], 96 B
-//   lights  : int slot + float area per emitter
+//   lights  : LightRec per emitter (64 B)
 #pragma once
 
 #include <stdint.h>
@@ -46,6 +46,22 @@ enum PrimFlag : uint32_t { F_VISIBLE = 1u << 24, F_NORMALS = 1u << 25, F_EMITTER
 // info = kind << 28 | flags | material (bits 0..23)
 struct alignas(16) PrimRec {
    float4 a, b, c;
+};
+
+// Shading features of a scene beyond the lean set (triangles, spheres, rectangles; Lambert,
+// mirror, metal, glass, dielectric, light, constant, normals, GGX conductor and
+// dielectric). Kernels instantiated without a feature leave its code out, which frees
+// registers; a pass takes such a kernel only when no scene in it uses the feature. The
+// code kept is the same source in every instantiation, so a sample's bits do not depend on
+// which one ran it.
+enum SceneFeature : uint32_t {
+   SF_DISPLACED = 1u,  // golf-ball sphere
+   SF_SDF = 2u,        // SDF field
+   SF_DOTS = 4u,       // Fibonacci-dots pattern
+   SF_LAYERED = 8u,    // thin film, clear coat
+   SF_ANISO = 16u,     // anisotropic metal
+   SF_ALL = 0xFFFFFFFFu,
+   SF_LEAN = 0u,
 };
 
 RY_HD inline uint32_t info_kind(uint32_t info) { return info >> 28; }
@@ -92,6 +108,13 @@ struct alignas(16) MatRec {
    float4 m5;  // pattern_color, 0
 };
 
+// Emitter record for next-event estimation: the emitter's primitive record (as in prims)
+// and its material's emission with the emitter's area, so a light sample is one load.
+struct alignas(16) LightRec {
+   PrimRec g;
+   float4 e;  // emission (the material's m1.xyz), area
+};
+
 // SDF primitive in rounded-box form: distance = |max(q,0)| + min(max q, 0) - rounding with
 // q = |a| - half, where a = p - center, or (|xz| - R, y, 0) for a torus.
 //   sphere r        : half (0,0,0), rounding r, R = -1
@@ -108,8 +131,7 @@ struct SceneView {
    const float4* normals;
    const Node2* nodes;
    const MatRec* mats;
-   const int* light_slots;
-   const float* light_area;
+   const LightRec* lights;  // NEE emitter table
    const SdfRec* sdf;
    const float4* sdf_sph;  // per SDF primitive: bounding sphere centre and radius
    const WNode* wnodes;  // 8-wide quantized tree, or NULL to traverse the binary pair nodes

@@ -41,8 +41,8 @@ __device__ inline void nee_request(const JobView& J, const Surf& s, V3 color, V3
    const SceneView& S = J.scene;
    float ul = R.next(), ua = R.next(), ub = R.next();
    int li = min((int)(ul * (float)S.num_lights), S.num_lights - 1);
-   const int slot = gload(S.light_slots + li);
-   const PrimRec lp = gload(S.prims + slot);
+   const LightRec lr = gload(S.lights + li);
+   const PrimRec& lp = lr.g;
    if constexpr (CNT) ++pc.light_prims;
    V3 q, nl;
    bool one_sided;
@@ -57,10 +57,9 @@ __device__ inline void nee_request(const JobView& J, const Surf& s, V3 color, V3
    sh.valid = cs > 0.0f && cl > 0.0f;
    if (sh.valid) {
       ++pc.shadow;
-      const MatRec LM = gload(S.mats + info_mat(__float_as_uint(lp.a.w)));
       if constexpr (CNT) ++pc.mats;
-      float g = cs * cl / d2 * gload(S.light_area + li) * (float)S.num_lights * (1.0f / kPiF);
-      sh.contrib = mul_rn(T, (g * J.light_intensity) * (color * mk(LM.m1)));
+      float g = cs * cl / d2 * lr.e.w * (float)S.num_lights * (1.0f / kPiF);
+      sh.contrib = mul_rn(T, (g * J.light_intensity) * (color * mk(lr.e)));
    }
 }
 
@@ -69,7 +68,7 @@ __device__ inline void nee_request(const JobView& J, const Surf& s, V3 color, V3
 // shadow request (NEE on Lambertian surfaces) and Russian roulette from bounce 1.
 // Returns true while the path continues with ps.o / ps.d. Shared by the megakernel and
 // the wavefront shade stage so both compute identical per-sample radiance.
-template <bool CNT>
+template <bool CNT, uint32_t FT = SF_ALL>
 __device__ inline bool shade_step(const JobView& J, const HitMin& h, int bounce, PathState& ps, Rng& R,
                                   ShadowReq& sh, PathCounts& pc) {
    const SceneView& S = J.scene;
@@ -79,16 +78,16 @@ __device__ inline bool shade_step(const JobView& J, const HitMin& h, int bounce,
       ps.L = ps.L + ps.T * sky(J, ps.d);
       return false;
    }
-   const Surf s = surface(S, h, ps.o, ps.d);
+   const Surf s = surface<FT>(S, h, ps.o, ps.d);
    if (!(s.info & F_VISIBLE) && bounce == 0) {  // cuda_raytracer.cuh:879-883
       ps.o = s.p + 0.01f * ps.d;
       return true;
    }
    const MatRec M = gload(S.mats + info_mat(s.info));
    if constexpr (CNT) ++pc.mats;
-   const V3 color = surface_color(M, s.p, s.center);
+   const V3 color = surface_color<FT>(M, s.p, s.center);
    Scatter sc;
-   scatter(M, color, J, s, ps.d, R, sc);
+   scatter<Rng, FT>(M, color, J, s, ps.d, R, sc);
    // An emitter that NEE samples (F_EMITTER) was already counted by the shadow ray of the
    // Lambertian bounce that led here; every other emitter is reached only this way.
    if (len2(sc.emitted) > 0.0f && !(nee && ps.prev_lambert && (s.info & F_EMITTER))) ps.L = ps.L + ps.T * sc.emitted;

@@ -4,16 +4,23 @@
 // This is the MI355X replacement for running renderAccKernel once per job
 // (shaders/render_acc_kernel.cu:128-282): the per-thread megakernel loop is split into
 // stages so that each stage keeps few registers (traversal needs the LDS stack, shading
-// does not), waves stay full as paths terminate (queues are compacted with one
-// wave-aggregated atomic per 64 lanes), and jobs of different sizes share every launch.
+// does not), waves stay full as paths terminate, and jobs of different sizes share every
+// launch.
 //
 // Work of one pass = for every job, `samples` samples of every pixel, enumerated
 // job-major, then sample, then pixel. It is processed in rounds of at most `capacity`
-// paths; path slot i of a round holds work item w0 + i. Queues carry slot indices. The
-// extend/shade/shadow kernels are persistent: a fixed grid of workgroups whose waves pull
-// 64 queue entries at a time until the queue (whose length the previous stage left in
-// device memory) is drained, so a whole pass needs no host synchronisation and can be
-// captured once into a hipGraph and replayed.
+// paths; path slot i of a round holds work item w0 + i, and its radiance is summed at
+// index i. The live paths of a bounce level are packed densely: the shade stage writes
+// each continuing path's state, with its slot, to the next position of the other of two
+// state sets, and each shadow ray to the next position of the shadow arrays, so every
+// stage reads its rays' state contiguously however few paths remain (indexing by slot
+// leaves deep levels reading one live record per cache line). Positions come from
+// per-wave reservations (one atomic per reservation); positions a wave reserved and did
+// not fill are holes, marked so the next stage skips them. The extend/shade/shadow
+// kernels are persistent: a fixed grid of workgroups whose waves pull 64 positions at a
+// time until the level (whose length the previous stage left in device memory) is
+// drained, so a whole pass needs no host synchronisation and can be captured once into a
+// hipGraph and replayed.
 //
 // Determinism: a path's random numbers depend only on (job seed, pixel, sample); its
 // radiance is accumulated into the pixel by the accumulate stage in sample order. A job
@@ -46,26 +53,25 @@ constexpr int kSdfSlice = 8;   // steps of the first marching slice; slice k tak
 constexpr int kSdfSlices = 8;  // slices per level at most (the last one runs to the end)
 constexpr long long kMegaSceneBytes = 8ll << 20;  // automatic path choice: megakernel up to this scene size
 constexpr int kCtl = 8;  // control words per level
-constexpr uint32_t kSkip = 1u;        // ids.w bit: pixel converged, no path this sample
-constexpr uint32_t kPrevLambert = 2u; // ids.w bit: the last scatter was Lambertian
-// ids.w bits 8..19: extension segments, bits 20..31: shadow segments
+// Path flags, kept in the w word of the throughput record
+constexpr uint32_t kPrevLambert = 2u;  // the last scatter was Lambertian
+constexpr uint32_t kHole = 0xFFFFFFFFu;  // job word of an unused position
 
 struct WfView {
-   float4* o;     // origin, rng state (bits)
-   float4* d;     // direction
-   float4* thr;   // throughput
-   float4* rad;   // radiance
-   uint4* ids;    // job, pixel, sample, flags
-   float4* hit;   // t, primitive slot (bits), u, v
-   float4* so;    // shadow ray of a path slot: origin
-   float4* sd;    // direction
-   float4* sc;    // contribution if unoccluded
-   int* q0;       // extension queues (path slots), alternating by level
-   int* q1;
-   int* sq;       // shadow queue (path slots)
-   unsigned* ctl;           // per level (stride kCtl): queue count, extend head, shade head,
-                            // shadow count, shadow head
-   int* qs0;                // SDF marching queues (path slots), alternating by slice
+   // Path state of the rays of a level, at dense positions; set (level & 1). The job and
+   // the flags travel in the otherwise unused w words.
+   float4* o[2];    // origin, rng state (bits)
+   float4* d[2];    // direction, job (bits; kHole at an unused position)
+   float4* thr[2];  // throughput, path flags (bits)
+   int* ps[2];      // path slot (work item index in the round), -1 at an unused position
+   float4* rad;     // radiance, by path slot
+   float4* hit;     // closest hit of the level's ray at each position: t, primitive slot (bits), u, v
+   float4* so;      // shadow rays of a level at dense positions: origin, path slot (bits; -1 unused)
+   float4* sd;      // direction, job (bits)
+   float4* sc;      // contribution if unoccluded
+   unsigned* ctl;           // per level (stride kCtl): positions, extend head, shade head,
+                            // shadow positions, shadow head
+   int* qs0;                // SDF marching queues (positions of the level), alternating by slice
    int* qs1;
    unsigned* sctl;          // per level and slice (stride 2): marching queue count, head
    const JobView* jobs;
@@ -99,7 +105,7 @@ struct Wave {
    int opt_chunk = 4;
    int opt_shade_chunk = 16;
    int opt_refill = 16;
-   int occ_trace = 0, occ_trace_wide = 0, occ_trace_short = 0, occ_trace_wide_short = 0, occ_shade = 0, occ_sdf = 0, occ_sdf_rest = 0;
+   int occ_trace = 0, occ_trace_wide = 0, occ_trace_short = 0, occ_trace_wide_short = 0, occ_shade = 0, occ_shade_lean = 0, occ_sdf = 0, occ_sdf_rest = 0;
    struct Pending {
       int stage;
       hipEvent_t e0, e1;
@@ -147,6 +153,10 @@ constexpr int kMarchRes = 128;  // marching-queue positions a trace wave reserve
 // wave of a trace launch may leave at most kMarchRes - 1 reserved positions unfilled
 // (at most 32 workgroups of 4 waves per CU).
 constexpr long long kMarchHeadroomPerCu = 32ll * (kWfBlock / 64) * kMarchRes;
+// Positions beyond a level's paths that its arrays must hold: every wave of a shade launch
+// may leave at most the 256 positions of one reservation unfilled (at most 32 workgroups
+// of 4 waves per CU).
+constexpr long long kPosHeadroomPerCu = 32ll * (kWfBlock / 64) * 256;
 
 // Appends to a marching queue through positions the wave reserves kMarchRes at a time, so
 // the queue's head counter, shared by every wave of a launch, takes one atomic per
@@ -176,6 +186,42 @@ struct QueueRes {
       for (unsigned i = next + (unsigned)lane_id(); i < end; i += 64) q[i] = -1;
    }
 };
+
+// Positions for a wave's outputs (continuing paths, shadow rays) from reservations of
+// `res` positions (a multiple of 64, wave-uniform) taken with one atomic on the level's
+// shared counter; what a wave reserved and did not fill it marks unused when it ends.
+struct PosRes {
+   unsigned next = 0, end = 0;  // wave-uniform
+   // Position of each lane with `pred` set, else -1.
+   __device__ __forceinline__ int take(unsigned* head, bool pred, unsigned res) {
+      const unsigned long long m = __ballot(pred);
+      if (m == 0) return -1;
+      const unsigned k = (unsigned)__popcll(m);
+      const unsigned rank = (unsigned)__popcll(m & ((1ull << lane_id()) - 1ull));
+      const unsigned have = end - next;
+      int pos = (pred && rank < have) ? (int)(next + rank) : -1;
+      if (k > have) {
+         unsigned base = 0;
+         if (lane_id() == 0) base = atomicAdd(head, res);
+         base = __shfl(base, 0);
+         if (pred && rank >= have) pos = (int)(base + (rank - have));
+         next = base + (k - have);
+         end = base + res;
+      } else {
+         next += k;
+      }
+      return pos;
+   }
+};
+
+// Reservation size for the outputs of a level of `count` positions: 64 per wave while
+// waves have little work (holes then stay a few percent of the next level), up to 256
+// when each wave handles more than 16 batches of 64.
+__device__ __forceinline__ unsigned pos_res(unsigned count) {
+   const unsigned waves = gridDim.x * (blockDim.x >> 6);
+   const unsigned per = count / (64u * 16u * waves);
+   return 64u * min(max(per, 1u), 4u);
+}
 
 // Persistent-worker loop: each wave claims `chunk` batches of 64 queue entries with one
 // atomic on the queue head and runs body(index) on each; body sees indices >= count on
@@ -244,10 +290,11 @@ __device__ __forceinline__ int find_job(const WfView& V, long long w) {
 // stages
 // ---------------------------------------------------------------------------------
 // dense: no job of the pass can skip a pixel (no adaptive test), so every slot is live
-// and the level-0 queue is the identity.
+// and its level-0 position is the slot itself; otherwise the live paths are packed.
 __global__ __launch_bounds__(kWfBlock) void k_wf_generate(WfView V, long long w0, int n, int dense) {
    const int i = blockIdx.x * kWfBlock + threadIdx.x;
    bool live = false;
+   float4 o4, d4;
    if (i < n) {
       const long long w = w0 + i;
       const int j = find_job(V, w);
@@ -257,36 +304,37 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_generate(WfView V, long long w0
       split_work(w - V.base[j], units, s, p);
       if (J.nlive >= 0) p = J.live[p];
       const int sample = J.sample_base + s;
-      uint32_t flags = 0;
-      if (J.counts[p] < 0) {
-         flags = kSkip;
-      } else {
+      if (J.counts[p] >= 0) {  // a converged pixel takes no path (the accumulate stage skips it)
          Rng R{counter_init(J.seed_lo, J.seed_hi, (uint32_t)p, (uint32_t)sample), false};
          V3 o, d;
          camera_ray(J, p % J.width, p / J.width, R, o, d);
-         V.o[i] = make_float4(o.x, o.y, o.z, __uint_as_float(R.s));
-         V.d[i] = make_float4(d.x, d.y, d.z, 0.0f);
-         V.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+         o4 = make_float4(o.x, o.y, o.z, __uint_as_float(R.s));
+         d4 = make_float4(d.x, d.y, d.z, __uint_as_float((uint32_t)j));
          V.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
          live = true;
       }
-      V.ids[i] = make_uint4((uint32_t)j, (uint32_t)p, (uint32_t)sample, flags);
    }
+   int pos = i;
    if (dense) {
-      if (i < n) V.q0[i] = i;
       if (i == 0) V.ctl[0] = (unsigned)n;
-      return;
+   } else {
+      pos = wave_push(&V.ctl[0], live);
    }
-   const int pos = wave_push(&V.ctl[0], live);
-   if (live) V.q0[pos] = i;
+   if (live) {
+      V.o[0][pos] = o4;
+      V.d[0][pos] = d4;
+      V.thr[0][pos] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
+      V.ps[0][pos] = i;
+   }
 }
 
 // SDF marching stage of extension rays: slice `it` takes up to kSdfSlice << it steps of
 // every ray in its queue (the last slice runs to the end); rays still marching go to the next
 // slice's queue, so each slice's waves hold only live rays. A hit replaces the ray's hit
 // record (the march ends at the closest other hit, and an exact tie goes to the field,
-// which has the largest original index). The march state lives in the shadow-ray arrays
-// of the slot, free between the trace and shade stages.
+// which has the largest original index). Queues hold the rays' positions in the level; the
+// march state lives in the shadow-ray arrays at the same position, free between the trace
+// stage (which consumed the previous level's shadow rays) and the shade stage.
 template <bool CNT>
 __global__ __launch_bounds__(kWfBlock) void k_wf_sdf(WfView V, int level, int it, int last) {
    unsigned long long n_begin = 0, n_steps = 0, n_evals = 0, n_slots = 0;
@@ -302,8 +350,9 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_sdf(WfView V, int level, int it
       unsigned taken = 0;
       if (idx < count) slot = qin[idx];  // -1: a position a trace wave reserved and left unused
       if (slot >= 0) {
-         const SceneView& S = V.jobs[V.ids[slot].x].scene;
-         const V3 o = mk(V.o[slot]), d = mk(V.d[slot]);
+         const float4 d4 = V.d[level & 1][slot];
+         const SceneView& S = V.jobs[__float_as_uint(d4.w)].scene;
+         const V3 o = mk(V.o[level & 1][slot]), d = mk(d4);
          SdfMarch m;
          bool live = true;
          if (it == 0) {
@@ -425,9 +474,10 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_sdf_rest(WfView V, int level) {
          const int s = idx >= 0 ? V.qs1[idx] : -1;  // -1 also marks unused reserved positions
          if (s >= 0) {
             slot = s;
-            S = &V.jobs[V.ids[slot].x].scene;
-            o = mk(V.o[slot]);
-            d = mk(V.d[slot]);
+            const float4 d4 = V.d[level & 1][slot];
+            S = &V.jobs[__float_as_uint(d4.w)].scene;
+            o = mk(V.o[level & 1][slot]);
+            d = mk(d4);
             const float4 a = V.so[slot], b = V.sc[slot];
             m.tw = a.x;
             m.tend = a.y;
@@ -470,11 +520,12 @@ template <bool CNT, int STACK>
 __device__ void ext_rays_bin(const WfView& V, int level, int* lds, TravCount& tc) {
    unsigned* ctl = V.ctl + kCtl * level;
    const unsigned count = ctl[0];
-   const int* q = (level & 1) ? V.q1 : V.q0;
+   const float4* O = V.o[level & 1];
+   const float4* D = V.d[level & 1];
    unsigned* mctl = V.sctl + 2 * (kSdfSlices + 1) * level;
    const unsigned step = 64u * (unsigned)fit_chunk(count, V.chunk);
    WavePool pool = start_pool(&ctl[1], count);
-   int slot = -1, cur = kSentinel;
+   int slot = -1, cur = kSentinel;  // slot: the lane's ray's position in the level
    // slot of a finished ray whose SDF march is still to be queued. The lanes' marches are
    // queued together before each refill, into positions the wave reserved in the marching
    // queue kMarchRes at a time (one atomic per reservation, not per refill): the queue head
@@ -495,13 +546,14 @@ __device__ void ext_rays_bin(const WfView& V, int level, int* lds, TravCount& tc
       if (!pool.drained || pool.next != pool.end) {
          if (__popcll(__ballot(slot < 0)) >= V.refill) queue_marches();
          const long long idx = pool.refill(&ctl[1], count, step, slot < 0, V.refill);
-         if (idx >= 0) {
-            slot = q[idx];
-            const SceneView& S = V.jobs[V.ids[slot].x].scene;
+         const float4 d4 = idx >= 0 ? D[idx] : make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kHole));
+         if (__float_as_uint(d4.w) != kHole) {
+            slot = (int)idx;
+            const SceneView& S = V.jobs[__float_as_uint(d4.w)].scene;
             nodes = S.nodes;
             prims = S.prims;
             cur = S.root;
-            r = make_rayinv(mk(V.o[slot]), mk(V.d[slot]));
+            r = make_rayinv(mk(O[idx]), mk(d4));
             best.t = 3.402823466e38f;
             best.slot = -1;
             best.orig = -1;
@@ -511,7 +563,10 @@ __device__ void ext_rays_bin(const WfView& V, int level, int* lds, TravCount& tc
             st.top = kSentinel;
          }
       }
-      if (__ballot(slot >= 0) == 0) break;
+      if (__ballot(slot >= 0) == 0) {  // the entries handed out may all have been holes
+         if (pool.drained && pool.next == pool.end) break;
+         continue;
+      }
       if (slot >= 0) {
          bool finished;
          // Internal nodes until every lane of the wave holds a leaf or has finished: the
@@ -603,16 +658,21 @@ __device__ void shadow_rays_bin(const WfView& V, int level, int* lds, TravCount&
    for (;;) {
       if (!pool.drained || pool.next != pool.end) {
          const long long idx = pool.refill(&sctl[1], count, step, slot < 0, V.refill);
-         if (idx >= 0) {
-            slot = V.sq[idx];
-            S = &V.jobs[V.ids[slot].x].scene;
+         const float4 o4 = idx >= 0 ? V.so[idx] : make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+         if (__float_as_int(o4.w) >= 0) {
+            slot = (int)idx;  // the shadow ray's position; o4.w holds its path's slot
+            const float4 d4 = V.sd[idx];
+            S = &V.jobs[__float_as_uint(d4.w)].scene;
             cur = S->root;
-            r = make_rayinv(mk(V.so[slot]), mk(V.sd[slot]));
+            r = make_rayinv(mk(o4), mk(d4));
             st.n = 0;
             st.top = kSentinel;
          }
       }
-      if (__ballot(slot >= 0) == 0) break;
+      if (__ballot(slot >= 0) == 0) {  // the entries handed out may all have been holes
+         if (pool.drained && pool.next == pool.end) break;
+         continue;
+      }
       if (slot >= 0) {
          bool occluded = false;
          int leaf = kSentinel;  // postponed leaves, as for extension rays
@@ -664,10 +724,11 @@ __device__ void shadow_rays_bin(const WfView& V, int level, int* lds, TravCount&
          }
          if (cur == kSentinel) {
             if (!occluded) {
+               const int ps = __float_as_int(V.so[slot].w);
                const float4 c = V.sc[slot];
-               const float4 rd = V.rad[slot];
+               const float4 rd = V.rad[ps];
                const V3 L = mk(rd) + mk(c);
-               V.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+               V.rad[ps] = make_float4(L.x, L.y, L.z, 0.0f);
             }
             slot = -1;
          }
@@ -735,11 +796,12 @@ template <bool CNT, int STACK>
 __device__ void ext_rays_wide(const WfView& V, int level, int* lds, TravCount& tc) {
    unsigned* ctl = V.ctl + kCtl * level;
    const unsigned count = ctl[0];
-   const int* q = (level & 1) ? V.q1 : V.q0;
+   const float4* O = V.o[level & 1];
+   const float4* D = V.d[level & 1];
    unsigned* mctl = V.sctl + 2 * (kSdfSlices + 1) * level;
    const unsigned step = 64u * (unsigned)fit_chunk(count, V.chunk);
    WavePool pool = start_pool(&ctl[1], count);
-   int slot = -1, pend = -1, sp = 0;
+   int slot = -1, pend = -1, sp = 0;  // slot: the lane's ray's position in the level
    uint32_t node = kWideDone;
    uint32_t held_masks = kNoHeld;  // child test of the node set aside for the lane's leaves
    QueueRes res;
@@ -754,10 +816,11 @@ __device__ void ext_rays_wide(const WfView& V, int level, int* lds, TravCount& t
       if (!pool.drained || pool.next != pool.end) {
          if (__popcll(__ballot(slot < 0)) >= V.refill) queue_marches();
          const long long idx = pool.refill(&ctl[1], count, step, slot < 0, V.refill);
-         if (idx >= 0) {
-            slot = q[idx];
-            S = &V.jobs[V.ids[slot].x].scene;
-            r = make_wideray(mk(V.o[slot]), mk(V.d[slot]));
+         const float4 d4 = idx >= 0 ? D[idx] : make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kHole));
+         if (__float_as_uint(d4.w) != kHole) {
+            slot = (int)idx;
+            S = &V.jobs[__float_as_uint(d4.w)].scene;
+            r = make_wideray(mk(O[idx]), mk(d4));
             best.t = 3.402823466e38f;
             best.slot = -1;
             best.orig = -1;
@@ -768,7 +831,10 @@ __device__ void ext_rays_wide(const WfView& V, int level, int* lds, TravCount& t
             held_masks = kNoHeld;
          }
       }
-      if (__ballot(slot >= 0) == 0) break;
+      if (__ballot(slot >= 0) == 0) {  // the entries handed out may all have been holes
+         if (pool.drained && pool.next == pool.end) break;
+         continue;
+      }
       if (slot >= 0) {
          const WNode* wn = S->wnodes;
          // Postponed leaves, as on binary trees: the hit leaf children of the first node that
@@ -841,16 +907,21 @@ __device__ void shadow_rays_wide(const WfView& V, int level, int* lds, TravCount
    for (;;) {
       if (!pool.drained || pool.next != pool.end) {
          const long long idx = pool.refill(&sctl[1], count, step, slot < 0, V.refill);
-         if (idx >= 0) {
-            slot = V.sq[idx];
-            S = &V.jobs[V.ids[slot].x].scene;
-            r = make_wideray(mk(V.so[slot]), mk(V.sd[slot]));
+         const float4 o4 = idx >= 0 ? V.so[idx] : make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+         if (__float_as_int(o4.w) >= 0) {
+            slot = (int)idx;  // the shadow ray's position; o4.w holds its path's slot
+            const float4 d4 = V.sd[idx];
+            S = &V.jobs[__float_as_uint(d4.w)].scene;
+            r = make_wideray(mk(o4), mk(d4));
             sp = 0;
             node = 0;
             held_masks = kNoHeld;
          }
       }
-      if (__ballot(slot >= 0) == 0) break;
+      if (__ballot(slot >= 0) == 0) {  // the entries handed out may all have been holes
+         if (pool.drained && pool.next == pool.end) break;
+         continue;
+      }
       if (slot >= 0) {
          const WNode* wn = S->wnodes;
          bool occluded = false;
@@ -898,10 +969,11 @@ __device__ void shadow_rays_wide(const WfView& V, int level, int* lds, TravCount
          }
          if (node == kWideDone) {
             if (!occluded) {
+               const int ps = __float_as_int(V.so[slot].w);
                const float4 c = V.sc[slot];
-               const float4 rd = V.rad[slot];
+               const float4 rd = V.rad[ps];
                const V3 L = mk(rd) + mk(c);
-               V.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+               V.rad[ps] = make_float4(L.x, L.y, L.z, 0.0f);
             }
             slot = -1;
          }
@@ -915,15 +987,16 @@ __device__ void shadow_rays_drain(const WfView& V, int level, int* lds, TravCoun
    unsigned* sctl = V.ctl + kCtl * level + 3;
    const unsigned count = sctl[0];
    drain(&sctl[1], count, V.chunk, [&](unsigned idx) {
-      if (idx < count) {
-         const int slot = V.sq[idx];
-         const float4 o = V.so[slot], d = V.sd[slot];
-         const SceneView& S = V.jobs[V.ids[slot].x].scene;
+      const float4 o = idx < count ? V.so[idx] : make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+      const int ps = __float_as_int(o.w);  // the path's slot; -1 at an unused position
+      if (ps >= 0) {
+         const float4 d = V.sd[idx];
+         const SceneView& S = V.jobs[__float_as_uint(d.w)].scene;
          if (!any_hit<kWfBlock, CNT, LAYOUT, STACK>(S, mk(o), mk(d), 1e-4f, 1.0f - 1e-4f, lds, &ts)) {
-            const float4 c = V.sc[slot];
-            const float4 r = V.rad[slot];
+            const float4 c = V.sc[idx];
+            const float4 r = V.rad[ps];
             const V3 L = mk(r) + mk(c);
-            V.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+            V.rad[ps] = make_float4(L.x, L.y, L.z, 0.0f);
          }
       }
    });
@@ -963,15 +1036,16 @@ __global__ __launch_bounds__(kWfBlock, (LAYOUT == L_BIN && STACK == kShortStack)
    if (ext_level >= 0) {
       unsigned* ctl = V.ctl + kCtl * ext_level;
       const unsigned count = ctl[0];
-      const int* q = (ext_level & 1) ? V.q1 : V.q0;
+      const float4* O = V.o[ext_level & 1];
+      const float4* D = V.d[ext_level & 1];
       unsigned* mctl = V.sctl + 2 * (kSdfSlices + 1) * ext_level;
       drain(&ctl[1], count, V.chunk, [&](unsigned idx) {
-         int slot = -1;
+         const int slot = (int)idx;  // the ray's position in the level
          bool march = false;
-         if (idx < count) {
-            slot = q[idx];
-            const float4 o = V.o[slot], d = V.d[slot];
-            const SceneView& S = V.jobs[V.ids[slot].x].scene;
+         const float4 d = idx < count ? D[idx] : make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kHole));
+         if (__float_as_uint(d.w) != kHole) {
+            const float4 o = O[idx];
+            const SceneView& S = V.jobs[__float_as_uint(d.w)].scene;
             const HitMin h = closest_hit<kWfBlock, CNT, LAYOUT, true, STACK>(S, mk(o), mk(d), 0.001f,
                                                                             3.402823466e38f, stack + threadIdx.x, &te);
             V.hit[slot] = make_float4(h.t, __int_as_float(h.slot), h.u, h.v);
@@ -997,33 +1071,51 @@ __global__ __launch_bounds__(kWfBlock, (LAYOUT == L_BIN && STACK == kShortStack)
 
 constexpr int kMaxChunk = 16;
 
-// Appends a wave's staged queue entries with one atomic and coalesced stores.
-__device__ __forceinline__ void flush(const int* buf, int n, unsigned* counter, int* dst) {
-   if (n == 0) return;
-   unsigned b = 0;
-   if (lane_id() == 0) b = atomicAdd(counter, (unsigned)n);
-   b = __shfl(b, 0);
-   for (int i = lane_id(); i < n; i += 64) dst[b + i] = buf[i];
+// Copy of a record at a wave-uniform address, every 32-bit word moved to scalar registers
+// (words the caller never reads are dropped with their loads).
+template <class T>
+__device__ __forceinline__ T uniform_copy(const T* p) {
+   static_assert(sizeof(T) % 4 == 0, "records are whole 32-bit words");
+   constexpr int n = (int)(sizeof(T) / 4);
+   const uint32_t* s = (const uint32_t*)p;
+   uint32_t w[n];
+#pragma unroll
+   for (int k = 0; k < n; ++k) w[k] = (uint32_t)__builtin_amdgcn_readfirstlane((int)s[k]);
+   T v;
+   __builtin_memcpy(&v, w, sizeof(T));
+   return v;
 }
 
-// Shade stage: one bounce (shade_step) per queued path. The surviving paths and the
-// shadow requests of each wave's claimed chunk are staged in LDS and appended to the
-// next-level and shadow queues with one atomic each per chunk.
-template <bool CNT>
+// Adds a wave's segment tallies of one job to the job's counters (extension + shadow
+// segments, shadow segments), one atomic pair per wave and job.
+__device__ __forceinline__ void flush_segments(const WfView& V, int job, unsigned long long& ext,
+                                               unsigned long long& sh) {
+   if (job >= 0 && ext + sh != 0 && lane_id() == 0) {
+      unsigned long long* ctr = V.jobs[job].counters + 8 * (blockIdx.x % kCounterSlots);
+      atomicAdd(&ctr[0], ext + sh);
+      atomicAdd(&ctr[1], sh);
+   }
+   ext = sh = 0;
+}
+
+// Shade stage: one bounce (shade_step) per path of the level. Each continuing path's state
+// goes, with its slot, to the next position of the other state set, and each shadow ray to
+// the next position of the shadow arrays; positions come from per-wave reservations.
+template <bool CNT, uint32_t FT>
 __global__ __launch_bounds__(kWfBlock, 4) void k_wf_shade(WfView V, int level) {
-   __shared__ int stage_next[kWfBlock / 64][64 * kMaxChunk];
-   __shared__ int stage_sh[kWfBlock / 64][64 * kMaxChunk];
-   const int w = threadIdx.x >> 6, lane = lane_id();
+   const int lane = lane_id();
+   const int cur = level & 1, nxt = cur ^ 1;
    unsigned* ctl = V.ctl + kCtl * level;
    const unsigned count = ctl[0];
-   const int* q = (level & 1) ? V.q1 : V.q0;
-   int* qn = (level & 1) ? V.q0 : V.q1;
    unsigned* ctl_next = V.ctl + kCtl * (level + 1);
    unsigned* sctl = ctl + 3;
-   const int chunk = fit_chunk(count, min(V.shade_chunk, kMaxChunk));
-   const unsigned long long lt = (1ull << lane) - 1ull;
+   const int chunk = fit_chunk(count, V.shade_chunk);
+   const unsigned res = pos_res(count);
+   PosRes rn, rs;  // next-level and shadow positions
    PathCounts pc;
-   unsigned seen = count;  // skip the claims of an empty or drained queue, as drain() does
+   int tally_job = -1;  // segments shaded by this wave, not yet added to the job's counters
+   unsigned long long tally_ext = 0, tally_sh = 0;
+   unsigned seen = count;  // skip the claims of an empty or drained level, as drain() does
    if (count != 0 && lane == 0) seen = __hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
    bool more = __shfl(seen, 0) < count;
    while (more) {
@@ -1032,62 +1124,84 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_wf_shade(WfView V, int level) {
       base = __shfl(base, 0);
       if (base >= count) break;
       more = base + 64u * (unsigned)chunk < count;
-      int n_next = 0, n_sh = 0;
       for (int k = 0; k < chunk; ++k) {
          const unsigned bk = base + 64u * (unsigned)k;
          if (bk >= count) break;
          const unsigned idx = bk + (unsigned)lane;
-         bool cont = false, shadow = false;
          int slot = -1;
-         if (idx < count) {
-            slot = q[idx];
-            uint4 id = V.ids[slot];
-            const JobView& J = V.jobs[id.x];
-            const float4 o = V.o[slot], d = V.d[slot], t = V.thr[slot], hf = V.hit[slot];
-            // the step's radiance starts at zero and is added to the slot's only if it
-            // adds any (one term per step: rad + (0 + T*e) is rad + T*e bit for bit)
-            PathState ps{mk(o), mk(d), mk(t), mk(0.0f, 0.0f, 0.0f), (id.w & kPrevLambert) != 0};
-            Rng R{__float_as_uint(o.w), false};
-            HitMin h;
-            h.t = hf.x;
-            h.slot = __float_as_int(hf.y);
-            h.u = hf.z;
-            h.v = hf.w;
-            ShadowReq sh;
-            cont = shade_step<CNT>(J, h, level, ps, R, sh, pc);
-            cont = cont && level + 1 < J.max_depth;
-            shadow = sh.valid;
-            id.w = (id.w & ~kPrevLambert) | (ps.prev_lambert ? kPrevLambert : 0u);
-            id.w += 1u << 8;
-            if (shadow) {
-               id.w += 1u << 20;
-               V.so[slot] = make_float4(sh.o.x, sh.o.y, sh.o.z, 0.0f);
-               V.sd[slot] = make_float4(sh.d.x, sh.d.y, sh.d.z, 0.0f);
-               V.sc[slot] = make_float4(sh.contrib.x, sh.contrib.y, sh.contrib.z, 0.0f);
-            }
-            V.ids[slot] = id;
-            if (cont) {  // a finished path needs only its radiance and ids
-               V.o[slot] = make_float4(ps.o.x, ps.o.y, ps.o.z, __uint_as_float(R.s));
-               V.d[slot] = make_float4(ps.d.x, ps.d.y, ps.d.z, 0.0f);
-               V.thr[slot] = make_float4(ps.T.x, ps.T.y, ps.T.z, 0.0f);
-            }
-            if (ps.L.x != 0.0f || ps.L.y != 0.0f || ps.L.z != 0.0f) {
-               const float4 r = V.rad[slot];
-               const V3 L = mk(r) + ps.L;
-               V.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
-            }
+         float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f), d = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kHole));
+         float4 t = o, hf = o;
+         if (idx < count) {  // the whole path state at once: the loads overlap
+            slot = V.ps[cur][idx];
+            o = V.o[cur][idx];
+            d = V.d[cur][idx];
+            t = V.thr[cur][idx];
+            hf = V.hit[idx];
          }
-         const unsigned long long mc = __ballot(cont), ms = __ballot(shadow);
-         if (cont) stage_next[w][n_next + __popcll(mc & lt)] = slot;
-         if (shadow) stage_sh[w][n_sh + __popcll(ms & lt)] = slot;
-         n_next += __popcll(mc);
-         n_sh += __popcll(ms);
+         const uint32_t job = __float_as_uint(d.w);
+         bool pending = slot >= 0;
+         bool cont = false, shadow = false;
+         PathState ps;
+         ShadowReq sh;
+         Rng R{__float_as_uint(o.w), false};
+         uint32_t flags = __float_as_uint(t.w);
+         // One job at a time: the lanes of the first pending lane's job shade together with
+         // that job's view read once into scalar registers (a wave's positions nearly always
+         // belong to one or two jobs: the levels keep work order).
+         for (;;) {
+            const unsigned long long m = __ballot(pending);
+            if (m == 0) break;
+            const int j0 = __builtin_amdgcn_readlane((int)job, __ffsll((long long)m) - 1);
+            const bool mine = pending && (int)job == j0;
+            if (j0 != tally_job) flush_segments(V, tally_job, tally_ext, tally_sh);
+            tally_job = j0;
+            if (mine) {
+               pending = false;
+               const JobView J = uniform_copy(V.jobs + j0);
+               // the step's radiance starts at zero and is added to the slot's only if it
+               // adds any (one term per step: rad + (0 + T*e) is rad + T*e bit for bit)
+               ps = PathState{mk(o), mk(d), mk(t), mk(0.0f, 0.0f, 0.0f), (flags & kPrevLambert) != 0};
+               HitMin h;
+               h.t = hf.x;
+               h.slot = __float_as_int(hf.y);
+               h.u = hf.z;
+               h.v = hf.w;
+               cont = shade_step<CNT, FT>(J, h, level, ps, R, sh, pc);
+               cont = cont && level + 1 < J.max_depth;
+               shadow = sh.valid;
+               flags = (flags & ~kPrevLambert) | (ps.prev_lambert ? kPrevLambert : 0u);
+               if (ps.L.x != 0.0f || ps.L.y != 0.0f || ps.L.z != 0.0f) {
+                  const float4 r = V.rad[slot];
+                  const V3 L = mk(r) + ps.L;
+                  V.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+               }
+            }
+            tally_ext += (unsigned long long)__popcll(__ballot(mine));
+            tally_sh += (unsigned long long)__popcll(__ballot(mine && shadow));
+         }
+         const int pn = rn.take(&ctl_next[0], cont, res);
+         if (cont) {
+            V.o[nxt][pn] = make_float4(ps.o.x, ps.o.y, ps.o.z, __uint_as_float(R.s));
+            V.d[nxt][pn] = make_float4(ps.d.x, ps.d.y, ps.d.z, d.w);
+            V.thr[nxt][pn] = make_float4(ps.T.x, ps.T.y, ps.T.z, __uint_as_float(flags));
+            V.ps[nxt][pn] = slot;
+         }
+         const int psh = rs.take(sctl, shadow, res);
+         if (shadow) {
+            V.so[psh] = make_float4(sh.o.x, sh.o.y, sh.o.z, __int_as_float(slot));
+            V.sd[psh] = make_float4(sh.d.x, sh.d.y, sh.d.z, d.w);
+            V.sc[psh] = make_float4(sh.contrib.x, sh.contrib.y, sh.contrib.z, 0.0f);
+         }
       }
-      __builtin_amdgcn_wave_barrier();
-      flush(stage_next[w], n_next, &ctl_next[0], qn);
-      flush(stage_sh[w], n_sh, sctl, V.sq);
-      __builtin_amdgcn_wave_barrier();
    }
+   flush_segments(V, tally_job, tally_ext, tally_sh);
+   // positions reserved and not filled: holes the next stages skip
+   for (unsigned i = rn.next + (unsigned)lane; i < rn.end; i += 64) {
+      V.d[nxt][i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kHole));
+      V.ps[nxt][i] = -1;
+   }
+   for (unsigned i = rs.next + (unsigned)lane; i < rs.end; i += 64)
+      V.so[i] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
    if constexpr (CNT) {
       unsigned long long a = wave_sum((unsigned long long)pc.mats), b = wave_sum((unsigned long long)pc.light_prims);
       if (lane == 0) {
@@ -1098,12 +1212,14 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_wf_shade(WfView V, int level) {
 }
 
 // One thread per pixel present in the round: adds that pixel's samples of the round in
-// sample order (float-float compensated sums), updates its count and the job's counters.
+// sample order (float-float compensated sums), updates its count and the job's count of
+// primary samples. A pixel that had converged before the round took no paths (its count
+// does not change during a pass's rounds, so the generate stage saw the same sign).
 __global__ __launch_bounds__(kWfBlock) void k_wf_accumulate(WfView V, long long w0, int n) {
    const int i = blockIdx.x * kWfBlock + threadIdx.x;
    bool active = false;
    int j = -1;
-   unsigned long long segs = 0, shadow = 0, prim = 0;
+   unsigned long long prim = 0;
    if (i < n) {
       const long long w = w0 + i;
       j = find_job(V, w);
@@ -1115,25 +1231,21 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_accumulate(WfView V, long long 
          int s, p;
          split_work(l, (unsigned)units, s, p);
          if (J.nlive >= 0) p = J.live[p];
-         const long long end = min(w0 + n, V.base[j + 1]);
-         float4 a = J.acc[p], c = J.comp[p];
-         int added = 0;
-         for (long long wk = w; wk < end; wk += units) {
-            const int slot = (int)(wk - w0);
-            const uint32_t f = V.ids[slot].w;
-            if (f & kSkip) continue;
-            const float4 r = V.rad[slot];
-            accumulate(a, c, mk(r));
-            ++added;
-            segs += (f >> 8) & 0xFFFu;
-            shadow += f >> 20;
-         }
-         if (added) {
+         const int n0 = J.counts[p];
+         if (n0 >= 0) {
+            const long long end = min(w0 + n, V.base[j + 1]);
+            float4 a = J.acc[p], c = J.comp[p];
+            int added = 0;
+            for (long long wk = w; wk < end; wk += units) {
+               const float4 r = V.rad[wk - w0];
+               accumulate(a, c, mk(r));
+               ++added;
+            }
             J.acc[p] = a;
             J.comp[p] = c;
-            J.counts[p] += added;
+            J.counts[p] = n0 + added;
+            prim = (unsigned long long)added;
          }
-         prim = (unsigned long long)added;
       }
    }
    // per-job counters: one atomic per wave when the wave's pixels belong to one job
@@ -1143,18 +1255,10 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_accumulate(WfView V, long long 
    const int jl = __shfl(j, leader);
    const bool uniform = __ballot(active && j != jl) == 0;
    if (uniform) {
-      const unsigned long long s0 = wave_sum(segs + shadow), s1 = wave_sum(shadow), s2 = wave_sum(prim);
-      if (lane_id() == leader) {
-         unsigned long long* ctr = V.jobs[jl].counters + 8 * (blockIdx.x % kCounterSlots);
-         atomicAdd(&ctr[0], s0);
-         atomicAdd(&ctr[1], s1);
-         atomicAdd(&ctr[2], s2);
-      }
-   } else if (active) {
-      unsigned long long* ctr = V.jobs[j].counters + 8 * (blockIdx.x % kCounterSlots);
-      atomicAdd(&ctr[0], segs + shadow);
-      atomicAdd(&ctr[1], shadow);
-      atomicAdd(&ctr[2], prim);
+      const unsigned long long s2 = wave_sum(prim);
+      if (lane_id() == leader && s2) atomicAdd(&V.jobs[jl].counters[8 * (blockIdx.x % kCounterSlots) + 2], s2);
+   } else if (active && prim) {
+      atomicAdd(&V.jobs[j].counters[8 * (blockIdx.x % kCounterSlots) + 2], prim);
    }
 }
 
@@ -1216,8 +1320,10 @@ static rayon_status wave_reserve(rayon_ctx* ctx, int capacity, int levels, int n
       W->occ_trace_short = n > 0 ? n : 1;
       RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_trace<false, L_WIDE, kShortWide>, kWfBlock, 0));
       W->occ_trace_wide_short = n > 0 ? n : 1;
-      RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_shade<false>, kWfBlock, 0));
+      RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_shade<false, SF_ALL>, kWfBlock, 0));
       W->occ_shade = n > 0 ? n : 1;
+      RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_shade<false, SF_LEAN>, kWfBlock, 0));
+      W->occ_shade_lean = n > 0 ? n : 1;
       RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_sdf<false>, kWfBlock, 0));
       W->occ_sdf = n > 0 ? n : 1;
       RY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_sdf_rest<false>, kWfBlock, 0));
@@ -1254,18 +1360,21 @@ static rayon_status wave_reserve(rayon_ctx* ctx, int capacity, int levels, int n
    if (capacity > W->capacity) {
       release(W->bufs_slots);
       const int cap = (int)std::max<long long>(capacity, std::min<long long>(2ll * W->capacity, W->opt_capacity));
-      float4** f4[] = {&W->v.o, &W->v.d, &W->v.thr, &W->v.rad, &W->v.hit, &W->v.so, &W->v.sd, &W->v.sc};
+      // positions of a level: its paths plus the holes of the shade waves' reservations
+      const size_t npos = (size_t)cap + (size_t)(kPosHeadroomPerCu * W->cus);
+      if ((st = alloc(W->bufs_slots, (size_t)cap * sizeof(float4), &p)) != RAYON_OK) return st;
+      W->v.rad = (float4*)p;
+      float4** f4[] = {&W->v.o[0], &W->v.o[1], &W->v.d[0], &W->v.d[1], &W->v.thr[0], &W->v.thr[1],
+                       &W->v.hit, &W->v.so, &W->v.sd, &W->v.sc};
       for (float4** f : f4) {
-         if ((st = alloc(W->bufs_slots, (size_t)cap * sizeof(float4), &p)) != RAYON_OK) return st;
+         if ((st = alloc(W->bufs_slots, npos * sizeof(float4), &p)) != RAYON_OK) return st;
          *f = (float4*)p;
       }
-      if ((st = alloc(W->bufs_slots, (size_t)cap * sizeof(uint4), &p)) != RAYON_OK) return st;
-      W->v.ids = (uint4*)p;
-      int** qs[] = {&W->v.q0, &W->v.q1, &W->v.sq, &W->v.qs0, &W->v.qs1};
+      int** qs[] = {&W->v.ps[0], &W->v.ps[1], &W->v.qs0, &W->v.qs1};
       for (int** q : qs) {
-         // the marching queues also hold the unused tails of the waves' reservations
+         // the marching queues also hold the unused tails of the trace waves' reservations
          const bool marching = q == &W->v.qs0 || q == &W->v.qs1;
-         const size_t n = (size_t)cap + (marching ? (size_t)(kMarchHeadroomPerCu * W->cus) : 0);
+         const size_t n = npos + (marching ? (size_t)(kMarchHeadroomPerCu * W->cus) : 0);
          if ((st = alloc(W->bufs_slots, n * sizeof(int), &p)) != RAYON_OK) return st;
          *q = (int*)p;
       }
@@ -1379,6 +1488,11 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
       return std::min(needed, W->cus * (W->opt_wg_per_cu > 0 ? W->opt_wg_per_cu : occ));
    };
    const dim3 g_shade(grid_of(W->occ_shade)), blk(kWfBlock);
+   // the shade kernel without the code of features no scene of the pass uses
+   uint32_t feat = 0;
+   for (rayon_job* j : jobs) feat |= j->scene->features;
+   const bool lean = feat == SF_LEAN;
+   const dim3 g_shade_lean(grid_of(W->occ_shade_lean));
    // one layout for the whole pass when every scene shares it
    int nwide = 0;
    for (rayon_job* j : jobs) nwide += j->scene->view.wnodes != nullptr;
@@ -1447,9 +1561,11 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
       march(0);
       for (int lv = 0; lv < levels; ++lv) {
          if (W->opt_count)
-            launch_stage(W, 2, k_wf_shade<true>, g_shade, blk, st, V, lv);
+            launch_stage(W, 2, k_wf_shade<true, SF_ALL>, g_shade, blk, st, V, lv);
+         else if (lean)
+            launch_stage(W, 2, k_wf_shade<false, SF_LEAN>, g_shade_lean, blk, st, V, lv);
          else
-            launch_stage(W, 2, k_wf_shade<false>, g_shade, blk, st, V, lv);
+            launch_stage(W, 2, k_wf_shade<false, SF_ALL>, g_shade, blk, st, V, lv);
          trace(lv + 1 < levels ? lv + 1 : -1, lv);
          march(lv + 1 < levels ? lv + 1 : -1);
       }
@@ -1666,6 +1782,9 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
       int wdepth = 0;  // picks the quantized-tree kernel's stack size
       for (rayon_job* j : active) wdepth = std::max(wdepth, j->scene->wide_depth);
       key.push_back(wdepth <= kShortWide / 2);
+      uint32_t feat = 0;  // picks the shade kernel's feature set
+      for (rayon_job* j : active) feat |= j->scene->features;
+      key.push_back(feat == SF_LEAN);
       key.push_back(-1);
       key.push_back(nmega);
       key.push_back(mtiles);
