@@ -83,6 +83,7 @@ struct WfView {
    int shade_chunk;  // the same for the shade stage
    int shadow_refill;  // shadow rays with lane refill (passes without SDF fields)
    int refill;         // idle lanes (of 64) at which a trace wave takes new rays
+   unsigned res_most;  // largest output reservation of a shade wave (multiple of 64)
 };
 
 struct Wave {
@@ -153,10 +154,11 @@ constexpr int kMarchRes = 128;  // marching-queue positions a trace wave reserve
 // wave of a trace launch may leave at most kMarchRes - 1 reserved positions unfilled
 // (at most 32 workgroups of 4 waves per CU).
 constexpr long long kMarchHeadroomPerCu = 32ll * (kWfBlock / 64) * kMarchRes;
-// Positions beyond a level's paths that its arrays must hold: every wave of a shade launch
-// may leave at most the 256 positions of one reservation unfilled (at most 32 workgroups
+// Positions beyond a level's paths that its arrays hold: every wave of a shade launch may
+// leave its last reservation partly unfilled, so the reservations are capped at this
+// headroom divided by the launch's waves (1024 for 16 waves per CU, 128 at 32 workgroups
 // of 4 waves per CU).
-constexpr long long kPosHeadroomPerCu = 32ll * (kWfBlock / 64) * 256;
+constexpr long long kPosHeadroomPerCu = 16ll * 1024;
 
 // Appends to a marching queue through positions the wave reserves kMarchRes at a time, so
 // the queue's head counter, shared by every wave of a launch, takes one atomic per
@@ -214,13 +216,15 @@ struct PosRes {
    }
 };
 
-// Reservation size for the outputs of a level of `count` positions: 64 per wave while
-// waves have little work (holes then stay a few percent of the next level), up to 256
-// when each wave handles more than 16 batches of 64.
-__device__ __forceinline__ unsigned pos_res(unsigned count) {
+// Reservation size for the outputs of a level of `count` positions: what a wave shades, in
+// multiples of 64, at most `most`. Atomics on one address serialise (about 13 ns each), so
+// small reservations made the counter the bottleneck of busy levels; the holes they leave
+// come as runs at the ends of the waves' last reservations, which the next stages skip a
+// batch at a time.
+__device__ __forceinline__ unsigned pos_res(unsigned count, unsigned most) {
    const unsigned waves = gridDim.x * (blockDim.x >> 6);
-   const unsigned per = count / (64u * 16u * waves);
-   return 64u * min(max(per, 1u), 4u);
+   const unsigned per = count / (64u * waves);
+   return min(64u * max(per, 1u), most);
 }
 
 // Persistent-worker loop: each wave claims `chunk` batches of 64 queue entries with one
@@ -1110,7 +1114,7 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_wf_shade(WfView V, int level) {
    unsigned* ctl_next = V.ctl + kCtl * (level + 1);
    unsigned* sctl = ctl + 3;
    const int chunk = fit_chunk(count, V.shade_chunk);
-   const unsigned res = pos_res(count);
+   const unsigned res = pos_res(count, V.res_most);
    PosRes rn, rs;  // next-level and shadow positions
    PathCounts pc;
    int tally_job = -1;  // segments shaded by this wave, not yet added to the job's counters
@@ -1493,6 +1497,12 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
    for (rayon_job* j : jobs) feat |= j->scene->features;
    const bool lean = feat == SF_LEAN;
    const dim3 g_shade_lean(grid_of(W->occ_shade_lean));
+   // output reservations of the shade waves: their unfilled tails must fit the headroom
+   {
+      const long long waves = (long long)std::max(g_shade.x, g_shade_lean.x) * (kWfBlock / 64);
+      const long long most = (kPosHeadroomPerCu * W->cus / waves) / 64 * 64;
+      V.res_most = (unsigned)std::min<long long>(1024, std::max<long long>(64, most));
+   }
    // one layout for the whole pass when every scene shares it
    int nwide = 0;
    for (rayon_job* j : jobs) nwide += j->scene->view.wnodes != nullptr;
