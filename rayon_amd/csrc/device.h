@@ -450,11 +450,15 @@ __device__ inline HitMin closest_hit_bin(const SceneView& S, V3 o, V3 d, float t
    best.sdf = -1;
    int cur = S.root;
    if (cur == kSentinel) return best;
+   // the scene's arrays in registers: S may live in memory the compiler cannot prove
+   // unchanged, and a reload before each fetch would add a dependent load to every visit
+   const Node2* const nodes = S.nodes;
+   const PrimRec* const prims = S.prims;
    const RayInv r = make_rayinv(o, d);
    BinStack<STRIDE, STACK> st(stack);
    for (;;) {
       while (cur >= 0) {
-         const Node2 n = gload(S.nodes + cur);
+         const Node2 n = gload(nodes + cur);
          if constexpr (CNT) ++tc->nodes;
          float tn0, tn1;
          bool h0, h1;
@@ -481,7 +485,7 @@ __device__ inline HitMin closest_hit_bin(const SceneView& S, V3 o, V3 d, float t
       const int packed = ~cur;
       const int first = packed >> 3, cnt = (packed & 7) + 1;
       for (int i = first; i < first + cnt; ++i) {
-         const PrimRec p = gload(S.prims + i);
+         const PrimRec p = gload(prims + i);
          if constexpr (CNT) ++tc->prims;
          const uint32_t info = __float_as_uint(p.a.w);
          const uint32_t kind = info_kind(info);
@@ -522,11 +526,13 @@ __device__ inline bool any_hit_bin(const SceneView& S, V3 o, V3 d, float tmin, f
                                    TravCount* tc = nullptr) {
    int cur = S.root;
    if (cur == kSentinel) return false;
+   const Node2* const nodes = S.nodes;  // in registers, as in closest_hit_bin
+   const PrimRec* const prims = S.prims;
    const RayInv r = make_rayinv(o, d);
    BinStack<STRIDE, STACK> st(stack);
    for (;;) {
       while (cur >= 0) {
-         const Node2 n = gload(S.nodes + cur);
+         const Node2 n = gload(nodes + cur);
          if constexpr (CNT) ++tc->nodes;
          float tn0, tn1;
          bool h0, h1;
@@ -548,7 +554,7 @@ __device__ inline bool any_hit_bin(const SceneView& S, V3 o, V3 d, float tmin, f
       const int packed = ~cur;
       const int first = packed >> 3, cnt = (packed & 7) + 1;
       for (int i = first; i < first + cnt; ++i) {
-         const PrimRec p = gload(S.prims + i);
+         const PrimRec p = gload(prims + i);
          if constexpr (CNT) ++tc->prims;
          const uint32_t info = __float_as_uint(p.a.w);
          if (info & F_EMITTER) continue;
@@ -572,9 +578,9 @@ __device__ inline bool any_hit_bin(const SceneView& S, V3 o, V3 d, float tmin, f
 
 // Tests one primitive for the closest hit (shared by both tree layouts).
 template <bool CNT, bool DEFER = false>
-__device__ __forceinline__ void test_closest(const SceneView& S, int i, V3 o, V3 d, float tmin, HitMin& best,
-                                             TravCount* tc) {
-   const PrimRec p = gload(S.prims + i);
+__device__ __forceinline__ void test_closest(const SceneView& S, const PrimRec* prims, int i, V3 o, V3 d, float tmin,
+                                             HitMin& best, TravCount* tc) {
+   const PrimRec p = gload(prims + i);
    if constexpr (CNT) ++tc->prims;
    const uint32_t info = __float_as_uint(p.a.w);
    const uint32_t kind = info_kind(info);
@@ -605,9 +611,9 @@ __device__ __forceinline__ void test_closest(const SceneView& S, int i, V3 o, V3
 }
 
 template <bool CNT>
-__device__ __forceinline__ bool test_any(const SceneView& S, int i, V3 o, V3 d, float tmin, float tmax,
-                                         TravCount* tc) {
-   const PrimRec p = gload(S.prims + i);
+__device__ __forceinline__ bool test_any(const SceneView& S, const PrimRec* prims, int i, V3 o, V3 d, float tmin,
+                                         float tmax, TravCount* tc) {
+   const PrimRec p = gload(prims + i);
    if constexpr (CNT) ++tc->prims;
    const uint32_t info = __float_as_uint(p.a.w);
    if (info & F_EMITTER) return false;
@@ -709,8 +715,10 @@ __device__ inline HitMin closest_hit_wide(const SceneView& S, V3 o, V3 d, float 
    const uint32_t perm = 7u ^ ((d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u));
    int sp = 0;
    uint32_t node = 0;
+   const WNode* const wn = S.wnodes;  // in registers, as in closest_hit_bin
+   const PrimRec* const prims = S.prims;
    for (;;) {
-      const WNode n = load_wnode(S.wnodes, node);
+      const WNode n = load_wnode(wn, node);
       if constexpr (CNT) ++tc->nodes;
       uint32_t hit_int, hit_leaf;
       wide_children(n, o, inv, perm, tmin, best.t, hit_int, hit_leaf);
@@ -720,7 +728,7 @@ __device__ inline HitMin closest_hit_wide(const SceneView& S, V3 o, V3 d, float 
          const uint32_t m = ((sl < 4 ? n.meta[0] : n.meta[1]) >> (8 * (sl & 3))) & 0xFFu;
          const int first = (int)n.prim_base + (int)(m & 31u);
          const int cnt = (int)(m >> 5) + 1;
-         for (int i = first; i < first + cnt; ++i) test_closest<CNT, DEFER>(S, i, o, d, tmin, best, tc);
+         for (int i = first; i < first + cnt; ++i) test_closest<CNT, DEFER>(S, prims, i, o, d, tmin, best, tc);
       }
       const uint32_t imask = n.e_imask >> 24;
       uint32_t base = n.child_base;
@@ -760,8 +768,10 @@ __device__ inline bool any_hit_wide(const SceneView& S, V3 o, V3 d, float tmin, 
    const uint32_t perm = 7u ^ ((d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u));
    int sp = 0;
    uint32_t node = 0;
+   const WNode* const wn = S.wnodes;  // in registers, as in closest_hit_bin
+   const PrimRec* const prims = S.prims;
    for (;;) {
-      const WNode n = load_wnode(S.wnodes, node);
+      const WNode n = load_wnode(wn, node);
       if constexpr (CNT) ++tc->nodes;
       uint32_t hit_int, hit_leaf;
       wide_children(n, o, inv, perm, tmin, tmax, hit_int, hit_leaf);
@@ -772,7 +782,7 @@ __device__ inline bool any_hit_wide(const SceneView& S, V3 o, V3 d, float tmin, 
          const int first = (int)n.prim_base + (int)(m & 31u);
          const int cnt = (int)(m >> 5) + 1;
          for (int i = first; i < first + cnt; ++i)
-            if (test_any<CNT>(S, i, o, d, tmin, tmax, tc)) return true;
+            if (test_any<CNT>(S, prims, i, o, d, tmin, tmax, tc)) return true;
       }
       const uint32_t imask = n.e_imask >> 24;
       uint32_t base = n.child_base;

@@ -655,7 +655,9 @@ __device__ void shadow_rays_bin(const WfView& V, int level, int* lds, TravCount&
    const unsigned step = 64u * (unsigned)fit_chunk(count, V.chunk);
    WavePool pool = start_pool(&sctl[1], count);
    int slot = -1, cur = kSentinel;
-   const SceneView* S = nullptr;
+   const SceneView* S = nullptr;  // read for SDF fields only
+   const Node2* nodes = nullptr;  // the scene's arrays in registers, as in ext_rays_bin
+   const PrimRec* prims = nullptr;
    RayInv r;
    BinStack<kWfBlock, STACK> st(lds);
    constexpr float tmin = 1e-4f, tmax = 1.0f - 1e-4f;
@@ -667,6 +669,8 @@ __device__ void shadow_rays_bin(const WfView& V, int level, int* lds, TravCount&
             slot = (int)idx;  // the shadow ray's position; o4.w holds its path's slot
             const float4 d4 = V.sd[idx];
             S = &V.jobs[__float_as_uint(d4.w)].scene;
+            nodes = S->nodes;
+            prims = S->prims;
             cur = S->root;
             r = make_rayinv(mk(o4), mk(d4));
             st.n = 0;
@@ -682,7 +686,7 @@ __device__ void shadow_rays_bin(const WfView& V, int level, int* lds, TravCount&
          int leaf = kSentinel;  // postponed leaves, as for extension rays
          for (;;) {
             if (cur >= 0) {
-               const Node2 n = gload(S->nodes + cur);
+               const Node2 n = gload(nodes + cur);
                if constexpr (CNT) ++tc.nodes;
                float tn0, tn1;
                bool h0, h1;
@@ -709,7 +713,7 @@ __device__ void shadow_rays_bin(const WfView& V, int level, int* lds, TravCount&
             const int packed = ~leaf;
             const int first = packed >> 3, cnt = (packed & 7) + 1;
             for (int i = first; i < first + cnt && !occluded; ++i) {
-               const PrimRec p = gload(S->prims + i);
+               const PrimRec p = gload(prims + i);
                if constexpr (CNT) ++tc.prims;
                const uint32_t info = __float_as_uint(p.a.w);
                if (info & F_EMITTER) continue;
@@ -814,6 +818,8 @@ __device__ void ext_rays_wide(const WfView& V, int level, int* lds, TravCount& t
       pend = -1;
    };
    const SceneView* S = nullptr;
+   const WNode* wn = nullptr;  // the scene's arrays in registers, as in ext_rays_bin
+   const PrimRec* prims = nullptr;
    WideRay r;
    HitMin best;
    for (;;) {
@@ -824,6 +830,8 @@ __device__ void ext_rays_wide(const WfView& V, int level, int* lds, TravCount& t
          if (__float_as_uint(d4.w) != kHole) {
             slot = (int)idx;
             S = &V.jobs[__float_as_uint(d4.w)].scene;
+            wn = S->wnodes;
+            prims = S->prims;
             r = make_wideray(mk(O[idx]), mk(d4));
             best.t = 3.402823466e38f;
             best.slot = -1;
@@ -840,7 +848,6 @@ __device__ void ext_rays_wide(const WfView& V, int level, int* lds, TravCount& t
          continue;
       }
       if (slot >= 0) {
-         const WNode* wn = S->wnodes;
          // Postponed leaves, as on binary trees: the hit leaf children of the first node that
          // has any are set aside (mask, meta words, primitive base) and the lane goes on with
          // its next node, until every lane of the wave holds leaves or has finished. A lane
@@ -883,7 +890,8 @@ __device__ void ext_rays_wide(const WfView& V, int level, int* lds, TravCount& t
             const uint32_t m = ((sl < 4 ? pm0 : pm1) >> (8 * (sl & 3))) & 0xFFu;
             const int first = (int)pbase + (int)(m & 31u);
             const int cnt = (int)(m >> 5) + 1;
-            for (int i = first; i < first + cnt; ++i) test_closest<CNT, true>(*S, i, r.o, r.d, 0.001f, best, &tc);
+            for (int i = first; i < first + cnt; ++i)
+               test_closest<CNT, true>(*S, prims, i, r.o, r.d, 0.001f, best, &tc);
          }
          if (node == kWideDone) {
             V.hit[slot] = make_float4(best.t, __int_as_float(best.slot), best.u, best.v);
@@ -905,7 +913,9 @@ __device__ void shadow_rays_wide(const WfView& V, int level, int* lds, TravCount
    int slot = -1, sp = 0;
    uint32_t node = kWideDone;
    uint32_t held_masks = kNoHeld;  // as for extension rays
-   const SceneView* S = nullptr;
+   const SceneView* S = nullptr;  // read for SDF fields only
+   const WNode* wn = nullptr;     // the scene's arrays in registers, as in ext_rays_bin
+   const PrimRec* prims = nullptr;
    WideRay r;
    constexpr float tmin = 1e-4f, tmax = 1.0f - 1e-4f;
    for (;;) {
@@ -916,6 +926,8 @@ __device__ void shadow_rays_wide(const WfView& V, int level, int* lds, TravCount
             slot = (int)idx;  // the shadow ray's position; o4.w holds its path's slot
             const float4 d4 = V.sd[idx];
             S = &V.jobs[__float_as_uint(d4.w)].scene;
+            wn = S->wnodes;
+            prims = S->prims;
             r = make_wideray(mk(o4), mk(d4));
             sp = 0;
             node = 0;
@@ -927,7 +939,6 @@ __device__ void shadow_rays_wide(const WfView& V, int level, int* lds, TravCount
          continue;
       }
       if (slot >= 0) {
-         const WNode* wn = S->wnodes;
          bool occluded = false;
          uint32_t pmask = 0, pm0 = 0, pm1 = 0, pbase = 0;  // postponed leaves, as for extension rays
          bool held = false;
@@ -965,7 +976,7 @@ __device__ void shadow_rays_wide(const WfView& V, int level, int* lds, TravCount
             const int first = (int)pbase + (int)(m & 31u);
             const int cnt = (int)(m >> 5) + 1;
             for (int i = first; i < first + cnt && !occluded; ++i)
-               occluded = test_any<CNT>(*S, i, r.o, r.d, tmin, tmax, &tc);
+               occluded = test_any<CNT>(*S, prims, i, r.o, r.d, tmin, tmax, &tc);
          }
          if (occluded) {
             node = kWideDone;
