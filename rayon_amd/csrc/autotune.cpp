@@ -184,8 +184,8 @@ extern "C" rayon_status rayon_autotune(rayon_ctx* ctx, rayon_job* const* jobs, i
    // trace waves' refill threshold
    const long long round_cap = std::min<long long>(1ll << 28, std::max<long long>(1 << 16, work));
    const std::vector<std::vector<long long>> axes = {
-      {1, 2, 4, 8, 16},
-      {2, 4, 8, 16},
+      {2, 4, 8, 16, 32},
+      {4, 8, 16, 32, 64},
       {0, 1, 2, 4, 8},
       {1ll << 22, 1ll << 24, 1ll << 25, 1ll << 26, 1ll << 27},
       {8, 16, 32, 64},

@@ -103,7 +103,7 @@ struct Wave {
    // options
    int opt_capacity = 1 << 27;
    int opt_wg_per_cu = 0;  // 0: as many as fit (occupancy of each stage kernel)
-   int opt_chunk = 4;
+   int opt_chunk = 8;
    int opt_shade_chunk = 16;
    int opt_refill = 16;
    int occ_trace = 0, occ_trace_wide = 0, occ_trace_short = 0, occ_trace_wide_short = 0, occ_shade = 0, occ_shade_lean = 0, occ_sdf = 0, occ_sdf_rest = 0;
@@ -1084,7 +1084,7 @@ __global__ __launch_bounds__(kWfBlock, (LAYOUT == L_BIN && STACK == kShortStack)
    }
 }
 
-constexpr int kMaxChunk = 16;
+constexpr int kMaxChunk = 64;  // batches of 64 a wave may claim per queue atomic
 
 // Copy of a record at a wave-uniform address, every 32-bit word moved to scalar registers
 // (words the caller never reads are dropped with their loads).
@@ -1876,14 +1876,14 @@ rayon_status wave_set_option(rayon_ctx* ctx, int opt, long long value) {
       break;
    case RAYON_OPT_WAVE_CHUNK:
       if (value < 1 || value > kMaxChunk) {
-         set_error("RAYON_OPT_WAVE_CHUNK must be in [1, 16]");
+         set_error("RAYON_OPT_WAVE_CHUNK must be in [1, 64]");
          return RAYON_ERR_INVALID;
       }
       W->opt_chunk = (int)value;
       break;
    case RAYON_OPT_SHADE_CHUNK:
       if (value < 1 || value > kMaxChunk) {
-         set_error("RAYON_OPT_SHADE_CHUNK must be in [1, 16]");
+         set_error("RAYON_OPT_SHADE_CHUNK must be in [1, 64]");
          return RAYON_ERR_INVALID;
       }
       W->opt_shade_chunk = (int)value;
