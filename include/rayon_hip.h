@@ -140,7 +140,9 @@ typedef struct rayon_scene_desc {
  * RAYON_BVH_DEVICE builds the binary tree on the GPU instead (Morton-ordered linear BVH:
  * milliseconds for a million triangles, at some traversal cost against SAH); images are
  * identical to the host tree's. It falls back to the host build if the tree would be
- * deeper than the traversal stack; rayon_scene_info.builder says which one ran. */
+ * deeper than the traversal stack; rayon_scene_info.builder says which one ran. With both
+ * flags the device-built tree is read back, collapsed into the 8-wide layout by the same
+ * surface-area rule, and its records are permuted into the wide order on the device. */
 enum {
    RAYON_BVH_DEFAULT = 0,
    RAYON_BVH_QUANTIZED = 1 << 0,
@@ -257,8 +259,10 @@ rayon_status rayon_scene_info_get(const rayon_scene* scene, rayon_scene_info* ou
  * containing what lies below it, depth within the traversal stack). Fills counts. */
 rayon_status rayon_bvh_check(const rayon_scene_desc* desc, rayon_scene_info* out);
 /* The same invariants for a created scene's binary tree (either builder), read back from
- * the device and checked against the description it was created from; the depth of its
- * deepest leaf through `depth`. */
+ * the device and checked against the description it was created from, and for its 8-wide
+ * quantized tree when it has one (quantized boxes containing what lies below them, every
+ * primitive in exactly one wide leaf, the reported wide depth); the depth of the binary
+ * tree's deepest leaf through `depth`. */
 rayon_status rayon_scene_check(const rayon_scene* scene, const rayon_scene_desc* desc, int32_t* depth);
 /* Index (into rayon_scene_desc order: geometries first, then mesh triangles in order)
  * of the primitive stored at BVH slot `slot`; for host-side checks. */

@@ -311,14 +311,64 @@ static double ms_between(std::chrono::steady_clock::time_point a, std::chrono::s
    return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
+// Collapses a device-built binary tree (pair nodes at dnodes, records at dp/dn in its leaf
+// order) into the 8-wide quantized layout: the nodes are read back and collapsed by the host
+// builder's surface-area dynamic program, the records are permuted into the wide tree's order
+// on the device, and the pair nodes are rewritten to that order (both layouts stay valid).
+// The scene keeps the binary layout when the collapse does not apply (a root leaf, or a wide
+// tree deeper than the traversal stack), as the host build does.
+static rayon_status collapse_device_tree(rayon_ctx* ctx, std::vector<DevBuf>& bufs, int n, int64_t num_nodes,
+                                         Node2* dnodes, PrimRec*& dp, float4*& dn, WNode*& dw, BuildResult& br) {
+   if (num_nodes <= 0) return RAYON_OK;
+   std::vector<Node2> hn((size_t)num_nodes);
+   RY_HIP(hipMemcpy(hn.data(), dnodes, sizeof(Node2) * hn.size(), hipMemcpyDeviceToHost));
+   std::vector<WNode> wn;
+   std::vector<int32_t> perm;
+   int wdepth = 0;
+   if (!collapse_pair_nodes(hn, n, wn, perm, wdepth)) return RAYON_OK;
+   std::vector<DevBuf> tmp;
+   int* dperm = nullptr;
+   rayon_status st = upload(tmp, perm, &dperm);
+   PrimRec* nr = nullptr;
+   float4* nn = nullptr;
+   hipError_t e = hipSuccess;
+   if (st == RAYON_OK) e = hipMalloc((void**)&nr, sizeof(PrimRec) * (size_t)n);
+   if (st == RAYON_OK && e == hipSuccess && dn) e = hipMalloc((void**)&nn, sizeof(float4) * 3 * (size_t)n);
+   if (st == RAYON_OK && e == hipSuccess) e = launch_permute_recs(dp, dn, dperm, n, nr, nn, ctx->stream);
+   if (st == RAYON_OK && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+   if (st == RAYON_OK && e == hipSuccess)
+      e = hipMemcpy(dnodes, hn.data(), sizeof(Node2) * hn.size(), hipMemcpyHostToDevice);
+   free_all(tmp);
+   if (st != RAYON_OK || e != hipSuccess) {
+      if (nr) (void)hipFree(nr);
+      if (nn) (void)hipFree(nn);
+      return st != RAYON_OK ? st : hip_fail(e, "device tree collapse");
+   }
+   // the permuted records replace the binary order's in the scene's allocations
+   for (DevBuf& b : bufs) {
+      if (b.p == dp) {
+         (void)hipFree(b.p);
+         b.p = nr;
+      } else if (dn && b.p == dn) {
+         (void)hipFree(b.p);
+         b.p = nn;
+      }
+   }
+   dp = nr;
+   dn = nn;
+   if ((st = upload(bufs, wn, &dw)) != RAYON_OK) return st;
+   std::vector<int32_t> order((size_t)n);
+   for (int s = 0; s < n; ++s) order[s] = br.order[perm[s]];
+   br.order = std::move(order);
+   br.wnodes = std::move(wn);
+   br.wide_depth = wdepth;
+   return RAYON_OK;
+}
+
 rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* d, uint32_t bvh_flags, rayon_scene** out) {
    if (!ctx || !out) {
       set_error("rayon_scene_create: NULL argument");
       return RAYON_ERR_INVALID;
-   }
-   if ((bvh_flags & RAYON_BVH_DEVICE) && (bvh_flags & RAYON_BVH_QUANTIZED)) {
-      set_error("rayon_scene_create: RAYON_BVH_DEVICE builds the binary layout; the quantized layout is host-built");
-      return RAYON_ERR_UNSUPPORTED;
    }
    std::vector<HostPrim> prims;
    rayon_status st = gather_prims(d, prims);
@@ -385,6 +435,13 @@ rayon_status rayon_scene_create(rayon_ctx* ctx, const rayon_scene_desc* d, uint3
          builder = RAYON_BVH_BUILDER_DEVICE;
          upload_ms = ms_between(u0, u1);
          build_ms = ms_between(u1, u2);
+         if (bvh_flags & RAYON_BVH_QUANTIZED) {
+            // the 8-wide collapse of the device-built tree: read back, collapsed on the host
+            // (a dynamic program over the tree), the records permuted on the device
+            if ((st = collapse_device_tree(ctx, sc->bufs, (int)n, num_nodes, dnodes, dp, dn, dw, br)) != RAYON_OK)
+               return fail(st);
+            build_ms = ms_between(u1, clock::now());
+         }
       }
    }
    if (builder == RAYON_BVH_BUILDER_HOST) {

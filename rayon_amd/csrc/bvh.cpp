@@ -437,6 +437,64 @@ struct WideBuilder {
 
 void compute_bounds(HostPrim& p) { prim_bounds(p); }
 
+bool collapse_pair_nodes(std::vector<Node2>& nodes, int nslots, std::vector<WNode>& wnodes, std::vector<int32_t>& perm,
+                         int& wide_depth) {
+   const int np = (int)nodes.size();
+   if (np == 0) return false;  // a root leaf: nothing to collapse
+   // binary nodes: pair node p is node p; the leaf children follow, one node each; a node's
+   // box is the one its parent's pair node holds for it (the root's: both children's union)
+   std::vector<BNode> bn((size_t)np);
+   for (int p = 0; p < np; ++p) {
+      const Node2& n = nodes[p];
+      const int ref[2] = {n.meta.x, n.meta.y}, cnt[2] = {n.meta.z, n.meta.w};
+      const float lox[2] = {n.bx.x, n.bx.z}, hix[2] = {n.bx.y, n.bx.w};
+      const float loy[2] = {n.by.x, n.by.z}, hiy[2] = {n.by.y, n.by.w};
+      const float loz[2] = {n.bz.x, n.bz.z}, hiz[2] = {n.bz.y, n.bz.w};
+      int kid[2];
+      for (int c = 0; c < 2; ++c) {
+         if (cnt[c]) {
+            BNode leaf;
+            leaf.first = ref[c];
+            leaf.count = cnt[c];
+            kid[c] = (int)bn.size();
+            bn.push_back(leaf);
+         } else {
+            if (ref[c] <= 0 || ref[c] >= np) return false;  // an internal child is some other pair node
+            kid[c] = ref[c];
+         }
+         const float lo[3] = {lox[c], loy[c], loz[c]}, hi[3] = {hix[c], hiy[c], hiz[c]};
+         bn[kid[c]].box = Box{};
+         bn[kid[c]].box.grow(lo, hi);
+      }
+      bn[p].left = kid[0];
+      bn[p].right = kid[1];
+   }
+   bn[0].box = Box{};
+   bn[0].box.grow(bn[bn[0].left].box);
+   bn[0].box.grow(bn[bn[0].right].box);
+   std::vector<int32_t> ident((size_t)nslots);
+   for (int i = 0; i < nslots; ++i) ident[i] = i;
+   WideBuilder wb(bn, ident);
+   wb.cost.resize(bn.size());
+   wb.cut.resize(bn.size());
+   wb.wide_cut.assign(bn.size(), 1);
+   wb.plan(0);
+   wb.out.resize(1);
+   wb.order.reserve(nslots);
+   wb.emit(0, 0, 0);
+   if (wb.max_depth >= kMaxWideDepth || (int)wb.order.size() != nslots) return false;
+   // the binary leaves follow the new primitive order too
+   for (int p = 0; p < np; ++p) {
+      Node2& n = nodes[p];
+      if (n.meta.z) n.meta.x = bn[bn[p].left].first;
+      if (n.meta.w) n.meta.y = bn[bn[p].right].first;
+   }
+   wnodes = std::move(wb.out);
+   perm = std::move(wb.order);
+   wide_depth = wb.max_depth;
+   return true;
+}
+
 bool build_bvh(const std::vector<HostPrim>& prims, BuildResult& out, bool wide) {
    const int n = (int)prims.size();
    out.order.clear();
@@ -790,10 +848,6 @@ extern "C" rayon_status rayon_scene_check(const rayon_scene* sc, const rayon_sce
                 sc->order.size());
       return RAYON_ERR_INVALID;
    }
-   if (sc->view.wnodes) {
-      set_error("rayon_scene_check: checks the binary layout only");
-      return RAYON_ERR_UNSUPPORTED;
-   }
    BuildResult br;
    br.order = sc->order;
    br.root = sc->view.root;
@@ -805,6 +859,19 @@ extern "C" rayon_status rayon_scene_check(const rayon_scene* sc, const rayon_sce
    if (!check_binary(br, prims, dep)) {
       set_error("BVH invariant violated (containment, leaf size or slot coverage)");
       return RAYON_ERR_INVALID;
+   }
+   if (sc->view.wnodes) {  // the 8-wide tree over the same primitive order
+      br.wnodes.resize((size_t)sc->info.num_qnodes);
+      RY_HIP(hipMemcpy(br.wnodes.data(), sc->view.wnodes, br.wnodes.size() * sizeof(WNode), hipMemcpyDeviceToHost));
+      std::vector<int> wseen(prims.size(), 0);
+      int wdepth = 0;
+      double lo[3], hi[3];
+      bool wok = check_wide(br, prims, 0, wseen, 0, wdepth, lo, hi) && wdepth == sc->wide_depth;
+      for (int v : wseen) wok = wok && v == 1;
+      if (!wok) {
+         set_error("wide BVH invariant violated (quantized containment, leaf size, slot coverage or depth)");
+         return RAYON_ERR_INVALID;
+      }
    }
    if (depth) *depth = dep;
    return RAYON_OK;

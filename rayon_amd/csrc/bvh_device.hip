@@ -274,6 +274,14 @@ hipError_t salloc(T** p, size_t count, hipStream_t st) {
 
 }  // namespace
 
+hipError_t launch_permute_recs(const PrimRec* src, const float4* nsrc, const int* perm, int n, PrimRec* dst,
+                               float4* ndst, hipStream_t st) {
+   if (n <= 0) return hipSuccess;
+   hipLaunchKernelGGL(k_permute, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, src, nsrc, perm, n,
+                      dst, ndst);
+   return hipGetLastError();
+}
+
 rayon_status build_bvh_device(rayon_ctx* ctx, const float4* d_lo, const float4* d_hi, const PrimRec* d_recs,
                               const float4* d_normals, int n, DeviceTree& out) {
    hipStream_t st = ctx->stream;

@@ -52,6 +52,12 @@ struct BuildResult {
 constexpr int kMaxDepth = 32;
 bool build_bvh(const std::vector<HostPrim>& prims, BuildResult& out, bool wide = false);
 constexpr int kMaxWideDepth = 16;  // device stack entries of the wide traversal
+// The same 8-wide collapse of a binary tree given as pair nodes (root = node 0: a device-built
+// tree read back). perm receives, for each primitive slot of the
+// wide tree's order, the slot it had in the binary tree; the pair nodes' leaf references are
+// rewritten to the new order. False when the root is a leaf or the wide tree would be too deep.
+bool collapse_pair_nodes(std::vector<Node2>& nodes, int nslots, std::vector<WNode>& wnodes, std::vector<int32_t>& perm,
+                         int& wide_depth);
 
 // Linear BVH built on the device (bvh_device.hip) over primitive boxes and records in host
 // order. The outputs are plain device allocations that the scene takes over.

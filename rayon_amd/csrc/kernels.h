@@ -49,6 +49,9 @@ hipError_t launch_resolve_u8_raw(const float4* acc, const float4* comp, const in
                                  uint8_t* out, int channels, float gamma, hipStream_t st);
 hipError_t launch_heatmap(const int* counts, int npix, uint8_t* out, int channels, int max_samples, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t st);
+// dst[s] = src[perm[s]] for n primitive records (and their 3 vertex normals each when nsrc).
+hipError_t launch_permute_recs(const PrimRec* src, const float4* nsrc, const int* perm, int n, PrimRec* dst,
+                               float4* ndst, hipStream_t st);
 // One launch per kPreviewJobs jobs: `samples` paths per pixel from sample 0, resolved with
 // `gamma` into each job's RGBA8 frame. A job with a `parts` entry (NULL: none) also stores
 // those samples as its first `samples` accumulated samples: its sums, compensations and

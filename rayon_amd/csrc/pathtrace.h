@@ -57,7 +57,6 @@ __device__ inline void nee_request(const JobView& J, const Surf& s, V3 color, V3
    sh.valid = cs > 0.0f && cl > 0.0f;
    if (sh.valid) {
       ++pc.shadow;
-      if constexpr (CNT) ++pc.mats;
       float g = cs * cl / d2 * lr.e.w * (float)S.num_lights * (1.0f / kPiF);
       sh.contrib = mul_rn(T, (g * J.light_intensity) * (color * mk(lr.e)));
    }

@@ -113,7 +113,52 @@ def test_device_tree_small_and_degenerate_scenes(ctx, name):
     del keep
 
 
-def test_device_build_rejects_quantized_layout(ctx):
-    g = ra.Generated.generate(ra.GEN.BOOK_SPHERES, 3)
-    with pytest.raises(ra.RayonError, match="UNSUPPORTED"):
-        ctx.scene(g.desc_ptr(), ra.BVH.DEVICE | ra.BVH.QUANTIZED)
+@pytest.mark.parametrize("name", sorted(GENS))
+def test_device_built_quantized_tree(ctx, name):
+    """RAYON_BVH_DEVICE | RAYON_BVH_QUANTIZED: the device-built tree collapsed into 8-wide
+    quantized nodes passes the wide and binary invariants (rayon_scene_check) and renders the
+    host binary tree's images bit for bit, on the plain kernel and the batched path."""
+    kind, seed, detail, flags = GENS[name]
+    print(f"[bvh_device quantized] {name}", flush=True)
+    g = ra.Generated.generate(kind, seed, detail=detail)
+    host = ctx.scene(g.desc_ptr())
+    dq = ctx.scene(g.desc_ptr(), ra.BVH.DEVICE | ra.BVH.QUANTIZED)
+    qi = dq.info()
+    assert qi.builder == ra.BVH.BUILDER_DEVICE
+    assert qi.num_qnodes > 0 and 0 < qi.wide_depth < 16
+    assert dq.check(g.desc_ptr()) == qi.max_depth
+    lf, la, up, fov = g.camera()
+    w, h = 96, 64
+    cam = ra.camera_frame(lf, la, up, fov, w, h)
+    for batched in (False, True):
+        a = render(ctx, host, cam, w, h, 4, batched, flags)
+        b = render(ctx, dq, cam, w, h, 4, batched, flags)
+        np.testing.assert_array_equal(a, b)
+    host.close()
+    dq.close()
+
+
+def test_device_built_quantized_closest_hits_identical(ctx):
+    g = ra.Generated.generate(ra.GEN.CORNELL_ICOSPHERE, 7, detail=6)
+    host = ctx.scene(g.desc_ptr())
+    dq = ctx.scene(g.desc_ptr(), ra.BVH.DEVICE | ra.BVH.QUANTIZED)
+    assert dq.info().num_qnodes > 0
+    rng = np.random.default_rng(5)
+    n = 20000
+    o = rng.uniform(-0.5, 0.5, (n, 3)).astype(np.float32) + np.float32([278, 278, 278])
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    ph, th = host.trace_closest(o, d)
+    pq, tq = dq.trace_closest(o, d)
+    assert (ph >= 0).mean() > 0.9
+    np.testing.assert_array_equal(ph, pq)
+    np.testing.assert_array_equal(th, tq)
+    host.close()
+    dq.close()
+
+
+def test_scene_check_covers_host_quantized_tree(ctx):
+    g = ra.Generated.generate(ra.GEN.CORNELL_ICOSPHERE, 7, detail=5)
+    q = ctx.scene(g.desc_ptr(), ra.BVH.QUANTIZED)
+    assert q.info().num_qnodes > 0
+    assert q.check(g.desc_ptr()) == q.info().max_depth
+    q.close()
