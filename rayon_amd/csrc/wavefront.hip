@@ -293,8 +293,11 @@ __device__ __forceinline__ int find_job(const WfView& V, long long w) {
 // ---------------------------------------------------------------------------------
 // stages
 // ---------------------------------------------------------------------------------
-// dense: no job of the pass can skip a pixel (no adaptive test), so every slot is live
-// and its level-0 position is the slot itself; otherwise the live paths are packed.
+// A path's level-0 position is its slot. A converged pixel (possible only for adaptive jobs,
+// and rare: their passes cover the pixels still sampling when the pass starts) leaves a hole
+// there, which the trace and shade stages skip; packing the live paths instead took one
+// atomic per wave on the level's counter, which serialised (5.7 ms per serving pass).
+// dense: no job of the pass can skip a pixel (no adaptive test), so no hole is written.
 __global__ __launch_bounds__(kWfBlock) void k_wf_generate(WfView V, long long w0, int n, int dense) {
    const int i = blockIdx.x * kWfBlock + threadIdx.x;
    bool live = false;
@@ -318,17 +321,15 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_generate(WfView V, long long w0
          live = true;
       }
    }
-   int pos = i;
-   if (dense) {
-      if (i == 0) V.ctl[0] = (unsigned)n;
-   } else {
-      pos = wave_push(&V.ctl[0], live);
-   }
+   if (i == 0) V.ctl[0] = (unsigned)n;
    if (live) {
-      V.o[0][pos] = o4;
-      V.d[0][pos] = d4;
-      V.thr[0][pos] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
-      V.ps[0][pos] = i;
+      V.o[0][i] = o4;
+      V.d[0][i] = d4;
+      V.thr[0][i] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
+      V.ps[0][i] = i;
+   } else if (!dense && i < n) {
+      V.d[0][i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kHole));
+      V.ps[0][i] = -1;
    }
 }
 
