@@ -149,22 +149,33 @@ __device__ __forceinline__ int wave_push(unsigned* counter, bool pred) {
    return pred ? (int)(base + (unsigned)__popcll(below)) : -1;
 }
 
-constexpr int kMarchRes = 128;  // marching-queue positions a trace wave reserves at a time
-// Positions beyond the marches themselves that the slice-0 marching queue must hold: every
-// wave of a trace launch may leave at most kMarchRes - 1 reserved positions unfilled
+constexpr int kMarchResMax = 512;  // marching-queue positions a wave reserves at a time, at most
+// Positions beyond the marches themselves that a marching queue must hold: every wave of a
+// trace or marching launch may leave at most kMarchResMax - 1 reserved positions unfilled
 // (at most 32 workgroups of 4 waves per CU).
-constexpr long long kMarchHeadroomPerCu = 32ll * (kWfBlock / 64) * kMarchRes;
+constexpr long long kMarchHeadroomPerCu = 32ll * (kWfBlock / 64) * kMarchResMax;
 // Positions beyond a level's paths that its arrays hold: every wave of a shade launch may
 // leave its last reservation partly unfilled, so the reservations are capped at this
 // headroom divided by the launch's waves (1024 for 16 waves per CU, 128 at 32 workgroups
 // of 4 waves per CU).
 constexpr long long kPosHeadroomPerCu = 16ll * 1024;
 
-// Appends to a marching queue through positions the wave reserves kMarchRes at a time, so
-// the queue's head counter, shared by every wave of a launch, takes one atomic per
-// reservation instead of one per push. close() marks the positions the wave reserved and
-// did not fill with -1; the consumer skips them.
+// Reservation size for a wave's marching-queue pushes in a launch over `count` rays: half
+// of a wave's share (not every ray marches), in multiples of 64 from 64 to kMarchResMax.
+// Atomics on the shared head serialise, so a busy launch takes large reservations; the
+// unfilled tails stay a few percent of the queue.
+__device__ __forceinline__ unsigned march_res(unsigned count) {
+   const unsigned waves = gridDim.x * (blockDim.x >> 6);
+   const unsigned per = count / (128u * waves);
+   return min(64u * max(per, 1u), (unsigned)kMarchResMax);
+}
+
+// Appends to a marching queue through positions the wave reserves `res` at a time (a
+// multiple of 64, wave-uniform), so the queue's head counter, shared by every wave of a
+// launch, takes one atomic per reservation instead of one per push. close() marks the
+// positions the wave reserved and did not fill with -1; the consumer skips them.
 struct QueueRes {
+   unsigned res = 64;
    unsigned next = 0, end = 0;  // wave-uniform
    __device__ __forceinline__ void push(unsigned* head, int* q, bool pred, int value) {
       const unsigned long long m = __ballot(pred);
@@ -175,11 +186,11 @@ struct QueueRes {
       if (pred && rank < have) q[next + rank] = value;
       if (k > have) {
          unsigned base = 0;
-         if (lane_id() == 0) base = atomicAdd(head, (unsigned)kMarchRes);
+         if (lane_id() == 0) base = atomicAdd(head, res);
          base = __shfl(base, 0);
          if (pred && rank >= have) q[base + (rank - have)] = value;
          next = base + (k - have);
-         end = base + (unsigned)kMarchRes;
+         end = base + res;
       } else {
          next += k;
       }
@@ -349,6 +360,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_sdf(WfView V, int level, int it
    const int* qin = (it & 1) ? V.qs1 : V.qs0;
    int* qout = (it & 1) ? V.qs0 : V.qs1;
    QueueRes res;  // rays still marching after this slice
+   res.res = march_res(count);
    drain(&cin[1], count, V.chunk, [&](unsigned idx) {
       int slot = -1;
       bool more = false;
@@ -533,11 +545,12 @@ __device__ void ext_rays_bin(const WfView& V, int level, int* lds, TravCount& tc
    int slot = -1, cur = kSentinel;  // slot: the lane's ray's position in the level
    // slot of a finished ray whose SDF march is still to be queued. The lanes' marches are
    // queued together before each refill, into positions the wave reserved in the marching
-   // queue kMarchRes at a time (one atomic per reservation, not per refill): the queue head
+   // queue march_res(count) at a time (one atomic per reservation, not per refill): the queue head
    // is shared by every wave of the launch. Positions reserved but not filled by the wave's
    // end hold -1, which k_wf_sdf skips.
    int pend = -1;
    QueueRes res;
+   res.res = march_res(count);
    auto queue_marches = [&]() {
       res.push(&mctl[0], V.qs0, pend >= 0, pend);
       pend = -1;
@@ -814,6 +827,7 @@ __device__ void ext_rays_wide(const WfView& V, int level, int* lds, TravCount& t
    uint32_t node = kWideDone;
    uint32_t held_masks = kNoHeld;  // child test of the node set aside for the lane's leaves
    QueueRes res;
+   res.res = march_res(count);
    auto queue_marches = [&]() {  // as in ext_rays_bin
       res.push(&mctl[0], V.qs0, pend >= 0, pend);
       pend = -1;
