@@ -371,6 +371,13 @@ def main(argv=None) -> int:
                 "requested_note": "every node (64 B) and primitive (48 B) record the traversal fetches, plus ray "
                                   "state; most are served by L2 and the 256 MB MALL, so this rate exceeds HBM peak",
                 "requested_bytes_per_ray": round(ext_bytes / max(ext_rays + sh_rays, 1), 1),
+                "visits_per_ray": {
+                    "extension": {"nodes": round(st_c.extend_node_fetches / max(ext_rays, 1), 2),
+                                  "primitives": round(st_c.extend_prim_fetches / max(ext_rays, 1), 2),
+                                  "rays": int(ext_rays)},
+                    "shadow": {"nodes": round(st_c.shadow_node_fetches / max(sh_rays, 1), 2),
+                               "primitives": round(st_c.shadow_prim_fetches / max(sh_rays, 1), 2),
+                               "rays": int(sh_rays)}},
                 "requested_GBs_ref_layout": round(ref_achieved, 1),
                 "valu": valu}
     else:

@@ -82,6 +82,8 @@ struct WfView {
    int chunk;  // 64-entry batches a wave claims per queue-head atomic (trace, marching)
    int shade_chunk;  // the same for the shade stage
    int shadow_refill;  // shadow rays with lane refill (passes without SDF fields)
+   int dense;          // no job of the pass is adaptive: every level-0 position holds a path
+                       // whose slot is the position, throughput 1 and flags 0 (not stored)
    int refill;         // idle lanes (of 64) at which a trace wave takes new rays
    unsigned res_most;  // largest output reservation of a shade wave (multiple of 64)
 };
@@ -336,8 +338,10 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_generate(WfView V, long long w0
    if (live) {
       V.o[0][i] = o4;
       V.d[0][i] = d4;
-      V.thr[0][i] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
-      V.ps[0][i] = i;
+      if (!dense) {  // a dense pass's level 0 implies them (k_wf_shade)
+         V.thr[0][i] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
+         V.ps[0][i] = i;
+      }
    } else if (!dense && i < n) {
       V.d[0][i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kHole));
       V.ps[0][i] = -1;
@@ -1162,11 +1166,16 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_wf_shade(WfView V, int level) {
          float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f), d = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kHole));
          float4 t = o, hf = o;
          if (idx < count) {  // the whole path state at once: the loads overlap
-            slot = V.ps[cur][idx];
             o = V.o[cur][idx];
             d = V.d[cur][idx];
-            t = V.thr[cur][idx];
             hf = V.hit[idx];
+            if (level == 0 && V.dense) {  // fresh camera paths, at their slots
+               slot = (int)idx;
+               t = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
+            } else {
+               slot = V.ps[cur][idx];
+               t = V.thr[cur][idx];
+            }
          }
          const uint32_t job = __float_as_uint(d.w);
          bool pending = slot >= 0;
@@ -1579,6 +1588,7 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
    int dense = 1;
    for (rayon_job* j : jobs)
       if (j->params.flags & (RAYON_FLAG_ADAPTIVE_REF | RAYON_FLAG_ADAPTIVE_STDERR)) dense = 0;
+   V.dense = dense;
    for (size_t k = 0; k < jobs.size(); ++k) {
       rayon_job* j = jobs[k];
       if (todo[k] > 0 && (j->params.flags & RAYON_FLAG_ADAPTIVE_REF) && j->prev) {
