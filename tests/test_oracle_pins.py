@@ -88,7 +88,7 @@ def hemisphere_grid(n_theta, n_phi, lower=False):
 
 # ----------------------------------------------------------------------------- VNDF
 VNDF_CASES = [  # (alpha_x, alpha_y, theta_o deg, phi_o deg)
-    (0.259, 0.259, 30.0, 0.0),   # the C1 icosphere's roughness 0.509
+    (0.259, 0.259, 30.0, 0.0),   # the mesh benchmark icosphere's roughness 0.509
     (0.5, 0.5, 70.0, 40.0),
     (0.15, 0.6, 50.0, 30.0),
     (0.6, 0.15, 60.0, 120.0),
