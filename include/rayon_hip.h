@@ -300,8 +300,13 @@ enum {
    RAYON_OPT_COUNT_TRAVERSAL = 4, /* count node/primitive fetches of the queue stages (0)     */
    RAYON_OPT_WAVE_CHUNK = 5,      /* 64-ray batches a trace wave claims per queue atomic (8) */
    RAYON_OPT_SHADE_CHUNK = 6,     /* 64-hit batches a shade wave claims per queue atomic (16) */
-   RAYON_OPT_TRACE_REFILL = 7     /* idle lanes (of 64) at which a binary-tree trace wave takes
-                                     new rays from the queue (16); 64 traces whole batches     */
+   RAYON_OPT_TRACE_REFILL = 7,    /* idle lanes (of 64) at which a trace wave takes new rays
+                                     from the queue; 64 traces whole batches (0: 32 on binary
+                                     trees, 16 on quantized ones)                              */
+   RAYON_OPT_TRACE_LEAF_EXIT = 8  /* a trace wave tests the leaves its lanes hold once at most
+                                     min(this, lanes holding rays / 4) lanes (/ 3 on quantized
+                                     trees) are still descending to their first leaf; 0 waits
+                                     for every lane (-1: 12 on binary trees, 16 on quantized) */
 };
 rayon_status rayon_ctx_set_option(rayon_ctx* ctx, int32_t option, int64_t value);
 rayon_status rayon_ctx_get_option(rayon_ctx* ctx, int32_t option, int64_t* value);

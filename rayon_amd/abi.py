@@ -28,7 +28,7 @@ YAML = _NS(LIFT_CAPS=1)
 JOB = _NS(QUEUED=0, RUNNING=1, DONE=2, FAILED=3, CANCELLED=4)
 PATH = _NS(AUTO=0, WAVEFRONT=1, MEGAKERNEL=2)
 OPT = _NS(WAVE_CAPACITY=0, WAVE_WG_PER_CU=1, GRAPH=2, STAGE_TIMING=3, COUNT_TRAVERSAL=4, WAVE_CHUNK=5, SHADE_CHUNK=6,
-          TRACE_REFILL=7)
+          TRACE_REFILL=7, TRACE_LEAF_EXIT=8)
 
 
 class Material(C.Structure):

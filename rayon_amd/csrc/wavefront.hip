@@ -86,6 +86,8 @@ struct WfView {
                        // whose slot is the position, throughput 1 and flags 0 (not stored)
    int refill;         // idle lanes (of 64) at which a trace wave takes new rays
    unsigned res_most;  // largest output reservation of a shade wave (multiple of 64)
+   unsigned leaf_exit;  // a trace wave stops descending when at most
+   unsigned leaf_div;   // min(leaf_exit, lanes holding rays / leaf_div) lanes have no leaf
 };
 
 struct Wave {
@@ -107,7 +109,8 @@ struct Wave {
    int opt_wg_per_cu = 0;  // 0: as many as fit (occupancy of each stage kernel)
    int opt_chunk = 8;
    int opt_shade_chunk = 16;
-   int opt_refill = 16;
+   int opt_refill = 0;      // 0: kRefillBin or kRefillWide by the pass's tree layout
+   int opt_leaf_exit = -1;  // -1: kLeafExitBin or kLeafExitWide
    int occ_trace = 0, occ_trace_wide = 0, occ_trace_short = 0, occ_trace_wide_short = 0, occ_shade = 0, occ_shade_lean = 0, occ_sdf = 0, occ_sdf_rest = 0;
    struct Pending {
       int stage;
@@ -536,6 +539,20 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_sdf_rest(WfView V, int level) {
    }
 }
 
+// When a trace wave stops descending to test the leaves its lanes hold (postponed leaves):
+// once no more than min(leaf_exit, lanes holding rays / leaf_div) of its lanes are still
+// looking for their first leaf. Waiting for the last lane (a limit of 0) leaves two thirds
+// of the lanes idle per node step on the 1.31M-triangle mesh; the lanes still looking go on
+// after the leaf tests. Measured defaults per tree layout, with the refill threshold that
+// suits each (a wave that leaves the node loop more often refills better in larger groups).
+constexpr unsigned kLeafExitBin = 12, kLeafDivBin = 4, kLeafExitWide = 16, kLeafDivWide = 3;
+constexpr int kRefillBin = 32, kRefillWide = 16;
+
+__device__ __forceinline__ unsigned leaf_limit(const WfView& V) {
+   const unsigned lanes = (unsigned)__popcll(__ballot(1));  // the lanes holding rays
+   return min(V.leaf_exit, lanes / V.leaf_div);
+}
+
 // Extension rays of one level: closest hit (SDF fields deferred to k_wf_sdf).
 template <bool CNT, int STACK>
 __device__ void ext_rays_bin(const WfView& V, int level, int* lds, TravCount& tc) {
@@ -591,12 +608,13 @@ __device__ void ext_rays_bin(const WfView& V, int level, int* lds, TravCount& tc
       }
       if (slot >= 0) {
          bool finished;
-         // Internal nodes until every lane of the wave holds a leaf or has finished: the
-         // first leaf a lane meets is set aside and the lane goes on with its next node
-         // (postponed leaves, Aila and Laine 2009), so lanes that reach a leaf early keep
-         // working while the others descend. The closest hit does not depend on the order
-         // of the primitive tests (equal distances go to the larger original index).
+         // Internal nodes until nearly every lane of the wave holds a leaf or has finished
+         // (leaf_limit): the first leaf a lane meets is set aside and the lane goes on with
+         // its next node (postponed leaves, Aila and Laine 2009), so lanes that reach a leaf
+         // early keep working while the others descend. The closest hit does not depend on
+         // the order of the primitive tests (equal distances go to the larger original index).
          int leaf = kSentinel;
+         const unsigned lim = leaf_limit(V);
          for (;;) {
             if (cur >= 0) {
                const Node2 n = gload(nodes + cur);
@@ -621,7 +639,7 @@ __device__ void ext_rays_bin(const WfView& V, int level, int* lds, TravCount& tc
                leaf = cur;
                cur = st.pop();
             }
-            if (__all(leaf != kSentinel || cur == kSentinel)) break;
+            if (__popcll(__ballot(leaf == kSentinel && cur != kSentinel)) <= lim) break;
          }
          if (leaf != kSentinel) {  // one leaf
             const int packed = ~leaf;
@@ -702,6 +720,7 @@ __device__ void shadow_rays_bin(const WfView& V, int level, int* lds, TravCount&
       if (slot >= 0) {
          bool occluded = false;
          int leaf = kSentinel;  // postponed leaves, as for extension rays
+         const unsigned lim = leaf_limit(V);
          for (;;) {
             if (cur >= 0) {
                const Node2 n = gload(nodes + cur);
@@ -725,7 +744,7 @@ __device__ void shadow_rays_bin(const WfView& V, int level, int* lds, TravCount&
                leaf = cur;
                cur = st.pop();
             }
-            if (__all(leaf != kSentinel || cur == kSentinel)) break;
+            if (__popcll(__ballot(leaf == kSentinel && cur != kSentinel)) <= lim) break;
          }
          if (leaf != kSentinel) {  // one leaf
             const int packed = ~leaf;
@@ -869,13 +888,15 @@ __device__ void ext_rays_wide(const WfView& V, int level, int* lds, TravCount& t
       if (slot >= 0) {
          // Postponed leaves, as on binary trees: the hit leaf children of the first node that
          // has any are set aside (mask, meta words, primitive base) and the lane goes on with
-         // its next node, until every lane of the wave holds leaves or has finished. A lane
+         // its next node, until nearly every lane of the wave holds leaves or has finished
+         // (leaf_limit; the others go on after the leaf tests). A lane
          // that meets another node with hit leaves meanwhile waits on it, keeping that node's
          // child masks, and takes the node up again after its leaves without testing its
          // children again (masks found with a farther closest hit are a superset of the
          // current ones, so the closest hit is the same).
          uint32_t pmask = 0, pm0 = 0, pm1 = 0, pbase = 0;
          bool held = false;
+         const unsigned lim = leaf_limit(V);
          for (;;) {
             if (node != kWideDone && !held) {
                const WNode n = load_wnode(wn, node);
@@ -901,7 +922,7 @@ __device__ void ext_rays_wide(const WfView& V, int level, int* lds, TravCount& t
                   node = wide_next<STACK>(hit_int, n.child_base, n.e_imask >> 24, r.perm, lds, sp);
                }
             }
-            if (__all(pmask != 0 || node == kWideDone)) break;
+            if (__popcll(__ballot(pmask == 0 && node != kWideDone)) <= lim) break;
          }
          while (pmask) {
             const int sl = __ffs(pmask) - 1;
@@ -961,6 +982,7 @@ __device__ void shadow_rays_wide(const WfView& V, int level, int* lds, TravCount
          bool occluded = false;
          uint32_t pmask = 0, pm0 = 0, pm1 = 0, pbase = 0;  // postponed leaves, as for extension rays
          bool held = false;
+         const unsigned lim = leaf_limit(V);
          for (;;) {
             if (node != kWideDone && !held) {
                const WNode n = load_wnode(wn, node);
@@ -986,7 +1008,7 @@ __device__ void shadow_rays_wide(const WfView& V, int level, int* lds, TravCount
                   node = wide_next<STACK>(hit_int, n.child_base, n.e_imask >> 24, r.perm, lds, sp);
                }
             }
-            if (__all(pmask != 0 || node == kWideDone)) break;
+            if (__popcll(__ballot(pmask == 0 && node != kWideDone)) <= lim) break;
          }
          while (pmask && !occluded) {
             const int sl = __ffs(pmask) - 1;
@@ -1515,7 +1537,6 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
    V.levels = levels;
    V.chunk = W->opt_chunk;
    V.shade_chunk = W->opt_shade_chunk;
-   V.refill = W->opt_refill;
    V.shadow_refill = 1;
    for (rayon_job* j : jobs)
       if (j->scene->view.num_sdf > 0) V.shadow_refill = 0;
@@ -1542,6 +1563,10 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
    int nwide = 0;
    for (rayon_job* j : jobs) nwide += j->scene->view.wnodes != nullptr;
    const int layout = nwide == 0 ? L_BIN : (nwide == (int)jobs.size() ? L_WIDE : L_ANY);
+   const bool wide = layout == L_WIDE;
+   V.refill = W->opt_refill > 0 ? W->opt_refill : (wide ? kRefillWide : kRefillBin);
+   V.leaf_exit = W->opt_leaf_exit >= 0 ? (unsigned)W->opt_leaf_exit : (wide ? kLeafExitWide : kLeafExitBin);
+   V.leaf_div = wide ? kLeafDivWide : kLeafDivBin;
    // binary trees no deeper than the short stack take the kernel with the smaller LDS stack
    int depth = 0;
    for (rayon_job* j : jobs) depth = std::max(depth, j->scene->info.max_depth);
@@ -1816,6 +1841,7 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
       key.push_back(W->opt_chunk);
       key.push_back(W->opt_shade_chunk);
       key.push_back(W->opt_refill);
+      key.push_back(W->opt_leaf_exit);
       key.push_back(W->opt_count);
       for (rayon_job* j : active) key.push_back(j->scene->view.wnodes != nullptr);
       int sdf_steps = 0;  // the marching slices the pass enqueues
@@ -1914,11 +1940,18 @@ rayon_status wave_set_option(rayon_ctx* ctx, int opt, long long value) {
       W->opt_shade_chunk = (int)value;
       break;
    case RAYON_OPT_TRACE_REFILL:
-      if (value < 1 || value > 64) {
-         set_error("RAYON_OPT_TRACE_REFILL must be in [1, 64]");
+      if (value < 0 || value > 64) {
+         set_error("RAYON_OPT_TRACE_REFILL must be in [0, 64] (0: the tree layout's default)");
          return RAYON_ERR_INVALID;
       }
       W->opt_refill = (int)value;
+      break;
+   case RAYON_OPT_TRACE_LEAF_EXIT:
+      if (value < -1 || value > 64) {
+         set_error("RAYON_OPT_TRACE_LEAF_EXIT must be in [-1, 64] (-1: the tree layout's default)");
+         return RAYON_ERR_INVALID;
+      }
+      W->opt_leaf_exit = (int)value;
       break;
    case RAYON_OPT_GRAPH:
       W->opt_graph = value != 0;
@@ -1945,6 +1978,7 @@ rayon_status wave_get_option(rayon_ctx* ctx, int opt, long long* value) {
    case RAYON_OPT_WAVE_CHUNK: *value = W->opt_chunk; break;
    case RAYON_OPT_SHADE_CHUNK: *value = W->opt_shade_chunk; break;
    case RAYON_OPT_TRACE_REFILL: *value = W->opt_refill; break;
+   case RAYON_OPT_TRACE_LEAF_EXIT: *value = W->opt_leaf_exit; break;
    case RAYON_OPT_GRAPH: *value = W->opt_graph; break;
    case RAYON_OPT_STAGE_TIMING: *value = W->opt_timing; break;
    case RAYON_OPT_COUNT_TRAVERSAL: *value = W->opt_count; break;

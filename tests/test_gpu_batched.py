@@ -342,13 +342,59 @@ def test_quantized_wavefront_refill_equals_per_ray_traversal(ctx, scene, kw):
                 other.close()
             return img, segs
 
-        ref, segs = run(True, 16)
+        ref, segs = run(True, 0)
         log(f"quantized {scene}: per-ray traversal, {segs} segments")
-        for refill in (16, 1, 64):
+        for refill, leaf_exit in ((0, -1), (1, 0), (64, 64), (16, 0)):
+            ctx.set_option(ra.OPT.TRACE_LEAF_EXIT, leaf_exit)
             img, s2 = run(False, refill)
-            np.testing.assert_array_equal(img, ref, err_msg=f"refill {refill}")
+            np.testing.assert_array_equal(img, ref, err_msg=f"refill {refill}, leaf exit {leaf_exit}")
             assert s2 == segs
     finally:
-        ctx.set_option(ra.OPT.TRACE_REFILL, 16)
+        ctx.set_option(ra.OPT.TRACE_REFILL, 0)
+        ctx.set_option(ra.OPT.TRACE_LEAF_EXIT, -1)
         q.close()
         b.close()
+
+
+@pytest.mark.parametrize("scene,kw", [(CORN, dict(max_depth=8, flags=ra.FLAG.NEE)), (SDF, dict(max_depth=8)),
+                                      (BOOK, dict(max_depth=50))], ids=["cornell_nee", "sdf", "book"])
+def test_binary_wavefront_leaf_exit_and_refill_keep_images(ctx, scene, kw):
+    """Binary trees on the wavefront: when a trace wave stops descending to test its lanes'
+    postponed leaves (RAYON_OPT_TRACE_LEAF_EXIT) and how many idle lanes it refills at once
+    (RAYON_OPT_TRACE_REFILL) change only the order of the node visits and primitive tests,
+    so every setting gives the same image bit for bit, equal to the per-ray traversal (a
+    pass batched with a quantized-tree job)."""
+    g = ra.Generated.generate(scene[0], scene[1], detail=scene[2])
+    b = ctx.scene(g.desc_ptr())
+    gq = ra.Generated.generate(*BOOK[:2], detail=BOOK[2])
+    q = ctx.scene(gq.desc_ptr(), ra.BVH.QUANTIZED)
+    try:
+        b.set_batch_path(ra.PATH.WAVEFRONT)
+        q.set_batch_path(ra.PATH.WAVEFRONT)
+        lf, la, up, fov = g.camera()
+        P = ra.job_params(72, 40, ra.camera_frame(lf, la, up, fov, 72, 40), spp=12, seed=23, **kw)
+        lfq, laq, upq, fovq = gq.camera()
+        Pq = ra.job_params(40, 24, ra.camera_frame(lfq, laq, upq, fovq, 40, 24), spp=12, seed=3, max_depth=8)
+
+        def run(with_quantized):
+            j = ctx.job(b, P)
+            other = ctx.job(q, Pq) if with_quantized else None
+            ctx.render_batched([j] + ([other] if other else []), 12)
+            img, segs = j.resolve_linear(), j.stats().segments
+            j.close()
+            if other:
+                other.close()
+            return img, segs
+
+        ref, segs = run(True)
+        for refill, leaf_exit in ((0, -1), (1, 0), (64, 64), (8, 64), (32, 0)):
+            ctx.set_option(ra.OPT.TRACE_REFILL, refill)
+            ctx.set_option(ra.OPT.TRACE_LEAF_EXIT, leaf_exit)
+            img, s2 = run(False)
+            np.testing.assert_array_equal(img, ref, err_msg=f"refill {refill}, leaf exit {leaf_exit}")
+            assert s2 == segs
+    finally:
+        ctx.set_option(ra.OPT.TRACE_REFILL, 0)
+        ctx.set_option(ra.OPT.TRACE_LEAF_EXIT, -1)
+        b.close()
+        q.close()
