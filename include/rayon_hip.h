@@ -298,7 +298,8 @@ enum {
    RAYON_OPT_GRAPH = 2,           /* capture each pass into a hipGraph and replay it (1)      */
    RAYON_OPT_STAGE_TIMING = 3,    /* time every stage launch with HIP events (0; disables graphs) */
    RAYON_OPT_COUNT_TRAVERSAL = 4, /* count node/primitive fetches of the queue stages (0)     */
-   RAYON_OPT_WAVE_CHUNK = 5,      /* 64-ray batches a trace wave claims per queue atomic (8) */
+   RAYON_OPT_WAVE_CHUNK = 5,      /* 64-ray batches a trace wave claims per queue atomic
+                                     (0: 8 on binary trees, 32 on quantized ones)            */
    RAYON_OPT_SHADE_CHUNK = 6,     /* 64-hit batches a shade wave claims per queue atomic (16) */
    RAYON_OPT_TRACE_REFILL = 7,    /* idle lanes (of 64) at which a trace wave takes new rays
                                      from the queue; 64 traces whole batches (0: 32 on binary
