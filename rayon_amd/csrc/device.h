@@ -285,15 +285,37 @@ constexpr int kSdfMaskMax = 64;
 
 __device__ __forceinline__ float sdf_margin(const SceneView& S) { return fmaxf(10.0f * S.sdf_k, 1.0f); }
 
-__device__ __forceinline__ float sdf_field_masked(const SceneView& S, V3 p, uint64_t mask) {
+// Where the field's primitive records are read from: the scene's array in device memory,
+// or a copy in the workgroup's LDS (a marching kernel stages the records of a pass whose
+// fields all belong to one scene). Both hold the same values, so the field is the same.
+struct SdfGlobalRecs {
+   const SdfRec* p;
+   __device__ __forceinline__ SdfRec operator()(int i) const { return gload(p + i); }
+};
+struct SdfLdsRecs {
+   const __attribute__((address_space(3))) gvec4* p;  // LDS, two vectors per record: ds_read, not flat loads
+   __device__ __forceinline__ SdfRec operator()(int i) const {
+      const gvec4 w[2] = {p[2 * i], p[2 * i + 1]};
+      SdfRec r;
+      __builtin_memcpy(&r, w, sizeof(r));
+      return r;
+   }
+};
+
+template <class REC>
+__device__ __forceinline__ float sdf_field_masked(const SceneView& S, V3 p, uint64_t mask, REC rec) {
    const float k = S.sdf_k, ik = 1.0f / S.sdf_k;
    int i = __builtin_ctzll(mask);
-   float d = sdf_prim(gload(S.sdf + i), p);
+   float d = sdf_prim(rec(i), p);
    for (mask &= mask - 1; mask; mask &= mask - 1) {
       i = __builtin_ctzll(mask);
-      d = smooth_union(d, sdf_prim(gload(S.sdf + i), p), k, ik);
+      d = smooth_union(d, sdf_prim(rec(i), p), k, ik);
    }
    return d;
+}
+
+__device__ __forceinline__ float sdf_field_masked(const SceneView& S, V3 p, uint64_t mask) {
+   return sdf_field_masked(S, p, mask, SdfGlobalRecs{S.sdf});
 }
 
 // Primitives within `margin` of the segment o + t*dn, t in [t0, t1]; `far` receives the
@@ -381,13 +403,18 @@ __device__ __forceinline__ bool sdf_march_begin(const SceneView& S, const PrimRe
 
 // Up to `steps` more steps. Returns 1 on a hit (t = ray parameter), 0 when the march ended
 // without one, -1 when it needs more steps.
-__device__ __forceinline__ int sdf_march_run(const SceneView& S, V3 o, V3 d, SdfMarch& m, int steps, float& t) {
+// REC: the primitive records of a field of at most kSdfMaskMax primitives (larger fields
+// are read from the scene).
+template <class REC>
+__device__ __forceinline__ int sdf_march_run(const SceneView& S, V3 o, V3 d, SdfMarch& m, int steps, float& t,
+                                             REC rec) {
    const float dl = sqrtf(dot(d, d));
    const float idl = 1.0f / dl;
    const V3 dn = idl * d;
    const int stop = min(S.sdf_steps, m.used + steps);
    for (; m.used < stop; ++m.used) {
-      const float dist = sdf_field_near(S, o + m.tw * dn, m.mask);
+      const V3 q = o + m.tw * dn;
+      const float dist = S.num_sdf > kSdfMaskMax ? sdf_field(S, q) : sdf_field_masked(S, q, m.mask, rec);
       if (dist < S.sdf_eps) {
          t = m.tw * idl;
          return 1;
@@ -396,6 +423,10 @@ __device__ __forceinline__ int sdf_march_run(const SceneView& S, V3 o, V3 d, Sdf
       if (m.tw > m.tend) return 0;
    }
    return m.used >= S.sdf_steps ? 0 : -1;
+}
+
+__device__ __forceinline__ int sdf_march_run(const SceneView& S, V3 o, V3 d, SdfMarch& m, int steps, float& t) {
+   return sdf_march_run(S, o, d, m, steps, t, SdfGlobalRecs{S.sdf});
 }
 
 __device__ inline bool isect_sdf(const SceneView& S, const PrimRec& p, V3 o, V3 d, float tmin, float tmax, float& t) {

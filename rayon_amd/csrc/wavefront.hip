@@ -88,6 +88,8 @@ struct WfView {
    unsigned res_most;  // largest output reservation of a shade wave (multiple of 64)
    unsigned leaf_exit;  // a trace wave stops descending when at most
    unsigned leaf_div;   // min(leaf_exit, lanes holding rays / leaf_div) lanes have no leaf
+   const SdfRec* sdf_one;  // the field records when every SDF scene of the pass is this one
+   int sdf_one_n;          // (at most kSdfMaskMax records; null otherwise)
 };
 
 struct Wave {
@@ -358,8 +360,29 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_generate(WfView V, long long w0
 // which has the largest original index). Queues hold the rays' positions in the level; the
 // march state lives in the shadow-ray arrays at the same position, free between the trace
 // stage (which consumed the previous level's shadow rays) and the shade stage.
+// The marching kernels copy the field's records of a one-field pass into LDS once per
+// workgroup: every step of every march reads a record per primitive near the ray, and from
+// LDS those reads no longer wait on the cache hierarchy.
+__device__ __forceinline__ void stage_sdf_records(const WfView& V, SdfRec* lsdf) {
+   if (V.sdf_one)
+      for (int i = threadIdx.x; i < V.sdf_one_n; i += kWfBlock) lsdf[i] = gload(V.sdf_one + i);
+   __syncthreads();
+}
+
+// A march of `steps` more steps with the records from LDS when the ray's field is the
+// staged one.
+__device__ __forceinline__ int march_staged(const WfView& V, const SdfRec* lsdf, const SceneView& S, V3 o, V3 d,
+                                            SdfMarch& m, int steps, float& t) {
+   if (V.sdf_one != nullptr && S.sdf == V.sdf_one)
+      return sdf_march_run(S, o, d, m, steps, t,
+                           SdfLdsRecs{(const __attribute__((address_space(3))) gvec4*)lsdf});
+   return sdf_march_run(S, o, d, m, steps, t);
+}
+
 template <bool CNT>
 __global__ __launch_bounds__(kWfBlock) void k_wf_sdf(WfView V, int level, int it, int last) {
+   __shared__ SdfRec lsdf[kSdfMaskMax];
+   stage_sdf_records(V, lsdf);
    unsigned long long n_begin = 0, n_steps = 0, n_evals = 0, n_slots = 0;
    unsigned* cin = V.sctl + 2 * ((kSdfSlices + 1) * level + it);
    unsigned* cout = cin + 2;
@@ -392,7 +415,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_sdf(WfView V, int level, int it
          if (live) {
             float t = 0.0f;
             const int used0 = m.used;
-            const int r = sdf_march_run(S, o, d, m, last ? S.sdf_steps : (kSdfSlice << it), t);
+            const int r = march_staged(V, lsdf, S, o, d, m, last ? S.sdf_steps : (kSdfSlice << it), t);
             if constexpr (CNT) {
                // steps evaluated: the loop's count, plus the step that returned from inside it
                taken = (unsigned)(m.used - used0 + (r >= 0 && m.used < S.sdf_steps ? 1 : 0));
@@ -483,6 +506,8 @@ constexpr int kSdfBurst = 4;
 
 template <bool CNT>
 __global__ __launch_bounds__(kWfBlock) void k_wf_sdf_rest(WfView V, int level) {
+   __shared__ SdfRec lsdf[kSdfMaskMax];
+   stage_sdf_records(V, lsdf);
    unsigned long long n_steps = 0, n_evals = 0, n_slots = 0;
    unsigned* cin = V.sctl + 2 * ((kSdfSlices + 1) * level + 1);
    const unsigned count = cin[0];
@@ -518,7 +543,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_sdf_rest(WfView V, int level) {
       if (slot >= 0) {
          float t = 0.0f;
          const int used0 = m.used;
-         const int r = sdf_march_run(*S, o, d, m, kSdfBurst, t);
+         const int r = march_staged(V, lsdf, *S, o, d, m, kSdfBurst, t);
          if constexpr (CNT) {
             taken = (unsigned)(m.used - used0 + (r >= 0 && m.used < S->sdf_steps ? 1 : 0));
             n_steps += taken;
@@ -1558,6 +1583,23 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
    V.shadow_refill = 1;
    for (rayon_job* j : jobs)
       if (j->scene->view.num_sdf > 0) V.shadow_refill = 0;
+   // the marching kernels stage the field's records in LDS when the pass has one field
+   V.sdf_one = nullptr;
+   V.sdf_one_n = 0;
+   {
+      const SceneView* one = nullptr;
+      bool single = true;
+      for (rayon_job* j : jobs) {
+         const SceneView& sv = j->scene->view;
+         if (sv.num_sdf == 0) continue;
+         if (one && one->sdf != sv.sdf) single = false;
+         one = &sv;
+      }
+      if (one && single && one->num_sdf <= kSdfMaskMax) {
+         V.sdf_one = one->sdf;
+         V.sdf_one_n = one->num_sdf;
+      }
+   }
    // persistent grids: the resident workgroups, but no more than one lane per path of a
    // round (small passes then launch and drain quickly)
    const long long per_round = std::min<long long>(std::max(total, 1ll), W->round);
