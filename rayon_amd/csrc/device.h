@@ -322,12 +322,15 @@ __device__ __forceinline__ float sdf_field_masked(const SceneView& S, V3 p, uint
 // smallest distance bound of the others. Each primitive is tested through its bounding
 // sphere (S.sdf_sph: centre, radius), comparing squared distances with the sphere grown
 // by the margin, so only the distance bounds of the excluded primitives take a root.
+// SPH: the bounding spheres, from the scene (a float4 pointer in device memory) or from a
+// copy in LDS (a pointer in address space 3; every lane reads the same sphere at once).
+template <class SPH>
 __device__ __forceinline__ uint64_t sdf_ray_mask(const SceneView& S, V3 o, V3 dn, float t0, float t1, float margin,
-                                                 float& far) {
+                                                 float& far, SPH sph) {
    uint64_t mask = 0;
    far = 3.4e38f;
    for (int i = 0; i < S.num_sdf; ++i) {
-      const float4 b = gload(S.sdf_sph + i);
+      const float4 b = sph(i);
       const V3 oc = mk(b) - o;
       const float tc = fminf(fmaxf(dot(oc, dn), t0), t1);
       const V3 q = tc * dn - oc;
@@ -369,8 +372,21 @@ struct SdfMarch {
 };
 
 // Returns false when the ray cannot meet the field in [tmin, tmax].
+struct SdfGlobalSph {
+   const float4* p;
+   __device__ __forceinline__ float4 operator()(int i) const { return gload(p + i); }
+};
+struct SdfLdsSph {
+   const __attribute__((address_space(3))) gvec4* p;
+   __device__ __forceinline__ float4 operator()(int i) const {
+      const gvec4 w = p[i];
+      return make_float4(w.x, w.y, w.z, w.w);
+   }
+};
+
+template <class SPH>
 __device__ __forceinline__ bool sdf_march_begin(const SceneView& S, const PrimRec& p, V3 o, V3 d, float tmin,
-                                                float tmax, SdfMarch& m) {
+                                                float tmax, SdfMarch& m, SPH sph) {
    const float dl = sqrtf(dot(d, d));
    const V3 dn = (1.0f / dl) * d;
    // the march (start, primitive subset, step cap) depends on tmin and the box only, not
@@ -391,7 +407,7 @@ __device__ __forceinline__ bool sdf_march_begin(const SceneView& S, const PrimRe
    m.cap = 3.4e38f;
    if (S.num_sdf <= kSdfMaskMax) {
       float far;
-      m.mask = sdf_ray_mask(S, o, dn, t0, tb, sdf_margin(S), far);
+      m.mask = sdf_ray_mask(S, o, dn, t0, tb, sdf_margin(S), far, sph);
       if (!m.mask) return false;  // every primitive stays >= margin away: no surface on the segment
       m.cap = far - 2.0f * S.sdf_k;
    }
@@ -399,6 +415,11 @@ __device__ __forceinline__ bool sdf_march_begin(const SceneView& S, const PrimRe
    m.tend = t1;
    m.used = 0;
    return true;
+}
+
+__device__ __forceinline__ bool sdf_march_begin(const SceneView& S, const PrimRec& p, V3 o, V3 d, float tmin,
+                                                float tmax, SdfMarch& m) {
+   return sdf_march_begin(S, p, o, d, tmin, tmax, m, SdfGlobalSph{S.sdf_sph});
 }
 
 // Up to `steps` more steps. Returns 1 on a hit (t = ray parameter), 0 when the march ended

@@ -89,6 +89,7 @@ struct WfView {
    unsigned leaf_exit;  // a trace wave stops descending when at most
    unsigned leaf_div;   // min(leaf_exit, lanes holding rays / leaf_div) lanes have no leaf
    const SdfRec* sdf_one;  // the field records when every SDF scene of the pass is this one
+   const float4* sdf_one_sph;  // and their bounding spheres
    int sdf_one_n;          // (at most kSdfMaskMax records; null otherwise)
 };
 
@@ -363,9 +364,12 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_generate(WfView V, long long w0
 // The marching kernels copy the field's records of a one-field pass into LDS once per
 // workgroup: every step of every march reads a record per primitive near the ray, and from
 // LDS those reads no longer wait on the cache hierarchy.
-__device__ __forceinline__ void stage_sdf_records(const WfView& V, SdfRec* lsdf) {
+__device__ __forceinline__ void stage_sdf_records(const WfView& V, SdfRec* lsdf, float4* lsph = nullptr) {
    if (V.sdf_one)
-      for (int i = threadIdx.x; i < V.sdf_one_n; i += kWfBlock) lsdf[i] = gload(V.sdf_one + i);
+      for (int i = threadIdx.x; i < V.sdf_one_n; i += kWfBlock) {
+         lsdf[i] = gload(V.sdf_one + i);
+         if (lsph) lsph[i] = gload(V.sdf_one_sph + i);
+      }
    __syncthreads();
 }
 
@@ -382,7 +386,8 @@ __device__ __forceinline__ int march_staged(const WfView& V, const SdfRec* lsdf,
 template <bool CNT>
 __global__ __launch_bounds__(kWfBlock) void k_wf_sdf(WfView V, int level, int it, int last) {
    __shared__ SdfRec lsdf[kSdfMaskMax];
-   stage_sdf_records(V, lsdf);
+   __shared__ float4 lsph[kSdfMaskMax];  // bounding spheres, for each march's primitive subset
+   stage_sdf_records(V, lsdf, lsph);
    unsigned long long n_begin = 0, n_steps = 0, n_evals = 0, n_slots = 0;
    unsigned* cin = V.sctl + 2 * ((kSdfSlices + 1) * level + it);
    unsigned* cout = cin + 2;
@@ -403,7 +408,12 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_sdf(WfView V, int level, int it
          SdfMarch m;
          bool live = true;
          if (it == 0) {
-            live = sdf_march_begin(S, gload(S.prims + S.sdf_prim), o, d, 0.001f, V.hit[slot].x, m);
+            const PrimRec fp = gload(S.prims + S.sdf_prim);
+            if (V.sdf_one != nullptr && S.sdf == V.sdf_one)
+               live = sdf_march_begin(S, fp, o, d, 0.001f, V.hit[slot].x, m,
+                                      SdfLdsSph{(const __attribute__((address_space(3))) gvec4*)lsph});
+            else
+               live = sdf_march_begin(S, fp, o, d, 0.001f, V.hit[slot].x, m);
          } else {
             const float4 a = V.so[slot], b = V.sc[slot];
             m.tw = a.x;
@@ -1585,6 +1595,7 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
       if (j->scene->view.num_sdf > 0) V.shadow_refill = 0;
    // the marching kernels stage the field's records in LDS when the pass has one field
    V.sdf_one = nullptr;
+   V.sdf_one_sph = nullptr;
    V.sdf_one_n = 0;
    {
       const SceneView* one = nullptr;
@@ -1597,6 +1608,7 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
       }
       if (one && single && one->num_sdf <= kSdfMaskMax) {
          V.sdf_one = one->sdf;
+         V.sdf_one_sph = one->sdf_sph;
          V.sdf_one_n = one->num_sdf;
       }
    }
