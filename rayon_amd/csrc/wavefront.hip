@@ -88,9 +88,6 @@ struct WfView {
    unsigned res_most;  // largest output reservation of a shade wave (multiple of 64)
    unsigned leaf_exit;  // a trace wave stops descending when at most
    unsigned leaf_div;   // min(leaf_exit, lanes holding rays / leaf_div) lanes have no leaf
-   const SdfRec* sdf_one;  // the field records when every SDF scene of the pass is this one
-   const float4* sdf_one_sph;  // and their bounding spheres
-   int sdf_one_n;          // (at most kSdfMaskMax records; null otherwise)
 };
 
 struct Wave {
@@ -361,33 +358,53 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_generate(WfView V, long long w0
 // which has the largest original index). Queues hold the rays' positions in the level; the
 // march state lives in the shadow-ray arrays at the same position, free between the trace
 // stage (which consumed the previous level's shadow rays) and the shade stage.
-// The marching kernels copy the field's records of a one-field pass into LDS once per
-// workgroup: every step of every march reads a record per primitive near the ray, and from
-// LDS those reads no longer wait on the cache hierarchy.
-__device__ __forceinline__ void stage_sdf_records(const WfView& V, SdfRec* lsdf, float4* lsph = nullptr) {
-   if (V.sdf_one)
-      for (int i = threadIdx.x; i < V.sdf_one_n; i += kWfBlock) {
-         lsdf[i] = gload(V.sdf_one + i);
-         if (lsph) lsph[i] = gload(V.sdf_one_sph + i);
+// Each marching wave keeps one field of at most kSdfMaskMax primitives in its own LDS slot
+// (records, and for the set-up the bounding spheres): every step of every march reads a
+// record per primitive near the ray, and from LDS those reads no longer wait on the cache
+// hierarchy. A wave's rays nearly always share a field (queues keep work order); when the
+// first lane with a field has another one than the slot holds, the wave reloads the slot
+// (one record per lane) before any lane marches. Lanes of other fields read the scene.
+struct FieldSlot {
+   SdfRec* rec;
+   float4* sph;             // null: no spheres kept
+   const SdfRec* held = nullptr;  // the field in the slot (the same in every lane)
+};
+
+// True when this lane's field (S may be null for an idle lane) is the one in the slot.
+__device__ __forceinline__ bool hold_field(FieldSlot& f, const SceneView* S) {
+   const SdfRec* mine = (S != nullptr && S->num_sdf > 0 && S->num_sdf <= kSdfMaskMax) ? S->sdf : nullptr;
+   const unsigned long long has = __ballot(mine != nullptr);
+   if (has == 0) return false;
+   const int first = __ffsll((long long)has) - 1;
+   const SdfRec* want = (const SdfRec*)__shfl((unsigned long long)(uintptr_t)mine, first);
+   if (want != f.held) {
+      const SceneView* Sf = (const SceneView*)__shfl((unsigned long long)(uintptr_t)S, first);
+      const int n = Sf->num_sdf, i = lane_id();
+      if (i < n) {
+         f.rec[i] = gload(want + i);
+         if (f.sph) f.sph[i] = gload(Sf->sdf_sph + i);
       }
-   __syncthreads();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      f.held = want;
+   }
+   return mine != nullptr && mine == f.held;
 }
 
-// A march of `steps` more steps with the records from LDS when the ray's field is the
-// staged one.
-__device__ __forceinline__ int march_staged(const WfView& V, const SdfRec* lsdf, const SceneView& S, V3 o, V3 d,
-                                            SdfMarch& m, int steps, float& t) {
-   if (V.sdf_one != nullptr && S.sdf == V.sdf_one)
-      return sdf_march_run(S, o, d, m, steps, t,
-                           SdfLdsRecs{(const __attribute__((address_space(3))) gvec4*)lsdf});
+// A march of `steps` more steps, with the records from the wave's slot when `held`.
+__device__ __forceinline__ int march_held(const FieldSlot& f, bool held, const SceneView& S, V3 o, V3 d,
+                                          SdfMarch& m, int steps, float& t) {
+   if (held)
+      return sdf_march_run(S, o, d, m, steps, t, SdfLdsRecs{(const __attribute__((address_space(3))) gvec4*)f.rec});
    return sdf_march_run(S, o, d, m, steps, t);
 }
 
 template <bool CNT>
 __global__ __launch_bounds__(kWfBlock) void k_wf_sdf(WfView V, int level, int it, int last) {
-   __shared__ SdfRec lsdf[kSdfMaskMax];
-   __shared__ float4 lsph[kSdfMaskMax];  // bounding spheres, for each march's primitive subset
-   stage_sdf_records(V, lsdf, lsph);
+   __shared__ SdfRec lrec[kWfBlock / 64][kSdfMaskMax];
+   __shared__ float4 lsph[kWfBlock / 64][kSdfMaskMax];  // bounding spheres, for the set-up
+   FieldSlot fs{lrec[threadIdx.x / 64], it == 0 ? lsph[threadIdx.x / 64] : nullptr};
    unsigned long long n_begin = 0, n_steps = 0, n_evals = 0, n_slots = 0;
    unsigned* cin = V.sctl + 2 * ((kSdfSlices + 1) * level + it);
    unsigned* cout = cin + 2;
@@ -401,17 +418,23 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_sdf(WfView V, int level, int it
       bool more = false;
       unsigned taken = 0;
       if (idx < count) slot = qin[idx];  // -1: a position a trace wave reserved and left unused
+      float4 d4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      const SceneView* Sp = nullptr;
       if (slot >= 0) {
-         const float4 d4 = V.d[level & 1][slot];
-         const SceneView& S = V.jobs[__float_as_uint(d4.w)].scene;
+         d4 = V.d[level & 1][slot];
+         Sp = &V.jobs[__float_as_uint(d4.w)].scene;
+      }
+      const bool held = hold_field(fs, Sp);
+      if (slot >= 0) {
+         const SceneView& S = *Sp;
          const V3 o = mk(V.o[level & 1][slot]), d = mk(d4);
          SdfMarch m;
          bool live = true;
          if (it == 0) {
             const PrimRec fp = gload(S.prims + S.sdf_prim);
-            if (V.sdf_one != nullptr && S.sdf == V.sdf_one)
+            if (held)
                live = sdf_march_begin(S, fp, o, d, 0.001f, V.hit[slot].x, m,
-                                      SdfLdsSph{(const __attribute__((address_space(3))) gvec4*)lsph});
+                                      SdfLdsSph{(const __attribute__((address_space(3))) gvec4*)fs.sph});
             else
                live = sdf_march_begin(S, fp, o, d, 0.001f, V.hit[slot].x, m);
          } else {
@@ -425,7 +448,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_sdf(WfView V, int level, int it
          if (live) {
             float t = 0.0f;
             const int used0 = m.used;
-            const int r = march_staged(V, lsdf, S, o, d, m, last ? S.sdf_steps : (kSdfSlice << it), t);
+            const int r = march_held(fs, held, S, o, d, m, last ? S.sdf_steps : (kSdfSlice << it), t);
             if constexpr (CNT) {
                // steps evaluated: the loop's count, plus the step that returned from inside it
                taken = (unsigned)(m.used - used0 + (r >= 0 && m.used < S.sdf_steps ? 1 : 0));
@@ -516,8 +539,8 @@ constexpr int kSdfBurst = 4;
 
 template <bool CNT>
 __global__ __launch_bounds__(kWfBlock) void k_wf_sdf_rest(WfView V, int level) {
-   __shared__ SdfRec lsdf[kSdfMaskMax];
-   stage_sdf_records(V, lsdf);
+   __shared__ SdfRec lrec[kWfBlock / 64][kSdfMaskMax];
+   FieldSlot fs{lrec[threadIdx.x / 64], nullptr};
    unsigned long long n_steps = 0, n_evals = 0, n_slots = 0;
    unsigned* cin = V.sctl + 2 * ((kSdfSlices + 1) * level + 1);
    const unsigned count = cin[0];
@@ -550,10 +573,11 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_sdf_rest(WfView V, int level) {
          continue;
       }
       unsigned taken = 0;
+      const bool held = hold_field(fs, slot >= 0 ? S : nullptr);
       if (slot >= 0) {
          float t = 0.0f;
          const int used0 = m.used;
-         const int r = march_staged(V, lsdf, *S, o, d, m, kSdfBurst, t);
+         const int r = march_held(fs, held, *S, o, d, m, kSdfBurst, t);
          if constexpr (CNT) {
             taken = (unsigned)(m.used - used0 + (r >= 0 && m.used < S->sdf_steps ? 1 : 0));
             n_steps += taken;
@@ -1593,25 +1617,6 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
    V.shadow_refill = 1;
    for (rayon_job* j : jobs)
       if (j->scene->view.num_sdf > 0) V.shadow_refill = 0;
-   // the marching kernels stage the field's records in LDS when the pass has one field
-   V.sdf_one = nullptr;
-   V.sdf_one_sph = nullptr;
-   V.sdf_one_n = 0;
-   {
-      const SceneView* one = nullptr;
-      bool single = true;
-      for (rayon_job* j : jobs) {
-         const SceneView& sv = j->scene->view;
-         if (sv.num_sdf == 0) continue;
-         if (one && one->sdf != sv.sdf) single = false;
-         one = &sv;
-      }
-      if (one && single && one->num_sdf <= kSdfMaskMax) {
-         V.sdf_one = one->sdf;
-         V.sdf_one_sph = one->sdf_sph;
-         V.sdf_one_n = one->num_sdf;
-      }
-   }
    // persistent grids: the resident workgroups, but no more than one lane per path of a
    // round (small passes then launch and drain quickly)
    const long long per_round = std::min<long long>(std::max(total, 1ll), W->round);
