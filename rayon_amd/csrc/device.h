@@ -302,6 +302,19 @@ struct SdfLdsRecs {
    }
 };
 
+// The bounding spheres, from the scene or from an LDS copy.
+struct SdfGlobalSph {
+   const float4* p;
+   __device__ __forceinline__ float4 operator()(int i) const { return gload(p + i); }
+};
+struct SdfLdsSph {
+   const __attribute__((address_space(3))) gvec4* p;
+   __device__ __forceinline__ float4 operator()(int i) const {
+      const gvec4 w = p[i];
+      return make_float4(w.x, w.y, w.z, w.w);
+   }
+};
+
 template <class REC>
 __device__ __forceinline__ float sdf_field_masked(const SceneView& S, V3 p, uint64_t mask, REC rec) {
    const float k = S.sdf_k, ik = 1.0f / S.sdf_k;
@@ -343,21 +356,29 @@ __device__ __forceinline__ uint64_t sdf_ray_mask(const SceneView& S, V3 o, V3 dn
 }
 
 // Field at a point from the primitives within `margin` of it (all of them for large fields).
+template <class REC>
+__device__ __forceinline__ float sdf_field_near(const SceneView& S, V3 p, uint64_t mask, REC rec) {
+   return S.num_sdf > kSdfMaskMax ? sdf_field(S, p) : sdf_field_masked(S, p, mask, rec);
+}
 __device__ __forceinline__ float sdf_field_near(const SceneView& S, V3 p, uint64_t mask) {
-   return S.num_sdf > kSdfMaskMax ? sdf_field(S, p) : sdf_field_masked(S, p, mask);
+   return sdf_field_near(S, p, mask, SdfGlobalRecs{S.sdf});
 }
 
-__device__ __forceinline__ uint64_t sdf_point_mask(const SceneView& S, V3 p) {
+template <class SPH>
+__device__ __forceinline__ uint64_t sdf_point_mask(const SceneView& S, V3 p, SPH sph) {
    if (S.num_sdf > kSdfMaskMax) return ~0ull;
    const float margin = sdf_margin(S);
    uint64_t mask = 0;
    for (int i = 0; i < S.num_sdf; ++i) {
-      const float4 b = gload(S.sdf_sph + i);
+      const float4 b = sph(i);
       const V3 q = p - mk(b);
       const float r = margin + b.w;
       mask |= dot(q, q) <= r * r ? 1ull << i : 0ull;
    }
    return mask ? mask : 1ull;
+}
+__device__ __forceinline__ uint64_t sdf_point_mask(const SceneView& S, V3 p) {
+   return sdf_point_mask(S, p, SdfGlobalSph{S.sdf_sph});
 }
 
 // Sphere tracing from the entry of the field's box along the unit direction
@@ -372,18 +393,6 @@ struct SdfMarch {
 };
 
 // Returns false when the ray cannot meet the field in [tmin, tmax].
-struct SdfGlobalSph {
-   const float4* p;
-   __device__ __forceinline__ float4 operator()(int i) const { return gload(p + i); }
-};
-struct SdfLdsSph {
-   const __attribute__((address_space(3))) gvec4* p;
-   __device__ __forceinline__ float4 operator()(int i) const {
-      const gvec4 w = p[i];
-      return make_float4(w.x, w.y, w.z, w.w);
-   }
-};
-
 template <class SPH>
 __device__ __forceinline__ bool sdf_march_begin(const SceneView& S, const PrimRec& p, V3 o, V3 d, float tmin,
                                                 float tmax, SdfMarch& m, SPH sph) {
@@ -923,9 +932,26 @@ __device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
    return t * t * (3.0f - 2.0f * t);
 }
 
+// Where the normal of an SDF hit reads the field from: the scene's arrays, or the copy of
+// this field that the calling wave holds in LDS (`held`; the same values, so the same normal).
+struct ShadeField {
+   const __attribute__((address_space(3))) gvec4* rec = nullptr;
+   const __attribute__((address_space(3))) gvec4* sph = nullptr;
+   bool held = false;
+};
+
+// Central differences of the field at p (sdf.hpp:499-507).
+template <class SPH, class REC>
+__device__ __forceinline__ V3 sdf_gradient(const SceneView& S, V3 p, float e, SPH sph, REC rec) {
+   const uint64_t m = sdf_point_mask(S, p, sph);
+   return mk(sdf_field_near(S, p + mk(e, 0, 0), m, rec) - sdf_field_near(S, p - mk(e, 0, 0), m, rec),
+             sdf_field_near(S, p + mk(0, e, 0), m, rec) - sdf_field_near(S, p - mk(0, e, 0), m, rec),
+             sdf_field_near(S, p + mk(0, 0, e), m, rec) - sdf_field_near(S, p - mk(0, 0, e), m, rec));
+}
+
 // FT: the scene features (SceneFeature) whose code is compiled in.
 template <uint32_t FT = SF_ALL>
-__device__ inline Surf surface(const SceneView& S, const HitMin& h, V3 o, V3 d) {
+__device__ inline Surf surface(const SceneView& S, const HitMin& h, V3 o, V3 d, ShadeField fld = ShadeField{}) {
    const PrimRec p = gload(S.prims + h.slot);
    Surf s;
    s.info = __float_as_uint(p.a.w);
@@ -954,10 +980,8 @@ __device__ inline Surf surface(const SceneView& S, const HitMin& h, V3 o, V3 d) 
       s.tangent = unit(mk(p.b));
    } else if ((FT & SF_SDF) && kind == K_SDF) {  // central differences, sdf.hpp:499-507
       const float e = S.sdf_eps;
-      const uint64_t m = sdf_point_mask(S, s.p);
-      V3 g = mk(sdf_field_near(S, s.p + mk(e, 0, 0), m) - sdf_field_near(S, s.p - mk(e, 0, 0), m),
-                sdf_field_near(S, s.p + mk(0, e, 0), m) - sdf_field_near(S, s.p - mk(0, e, 0), m),
-                sdf_field_near(S, s.p + mk(0, 0, e), m) - sdf_field_near(S, s.p - mk(0, 0, e), m));
+      const V3 g = fld.held ? sdf_gradient(S, s.p, e, SdfLdsSph{fld.sph}, SdfLdsRecs{fld.rec})
+                            : sdf_gradient(S, s.p, e, SdfGlobalSph{S.sdf_sph}, SdfGlobalRecs{S.sdf});
       V3 n = unit(g);
       s.front = dot(d, n) < 0.0f;
       s.n = s.front ? n : -n;

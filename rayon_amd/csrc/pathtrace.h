@@ -69,7 +69,7 @@ __device__ inline void nee_request(const JobView& J, const Surf& s, V3 color, V3
 // the wavefront shade stage so both compute identical per-sample radiance.
 template <bool CNT, uint32_t FT = SF_ALL>
 __device__ inline bool shade_step(const JobView& J, const HitMin& h, int bounce, PathState& ps, Rng& R,
-                                  ShadowReq& sh, PathCounts& pc) {
+                                  ShadowReq& sh, PathCounts& pc, ShadeField fld = ShadeField{}) {
    const SceneView& S = J.scene;
    const bool nee = (J.flags & RAYON_FLAG_NEE) && S.num_lights > 0;
    sh.valid = false;
@@ -77,7 +77,7 @@ __device__ inline bool shade_step(const JobView& J, const HitMin& h, int bounce,
       ps.L = ps.L + ps.T * sky(J, ps.d);
       return false;
    }
-   const Surf s = surface<FT>(S, h, ps.o, ps.d);
+   const Surf s = surface<FT>(S, h, ps.o, ps.d, fld);
    if (!(s.info & F_VISIBLE) && bounce == 0) {  // cuda_raytracer.cuh:879-883
       ps.o = s.p + 0.01f * ps.d;
       return true;

@@ -1237,6 +1237,11 @@ __device__ __forceinline__ void flush_segments(const WfView& V, int job, unsigne
 // the next position of the shadow arrays; positions come from per-wave reservations.
 template <bool CNT, uint32_t FT>
 __global__ __launch_bounds__(kWfBlock, 4) void k_wf_shade(WfView V, int level) {
+   // SDF normals read the job's field from a per-wave LDS slot (kernels with SDF code only)
+   constexpr int kSlots = (FT & SF_SDF) ? kWfBlock / 64 : 1, kRecs = (FT & SF_SDF) ? kSdfMaskMax : 1;
+   __shared__ SdfRec frec[kSlots][kRecs];
+   __shared__ float4 fsph[kSlots][kRecs];
+   FieldSlot fs{frec[(FT & SF_SDF) ? threadIdx.x / 64 : 0], fsph[(FT & SF_SDF) ? threadIdx.x / 64 : 0]};
    const int lane = lane_id();
    const int cur = level & 1, nxt = cur ^ 1;
    unsigned* ctl = V.ctl + kCtl * level;
@@ -1294,6 +1299,12 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_wf_shade(WfView V, int level) {
             const bool mine = pending && (int)job == j0;
             if (j0 != tally_job) flush_segments(V, tally_job, tally_ext, tally_sh);
             tally_job = j0;
+            ShadeField fld;
+            if constexpr ((FT & SF_SDF) != 0) {
+               fld.held = hold_field(fs, mine ? &V.jobs[j0].scene : nullptr);
+               fld.rec = (const __attribute__((address_space(3))) gvec4*)fs.rec;
+               fld.sph = (const __attribute__((address_space(3))) gvec4*)fs.sph;
+            }
             if (mine) {
                pending = false;
                const JobView J = uniform_copy(V.jobs + j0);
@@ -1305,7 +1316,7 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_wf_shade(WfView V, int level) {
                h.slot = __float_as_int(hf.y);
                h.u = hf.z;
                h.v = hf.w;
-               cont = shade_step<CNT, FT>(J, h, level, ps, R, sh, pc);
+               cont = shade_step<CNT, FT>(J, h, level, ps, R, sh, pc, fld);
                cont = cont && level + 1 < J.max_depth;
                shadow = sh.valid;
                flags = (flags & ~kPrevLambert) | (ps.prev_lambert ? kPrevLambert : 0u);
