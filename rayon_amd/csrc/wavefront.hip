@@ -486,9 +486,9 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_sdf(WfView V, int level, int it
 // queue entries it has claimed; whenever at least V.refill of its lanes have no ray, those
 // lanes take the next entries of the pool (claiming another chunk when it runs out), so a
 // wave is not held by the slowest of 64 rays. Each lane visits the nodes closest_hit_bin /
-// any_hit_bin visit, with leaves postponed until every lane of the wave holds one, and tests
-// one leaf per iteration with the same tie rule, so every ray's result is what the
-// run-to-completion traversal returns.
+// any_hit_bin visit, with leaves postponed until nearly every lane of the wave holds one
+// (leaf_limit), and tests one leaf per iteration with the same tie rule, so every ray's
+// result is what the run-to-completion traversal returns.
 
 struct WavePool {
    unsigned next = 0, end = 0;  // wave-uniform: entries claimed and not yet handed out
