@@ -398,3 +398,34 @@ def test_binary_wavefront_leaf_exit_and_refill_keep_images(ctx, scene, kw):
         ctx.set_option(ra.OPT.TRACE_LEAF_EXIT, -1)
         b.close()
         q.close()
+
+
+def test_several_sdf_fields_in_one_pass_equal_alone(ctx):
+    """Marching and shading waves keep one SDF field in LDS and reload it when their rays
+    move to another field: jobs of three different fields batched in one pass render as
+    they do alone, bit for bit."""
+    gens = [ra.Generated.generate(SDF[0], seed, detail=SDF[2]) for seed in (11, 12, 13)]
+    scs = [ctx.scene(g.desc_ptr()) for g in gens]
+    try:
+        for sc in scs:
+            sc.set_batch_path(ra.PATH.WAVEFRONT)
+
+        def params(g, seed):
+            lf, la, up, fov = g.camera()
+            return ra.job_params(48, 32, ra.camera_frame(lf, la, up, fov, 48, 32), spp=16, seed=seed, max_depth=8)
+
+        alone = []
+        for k, (g, sc) in enumerate(zip(gens, scs)):
+            j = ctx.job(sc, params(g, 100 + k))
+            ctx.render_batched([j], 16)
+            alone.append((j.resolve_linear(), j.stats().segments))
+            j.close()
+        jobs = [ctx.job(sc, params(g, 100 + k)) for k, (g, sc) in enumerate(zip(gens, scs))]
+        ctx.render_batched(jobs, 16)
+        for j, (ref, segs) in zip(jobs, alone):
+            np.testing.assert_array_equal(j.resolve_linear(), ref)
+            assert j.stats().segments == segs
+            j.close()
+    finally:
+        for sc in scs:
+            sc.close()
