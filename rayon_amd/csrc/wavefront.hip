@@ -81,7 +81,7 @@ struct WfView {
    int levels;
    int chunk;  // 64-entry batches a wave claims per queue-head atomic (trace, marching)
    int shade_chunk;  // the same for the shade stage
-   int shadow_refill;  // shadow rays with lane refill (passes without SDF fields)
+   int shadow_refill;  // shadow rays with lane refill (no shadow ray may cross an SDF field)
    int dense;          // no job of the pass is adaptive: every level-0 position holds a path
                        // whose slot is the position, throughput 1 and flags 0 (not stored)
    int refill;         // idle lanes (of 64) at which a trace wave takes new rays
@@ -1599,6 +1599,17 @@ void launch_stage(Wave* W, int stage, K kernel, dim3 grid, dim3 block, hipStream
 
 }  // namespace
 
+// Shadow rays refill lanes unless one of them may cross an SDF field: a field marched inline
+// by lanes refilled at different times runs one lane at a time. Only jobs with next-event
+// estimation (and lights) cast shadow rays, so fields in other jobs' scenes do not matter
+// (the 256-job mix: Cornell jobs' shadow rays beside SDF jobs without NEE).
+static bool shadow_refill_ok(const std::vector<rayon_job*>& jobs) {
+   for (rayon_job* j : jobs)
+      if (j->scene->view.num_sdf > 0 && (j->params.flags & RAYON_FLAG_NEE) && j->scene->view.num_lights > 0)
+         return false;
+   return true;
+}
+
 // Enqueues every launch of one pass (the sequence a graph captures).
 static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jobs, const std::vector<int>& todo,
                                long long total, int levels, int nmega, int mega_tiles, int mbin, int mbin_tiles) {
@@ -1625,9 +1636,7 @@ static hipError_t enqueue_pass(rayon_ctx* ctx, const std::vector<rayon_job*>& jo
    V.njobs = (int)jobs.size();
    V.levels = levels;
    V.shade_chunk = W->opt_shade_chunk;
-   V.shadow_refill = 1;
-   for (rayon_job* j : jobs)
-      if (j->scene->view.num_sdf > 0) V.shadow_refill = 0;
+   V.shadow_refill = shadow_refill_ok(jobs) ? 1 : 0;
    // persistent grids: the resident workgroups, but no more than one lane per path of a
    // round (small passes then launch and drain quickly)
    const long long per_round = std::min<long long>(std::max(total, 1ll), W->round);
@@ -1937,6 +1946,7 @@ rayon_status render_batched(rayon_ctx* ctx, rayon_job* const* jobs_in, int n, in
       for (rayon_job* j : active)
          if (j->scene->view.num_sdf > 0) sdf_steps = std::max(sdf_steps, j->scene->view.sdf_steps);
       key.push_back(sdf_steps);
+      key.push_back(shadow_refill_ok(active));
       int depth = 0;  // picks the trace kernel's stack size
       for (rayon_job* j : active) depth = std::max(depth, j->scene->info.max_depth);
       key.push_back(depth <= kShortStack);
