@@ -351,13 +351,6 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_generate(WfView V, long long w0
    }
 }
 
-// SDF marching stage of extension rays: slice `it` takes up to kSdfSlice << it steps of
-// every ray in its queue (the last slice runs to the end); rays still marching go to the next
-// slice's queue, so each slice's waves hold only live rays. A hit replaces the ray's hit
-// record (the march ends at the closest other hit, and an exact tie goes to the field,
-// which has the largest original index). Queues hold the rays' positions in the level; the
-// march state lives in the shadow-ray arrays at the same position, free between the trace
-// stage (which consumed the previous level's shadow rays) and the shade stage.
 // Each marching wave keeps one field of at most kSdfMaskMax primitives in its own LDS slot
 // (records, and for the set-up the bounding spheres): every step of every march reads a
 // record per primitive near the ray, and from LDS those reads no longer wait on the cache
@@ -400,6 +393,13 @@ __device__ __forceinline__ int march_held(const FieldSlot& f, bool held, const S
    return sdf_march_run(S, o, d, m, steps, t);
 }
 
+// SDF marching stage of extension rays: slice `it` takes up to kSdfSlice << it steps of
+// every ray in its queue (the last slice runs to the end); rays still marching go to the next
+// slice's queue, so each slice's waves hold only live rays. A hit replaces the ray's hit
+// record (the march ends at the closest other hit, and an exact tie goes to the field,
+// which has the largest original index). Queues hold the rays' positions in the level; the
+// march state lives in the shadow-ray arrays at the same position, free between the trace
+// stage (which consumed the previous level's shadow rays) and the shade stage.
 template <bool CNT>
 __global__ __launch_bounds__(kWfBlock) void k_wf_sdf(WfView V, int level, int it, int last) {
    __shared__ SdfRec lrec[kWfBlock / 64][kSdfMaskMax];
